@@ -1,0 +1,212 @@
+/* pk.h — C ABI of the MI355X wavefront path tracer (libpike_amd.so).
+ *
+ * The JavaScript render passes (pike_amd/js, through the N-API addon in
+ * pike_amd/napi) and the Python harness (pike_amd/__init__.py, ctypes) call
+ * only these entry points.  Every call returns PK_OK (0) or a negative PK_E_*
+ * code; pk_last_error() gives the message of the last failure on the calling
+ * thread.  Handles are opaque.  Host arrays passed in are copied (or read to
+ * completion) before the call returns.  Device work is asynchronous on the
+ * context's stream; pk_sync() and the pk_read_* calls are the blocking ones.
+ * A context is thread-compatible, not thread-safe (one JS thread drives it,
+ * as in the reference).
+ *
+ * Reference interfaces each group replaces (apbodnar/Pike, paths relative to
+ * the repository root):
+ *   context            render/pike.js:92-118 (initWebGpu: adapter + device)
+ *   scene upload       render/passes/trace_pass.js:220-263 (initBVHBuffer),
+ *                      render/passes/shade_hit_pass.js:58-81,293-368 (atlas,
+ *                      attributes, materials, env tables),
+ *                      render/env_sampler.js:7-157 (EnvironmentGenerator)
+ *   BVH build          render/bvh.js:7-108 (BVH), :110-275 (Node)
+ *   frame / passes     render/passes/camera_pass.js:3-77 (RenderState),
+ *                      :79-221 (CameraPass), trace_pass.js:265-276,
+ *                      trace_shadow_pass.js:42-52, shade_hit_pass.js:369-382,
+ *                      shade_miss_pass.js:68-80, accumulate_pass.js:75-85,
+ *                      render/pike.js:120-162 (tick)
+ */
+#ifndef PK_H_
+#define PK_H_
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_OK 0
+#define PK_E_INVALID (-1)     /* bad argument or handle */
+#define PK_E_NO_DEVICE (-2)   /* HIP device missing or unusable */
+#define PK_E_OOM (-3)         /* device or host allocation failed */
+#define PK_E_HIP (-4)         /* a HIP runtime call failed */
+#define PK_E_LIMIT (-5)       /* size beyond a documented limit */
+#define PK_E_STATE (-6)       /* call out of order (e.g. trace before raygen) */
+#define PK_E_OVERFLOW (-7)    /* traversal stack overflow flagged by a kernel */
+
+/* texture atlas formats (render/util/texture_packer.js:118-136 produces RGBA8) */
+#define PK_TEX_RGBA8 0
+#define PK_TEX_RGBA32F 1
+/* environment formats: RGBE-in-alpha RGBA8 as env_sampler.js:8-48 reads it,
+ * or linear float RGB */
+#define PK_ENV_NONE 0
+#define PK_ENV_RGBE8 1
+#define PK_ENV_RGB32F 2
+/* environment importance sampler */
+#define PK_ENV_SAMPLER_BINS 0 /* equal-radiance bins, env_sampler.js:74-157 */
+#define PK_ENV_SAMPLER_CDF 1  /* marginal/conditional CDF, device binary search */
+
+typedef struct pk_ctx pk_ctx;
+typedef struct pk_scene pk_scene;
+typedef struct pk_frame pk_frame;
+typedef struct pk_bvh pk_bvh;
+
+/* Scene in the reference's host representation (render/scene.js:120-181):
+ * positions are world-space float64 (attributes[].pos), triangles are
+ * scene.indices {i0, i1, i2, matId}, attributes are the quantized
+ * {tangent, bitangent, normal (snorm8x4), uv (snorm16x2)} words, materials are
+ * {diffMap, metRoughMap, normMap, emitMap} atlas layer ids.  The atlas is
+ * tex_layers layers of tex_width x tex_height texels. */
+typedef struct pk_scene_desc {
+  uint32_t num_vertices;
+  const double* positions;    /* [num_vertices][3] */
+  const uint32_t* attributes; /* [num_vertices][4] */
+  uint32_t num_triangles;
+  const int32_t* triangles;   /* [num_triangles][4] */
+  uint32_t num_materials;
+  const int32_t* materials;   /* [num_materials][4] */
+  uint32_t tex_width, tex_height, tex_layers, tex_format;
+  const void* tex_data;       /* RGBA8: 4 B/texel, RGBA32F: 16 B/texel */
+  uint32_t env_width, env_height, env_format, env_sampler;
+  const void* env_data;       /* RGBE8: 4 B/texel, RGB32F: 12 B/texel */
+} pk_scene_desc;
+
+/* CameraState (render/util/structs.js:235-249, camera.wgsl:5-14), in the
+ * normalized scene space the tracer works in. */
+typedef struct pk_camera {
+  float origin[3];
+  float dir[3];
+  float fov;          /* radians */
+  float focal_depth;
+  float aperture;
+  float distortion;
+  float bokeh;
+} pk_camera;
+
+typedef struct pk_render_params {
+  uint32_t width, height;
+  uint32_t max_depth;     /* numBounces (render/pike.js:20) */
+  uint32_t first_sample;  /* RenderState.samples at the first frame */
+  uint32_t num_frames;    /* one sample per pixel per frame */
+  float env_theta;        /* RenderState.env_theta */
+  uint32_t use_graph;     /* replay frames through one captured hipGraph */
+} pk_render_params;
+
+/* Per-frame device counters, read back by pk_read_counters (the reference
+ * keeps num_hits/num_misses/num_rays/numShadowRays in RenderState). */
+typedef struct pk_counters {
+  uint64_t rays_traced;    /* closest-hit rays over the frames rendered */
+  uint64_t shadow_rays;    /* any-hit rays */
+  uint64_t hits;
+  uint64_t nodes_visited;  /* closest-hit kernel; only when node counting is enabled */
+  uint64_t tris_tested;
+  uint64_t shadow_nodes;   /* any-hit kernel */
+  uint64_t shadow_tris;
+  uint32_t stack_overflow; /* nonzero if any ray overflowed its stack */
+  uint32_t frames;
+} pk_counters;
+
+const char* pk_last_error(void);
+const char* pk_version(void);
+
+/* ---- context (render/pike.js:92-97) ---- */
+int pk_ctx_create(int device, pk_ctx** out);
+int pk_ctx_destroy(pk_ctx* ctx);
+int pk_sync(pk_ctx* ctx);
+
+/* ---- device buffers (GPUDevice.createBuffer / queue.writeBuffer /
+ * mapAsync in the reference's WebGPU use, e.g. structs.js:148-157) ---- */
+int pk_buffer_create(pk_ctx* ctx, size_t bytes, void** device_ptr);
+int pk_buffer_destroy(pk_ctx* ctx, void* device_ptr);
+int pk_buffer_write(pk_ctx* ctx, void* device_dst, const void* host_src, size_t bytes);  /* synchronous */
+int pk_buffer_read(pk_ctx* ctx, void* host_dst, const void* device_src, size_t bytes);   /* synchronous */
+
+/* ---- BVH (render/bvh.js) : host-side, no device needed ----
+ * Builds the SAH binary BVH with bvh.js's decisions (full sweep, float64,
+ * first minimum, leaf iff cost > N) and collapses it into the 128-byte wide
+ * layout the kernels traverse. */
+int pk_bvh_build(const double* positions, uint32_t num_vertices, const int32_t* triangles,
+                 uint32_t num_triangles, int num_threads, pk_bvh** out);
+int pk_bvh_destroy(pk_bvh* bvh);
+/* sizes: [0] binary nodes, [1] wide nodes, [2] leaf triangles, [3] binary depth,
+ * [4] wide depth, [5] largest leaf */
+int pk_bvh_info(const pk_bvh* bvh, int64_t* info6);
+/* The reference's flattened buffers (trace_pass.js:220-263): BFS nodes of 5
+ * words {child_base_idx, triangles, x, y, z} with snorm16 boxes, triangles
+ * {i0,i1,i2,mat} in leaf order, normalized float32 vertices padded to 16 B,
+ * and the normalizer (center xyz, scale) it applied. */
+int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, float* verts4,
+                            double* normalizer4);
+
+/* ---- scene (TracePass / ShadeHitPass / ShadeMissPass resources) ---- */
+int pk_scene_upload(pk_ctx* ctx, const pk_scene_desc* desc, const pk_bvh* bvh, pk_scene** out);
+int pk_scene_destroy(pk_scene* scene);
+/* environment sampling tables as built (for inspection): bins count, coords count */
+int pk_scene_env_info(const pk_scene* scene, int64_t* info4);
+/* Host-only environment tables (EnvironmentGenerator, env_sampler.js:74-157:
+ * createHistogramBuffer / createEnvCoordBuffer / createPdfTexture).  Call with
+ * NULL arrays to get info4 = {bins, coords, width, height}; then bins2 is
+ * [bins][2] {h0, h1}, coords2 [coords][2] {x, y}, pdf [height][width]. */
+int pk_env_tables(const pk_scene_desc* desc, int64_t* info4, int32_t* bins2, int32_t* coords2, float* pdf);
+
+/* ---- frame: RenderState + per-pass entry points ---- */
+int pk_frame_create(pk_ctx* ctx, uint32_t width, uint32_t height, pk_frame** out);
+int pk_frame_destroy(pk_frame* frame);
+int pk_frame_reset(pk_frame* frame);                       /* RenderState.resetSamples */
+int pk_frame_set_camera(pk_frame* frame, const pk_camera* cam);  /* CameraPass.setCameraState */
+int pk_frame_set_env_theta(pk_frame* frame, float theta); /* RenderState.setEnvRotation */
+int pk_frame_begin(pk_frame* frame, uint32_t sample);      /* RenderState.generateCommands + clears */
+int pk_raygen(pk_frame* frame);                            /* CameraPass.generateCommands */
+int pk_trace_closest(pk_frame* frame, pk_scene* scene, uint32_t bounce);  /* TracePass + miss shading */
+int pk_shade(pk_frame* frame, pk_scene* scene, uint32_t bounce);          /* ShadeHitPass */
+int pk_trace_shadow(pk_frame* frame, pk_scene* scene);     /* TraceShadowPass + miss shading */
+int pk_accumulate(pk_frame* frame);                        /* AccumulatePass */
+/* A whole run of frames (tick() x num_frames), optionally graph-replayed. */
+int pk_render(pk_frame* frame, pk_scene* scene, const pk_render_params* params);
+/* Running-mean image, float RGBA, width*height*16 bytes. */
+int pk_read_image(pk_frame* frame, float* rgba, size_t bytes);
+/* The current frame's color buffer (RenderState.colorBuffer) before accumulation. */
+int pk_read_color(pk_frame* frame, float* rgba, size_t bytes);
+int pk_read_counters(pk_frame* frame, pk_counters* out);
+/* Count BVH nodes visited and triangles tested in this frame's trace kernels
+ * (adds per-lane counters; the uncounted kernels are the timed ones). */
+int pk_frame_count_nodes(pk_frame* frame, int enable);
+/* Kernel-level timing of num_frames whole frames (no graph): HIP events
+ * around every launch on the context stream.  ms5 receives the total
+ * milliseconds spent in raygen, trace_closest, shade, trace_shadow and
+ * accumulate.  Renders exactly what pk_render renders. */
+int pk_render_profile(pk_frame* frame, pk_scene* scene, const pk_render_params* params, double* ms5);
+
+/* ---- standalone ray queries (traversal microbenchmark) ----
+ * rays: [n][8] floats {ox, oy, oz, 0, dx, dy, dz, 0} in normalized space.
+ * Outputs per ray: original triangle id (-1 on miss), t, bary mask
+ * (pack2x16unorm(u, v)).  any_hit selects occlusion (trace_shadow.wgsl
+ * semantics: epsilon 1e-7, returns whether anything is hit; id is -1 or the
+ * id of some hit triangle). */
+int pk_trace_rays(pk_ctx* ctx, pk_scene* scene, const float* rays, int64_t n, int any_hit,
+                  int32_t* ids, float* t, uint32_t* bary);
+/* Device-resident variant: pointers are device addresses (hipMalloc'd or
+ * torch tensors); asynchronous on the context stream.  count_nodes adds the
+ * per-ray node/triangle totals to the context's traversal counters. */
+int pk_trace_rays_device(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit,
+                         int32_t* d_ids, float* d_t, uint32_t* d_bary, int count_nodes);
+/* Traversal statistics of the last counted pk_trace_rays_device call:
+ * [0] nodes visited, [1] triangles tested. */
+int pk_ctx_trace_stats(pk_ctx* ctx, uint64_t* stats2);
+/* Time the device-resident trace kernel: runs it `iters` times with events on
+ * the context stream and returns the mean kernel milliseconds. */
+int pk_time_trace_rays(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit,
+                       int32_t* d_ids, float* d_t, uint32_t* d_bary, int iters, float* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+/* oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of apbodnar/Pike's wavefront path-tracing path, used by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker.  The product (libpike_amd.so) never links, loads or calls it.
+ *
+ * What it restates (reference file:line):
+ *   BVH build           render/bvh.js:7-215, render/primitives.js:3-44
+ *   flatten/quantize    render/passes/trace_pass.js:166-170, 202-263, bvh.js:46-74
+ *   closest hit         render/shader/trace.wgsl:83-213
+ *   any hit             render/shader/trace_shadow.wgsl:1, 146-207
+ *   camera              render/shader/camera.wgsl:46-92
+ *   shading             render/shader/shade_hit.wgsl:153-483
+ *   miss shading        render/shader/shade_miss.wgsl:37-57
+ *   accumulate          render/shader/accumulate.wgsl:20-34
+ *   env tables          render/env_sampler.js:31-157
+ *   frame loop          render/pike.js:120-162
+ * Deliberate, documented deviations from the reference (its defects) are the
+ * same ones the product makes: deterministic shading seeds, race-free
+ * accumulation, every miss shaded, correct last pixel.
+ */
+#ifndef ORACLE_H_
+#define ORACLE_H_
+#include <stdint.h>
+#include "../include/pk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_scene or_scene;
+
+or_scene* or_scene_create(const pk_scene_desc* desc);
+void or_scene_destroy(or_scene* s);
+/* Literal bvh.js build (float64 SAH); returns 0 or -1 on a degenerate node. */
+int or_bvh_build(or_scene* s, int num_threads);
+/* [0] nodes, [1] leaf triangles, [2] depth, [3] largest leaf */
+void or_bvh_info(const or_scene* s, int64_t* info4);
+/* BFS flatten + normalize + snorm16 quantize exactly as trace_pass.js does. */
+int or_bvh_serialize(const or_scene* s, int32_t* nodes5, int32_t* tris4, float* verts4, double* normalizer4);
+/* Ray queries. rays: [n][8] {ox,oy,oz,_,dx,dy,dz,_} (normalized space, f32).
+ * mode 0: literal reference traversal of the serialized 20-byte nodes
+ *         (ids are original triangle ids mapped through leaf order),
+ * mode 1: brute force over all triangles, canonical argmin (t, id),
+ * mode 2: canonical closest hit through a conservative float64-box BVH.
+ * any_hit selects trace_shadow semantics. flags[i] bit0 = literal stack overflow. */
+int or_trace(const or_scene* s, const float* rays, int64_t n, int mode, int any_hit, int num_threads,
+             int32_t* ids, float* t, uint32_t* bary, uint32_t* flags);
+/* Environment tables (env_sampler.js). info: [0] bins, [1] coords, [2] width, [3] height */
+int or_env_info(const or_scene* s, int64_t* info4);
+int or_env_tables(const or_scene* s, int32_t* bins2, int32_t* coords2, float* pdf);
+
+typedef struct or_render_params {
+  uint32_t width, height, max_depth, first_sample, num_frames;
+  float env_theta;
+  int precision;   /* 32: FP32 restatement bit-for-bit with the product, 64: FP64 oracle */
+  int num_threads;
+  /* optional pixel window [x0, x1) x [y0, y1) (all zero = whole image) */
+  uint32_t x0, x1, y0, y1;
+} or_render_params;
+
+/* Renders num_frames frames; writes the running mean (RGBA f32, A=1) over
+ * the window and, if color_out != NULL, the last frame's color buffer (rgb sum
+ * + contribution count).  Counters: [0] closest-hit rays, [1] shadow rays, [2] hits. */
+int or_render(const or_scene* s, const pk_camera* cam, const or_render_params* p, float* accum_inout,
+              float* color_out, uint64_t* counters3);
+
+/* pk_math contract probes for tests */
+void or_math_eval(int fn, const float* x, const float* y, float* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,149 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes wrapper of the oracle (oracle/liboracle.so).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+as the checker of the HIP path; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    lib.or_scene_create.restype = vp
+    lib.or_scene_create.argtypes = [vp]
+    lib.or_scene_destroy.argtypes = [vp]
+    lib.or_bvh_build.restype = i32
+    lib.or_bvh_build.argtypes = [vp, i32]
+    lib.or_bvh_info.argtypes = [vp, vp]
+    lib.or_bvh_serialize.restype = i32
+    lib.or_bvh_serialize.argtypes = [vp, vp, vp, vp, vp]
+    lib.or_trace.restype = i32
+    lib.or_trace.argtypes = [vp, vp, i64, i32, i32, i32, vp, vp, vp, vp]
+    lib.or_env_info.restype = i32
+    lib.or_env_info.argtypes = [vp, vp]
+    lib.or_env_tables.restype = i32
+    lib.or_env_tables.argtypes = [vp, vp, vp, vp]
+    lib.or_render.restype = i32
+    lib.or_render.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.or_math_eval.argtypes = [i32, vp, vp, vp, i64]
+    _lib = lib
+    return lib
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("first_sample", ctypes.c_uint32), ("num_frames", ctypes.c_uint32), ("env_theta", ctypes.c_float),
+                ("precision", ctypes.c_int), ("num_threads", ctypes.c_int), ("x0", ctypes.c_uint32),
+                ("x1", ctypes.c_uint32), ("y0", ctypes.c_uint32), ("y1", ctypes.c_uint32)]
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+MATH_FNS = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "acos": 4, "log2": 5, "exp2": 6, "pow": 7, "sqrt": 8,
+            "div": 9, "pcg": 10}
+
+
+def math_eval(fn, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(x if y is None else y, np.float32)
+    out = np.empty_like(x)
+    load().or_math_eval(MATH_FNS[fn], _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+class OracleScene:
+    """The restatement's view of a scene (same pk_scene_desc as the product)."""
+
+    def __init__(self, data, threads=None):
+        self.lib = load()
+        self.data = data
+        self.desc = data.desc()
+        self.h = self.lib.or_scene_create(ctypes.byref(self.desc))
+        self.threads = threads or min(16, os.cpu_count() or 1)
+        self.built = False
+
+    def build_bvh(self):
+        rc = self.lib.or_bvh_build(self.h, self.threads)
+        self.built = True
+        return rc
+
+    def info(self):
+        a = np.zeros(4, np.int64)
+        self.lib.or_bvh_info(self.h, _p(a))
+        return dict(nodes=int(a[0]), leaf_triangles=int(a[1]), depth=int(a[2]), largest_leaf=int(a[3]))
+
+    def serialize(self):
+        info = self.info()
+        nodes = np.zeros((info["nodes"], 5), np.int32)
+        tris = np.zeros((self.data.num_triangles, 4), np.int32)
+        verts = np.zeros((self.data.positions.shape[0], 4), np.float32)
+        norm = np.zeros(4, np.float64)
+        assert self.lib.or_bvh_serialize(self.h, _p(nodes), _p(tris), _p(verts), _p(norm)) == 0
+        return nodes, tris, verts, norm
+
+    def trace(self, rays, mode, any_hit=False):
+        """mode: 'literal' (reference traversal), 'brute', 'canonical'."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        ids = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        bary = np.empty(n, np.uint32)
+        flags = np.empty(n, np.uint32)
+        m = {"literal": 0, "brute": 1, "canonical": 2}[mode]
+        assert self.lib.or_trace(self.h, _p(rays), n, m, int(any_hit), self.threads, _p(ids), _p(t), _p(bary),
+                                 _p(flags)) == 0
+        return ids, t, bary, flags
+
+    def env_tables(self):
+        info = np.zeros(4, np.int64)
+        self.lib.or_env_info(self.h, _p(info))
+        bins = np.zeros((int(info[0]), 2), np.int32)
+        coords = np.zeros((int(info[1]), 2), np.int32)
+        pdf = np.zeros((int(info[3]), int(info[2])), np.float32)
+        self.lib.or_env_tables(self.h, _p(bins), _p(coords), _p(pdf))
+        return bins, coords, pdf
+
+    def render(self, cam, width, height, max_depth, first_sample=0, num_frames=1, env_theta=0.0, precision=32,
+               window=None, accum=None):
+        """Returns (running mean RGBA, last frame color RGBA, counters)."""
+        x0, x1, y0, y1 = window if window else (0, 0, 0, 0)
+        ww, wh = (x1 - x0, y1 - y0) if window else (width, height)
+        if accum is None:
+            accum = np.zeros((wh, ww, 4), np.float32)
+        color = np.zeros((wh, ww, 4), np.float32)
+        cnt = np.zeros(3, np.uint64)
+        p = RenderParams(width, height, max_depth, first_sample, num_frames, env_theta, precision, self.threads,
+                         x0, x1, y0, y1)
+        assert self.lib.or_render(self.h, ctypes.byref(cam), ctypes.byref(p), _p(accum), _p(color), _p(cnt)) == 0
+        return accum, color, cnt
+
+    def close(self):
+        if self.h:
+            self.lib.or_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,285 @@
+"""pike_amd — MI355X-native wavefront path tracer (Python harness).
+
+Thin, typed wrappers over the C ABI in include/pk.h.  Every call goes to
+libpike_amd.so (HIP kernels for gfx950); there is no CPU fallback: a missing
+library raises ImportError on use, a missing GPU raises PikeError with
+PK_E_NO_DEVICE.
+
+The object model mirrors the reference's render passes
+(render/passes/*.js): a Context (the WebGPU device, pike.js:92-97), a BVH
+(bvh.js), a Scene (TracePass/ShadeHitPass/ShadeMissPass resources), and a
+Frame (RenderState + CameraPass state) whose methods are the passes'
+generateCommands.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import Camera, Counters, RenderParams  # noqa: F401
+
+__all__ = ["PikeError", "Context", "BVH", "Scene", "Frame", "lib"]
+
+
+class PikeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+def lib():
+    return _ffi.load()
+
+
+def _check(rc, what):
+    if rc != _ffi.PK_OK:
+        err = lib().pk_last_error()
+        raise PikeError(rc, f"{what}: {err.decode() if err else 'error'}")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """A HIP device context (one stream)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().pk_ctx_create(device, ctypes.byref(h)), "pk_ctx_create")
+        self.h = h
+
+    def sync(self):
+        _check(lib().pk_sync(self.h), "pk_sync")
+
+    def close(self):
+        if self.h:
+            lib().pk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace_rays(self, scene, rays, any_hit=False):
+        """Closest (or any) hit of rays (n, 8) float32 {o, _, d, _} in the
+        normalized space.  Returns (ids int32, t float32, bary uint32)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        ids = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        bary = np.empty(n, np.uint32)
+        _check(lib().pk_trace_rays(self.h, scene.h, _ptr(rays), n, int(any_hit), _ptr(ids), _ptr(t), _ptr(bary)),
+               "pk_trace_rays")
+        return ids, t, bary
+
+    def trace_rays_device(self, scene, rays_ptr, n, any_hit, ids_ptr, t_ptr, bary_ptr, count_nodes=False):
+        _check(lib().pk_trace_rays_device(self.h, scene.h, rays_ptr, n, int(any_hit), ids_ptr, t_ptr, bary_ptr,
+                                          int(count_nodes)), "pk_trace_rays_device")
+
+    def trace_stats(self):
+        s = (ctypes.c_uint64 * 2)()
+        _check(lib().pk_ctx_trace_stats(self.h, s), "pk_ctx_trace_stats")
+        return int(s[0]), int(s[1])
+
+    def time_trace_rays(self, scene, rays_ptr, n, any_hit, ids_ptr, t_ptr, bary_ptr, iters):
+        ms = ctypes.c_float()
+        _check(lib().pk_time_trace_rays(self.h, scene.h, rays_ptr, n, int(any_hit), ids_ptr, t_ptr, bary_ptr, iters,
+                                        ctypes.byref(ms)), "pk_time_trace_rays")
+        return float(ms.value)
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context (createBuffer/writeBuffer/mapAsync)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().pk_buffer_create(ctx.h, self.nbytes, ctypes.byref(p)), "pk_buffer_create")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(ctx, arr.nbytes)
+        b.write(arr)
+        return b
+
+    def write(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().pk_buffer_write(self.ctx.h, self.ptr, _ptr(arr), arr.nbytes), "pk_buffer_write")
+
+    def read(self, dtype, count):
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().pk_buffer_read(self.ctx.h, _ptr(out), self.ptr, out.nbytes), "pk_buffer_read")
+        return out
+
+    def close(self):
+        if self.ptr and self.ctx.h:
+            lib().pk_buffer_destroy(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BVH:
+    """bvh.js SAH binary BVH + its 128-byte wide collapse (host side)."""
+
+    def __init__(self, positions, triangles, threads=0):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        tri = np.ascontiguousarray(triangles, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().pk_bvh_build(_ptr(pos), pos.shape[0], _ptr(tri), tri.shape[0], threads, ctypes.byref(h)),
+               "pk_bvh_build")
+        self.h = h
+        self.num_vertices = pos.shape[0]
+        self.num_triangles = tri.shape[0]
+
+    def info(self):
+        a = np.zeros(6, np.int64)
+        _check(lib().pk_bvh_info(self.h, _ptr(a)), "pk_bvh_info")
+        keys = ["binary_nodes", "wide_nodes", "leaf_triangles", "binary_depth", "wide_depth", "largest_leaf"]
+        return dict(zip(keys, (int(x) for x in a)))
+
+    def export_reference(self):
+        """The reference's flattened buffers (trace_pass.js:220-263)."""
+        info = self.info()
+        nodes = np.zeros((info["binary_nodes"], 5), np.int32)
+        tris = np.zeros((self.num_triangles, 4), np.int32)
+        verts = np.zeros((self.num_vertices, 4), np.float32)
+        norm = np.zeros(4, np.float64)
+        _check(lib().pk_bvh_export_reference(self.h, _ptr(nodes), _ptr(tris), _ptr(verts), _ptr(norm)),
+               "pk_bvh_export_reference")
+        return nodes, tris, verts, norm
+
+    def normalizer(self):
+        norm = np.zeros(4, np.float64)
+        _check(lib().pk_bvh_export_reference(self.h, None, None, None, _ptr(norm)), "pk_bvh_export_reference")
+        return norm
+
+    def close(self):
+        if self.h:
+            lib().pk_bvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Device-resident scene (geometry, wide BVH, materials, atlas, env tables)."""
+
+    def __init__(self, ctx, data, bvh):
+        self.ctx = ctx
+        self.desc = data.desc()
+        self._data = data
+        h = ctypes.c_void_p()
+        _check(lib().pk_scene_upload(ctx.h, ctypes.byref(self.desc), bvh.h, ctypes.byref(h)), "pk_scene_upload")
+        self.h = h
+
+    def env_info(self):
+        a = np.zeros(4, np.int64)
+        _check(lib().pk_scene_env_info(self.h, _ptr(a)), "pk_scene_env_info")
+        return dict(bins=int(a[0]), coords=int(a[1]), width=int(a[2]), height=int(a[3]))
+
+    def close(self):
+        if self.h:
+            lib().pk_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Frame:
+    """RenderState + camera + ray/hit queues for one resolution."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        h = ctypes.c_void_p()
+        _check(lib().pk_frame_create(ctx.h, width, height, ctypes.byref(h)), "pk_frame_create")
+        self.h = h
+
+    def set_camera(self, cam):
+        self._cam = cam
+        _check(lib().pk_frame_set_camera(self.h, ctypes.byref(cam)), "pk_frame_set_camera")
+
+    def set_env_theta(self, theta):
+        _check(lib().pk_frame_set_env_theta(self.h, float(theta)), "pk_frame_set_env_theta")
+
+    def reset(self):
+        _check(lib().pk_frame_reset(self.h), "pk_frame_reset")
+
+    def count_nodes(self, enable=True):
+        _check(lib().pk_frame_count_nodes(self.h, int(enable)), "pk_frame_count_nodes")
+
+    # per-pass entry points (CameraPass / TracePass / ShadeHitPass / ...)
+    def begin(self, sample):
+        _check(lib().pk_frame_begin(self.h, sample), "pk_frame_begin")
+
+    def raygen(self):
+        _check(lib().pk_raygen(self.h), "pk_raygen")
+
+    def trace_closest(self, scene, bounce):
+        _check(lib().pk_trace_closest(self.h, scene.h, bounce), "pk_trace_closest")
+
+    def shade(self, scene, bounce):
+        _check(lib().pk_shade(self.h, scene.h, bounce), "pk_shade")
+
+    def trace_shadow(self, scene):
+        _check(lib().pk_trace_shadow(self.h, scene.h), "pk_trace_shadow")
+
+    def accumulate(self):
+        _check(lib().pk_accumulate(self.h), "pk_accumulate")
+
+    def render(self, scene, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0, use_graph=True):
+        p = RenderParams(self.width, self.height, max_depth, first_sample, num_frames, env_theta, int(use_graph))
+        _check(lib().pk_render(self.h, scene.h, ctypes.byref(p)), "pk_render")
+
+    def profile(self, scene, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0):
+        p = RenderParams(self.width, self.height, max_depth, first_sample, num_frames, env_theta, 0)
+        ms = np.zeros(5, np.float64)
+        _check(lib().pk_render_profile(self.h, scene.h, ctypes.byref(p), _ptr(ms)), "pk_render_profile")
+        return dict(zip(["raygen", "trace_closest", "shade", "trace_shadow", "accumulate"], ms.tolist()))
+
+    def read_image(self):
+        out = np.empty((self.height, self.width, 4), np.float32)
+        _check(lib().pk_read_image(self.h, _ptr(out), out.nbytes), "pk_read_image")
+        return out
+
+    def read_color(self):
+        out = np.empty((self.height, self.width, 4), np.float32)
+        _check(lib().pk_read_color(self.h, _ptr(out), out.nbytes), "pk_read_color")
+        return out
+
+    def counters(self):
+        c = Counters()
+        _check(lib().pk_read_counters(self.h, ctypes.byref(c)), "pk_read_counters")
+        return {k: int(getattr(c, k)) for k, _ in Counters._fields_}
+
+    def close(self):
+        if self.h:
+            lib().pk_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

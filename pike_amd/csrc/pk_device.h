@@ -1,0 +1,296 @@
+// pk_device.h — gfx950 device code: wide-BVH traversal, intersection and
+// shading helpers used by the wavefront kernels in pk_kernels.hip.
+//
+// The f32 arithmetic follows the contract in pk_math.h (no contraction,
+// correctly rounded divide/sqrt, project-written transcendentals), so the
+// values written here equal the CPU restatement's bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pk_layout.h"
+#include "pk_math.h"
+
+namespace pk {
+
+using pkm::v3;
+
+struct DevScene {
+  const WNode* nodes;
+  const float4* tris;        // TriRecord as 3 float4, leaf order
+  const int4* tri_index;     // {i0, i1, i2, mat}, leaf order
+  const uint4* attrs;        // quantized vertex attributes
+  const int4* materials;     // {diffMap, metRoughMap, normMap, emitMap}
+  const void* tex;
+  uint32_t tw, th, tl, tfmt;
+  const float4* env_rad;     // decoded radiance (rgb, 0)
+  const int2* env_bins;
+  const int2* env_coords;
+  const float* env_pdf;
+  const float* row_cdf;
+  const float* col_cdf;
+  uint32_t ew, eh, nbins;
+  int env_mode;              // 0 none, 1 bins, 2 cdf
+};
+
+struct Hit {
+  float t;
+  uint32_t id;    // original triangle id (0xffffffff: none)
+  uint32_t leaf;  // leaf-order triangle index
+  uint32_t bary;
+};
+
+__device__ inline v3 ld3(float4 a) { return pkm::mk(a.x, a.y, a.z); }
+
+// Möller–Trumbore exactly as trace.wgsl:97-114, with e1/e2 precomputed.
+// Returns the distance if accepted (eps < t < kMaxT), otherwise kMaxT.
+__device__ inline float intersect_tri(const float4* tris, uint32_t k, v3 o, v3 d, float eps, uint32_t* id,
+                                      uint32_t* bary) {
+  float4 a = tris[3 * k], b = tris[3 * k + 1], c = tris[3 * k + 2];
+  v3 p0 = ld3(a), e1 = ld3(b), e2 = ld3(c);
+  *id = __float_as_uint(a.w);
+  v3 p = pkm::cross(d, e2);
+  float det = pkm::dot(e1, p);
+  if (pkm::abs_(det) < eps) return kMaxT;
+  float inv_det = 1.0f / det;
+  v3 tv = pkm::sub(o, p0);
+  float u = pkm::dot(tv, p) * inv_det;
+  if (u < 0.0f || u > 1.0f) return kMaxT;
+  v3 q = pkm::cross(tv, e1);
+  float v = pkm::dot(d, q) * inv_det;
+  if (v < 0.0f || u + v > 1.0f) return kMaxT;
+  float dist = pkm::dot(e2, q) * inv_det;
+  *bary = pkm::pack2x16unorm(u, v);
+  return dist > eps ? dist : kMaxT;
+}
+
+constexpr int kStackLds = 16;     // LDS entries per lane
+constexpr int kStackMax = 64;     // total entries per lane (LDS + global spill)
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// Culling bound: a box entered beyond the best hit by more than the
+// intersector's rounding cannot hold a better (t, id) hit.
+__device__ inline float cull_bound(float tbest) { return fmaf(tbest, 0x1.004p+0f, 0x1.0p-20f); }
+
+struct TraverseStats {
+  uint32_t nodes, tris;
+};
+
+// Wide-BVH traversal for one ray.  any_hit: stop at the first accepted
+// triangle (trace_shadow.wgsl semantics); otherwise the canonical closest hit
+// argmin (t, id).  The per-lane short stack lives in LDS (lds_node/lds_t,
+// [kStackLds][64] per wave) and spills to spill[] (kStackMax - kStackLds
+// entries per lane) when deeper.  Returns false on stack overflow.
+template <bool kAnyHit, bool kCount>
+__device__ inline bool traverse(const DevScene& S, v3 o, v3 d, Hit* out, uint32_t* lds_node, float* lds_t,
+                                uint2* spill, int lane, TraverseStats* st) {
+  const float eps = kAnyHit ? kEpsShadow : kEpsTrace;
+  float invx = 1.0f / (pkm::abs_(d.x) < 1e-30f ? pkm::copysign_(1e-30f, d.x) : d.x);
+  float invy = 1.0f / (pkm::abs_(d.y) < 1e-30f ? pkm::copysign_(1e-30f, d.y) : d.y);
+  float invz = 1.0f / (pkm::abs_(d.z) < 1e-30f ? pkm::copysign_(1e-30f, d.z) : d.z);
+  const bool negx = invx < 0.0f, negy = invy < 0.0f, negz = invz < 0.0f;
+  float tbest = kMaxT;
+  uint32_t best_id = 0xffffffffu, best_leaf = 0xffffffffu, best_bary = 0;
+  float cull = kAnyHit ? kMaxT * 2.0f : cull_bound(kMaxT);
+  int sp = 0;
+  bool overflow = false;
+  uint32_t node = 0;
+  bool have = true;
+  while (true) {
+    if (!have) {
+      // pop, skipping entries whose box starts beyond the culling bound
+      while (sp > 0) {
+        --sp;
+        uint32_t n;
+        float tn;
+        if (sp < kStackLds) {
+          n = lds_node[sp * 64 + lane];
+          tn = lds_t[sp * 64 + lane];
+        } else {
+          uint2 e = spill[(sp - kStackLds) * 64 + lane];
+          n = e.x;
+          tn = __uint_as_float(e.y);
+        }
+        if (tn <= cull) {
+          node = n;
+          have = true;
+          break;
+        }
+      }
+      if (!have) break;
+    }
+    have = false;
+    if (kCount) st->nodes++;
+    const uint4* np = reinterpret_cast<const uint4*>(S.nodes + node);
+    uint4 L0 = np[0], L1 = np[1], L2 = np[2], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
+    const float ox = __uint_as_float(L0.x), oy = __uint_as_float(L0.y), oz = __uint_as_float(L0.z);
+    const uint32_t ew = L0.w;
+    const int nchild = (int)(ew >> 24);
+    const float sx = __uint_as_float((ew & 0xffu) << 23);
+    const float sy = __uint_as_float(((ew >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((ew >> 16) & 0xffu) << 23);
+    const float ax = sx * invx, ay = sy * invy, az = sz * invz;
+    const float bx = (ox - o.x) * invx, by = (oy - o.y) * invy, bz = (oz - o.z) * invz;
+    // near/far plane bytes per axis by ray direction sign
+    const uint32_t nx0 = negx ? L4.x : L3.z, nx1 = negx ? L4.y : L3.w;
+    const uint32_t fx0 = negx ? L3.z : L4.x, fx1 = negx ? L3.w : L4.y;
+    const uint32_t ny0 = negy ? L5.x : L4.z, ny1 = negy ? L5.y : L4.w;
+    const uint32_t fy0 = negy ? L4.z : L5.x, fy1 = negy ? L4.w : L5.y;
+    const uint32_t nz0 = negz ? L6.x : L5.z, nz1 = negz ? L6.y : L5.w;
+    const uint32_t fz0 = negz ? L5.z : L6.x, fz1 = negz ? L5.w : L6.y;
+    const uint32_t child[8] = {L1.x, L1.y, L1.z, L1.w, L2.x, L2.y, L2.z, L2.w};
+    const uint32_t meta0 = L3.x, meta1 = L3.y;
+    float tn[8];
+    uint32_t hitmask = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int sh = (c & 3) * 8;
+      const uint32_t wnx = c < 4 ? nx0 : nx1, wfx = c < 4 ? fx0 : fx1;
+      const uint32_t wny = c < 4 ? ny0 : ny1, wfy = c < 4 ? fy0 : fy1;
+      const uint32_t wnz = c < 4 ? nz0 : nz1, wfz = c < 4 ? fz0 : fz1;
+      const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bx);
+      const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, by);
+      const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bz);
+      const float tfx = fmaf((float)((wfx >> sh) & 0xffu), ax, bx);
+      const float tfy = fmaf((float)((wfy >> sh) & 0xffu), ay, by);
+      const float tfz = fmaf((float)((wfz >> sh) & 0xffu), az, bz);
+      const float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+      const float tfar = fminf(fminf(tfx, tfy), tfz);
+      tn[c] = tnear;
+      const bool h = (c < nchild) && tnear <= tfar && tnear <= cull;
+      hitmask |= h ? (1u << c) : 0u;
+    }
+    // leaf children first
+    uint32_t leafmask = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
+      if (m) leafmask |= 1u << c;
+    }
+    uint32_t lm = hitmask & leafmask;
+    while (lm) {
+      const int c = __builtin_ctz(lm);
+      lm &= lm - 1;
+      const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
+      uint32_t first = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j == c) first = child[j];
+      for (uint32_t k = first; k < first + m; ++k) {
+        if (kCount) st->tris++;
+        uint32_t id, bary = 0;
+        const float t = intersect_tri(S.tris, k, o, d, eps, &id, &bary);
+        if (t < kMaxT && (t < tbest || (t == tbest && id < best_id))) {
+          tbest = t;
+          best_id = id;
+          best_leaf = k;
+          best_bary = bary;
+          if (kAnyHit) break;
+          cull = cull_bound(tbest);
+        }
+      }
+      if (kAnyHit && best_id != 0xffffffffu) break;
+    }
+    if (kAnyHit && best_id != 0xffffffffu) break;
+    // internal children: nearest next, the rest onto the stack
+    uint32_t im = hitmask & ~leafmask;
+    // re-check against a cull bound tightened by the leaves just tested
+    float near_t = 3.0e38f;
+    int near_c = -1;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if ((im >> c) & 1u) {
+        if (tn[c] > cull) {
+          im &= ~(1u << c);
+        } else if (tn[c] < near_t) {
+          near_t = tn[c];
+          near_c = c;
+        }
+      }
+    }
+    if (near_c >= 0) {
+      im &= ~(1u << near_c);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (c == near_c) node = child[c];
+      }
+      have = true;
+#pragma unroll
+      for (int c = 7; c >= 0; --c) {
+        if ((im >> c) & 1u) {
+          if (sp < kStackLds) {
+            lds_node[sp * 64 + lane] = child[c];
+            lds_t[sp * 64 + lane] = tn[c];
+          } else if (sp < kStackMax) {
+            spill[(sp - kStackLds) * 64 + lane] = make_uint2(child[c], __float_as_uint(tn[c]));
+          } else {
+            overflow = true;
+            --sp;
+          }
+          ++sp;
+        }
+      }
+    }
+  }
+  out->t = tbest;
+  out->id = best_id;
+  out->leaf = best_leaf;
+  out->bary = best_bary;
+  return !overflow;
+}
+
+// ---------------------------------------------------------------------------
+// Shading helpers (shade_hit.wgsl / shade_miss.wgsl semantics)
+// ---------------------------------------------------------------------------
+__device__ inline int wrapi(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// shade_miss.wgsl envColor (:37-43): nearest sampler, repeat addressing
+__device__ inline v3 env_color(const DevScene& S, float theta, v3 dir) {
+  if (S.env_mode == 0) return pkm::mk(0.0f, 0.0f, 0.0f);
+  float u = theta + pkm::atan2f_(dir.z, dir.x) / pkm::kTau;
+  float v = pkm::acosf_(dir.y) * pkm::kInvPi;
+  int ix = wrapi((int)floorf(u * (float)S.ew), (int)S.ew);
+  int iy = wrapi((int)floorf(v * (float)S.eh), (int)S.eh);
+  float4 L = S.env_rad[(size_t)iy * S.ew + ix];
+  return pkm::mk(L.x, L.y, L.z);
+}
+
+// envPdf (shade_hit.wgsl:202-210); texel index clamped into the map
+__device__ inline float env_pdf(const DevScene& S, float theta, v3 dir) {
+  float a = 1.0f + theta + pkm::atan2f_(dir.z, dir.x) / pkm::kTau;
+  float u = a - truncf(a);
+  float v = pkm::acosf_(dir.y) * pkm::kInvPi;
+  int ix = (int)(u * (float)S.ew), iy = (int)(v * (float)S.eh);
+  ix = min(max(ix, 0), (int)S.ew - 1);
+  iy = min(max(iy, 0), (int)S.eh - 1);
+  float sin_phi = pkm::sinf_(v * pkm::kPi);
+  return S.env_pdf[(size_t)iy * S.ew + ix] / sin_phi;
+}
+
+__device__ inline float texel(const DevScene& S, int layer, int x, int y, int c) {
+  size_t i = (((size_t)layer * S.th + (size_t)y) * S.tw + (size_t)x) * 4 + c;
+  if (S.tfmt == 0) return (float)((const uint8_t*)S.tex)[i] / 255.0f;
+  return ((const float*)S.tex)[i];
+}
+// textureSampleLevel, linear filter, repeat, level 0
+__device__ inline float4 sample_tex(const DevScene& S, int layer, float u, float v) {
+  float x = u * (float)S.tw - 0.5f, y = v * (float)S.th - 0.5f;
+  float fx0 = floorf(x), fy0 = floorf(y);
+  float fx = x - fx0, fy = y - fy0;
+  int x0 = wrapi((int)fx0, (int)S.tw), x1 = wrapi((int)fx0 + 1, (int)S.tw);
+  int y0 = wrapi((int)fy0, (int)S.th), y1 = wrapi((int)fy0 + 1, (int)S.th);
+  float r[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float a = texel(S, layer, x0, y0, c), b = texel(S, layer, x1, y0, c);
+    float cc = texel(S, layer, x0, y1, c), dd = texel(S, layer, x1, y1, c);
+    float top = a + (b - a) * fx;
+    float bot = cc + (dd - cc) * fx;
+    r[c] = top + (bot - top) * fy;
+  }
+  return make_float4(r[0], r[1], r[2], r[3]);
+}
+
+}  // namespace pk

@@ -1,0 +1,53 @@
+// pk_internal.h — host-side structures shared by the runtime translation units.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pk.h"
+#include "pk_layout.h"
+
+// Binary BVH node (render/bvh.js Node): float64 world-space bounds.
+struct BNode {
+  double lo[3], hi[3];
+  int32_t left = -1, right = -1;  // children (internal nodes)
+  uint32_t begin = 0, count = 0;  // leaf: range into pk_bvh::leaf_tris
+  uint8_t leaf = 0;
+  uint8_t split_axis = 0;
+  // Oversized leaves (count > kLeafMax) get virtual halves that only the wide
+  // collapse follows; the reference export still sees the original leaf.
+  int32_t vleft = -1, vright = -1;
+};
+
+struct pk_bvh {
+  uint32_t nv = 0, nt = 0;
+  std::vector<int32_t> tris;      // scene.indices [nt][4]
+  std::vector<BNode> nodes;       // root at 0
+  std::vector<uint32_t> leaf_tris;
+  int degenerate = 0;
+  int binary_depth = 0, wide_depth = 0, largest_leaf = 0;
+  int64_t num_binary = 0;         // nodes of the bvh.js tree (before virtual splits)
+  double norm_center[3] = {0, 0, 0};
+  double norm_scale = 1;
+  std::vector<float> verts;       // normalized f32 [nv][3]
+  // wide layout
+  std::vector<pk::WNode> wnodes;
+  std::vector<pk::TriRecord> wtris;
+  std::vector<int32_t> wtri_index;  // leaf order -> {i0,i1,i2,mat} [n][4]
+};
+
+// Environment importance-sampling tables (render/env_sampler.js).
+struct EnvTables {
+  uint32_t width = 0, height = 0;
+  std::vector<float> radiance;      // decoded rgb f32 [w*h][3] (shade_miss lookup)
+  std::vector<int32_t> bins;        // [nbins][2] {h0, h1}
+  std::vector<int32_t> coords;      // [w*h][2] {x, y}, radiance-descending
+  std::vector<float> pdf;           // [w*h]
+  // marginal/conditional CDF sampler
+  std::vector<float> row_cdf;       // [h+1]
+  std::vector<float> col_cdf;       // [h][w+1]
+};
+
+int pk_build_env_tables(const pk_scene_desc* d, EnvTables* out, std::string* err);
+void pk_set_error(const std::string& msg);

@@ -1,0 +1,72 @@
+// pk_kernels.h — device frame state and the host launch wrappers of
+// pk_kernels.hip (the runtime in pk_runtime.hip calls only these).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/pk.h"
+#include "pk_device.h"
+
+namespace pk {
+
+// device counters (uint32)
+enum : int {
+  kCntRays0 = 0,      // ray queue 0 length
+  kCntRays1 = 1,      // ray queue 1 length
+  kCntHits = 2,       // hit queue length
+  kCntShadow = 3,     // shadow queue length
+  kCntWorkTrace = 4,  // trace_closest work cursor
+  kCntWorkShadow = 5, // trace_shadow work cursor
+  kCntOverflow = 6,   // stack overflow flag
+  kCntRaysCursor = 7, // pk_trace_rays work cursor
+  kNumCounters = 16
+};
+// device statistics (uint64)
+enum : int {
+  kStatRays = 0,
+  kStatShadow = 1,
+  kStatHits = 2,
+  kStatNodes = 3,
+  kStatTris = 4,
+  kStatFrames = 5,
+  kStatShadowNodes = 6,
+  kStatShadowTris = 7,
+  kNumStats = 8
+};
+
+struct FrameParams {
+  uint32_t sample;
+  float env_theta;
+  pk_camera cam;
+};
+
+struct FrameDev {
+  float4* qo[2];   // ray queues: origin.xyz, pixel id bits
+  float4* qd[2];   //             direction.xyz
+  float4* thr;     // path throughput per pixel
+  uint4* hits;     // {ray slot, leaf triangle, t bits, bary mask}
+  float4* so;      // shadow queue: origin.xyz, pixel id bits
+  float4* sd;      //               direction.xyz
+  float4* sthr;    //               NEE throughput
+  float4* color;   // RenderState.colorBuffer (rgb sum, contribution count)
+  float4* accum;   // running mean
+  uint32_t* cnt;
+  unsigned long long* stats;
+  FrameParams* params;
+  uint2* spill;    // traversal stack spill, per wave
+  uint32_t W, H;
+};
+
+void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s);
+void launch_advance(const FrameDev& F, hipStream_t s);
+void launch_raygen(const FrameDev& F, hipStream_t s);
+void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, uint32_t grid, bool count, hipStream_t s);
+void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s);
+void launch_trace_shadow(const DevScene& S, const FrameDev& F, uint32_t grid, bool count, hipStream_t s);
+void launch_accumulate(const FrameDev& F, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
+void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
+                       uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats, uint32_t* overflow,
+                       uint32_t grid, bool count, hipStream_t s);
+
+}  // namespace pk

@@ -1,0 +1,630 @@
+// pk_kernels.hip — the wavefront path-tracing kernels for gfx950.
+//
+// One frame (render/pike.js:120-162) is
+//   raygen -> { trace_closest -> shade -> trace_shadow } x depth -> accumulate
+// with path state in SoA queues in HBM.  Differences from the reference's
+// WGSL dispatch structure, all MI355X-driven:
+//   * ray, hit and shadow queues are SoA float4/uint4 arrays written through
+//     wave-ballot compaction (one atomic per wave, coalesced stores);
+//   * miss shading is fused into the two trace kernels (no miss queue), and
+//     each pixel's color is written only by the thread that owns its path in
+//     that stage, so accumulation is race-free and deterministic
+//     (shade_miss.wgsl:57 is a racy +=);
+//   * the trace kernels are persistent: waves pull 64-ray batches from a
+//     device counter until the queue drains, so launch sizes never depend on
+//     the data-dependent ray counts;
+//   * counters are reset by the kernel that runs before their consumer, so a
+//     whole frame is kernels only and can be captured in one hipGraph.
+#include "pk_device.h"
+#include "pk_kernels.h"
+
+namespace pk {
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Wave-level compaction: returns this lane's slot in a queue whose counter is
+// `counter` (valid only where pred is true).
+__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  const uint32_t total = (uint32_t)__popcll(m);
+  const int lane = (int)__lane_id();
+  uint32_t base = 0;
+  if (total) {
+    const int leader = __ffsll((long long)m) - 1;
+    if (lane == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+  }
+  const unsigned long long below = m & ((1ull << lane) - 1ull);
+  return base + (uint32_t)__popcll(below);
+}
+
+// ---------------------------------------------------------------------------
+// camera.wgsl main (:75-92) / createPrimaryRay (:58-73)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
+  const uint32_t W = F.W, H = F.H, B = W * H;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const FrameParams P = *F.params;
+  if (idx == 0) {
+    F.cnt[kCntRays0] = B;
+    F.cnt[kCntHits] = 0;
+    F.cnt[kCntWorkTrace] = 0;
+  }
+  if (idx >= B) return;
+  const uint32_t px = idx % W, py = idx / W;
+  uint32_t seed = (px * 1973u + py * 9277u + P.sample * 26699u) | 1u;
+  seed = pkm::pcg_next(&seed);
+  const float k = P.cam.distortion;
+  const float resx = (float)W, resy = (float)H;
+  const float jx = pkm::pcg_rand(&seed);
+  const float jy = pkm::pcg_rand(&seed);
+  float ux = (2.0f * (((float)px + jx) / resx) - 1.0f) * (resx / resy);
+  float uy = (2.0f * (((float)py + jy) / resy) - 1.0f) * -1.0f;
+  const float rd = pkm::sqrt_(ux * ux + uy * uy);
+  const float ru = rd * (1.0f + k * rd * rd);
+  const float l2 = pkm::sqrt_(ux * ux + uy * uy);
+  ux = ru * (ux / l2);
+  uy = ru * (uy / l2);
+  const v3 up = pkm::mk(0.0f, 1.0f, 0.0f);
+  const v3 dir = pkm::mk(P.cam.dir[0], P.cam.dir[1], P.cam.dir[2]);
+  const v3 org = pkm::mk(P.cam.origin[0], P.cam.origin[1], P.cam.origin[2]);
+  const float tf = pkm::tanf_(P.cam.fov * 0.5f);
+  const v3 bx = pkm::scale(pkm::normalize(pkm::cross(dir, up)), tf);
+  const v3 by = pkm::scale(pkm::normalize(pkm::cross(bx, dir)), tf);
+  const float theta = pkm::pcg_rand(&seed) * pkm::kTau;
+  const float pw = pkm::powf_(pkm::pcg_rand(&seed), P.cam.bokeh);
+  float st, ct;
+  pkm::sincosf_(theta, &st, &ct);
+  const v3 dof = pkm::scale(pkm::scale(pkm::add(pkm::scale(bx, ct), pkm::scale(by, st)), P.cam.aperture), pw);
+  const v3 screen = pkm::add(pkm::add(pkm::add(pkm::scale(bx, ux), pkm::scale(by, uy)), dir), org);
+  const v3 d = pkm::normalize(pkm::sub(pkm::add(screen, pkm::scale(dof, P.cam.focal_depth)), pkm::add(org, dof)));
+  const v3 o = pkm::add(dof, org);
+  F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
+  F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
+  F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__device__ inline void add_color(const FrameDev& F, uint32_t pid, v3 L) {
+  float4 c = F.color[pid];
+  c.x += L.x;
+  c.y += L.y;
+  c.z += L.z;
+  c.w += 1.0f;
+  F.color[pid] = c;
+}
+
+// ---------------------------------------------------------------------------
+// trace.wgsl main (:191-213) + shade_miss.wgsl (:45-57) for the misses
+// ---------------------------------------------------------------------------
+template <bool kCount>
+__global__ void __launch_bounds__(64) k_trace_closest(DevScene S, FrameDev F, int bounce) {
+  __shared__ uint32_t lds_node[kStackLds * 64];
+  __shared__ float lds_t[kStackLds * 64];
+  const int lane = (int)threadIdx.x;
+  const int in = bounce & 1;
+  if (blockIdx.x == 0 && lane == 0) {
+    F.cnt[kCntRays0 + (in ^ 1)] = 0;
+    F.cnt[kCntShadow] = 0;
+  }
+  const uint32_t n = F.cnt[kCntRays0 + in];
+  const float theta = F.params->env_theta;
+  uint2* spill = F.spill + (size_t)blockIdx.x * (kStackMax - kStackLds) * 64;
+  TraverseStats st = {0, 0};
+  uint32_t traced = 0, hits = 0, overflow = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&F.cnt[kCntWorkTrace], 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t slot = base + (uint32_t)lane;
+    const bool active = slot < n;
+    Hit h;
+    h.id = 0xffffffffu;
+    float4 o4 = make_float4(0, 0, 0, 0), d4 = make_float4(0, 0, 1, 0);
+    if (active) {
+      o4 = F.qo[in][slot];
+      d4 = F.qd[in][slot];
+      const v3 o = pkm::mk(o4.x, o4.y, o4.z), d = pkm::mk(d4.x, d4.y, d4.z);
+      if (!traverse<false, kCount>(S, o, d, &h, lds_node, lds_t, spill, lane, &st)) overflow = 1;
+      traced++;
+    }
+    const bool is_hit = active && h.id != 0xffffffffu;
+    const uint32_t hs = wave_append(&F.cnt[kCntHits], is_hit);
+    if (is_hit) {
+      F.hits[hs] = make_uint4(slot, h.leaf, __float_as_uint(h.t), h.bary);
+      hits++;
+    } else if (active) {
+      const uint32_t pid = __float_as_uint(o4.w);
+      const float4 t4 = F.thr[pid];
+      const v3 L = pkm::mul(env_color(S, theta, pkm::mk(d4.x, d4.y, d4.z)), pkm::mk(t4.x, t4.y, t4.z));
+      add_color(F, pid, L);
+    }
+  }
+  traced = wave_sum(traced);
+  hits = wave_sum(hits);
+  if (kCount) {
+    st.nodes = wave_sum(st.nodes);
+    st.tris = wave_sum(st.tris);
+  }
+  overflow = wave_sum(overflow);
+  if (lane == 0) {
+    atomicAdd(&F.stats[kStatRays], (unsigned long long)traced);
+    atomicAdd(&F.stats[kStatHits], (unsigned long long)hits);
+    if (kCount) {
+      atomicAdd(&F.stats[kStatNodes], (unsigned long long)st.nodes);
+      atomicAdd(&F.stats[kStatTris], (unsigned long long)st.tris);
+    }
+    if (overflow) atomicOr(&F.cnt[kCntOverflow], 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// shade_hit.wgsl main (:407-483)
+// ---------------------------------------------------------------------------
+struct SI {
+  pkm::m3 basis;
+  v3 wo, origin, normal, base, spec, emission;
+  float metallic, opacity, alpha;
+};
+
+__device__ inline float ggx_d(v3 m, float au, float av) {
+  float auv = au * av;
+  float tx = m.x / au, ty = m.y / av;
+  float e = tx * tx + ty * ty + m.z * m.z;
+  return 1.0f / (pkm::kPi * auv * e * e);
+}
+__device__ inline float ggx_g1(v3 w, v3 m, float au, float av) {
+  float ax = w.x * au, ay = w.y * av;
+  float axy2 = ax * ax + ay * ay;
+  float tan_theta = axy2 / (w.z * w.z);
+  float r = 2.0f / (1.0f + pkm::sqrt_(1.0f + tan_theta));
+  return pkm::dot(w, m) * pkm::dot(w, pkm::mk(0.0f, 0.0f, 1.0f)) <= 0.0f ? 0.0f : r;
+}
+__device__ inline float lambert_pdf(v3 wi) { return pkm::max_(wi.z, kEpsShadow) * pkm::kInvPi; }
+__device__ inline float specular_pdf(const SI& si, v3 m) {
+  return pkm::max_(kEpsShadow, ggx_d(m, si.alpha, si.alpha) * ggx_g1(si.wo, m, si.alpha, si.alpha) / (4.0f * si.wo.z));
+}
+__device__ inline v3 eval_lambert(const SI& si, v3 wi, float pdf) {
+  float k = pkm::max_(kEpsShadow, wi.z);
+  return pkm::mk(si.base.x * pkm::kInvPi * k / pdf, si.base.y * pkm::kInvPi * k / pdf,
+                 si.base.z * pkm::kInvPi * k / pdf);
+}
+__device__ inline v3 eval_specular(const SI& si, v3 wi, float pdf) {
+  v3 h = pkm::normalize(pkm::add(si.wo, wi));
+  float d = ggx_d(h, si.alpha, si.alpha);
+  float g = ggx_g1(wi, h, si.alpha, si.alpha) * ggx_g1(si.wo, h, si.alpha, si.alpha);
+  float k = pkm::max_(d * g / (4.0f * si.wo.z * pdf), 0.0f);
+  return pkm::scale(si.spec, k);
+}
+__device__ inline float power_heuristic(float a, float b) {
+  float a2 = a * a;
+  return a2 / (a2 + b * b);
+}
+__device__ inline float schlick(float cos_theta, float ior) {
+  float r0 = (1.0f - ior) / (1.0f + ior);
+  r0 *= r0;
+  float tmp = 1.0f - cos_theta;
+  float tmp2 = tmp * tmp;
+  return r0 + (1.0f - r0) * tmp2 * tmp2 * tmp;
+}
+__device__ inline pkm::m3 onb(v3 n) {
+  float sign = n.z > 0.0f ? 1.0f : -1.0f;
+  float a = -1.0f / (sign + n.z);
+  float b = n.x * n.y * a;
+  pkm::m3 m;
+  m.c0 = pkm::mk(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  m.c1 = pkm::mk(b, sign + n.y * n.y * a, -n.y);
+  m.c2 = n;
+  return m;
+}
+__device__ inline v3 sample_ggx(const SI& si, uint32_t* seed) {
+  float au = si.alpha, av = si.alpha;
+  v3 vh = pkm::normalize(pkm::mk(au * si.wo.x, av * si.wo.y, si.wo.z));
+  float lensq = vh.x * vh.x + vh.y * vh.y;
+  v3 t1v = lensq > 0.0f ? pkm::scale(pkm::mk(-vh.y, vh.x, 0.0f), 1.0f / pkm::sqrt_(lensq)) : pkm::mk(1.0f, 0.0f, 0.0f);
+  v3 t2v = pkm::cross(vh, t1v);
+  float r = pkm::sqrt_(pkm::pcg_rand(seed));
+  float phi = pkm::kTau * pkm::pcg_rand(seed);
+  float sp, cp;
+  pkm::sincosf_(phi, &sp, &cp);
+  float t1 = r * cp;
+  float t2 = r * sp;
+  float sv = 0.5f * (1.0f + vh.z);
+  t2 = (1.0f - sv) * pkm::sqrt_(1.0f - t1 * t1) + sv * t2;
+  v3 nh = pkm::add(pkm::add(pkm::scale(t1v, t1), pkm::scale(t2v, t2)),
+                   pkm::scale(vh, pkm::sqrt_(pkm::max_(0.0f, 1.0f - t1 * t1 - t2 * t2))));
+  return pkm::normalize(pkm::mk(au * nh.x, av * nh.y, pkm::max_(0.0f, nh.z)));
+}
+
+__device__ inline v3 attr_vec(const DevScene& S, int vi, int word) {
+  uint4 a = S.attrs[vi];
+  uint32_t w = word == 0 ? a.x : (word == 1 ? a.y : a.z);
+  return pkm::unpack4x8snorm_xyz(w);
+}
+
+// sampleEnv (shade_hit.wgsl:213-227) for the bins sampler; the CDF sampler
+// draws two uniform numbers from the same two hashes and binary-searches the
+// marginal and conditional tables.
+__device__ inline void sample_env(const DevScene& S, float theta, const pkm::m3& basis, uint32_t* seed, v3* wi,
+                                  float* pdf) {
+  int cx, cy;
+  if (S.env_mode == 1) {
+    uint32_t bi = pkm::pcg_next(seed) % S.nbins;
+    int2 bin = S.env_bins[bi];
+    // bins are never empty (env_sampler.js closes a bin only after adding a
+    // texel); the max() only keeps a malformed table from dividing by zero
+    int ci = (int)(pkm::pcg_next(seed) % (uint32_t)max(bin.y - bin.x, 1)) + bin.x;
+    int2 c = S.env_coords[ci];
+    cx = c.x;
+    cy = c.y;
+  } else {
+    const float u1 = pkm::min_(pkm::pcg_rand(seed), 0x1.fffffep-1f);
+    const float u2 = pkm::min_(pkm::pcg_rand(seed), 0x1.fffffep-1f);
+    int lo = 0, hi = (int)S.eh;  // largest y with row_cdf[y] <= u1
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (S.row_cdf[mid] <= u1) lo = mid;
+      else hi = mid;
+    }
+    cy = lo;
+    const float* cc = S.col_cdf + (size_t)cy * (S.ew + 1);
+    lo = 0;
+    hi = (int)S.ew;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (cc[mid] <= u2) lo = mid;
+      else hi = mid;
+    }
+    cx = lo;
+  }
+  float u = -theta + ((0.5f + (float)cx) / (float)S.ew);
+  float v = (0.5f + (float)cy) / (float)S.eh;
+  float th = u * pkm::kTau;
+  float ph = v * pkm::kPi;
+  float sph, cph, sth, cth;
+  pkm::sincosf_(ph, &sph, &cph);
+  pkm::sincosf_(th, &sth, &cth);
+  v3 dir = pkm::mk(cth * sph, cph, sth * sph);
+  *pdf = S.env_pdf[(size_t)cy * S.ew + cx] / sph;
+  *wi = pkm::vec_mul(dir, basis);
+}
+
+__global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounce) {
+  const int in = bounce & 1;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx == 0) F.cnt[kCntWorkShadow] = 0;
+  const uint32_t nh = F.cnt[kCntHits];
+  const FrameParams P = *F.params;
+  const bool active = idx < nh;
+  bool emit_bounce = false, emit_shadow = false;
+  v3 b_o, b_d, b_thr, s_o, s_d, s_thr;
+  uint32_t pid = 0;
+  if (active) {
+    const uint4 hr = F.hits[idx];
+    const uint32_t slot = hr.x, leaf = hr.y, bary_mask = hr.w;
+    const float t = __uint_as_float(hr.z);
+    const float4 o4 = F.qo[in][slot], d4 = F.qd[in][slot];
+    pid = __float_as_uint(o4.w);
+    const v3 ro = pkm::mk(o4.x, o4.y, o4.z), rd = pkm::mk(d4.x, d4.y, d4.z);
+    const float4 t4 = F.thr[pid];
+    const v3 thr = pkm::mk(t4.x, t4.y, t4.z);
+    // deterministic seed: bounce index in place of the compaction slot
+    uint32_t seed = ((uint32_t)bounce * 1973u + pid * 9277u + P.sample * 26699u) | 1u;
+    seed = pkm::pcg_next(&seed);
+    // getSurfaceInteraction (:179-200), interpolateVertexAttribute (:363-394)
+    const int4 tri = S.tri_index[leaf];
+    const float bu = (float)(bary_mask & 0xffffu) / 65535.0f, bv = (float)(bary_mask >> 16) / 65535.0f;
+    const v3 bary = pkm::mk(1.0f - bu - bv, bu, bv);
+    v3 tng, btg, nrm;
+    {
+      pkm::m3 m;
+      m.c0 = attr_vec(S, tri.x, 0);
+      m.c1 = attr_vec(S, tri.y, 0);
+      m.c2 = attr_vec(S, tri.z, 0);
+      tng = pkm::normalize(pkm::mat_mul(m, bary));
+      m.c0 = attr_vec(S, tri.x, 1);
+      m.c1 = attr_vec(S, tri.y, 1);
+      m.c2 = attr_vec(S, tri.z, 1);
+      btg = pkm::normalize(pkm::mat_mul(m, bary));
+      m.c0 = attr_vec(S, tri.x, 2);
+      m.c1 = attr_vec(S, tri.y, 2);
+      m.c2 = attr_vec(S, tri.z, 2);
+      nrm = pkm::normalize(pkm::mat_mul(m, bary));
+    }
+    const uint32_t w0 = S.attrs[tri.x].w, w1 = S.attrs[tri.y].w, w2 = S.attrs[tri.z].w;
+    const float uvx = pkm::unpack_snorm16(w0 & 0xffffu) * bary.x + pkm::unpack_snorm16(w1 & 0xffffu) * bary.y +
+                      pkm::unpack_snorm16(w2 & 0xffffu) * bary.z;
+    const float uvy = pkm::unpack_snorm16(w0 >> 16) * bary.x + pkm::unpack_snorm16(w1 >> 16) * bary.y +
+                      pkm::unpack_snorm16(w2 >> 16) * bary.z;
+    const int4 mi = S.materials[tri.w];
+    const float4 nm = sample_tex(S, mi.z, uvx, uvy);
+    const v3 map_normal = pkm::mk((nm.x - 0.5f) * 2.0f, (nm.y - 0.5f) * 2.0f, (nm.z - 0.0f) * 1.0f);
+    SI si;
+    {
+      pkm::m3 tbn;
+      tbn.c0 = tng;
+      tbn.c1 = btg;
+      tbn.c2 = nrm;
+      si.normal = pkm::normalize(pkm::mat_mul(tbn, map_normal));
+    }
+    const float eps40 = kEpsShadow * 40.0f;
+    si.origin = pkm::add(ro, pkm::scale(rd, t - eps40));
+    si.basis = onb(si.normal);
+    si.wo = pkm::vec_mul(pkm::neg(rd), si.basis);
+    const float4 dc = sample_tex(S, mi.x, uvx, uvy);
+    si.base = pkm::mk(dc.x, dc.y, dc.z);
+    si.opacity = dc.w;
+    const float4 mr = sample_tex(S, mi.y, uvx, uvy);
+    si.metallic = mr.z;
+    si.alpha = mr.y * mr.y;
+    const float4 em = sample_tex(S, mi.w, uvx, uvy);
+    si.emission = pkm::mk(em.x, em.y, em.z);
+    si.spec = pkm::mix3(pkm::mk(1.0f, 1.0f, 1.0f), si.base, si.metallic);
+    if (pkm::pcg_rand(&seed) > si.opacity) {
+      // alpha pass-through (:426-431)
+      b_o = pkm::add(ro, pkm::scale(rd, t + kEpsShadow * 40.0f));
+      b_d = rd;
+      b_thr = thr;
+      emit_bounce = true;
+    } else if (si.emission.x * 1.0f + si.emission.y * 1.0f + si.emission.z * 1.0f > 0.0f) {
+      // emission (:434-438)
+      const float att = pkm::max_(pkm::dot(si.normal, pkm::mat_mul(si.basis, si.wo)), 0.0f);
+      const v3 L = pkm::scale(pkm::mul(pkm::mul(pkm::mk(50.0f, 50.0f, 50.0f), si.emission), thr), att);
+      add_color(F, pid, L);
+    } else {
+      const v3 m = sample_ggx(si, &seed);
+      const float f = pkm::mix(schlick(pkm::max_(pkm::dot(si.wo, m), 0.0f), 1.5f), 1.0f, si.metallic);
+      v3 wi, bsdf;
+      float pdf;
+      if (pkm::pcg_rand(&seed) > f) {
+        // sampleLambert (:243-253)
+        const float r = pkm::sqrt_(pkm::pcg_rand(&seed));
+        const float phi = pkm::kTau * pkm::pcg_rand(&seed);
+        float sp, cp;
+        pkm::sincosf_(phi, &sp, &cp);
+        const float x = r * cp, y = r * sp;
+        const float z = pkm::sqrt_(pkm::max_(0.0f, 1.0f - x * x - y * y));
+        wi = pkm::mk(x, y, z);
+        pdf = lambert_pdf(wi);
+        bsdf = eval_lambert(si, wi, pdf);
+      } else {
+        wi = pkm::reflect(pkm::neg(si.wo), m);
+        pdf = specular_pdf(si, m);
+        bsdf = eval_specular(si, wi, pdf);
+      }
+      if (wi.z > 0.0f) {
+        const v3 dir = pkm::mat_mul(si.basis, wi);
+        const float weight = S.env_mode ? power_heuristic(pdf, env_pdf(S, P.env_theta, dir)) : 1.0f;
+        b_o = si.origin;
+        b_d = dir;
+        b_thr = pkm::scale(pkm::mul(bsdf, thr), weight);
+        emit_bounce = true;
+      }
+      if (S.env_mode) {
+        v3 ewi;
+        float epdf;
+        sample_env(S, P.env_theta, si.basis, &seed, &ewi, &epdf);
+        if (pkm::dot(ewi, m) > 0.0f && ewi.z > 0.0f) {
+          // createMaterialSampleRay (:339-348)
+          const float lw = power_heuristic(epdf, lambert_pdf(ewi));
+          const v3 el = eval_lambert(si, ewi, epdf);
+          v3 sc = pkm::mk((1.0f - f) * el.x * lw, (1.0f - f) * el.y * lw, (1.0f - f) * el.z * lw);
+          const v3 hh = pkm::normalize(pkm::add(si.wo, ewi));
+          const float sw = power_heuristic(epdf, specular_pdf(si, hh));
+          const v3 ev = eval_specular(si, ewi, epdf);
+          sc = pkm::add(sc, pkm::mk(f * ev.x * sw, f * ev.y * sw, f * ev.z * sw));
+          s_o = si.origin;
+          s_d = pkm::mat_mul(si.basis, ewi);
+          s_thr = pkm::mul(sc, thr);
+          emit_shadow = true;
+        }
+      }
+    }
+  }
+  const uint32_t bs = wave_append(&F.cnt[kCntRays0 + (in ^ 1)], emit_bounce);
+  if (emit_bounce) {
+    F.qo[in ^ 1][bs] = make_float4(b_o.x, b_o.y, b_o.z, __uint_as_float(pid));
+    F.qd[in ^ 1][bs] = make_float4(b_d.x, b_d.y, b_d.z, 0.0f);
+    F.thr[pid] = make_float4(b_thr.x, b_thr.y, b_thr.z, 0.0f);
+  }
+  const uint32_t ss = wave_append(&F.cnt[kCntShadow], emit_shadow);
+  if (emit_shadow) {
+    F.so[ss] = make_float4(s_o.x, s_o.y, s_o.z, __uint_as_float(pid));
+    F.sd[ss] = make_float4(s_d.x, s_d.y, s_d.z, 0.0f);
+    F.sthr[ss] = make_float4(s_thr.x, s_thr.y, s_thr.z, 0.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// trace_shadow.wgsl main (:192-207) + shade_miss for unoccluded shadow rays
+// ---------------------------------------------------------------------------
+template <bool kCount>
+__global__ void __launch_bounds__(64) k_trace_shadow(DevScene S, FrameDev F) {
+  __shared__ uint32_t lds_node[kStackLds * 64];
+  __shared__ float lds_t[kStackLds * 64];
+  const int lane = (int)threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) {
+    F.cnt[kCntHits] = 0;
+    F.cnt[kCntWorkTrace] = 0;
+  }
+  const uint32_t n = F.cnt[kCntShadow];
+  const float theta = F.params->env_theta;
+  uint2* spill = F.spill + (size_t)blockIdx.x * (kStackMax - kStackLds) * 64;
+  TraverseStats st = {0, 0};
+  uint32_t traced = 0, overflow = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&F.cnt[kCntWorkShadow], 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t slot = base + (uint32_t)lane;
+    if (slot < n) {
+      const float4 o4 = F.so[slot], d4 = F.sd[slot];
+      const v3 o = pkm::mk(o4.x, o4.y, o4.z), d = pkm::mk(d4.x, d4.y, d4.z);
+      Hit h;
+      if (!traverse<true, kCount>(S, o, d, &h, lds_node, lds_t, spill, lane, &st)) overflow = 1;
+      traced++;
+      if (h.id == 0xffffffffu) {
+        const float4 t4 = F.sthr[slot];
+        const v3 L = pkm::mul(env_color(S, theta, d), pkm::mk(t4.x, t4.y, t4.z));
+        add_color(F, __float_as_uint(o4.w), L);
+      }
+    }
+  }
+  traced = wave_sum(traced);
+  if (kCount) {
+    st.nodes = wave_sum(st.nodes);
+    st.tris = wave_sum(st.tris);
+  }
+  overflow = wave_sum(overflow);
+  if (lane == 0) {
+    atomicAdd(&F.stats[kStatShadow], (unsigned long long)traced);
+    if (kCount) {
+      atomicAdd(&F.stats[kStatShadowNodes], (unsigned long long)st.nodes);
+      atomicAdd(&F.stats[kStatShadowTris], (unsigned long long)st.tris);
+    }
+    if (overflow) atomicOr(&F.cnt[kCntOverflow], 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// accumulate.wgsl main (:20-34): running mean over frames
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
+  const uint32_t B = F.W * F.H;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B) return;
+  const float s = (float)F.params->sample;
+  const float s1 = (float)(F.params->sample + 1u);
+  const float4 c = F.color[idx];
+  float4 a = F.accum[idx];
+  a.x = (pkm::max_(c.x, 0.0f) + a.x * s) / s1;
+  a.y = (pkm::max_(c.y, 0.0f) + a.y * s) / s1;
+  a.z = (pkm::max_(c.z, 0.0f) + a.z * s) / s1;
+  a.w = 1.0f;
+  F.accum[idx] = a;
+}
+
+// Frame bookkeeping kernels (no memset/memcpy nodes, so frames can be
+// captured and replayed as one graph).
+__global__ void k_begin(FrameDev F, uint32_t sample, float theta, pk_camera cam) {
+  F.params->sample = sample;
+  F.params->env_theta = theta;
+  F.params->cam = cam;
+}
+__global__ void k_advance(FrameDev F) {
+  F.params->sample += 1u;
+  F.stats[kStatFrames] += 1ull;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f4(float4* p, float4 v, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Standalone ray queries (traversal microbenchmark, pk_trace_rays)
+// ---------------------------------------------------------------------------
+template <bool kAnyHit, bool kCount>
+__global__ void __launch_bounds__(64) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int32_t* ids,
+                                                    float* tout, uint32_t* bary, uint32_t* cursor, uint2* spill_all,
+                                                    unsigned long long* stats, uint32_t* overflow_flag) {
+  __shared__ uint32_t lds_node[kStackLds * 64];
+  __shared__ float lds_t[kStackLds * 64];
+  const int lane = (int)threadIdx.x;
+  uint2* spill = spill_all + (size_t)blockIdx.x * (kStackMax - kStackLds) * 64;
+  TraverseStats st = {0, 0};
+  uint32_t overflow = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(cursor, 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t i = base + (uint32_t)lane;
+    if (i < n) {
+      const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
+      Hit h;
+      if (!traverse<kAnyHit, kCount>(S, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), &h, lds_node, lds_t,
+                                     spill, lane, &st))
+        overflow = 1;
+      ids[i] = (int32_t)h.id;
+      tout[i] = h.t;
+      bary[i] = h.bary;
+    }
+  }
+  if (kCount) {
+    st.nodes = wave_sum(st.nodes);
+    st.tris = wave_sum(st.tris);
+  }
+  overflow = wave_sum(overflow);
+  if (lane == 0) {
+    if (kCount) {
+      atomicAdd(&stats[0], (unsigned long long)st.nodes);
+      atomicAdd(&stats[1], (unsigned long long)st.tris);
+    }
+    if (overflow) atomicOr(overflow_flag, 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launch wrappers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, s, F, sample, theta, cam);
+}
+void launch_advance(const FrameDev& F, hipStream_t s) { hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, F); }
+void launch_raygen(const FrameDev& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+}
+void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, uint32_t grid, bool count,
+                          hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(64), 0, s, S, F, bounce);
+  else hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(64), 0, s, S, F, bounce);
+}
+void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
+  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, S, F, bounce);
+}
+void launch_trace_shadow(const DevScene& S, const FrameDev& F, uint32_t grid, bool count, hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid), dim3(64), 0, s, S, F);
+  else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid), dim3(64), 0, s, S, F);
+}
+void launch_accumulate(const FrameDev& F, hipStream_t s) {
+  hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+}
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_f4, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
+                       uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,
+                       uint32_t* overflow, uint32_t grid, bool count, hipStream_t s) {
+  if (any_hit) {
+    if (count)
+      hipLaunchKernelGGL((k_trace_rays<true, true>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
+                         spill, stats, overflow);
+    else
+      hipLaunchKernelGGL((k_trace_rays<true, false>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
+                         spill, stats, overflow);
+  } else {
+    if (count)
+      hipLaunchKernelGGL((k_trace_rays<false, true>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
+                         spill, stats, overflow);
+    else
+      hipLaunchKernelGGL((k_trace_rays<false, false>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
+                         spill, stats, overflow);
+  }
+}
+
+}  // namespace pk

@@ -1,0 +1,56 @@
+// pk_layout.h — device data layout shared by the host runtime and the kernels.
+//
+// Scene geometry lives in the normalized space trace_pass.js:202-211 maps the
+// scene into (longest axis spans [-1, 1]).  Everything the traversal reads is
+// packed to HBM-line granularity:
+//   * wide BVH node: 128 bytes = one L2/HBM line, 8 children with 8-bit
+//     quantized boxes (conservative), explicit child references;
+//   * triangle record: 48 bytes = {v0, id}, {e1, 0}, {e2, 0} in leaf order,
+//     e1 = v1 - v0 and e2 = v2 - v0 precomputed in f32 — the same rounding the
+//     reference intersector performs (trace.wgsl:98-99), so the Möller–Trumbore
+//     result is bit-identical while saving two vertex gathers per test.
+#pragma once
+#include <stdint.h>
+
+namespace pk {
+
+constexpr int kWide = 8;            // children per node
+constexpr int kLeafMax = 255;       // triangles per leaf child (u8 count)
+constexpr float kMaxT = 100000.0f;  // trace.wgsl:2
+constexpr float kEpsTrace = 0.00000001f;   // trace.wgsl:1
+constexpr float kEpsShadow = 0.0000001f;   // trace_shadow.wgsl:1, shade_hit.wgsl:1
+// Box quantization margin in normalized units (the scene spans 2.0): covers
+// the slab test's f32 rounding for ray origins within a few scene radii.
+constexpr double kBoxMargin = 1.0 / 131072.0;
+
+// 128-byte wide node.  Child i (i < nchild) has box
+//   lo = origin + qlo[i] * 2^(e-127),  hi = origin + qhi[i] * 2^(e-127)
+// per axis; meta[i] == 0 marks an internal child (child[i] = node index),
+// meta[i] = k > 0 a leaf of k triangles starting at triangle child[i].
+struct alignas(128) WNode {
+  float ox, oy, oz;       // 0
+  uint8_t ex, ey, ez;     // 12: biased exponents
+  uint8_t nchild;         // 15
+  uint32_t child[8];      // 16
+  uint8_t meta[8];        // 48
+  uint8_t qlox[8];        // 56
+  uint8_t qhix[8];        // 64
+  uint8_t qloy[8];        // 72
+  uint8_t qhiy[8];        // 80
+  uint8_t qloz[8];        // 88
+  uint8_t qhiz[8];        // 96
+  uint32_t pad[6];        // 104..127
+};
+static_assert(sizeof(WNode) == 128, "wide node must be one 128-byte line");
+
+struct TriRecord {        // 48 bytes
+  float v0x, v0y, v0z;
+  uint32_t id;            // original scene.indices position
+  float e1x, e1y, e1z;
+  uint32_t pad0;
+  float e2x, e2y, e2z;
+  uint32_t pad1;
+};
+static_assert(sizeof(TriRecord) == 48, "triangle record must be 48 bytes");
+
+}  // namespace pk

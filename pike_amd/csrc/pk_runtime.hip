@@ -1,0 +1,855 @@
+// pk_runtime.hip — the C ABI of include/pk.h: context, scene upload, frame
+// state, per-pass entry points, whole-frame rendering (optionally replayed
+// through one captured hipGraph) and standalone ray queries.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pk_internal.h"
+#include "pk_kernels.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+void pk_set_error(const std::string& msg) { g_err = msg; }
+
+#define PK_HIP(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      pk_set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));                          \
+      return e_ == hipErrorOutOfMemory ? PK_E_OOM : PK_E_HIP;                                          \
+    }                                                                                                  \
+  } while (0)
+
+struct pk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  uint32_t grid = 0;  // persistent trace grid (one wave per block)
+  uint32_t* cursors = nullptr;  // per-launch work cursors for ray queries
+  uint32_t ncursors = 0;
+  uint2* spill = nullptr;
+  unsigned long long* stats = nullptr;
+  uint32_t* overflow = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct pk_scene {
+  pk_ctx* ctx = nullptr;
+  pk::DevScene dev{};
+  std::vector<void*> allocs;
+  int64_t env_info[4] = {0, 0, 0, 0};
+};
+
+struct pk_frame {
+  pk_ctx* ctx = nullptr;
+  uint32_t W = 0, H = 0;
+  pk::FrameDev dev{};
+  std::vector<void*> allocs;
+  pk_camera cam{};
+  bool has_camera = false;
+  float theta = 0;
+  bool begun = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const pk_scene* graph_scene = nullptr;
+  uint32_t graph_depth = 0;
+  bool count_nodes = false;
+};
+
+namespace {
+
+template <class T> int dev_alloc(std::vector<void*>& list, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    pk_set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                 hipGetErrorString(e));
+    return PK_E_OOM;
+  }
+  list.push_back(*p);
+  return PK_OK;
+}
+
+template <class T> int upload(std::vector<void*>& list, T** p, const void* src, size_t count, hipStream_t s) {
+  int rc = dev_alloc(list, p, count);
+  if (rc) return rc;
+  if (src && count) PK_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  return PK_OK;
+}
+
+void free_all(std::vector<void*>& list) {
+  for (void* p : list) (void)hipFree(p);
+  list.clear();
+}
+
+int ensure_cursors(pk_ctx* c, uint32_t n) {
+  if (c->ncursors >= n) return PK_OK;
+  if (c->cursors) (void)hipFree(c->cursors);
+  c->cursors = nullptr;
+  PK_HIP(hipMalloc((void**)&c->cursors, (size_t)n * sizeof(uint32_t)));
+  c->ncursors = n;
+  return PK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pk_last_error(void) { return g_err.c_str(); }
+const char* pk_version(void) { return "pike_amd 0.1 (gfx950)"; }
+
+int pk_ctx_create(int device, pk_ctx** out) {
+  if (!out) {
+    pk_set_error("pk_ctx_create: null output");
+    return PK_E_INVALID;
+  }
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    pk_set_error("pk_ctx_create: no HIP device is visible");
+    return PK_E_NO_DEVICE;
+  }
+  if (device < 0 || device >= n) {
+    pk_set_error("pk_ctx_create: device index out of range");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  PK_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    pk_set_error(std::string("pk_ctx_create: device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    return PK_E_NO_DEVICE;
+  }
+  auto* c = new pk_ctx();
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  c->grid = (uint32_t)c->num_cus * 16u;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->spill, (size_t)c->grid * (pk::kStackMax - pk::kStackLds) * 64 * sizeof(uint2));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->stats, 8 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->overflow, 4 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("pk_ctx_create: ") + hipGetErrorString(e));
+    pk_ctx_destroy(c);
+    return PK_E_HIP;
+  }
+  int rc = ensure_cursors(c, 64);
+  if (rc) {
+    pk_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return PK_OK;
+}
+
+int pk_ctx_destroy(pk_ctx* c) {
+  if (!c) return PK_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->spill) (void)hipFree(c->spill);
+  if (c->stats) (void)hipFree(c->stats);
+  if (c->overflow) (void)hipFree(c->overflow);
+  if (c->cursors) (void)hipFree(c->cursors);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PK_OK;
+}
+
+int pk_sync(pk_ctx* c) {
+  if (!c) {
+    pk_set_error("pk_sync: null context");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipStreamSynchronize(c->stream));
+  return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+int pk_buffer_create(pk_ctx* c, size_t bytes, void** p) {
+  if (!c || !p) {
+    pk_set_error("pk_buffer_create: null argument");
+    return PK_E_INVALID;
+  }
+  *p = nullptr;
+  PK_HIP(hipSetDevice(c->device));
+  PK_HIP(hipMalloc(p, bytes ? bytes : 1));
+  return PK_OK;
+}
+
+int pk_buffer_destroy(pk_ctx* c, void* p) {
+  if (!c) {
+    pk_set_error("pk_buffer_destroy: null context");
+    return PK_E_INVALID;
+  }
+  if (p) {
+    PK_HIP(hipStreamSynchronize(c->stream));
+    PK_HIP(hipFree(p));
+  }
+  return PK_OK;
+}
+
+int pk_buffer_write(pk_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) {
+    pk_set_error("pk_buffer_write: null argument");
+    return PK_E_INVALID;
+  }
+  if (bytes) {
+    PK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    PK_HIP(hipStreamSynchronize(c->stream));
+  }
+  return PK_OK;
+}
+
+int pk_buffer_read(pk_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) {
+    pk_set_error("pk_buffer_read: null argument");
+    return PK_E_INVALID;
+  }
+  if (bytes) {
+    PK_HIP(hipStreamSynchronize(c->stream));
+    PK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(hipStreamSynchronize(c->stream));
+  }
+  return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scene
+// ---------------------------------------------------------------------------
+int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_scene** out) {
+  if (!c || !d || !bvh || !out) {
+    pk_set_error("pk_scene_upload: null argument");
+    return PK_E_INVALID;
+  }
+  *out = nullptr;
+  if (d->num_triangles != bvh->nt || d->num_vertices != bvh->nv) {
+    pk_set_error("pk_scene_upload: the BVH was built for a different triangle/vertex count");
+    return PK_E_INVALID;
+  }
+  const uint32_t nm = d->materials ? d->num_materials : 1;
+  const uint32_t layers = d->tex_data ? d->tex_layers : 1;
+  if (d->tex_data && (d->tex_width == 0 || d->tex_height == 0 || d->tex_layers == 0)) {
+    pk_set_error("pk_scene_upload: texture atlas has a zero dimension");
+    return PK_E_INVALID;
+  }
+  if (d->tex_data && d->tex_format != PK_TEX_RGBA8 && d->tex_format != PK_TEX_RGBA32F) {
+    pk_set_error("pk_scene_upload: unknown texture format");
+    return PK_E_INVALID;
+  }
+  for (uint32_t t = 0; t < d->num_triangles; ++t) {
+    int32_t m = d->triangles[(size_t)t * 4 + 3];
+    if (d->materials && (m < 0 || (uint32_t)m >= nm)) {
+      pk_set_error("pk_scene_upload: triangle " + std::to_string(t) + " has material id out of range");
+      return PK_E_INVALID;
+    }
+  }
+  if (d->materials)
+    for (uint32_t i = 0; i < nm * 4; ++i)
+      if (d->materials[i] < 0 || (uint32_t)d->materials[i] >= layers) {
+        pk_set_error("pk_scene_upload: material " + std::to_string(i / 4) + " references a missing atlas layer");
+        return PK_E_INVALID;
+      }
+  EnvTables env;
+  std::string err;
+  int rc = pk_build_env_tables(d, &env, &err);
+  if (rc) {
+    pk_set_error("pk_scene_upload: " + err);
+    return rc;
+  }
+  PK_HIP(hipSetDevice(c->device));
+  auto* s = new pk_scene();
+  s->ctx = c;
+  pk::DevScene& D = s->dev;
+  hipStream_t st = c->stream;
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    free_all(s->allocs);
+    delete s;
+    return code;
+  };
+  pk::WNode* nodes;
+  float4* tris;
+  int4* tidx;
+  if ((rc = upload(s->allocs, &nodes, bvh->wnodes.data(), bvh->wnodes.size(), st))) return fail(rc);
+  if ((rc = upload(s->allocs, &tris, bvh->wtris.data(), bvh->wtris.size() * 3, st))) return fail(rc);
+  if ((rc = upload(s->allocs, &tidx, bvh->wtri_index.data(), bvh->wtri_index.size() / 4, st))) return fail(rc);
+  D.nodes = nodes;
+  D.tris = tris;
+  D.tri_index = tidx;
+  std::vector<uint32_t> zero_attr;
+  const void* attr_src = d->attributes;
+  if (!attr_src) {
+    zero_attr.assign((size_t)d->num_vertices * 4, 0u);
+    attr_src = zero_attr.data();
+  }
+  uint4* attrs;
+  if ((rc = upload(s->allocs, &attrs, attr_src, d->num_vertices, st))) return fail(rc);
+  D.attrs = attrs;
+  const int32_t default_mat[4] = {0, 0, 0, 0};
+  int4* mats;
+  if ((rc = upload(s->allocs, &mats, d->materials ? (const void*)d->materials : (const void*)default_mat, nm, st)))
+    return fail(rc);
+  D.materials = mats;
+  if (d->tex_data) {
+    const size_t texels = (size_t)d->tex_width * d->tex_height * d->tex_layers;
+    if (d->tex_format == PK_TEX_RGBA8) {
+      uint8_t* t8;
+      if ((rc = upload(s->allocs, &t8, d->tex_data, texels * 4, st))) return fail(rc);
+      D.tex = t8;
+    } else {
+      float* t32;
+      if ((rc = upload(s->allocs, &t32, d->tex_data, texels * 4, st))) return fail(rc);
+      D.tex = t32;
+    }
+    D.tw = d->tex_width;
+    D.th = d->tex_height;
+    D.tl = d->tex_layers;
+    D.tfmt = d->tex_format;
+  } else {
+    static const uint8_t white[4] = {255, 255, 255, 255};
+    uint8_t* t8;
+    if ((rc = upload(s->allocs, &t8, white, 4, st))) return fail(rc);
+    D.tex = t8;
+    D.tw = D.th = D.tl = 1;
+    D.tfmt = PK_TEX_RGBA8;
+  }
+  D.env_mode = 0;
+  if (d->env_format != PK_ENV_NONE) {
+    const size_t T = (size_t)env.width * env.height;
+    std::vector<float> rad4(T * 4);
+    for (size_t i = 0; i < T; ++i) {
+      rad4[i * 4 + 0] = env.radiance[i * 3 + 0];
+      rad4[i * 4 + 1] = env.radiance[i * 3 + 1];
+      rad4[i * 4 + 2] = env.radiance[i * 3 + 2];
+      rad4[i * 4 + 3] = 0.0f;
+    }
+    float4* er;
+    int2 *eb, *ec;
+    float *ep, *rc_cdf = nullptr, *cc_cdf = nullptr;
+    if ((rc = upload(s->allocs, &er, rad4.data(), T, st))) return fail(rc);
+    if ((rc = upload(s->allocs, &eb, env.bins.data(), env.bins.size() / 2, st))) return fail(rc);
+    if ((rc = upload(s->allocs, &ec, env.coords.data(), env.coords.size() / 2, st))) return fail(rc);
+    if ((rc = upload(s->allocs, &ep, env.pdf.data(), env.pdf.size(), st))) return fail(rc);
+    if (d->env_sampler == PK_ENV_SAMPLER_CDF) {
+      if ((rc = upload(s->allocs, &rc_cdf, env.row_cdf.data(), env.row_cdf.size(), st))) return fail(rc);
+      if ((rc = upload(s->allocs, &cc_cdf, env.col_cdf.data(), env.col_cdf.size(), st))) return fail(rc);
+    }
+    D.env_rad = er;
+    D.env_bins = eb;
+    D.env_coords = ec;
+    D.env_pdf = ep;
+    D.row_cdf = rc_cdf;
+    D.col_cdf = cc_cdf;
+    D.ew = env.width;
+    D.eh = env.height;
+    D.nbins = (uint32_t)(env.bins.size() / 2);
+    D.env_mode = d->env_sampler == PK_ENV_SAMPLER_CDF ? 2 : 1;
+    s->env_info[0] = (int64_t)D.nbins;
+    s->env_info[1] = (int64_t)(env.coords.size() / 2);
+    s->env_info[2] = env.width;
+    s->env_info[3] = env.height;
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("pk_scene_upload: ") + hipGetErrorString(e));
+    return fail(PK_E_HIP);
+  }
+  *out = s;
+  return PK_OK;
+}
+
+int pk_scene_destroy(pk_scene* s) {
+  if (!s) return PK_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  free_all(s->allocs);
+  delete s;
+  return PK_OK;
+}
+
+int pk_scene_env_info(const pk_scene* s, int64_t* info4) {
+  if (!s || !info4) {
+    pk_set_error("pk_scene_env_info: null argument");
+    return PK_E_INVALID;
+  }
+  std::memcpy(info4, s->env_info, sizeof(s->env_info));
+  return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// frame
+// ---------------------------------------------------------------------------
+int pk_frame_create(pk_ctx* c, uint32_t W, uint32_t H, pk_frame** out) {
+  if (!c || !out) {
+    pk_set_error("pk_frame_create: null argument");
+    return PK_E_INVALID;
+  }
+  *out = nullptr;
+  if (W == 0 || H == 0 || (uint64_t)W * H >= (1ull << 28)) {
+    pk_set_error("pk_frame_create: resolution must be nonzero and below 2^28 pixels");
+    return PK_E_LIMIT;
+  }
+  PK_HIP(hipSetDevice(c->device));
+  auto* f = new pk_frame();
+  f->ctx = c;
+  f->W = W;
+  f->H = H;
+  const size_t B = (size_t)W * H;
+  pk::FrameDev& F = f->dev;
+  F.W = W;
+  F.H = H;
+  int rc = 0;
+  auto fail = [&](int code) {
+    free_all(f->allocs);
+    delete f;
+    return code;
+  };
+  for (int q = 0; q < 2; ++q) {
+    if ((rc = dev_alloc(f->allocs, &F.qo[q], B))) return fail(rc);
+    if ((rc = dev_alloc(f->allocs, &F.qd[q], B))) return fail(rc);
+  }
+  if ((rc = dev_alloc(f->allocs, &F.thr, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.hits, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.so, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.sd, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.sthr, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.color, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.accum, B))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.cnt, pk::kNumCounters))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.stats, pk::kNumStats))) return fail(rc);
+  if ((rc = dev_alloc(f->allocs, &F.params, 1))) return fail(rc);
+  F.spill = c->spill;
+  pk::launch_fill_f4(F.color, make_float4(0, 0, 0, 0), (uint32_t)B, c->stream);
+  pk::launch_fill_f4(F.accum, make_float4(0, 0, 0, 0), (uint32_t)B, c->stream);
+  pk::launch_fill_u32(F.cnt, 0u, pk::kNumCounters, c->stream);
+  pk::launch_fill_u32((uint32_t*)F.stats, 0u, pk::kNumStats * 2, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("pk_frame_create: ") + hipGetErrorString(e));
+    return fail(PK_E_HIP);
+  }
+  *out = f;
+  return PK_OK;
+}
+
+static void drop_graph(pk_frame* f) {
+  if (f->exec) (void)hipGraphExecDestroy(f->exec);
+  if (f->graph) (void)hipGraphDestroy(f->graph);
+  f->exec = nullptr;
+  f->graph = nullptr;
+  f->graph_scene = nullptr;
+}
+
+int pk_frame_destroy(pk_frame* f) {
+  if (!f) return PK_OK;
+  (void)hipSetDevice(f->ctx->device);
+  (void)hipStreamSynchronize(f->ctx->stream);
+  drop_graph(f);
+  free_all(f->allocs);
+  delete f;
+  return PK_OK;
+}
+
+int pk_frame_reset(pk_frame* f) {
+  if (!f) {
+    pk_set_error("pk_frame_reset: null frame");
+    return PK_E_INVALID;
+  }
+  const uint32_t B = f->W * f->H;
+  pk::launch_fill_f4(f->dev.accum, make_float4(0, 0, 0, 0), B, f->ctx->stream);
+  pk::launch_fill_f4(f->dev.color, make_float4(0, 0, 0, 0), B, f->ctx->stream);
+  pk::launch_fill_u32((uint32_t*)f->dev.stats, 0u, pk::kNumStats * 2, f->ctx->stream);
+  pk::launch_fill_u32(f->dev.cnt, 0u, pk::kNumCounters, f->ctx->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+int pk_frame_set_camera(pk_frame* f, const pk_camera* cam) {
+  if (!f || !cam) {
+    pk_set_error("pk_frame_set_camera: null argument");
+    return PK_E_INVALID;
+  }
+  f->cam = *cam;
+  f->has_camera = true;
+  return PK_OK;
+}
+
+int pk_frame_set_env_theta(pk_frame* f, float theta) {
+  if (!f) {
+    pk_set_error("pk_frame_set_env_theta: null frame");
+    return PK_E_INVALID;
+  }
+  f->theta = theta;
+  return PK_OK;
+}
+
+int pk_frame_begin(pk_frame* f, uint32_t sample) {
+  if (!f) {
+    pk_set_error("pk_frame_begin: null frame");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_frame_begin: no camera set (pk_frame_set_camera)");
+    return PK_E_STATE;
+  }
+  pk::launch_begin(f->dev, sample, f->theta, f->cam, f->ctx->stream);
+  f->begun = true;
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+static int check_pass(pk_frame* f, pk_scene* s, const char* what) {
+  if (!f || (s == nullptr && what[0] != 'r' && what[0] != 'a')) {
+    pk_set_error(std::string(what) + ": null argument");
+    return PK_E_INVALID;
+  }
+  if (s && s->ctx != f->ctx) {
+    pk_set_error(std::string(what) + ": scene and frame belong to different contexts");
+    return PK_E_INVALID;
+  }
+  if (!f->begun) {
+    pk_set_error(std::string(what) + ": frame not begun (pk_frame_begin)");
+    return PK_E_STATE;
+  }
+  return PK_OK;
+}
+
+int pk_raygen(pk_frame* f) {
+  int rc = check_pass(f, nullptr, "raygen");
+  if (rc) return rc;
+  pk::launch_raygen(f->dev, f->ctx->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+int pk_trace_closest(pk_frame* f, pk_scene* s, uint32_t bounce) {
+  int rc = check_pass(f, s, "pk_trace_closest");
+  if (rc) return rc;
+  pk::launch_trace_closest(s->dev, f->dev, (int)bounce, f->ctx->grid, f->count_nodes, f->ctx->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+int pk_shade(pk_frame* f, pk_scene* s, uint32_t bounce) {
+  int rc = check_pass(f, s, "pk_shade");
+  if (rc) return rc;
+  pk::launch_shade(s->dev, f->dev, (int)bounce, f->ctx->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+int pk_trace_shadow(pk_frame* f, pk_scene* s) {
+  int rc = check_pass(f, s, "pk_trace_shadow");
+  if (rc) return rc;
+  pk::launch_trace_shadow(s->dev, f->dev, f->ctx->grid, f->count_nodes, f->ctx->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+int pk_accumulate(pk_frame* f) {
+  int rc = check_pass(f, nullptr, "accumulate");
+  if (rc) return rc;
+  pk::launch_accumulate(f->dev, f->ctx->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+static void enqueue_frame(pk_frame* f, pk_scene* s, uint32_t depth, hipStream_t st) {
+  pk::launch_raygen(f->dev, st);
+  for (uint32_t b = 0; b < depth; ++b) {
+    pk::launch_trace_closest(s->dev, f->dev, (int)b, f->ctx->grid, f->count_nodes, st);
+    pk::launch_shade(s->dev, f->dev, (int)b, st);
+    pk::launch_trace_shadow(s->dev, f->dev, f->ctx->grid, f->count_nodes, st);
+  }
+  pk::launch_accumulate(f->dev, st);
+  pk::launch_advance(f->dev, st);
+}
+
+int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
+  if (!f || !s || !p) {
+    pk_set_error("pk_render: null argument");
+    return PK_E_INVALID;
+  }
+  if (p->width != f->W || p->height != f->H) {
+    pk_set_error("pk_render: params resolution differs from the frame's");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_render: no camera set (pk_frame_set_camera)");
+    return PK_E_STATE;
+  }
+  if (p->max_depth > 64) {
+    pk_set_error("pk_render: max_depth above 64");
+    return PK_E_LIMIT;
+  }
+  hipStream_t st = f->ctx->stream;
+  f->theta = p->env_theta;
+  pk::launch_begin(f->dev, p->first_sample, p->env_theta, f->cam, st);
+  f->begun = true;
+  if (!p->use_graph) {
+    for (uint32_t i = 0; i < p->num_frames; ++i) enqueue_frame(f, s, p->max_depth, st);
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+  }
+  if (!f->exec || f->graph_scene != s || f->graph_depth != p->max_depth) {
+    drop_graph(f);
+    PK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_frame(f, s, p->max_depth, st);
+    hipError_t e = hipStreamEndCapture(st, &f->graph);
+    if (e != hipSuccess) {
+      pk_set_error(std::string("pk_render: graph capture failed: ") + hipGetErrorString(e));
+      return PK_E_HIP;
+    }
+    PK_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
+    f->graph_scene = s;
+    f->graph_depth = p->max_depth;
+  }
+  for (uint32_t i = 0; i < p->num_frames; ++i) PK_HIP(hipGraphLaunch(f->exec, st));
+  return PK_OK;
+}
+
+int pk_read_image(pk_frame* f, float* rgba, size_t bytes) {
+  if (!f || !rgba || bytes < (size_t)f->W * f->H * 16) {
+    pk_set_error("pk_read_image: null argument or buffer smaller than width*height*16");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipStreamSynchronize(f->ctx->stream));
+  PK_HIP(hipMemcpy(rgba, f->dev.accum, (size_t)f->W * f->H * 16, hipMemcpyDeviceToHost));
+  return PK_OK;
+}
+
+int pk_read_color(pk_frame* f, float* rgba, size_t bytes) {
+  if (!f || !rgba || bytes < (size_t)f->W * f->H * 16) {
+    pk_set_error("pk_read_color: null argument or buffer smaller than width*height*16");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipStreamSynchronize(f->ctx->stream));
+  PK_HIP(hipMemcpy(rgba, f->dev.color, (size_t)f->W * f->H * 16, hipMemcpyDeviceToHost));
+  return PK_OK;
+}
+
+int pk_read_counters(pk_frame* f, pk_counters* out) {
+  if (!f || !out) {
+    pk_set_error("pk_read_counters: null argument");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipStreamSynchronize(f->ctx->stream));
+  unsigned long long st[pk::kNumStats];
+  uint32_t cnt[pk::kNumCounters];
+  PK_HIP(hipMemcpy(st, f->dev.stats, sizeof(st), hipMemcpyDeviceToHost));
+  PK_HIP(hipMemcpy(cnt, f->dev.cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  out->rays_traced = st[pk::kStatRays];
+  out->shadow_rays = st[pk::kStatShadow];
+  out->hits = st[pk::kStatHits];
+  out->nodes_visited = st[pk::kStatNodes];
+  out->tris_tested = st[pk::kStatTris];
+  out->shadow_nodes = st[pk::kStatShadowNodes];
+  out->shadow_tris = st[pk::kStatShadowTris];
+  out->frames = (uint32_t)st[pk::kStatFrames];
+  out->stack_overflow = cnt[pk::kCntOverflow];
+  return PK_OK;
+}
+
+int pk_frame_count_nodes(pk_frame* f, int enable) {
+  if (!f) {
+    pk_set_error("pk_frame_count_nodes: null frame");
+    return PK_E_INVALID;
+  }
+  if ((bool)enable != f->count_nodes) drop_graph(f);
+  f->count_nodes = enable != 0;
+  return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// standalone ray queries
+// ---------------------------------------------------------------------------
+static int trace_rays_launch(pk_ctx* c, pk_scene* s, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
+                             float* d_t, uint32_t* d_bary, int count, uint32_t cursor_index) {
+  pk::launch_trace_rays(s->dev, (const float4*)d_rays, (uint32_t)n, any_hit, d_ids, d_t, d_bary,
+                        c->cursors + cursor_index, c->spill, c->stats, c->overflow, c->grid, count != 0, c->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
+}
+
+static int check_rays(pk_ctx* c, pk_scene* s, int64_t n, const char* what) {
+  if (!c || !s) {
+    pk_set_error(std::string(what) + ": null context or scene");
+    return PK_E_INVALID;
+  }
+  if (s->ctx != c) {
+    pk_set_error(std::string(what) + ": scene belongs to another context");
+    return PK_E_INVALID;
+  }
+  if (n < 0 || n >= (1ll << 31)) {
+    pk_set_error(std::string(what) + ": ray count must be in [0, 2^31)");
+    return PK_E_LIMIT;
+  }
+  return PK_OK;
+}
+
+int pk_trace_rays_device(pk_ctx* c, pk_scene* s, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
+                         float* d_t, uint32_t* d_bary, int count_nodes) {
+  int rc = check_rays(c, s, n, "pk_trace_rays_device");
+  if (rc) return rc;
+  if (n == 0) return PK_OK;
+  if (!d_rays || !d_ids || !d_t || !d_bary) {
+    pk_set_error("pk_trace_rays_device: null buffer");
+    return PK_E_INVALID;
+  }
+  pk::launch_fill_u32(c->cursors, 0u, 1, c->stream);
+  pk::launch_fill_u32((uint32_t*)c->stats, 0u, 4, c->stream);
+  pk::launch_fill_u32(c->overflow, 0u, 1, c->stream);
+  return trace_rays_launch(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, count_nodes, 0);
+}
+
+int pk_ctx_trace_stats(pk_ctx* c, uint64_t* stats2) {
+  if (!c || !stats2) {
+    pk_set_error("pk_ctx_trace_stats: null argument");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipStreamSynchronize(c->stream));
+  unsigned long long st[2];
+  uint32_t ov = 0;
+  PK_HIP(hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
+  PK_HIP(hipMemcpy(&ov, c->overflow, 4, hipMemcpyDeviceToHost));
+  stats2[0] = st[0];
+  stats2[1] = st[1];
+  if (ov) {
+    pk_set_error("traversal stack overflow");
+    return PK_E_OVERFLOW;
+  }
+  return PK_OK;
+}
+
+int pk_trace_rays(pk_ctx* c, pk_scene* s, const float* rays, int64_t n, int any_hit, int32_t* ids, float* t,
+                  uint32_t* bary) {
+  int rc = check_rays(c, s, n, "pk_trace_rays");
+  if (rc) return rc;
+  if (n == 0) return PK_OK;
+  if (!rays || !ids || !t || !bary) {
+    pk_set_error("pk_trace_rays: null buffer");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipSetDevice(c->device));
+  std::vector<void*> tmp;
+  float4* d_rays;
+  int32_t* d_ids;
+  float* d_t;
+  uint32_t* d_b;
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(c->stream);
+    free_all(tmp);
+    return code;
+  };
+  if ((rc = upload(tmp, &d_rays, rays, (size_t)n * 2, c->stream))) return fail(rc);
+  if ((rc = dev_alloc(tmp, &d_ids, (size_t)n))) return fail(rc);
+  if ((rc = dev_alloc(tmp, &d_t, (size_t)n))) return fail(rc);
+  if ((rc = dev_alloc(tmp, &d_b, (size_t)n))) return fail(rc);
+  if ((rc = pk_trace_rays_device(c, s, (const float*)d_rays, n, any_hit, d_ids, d_t, d_b, 0))) return fail(rc);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(ids, d_ids, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(bary, d_b, (size_t)n * 4, hipMemcpyDeviceToHost);
+  uint32_t ov = 0;
+  if (e == hipSuccess) e = hipMemcpy(&ov, c->overflow, 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("pk_trace_rays: ") + hipGetErrorString(e));
+    return fail(PK_E_HIP);
+  }
+  free_all(tmp);
+  if (ov) {
+    pk_set_error("pk_trace_rays: traversal stack overflow");
+    return PK_E_OVERFLOW;
+  }
+  return PK_OK;
+}
+
+int pk_time_trace_rays(pk_ctx* c, pk_scene* s, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
+                       float* d_t, uint32_t* d_bary, int iters, float* ms_out) {
+  int rc = check_rays(c, s, n, "pk_time_trace_rays");
+  if (rc) return rc;
+  if (iters <= 0 || !ms_out || !d_rays || !d_ids || !d_t || !d_bary) {
+    pk_set_error("pk_time_trace_rays: bad iteration count or null buffer");
+    return PK_E_INVALID;
+  }
+  if ((rc = ensure_cursors(c, (uint32_t)iters))) return rc;
+  pk::launch_fill_u32(c->cursors, 0u, (uint32_t)iters, c->stream);
+  pk::launch_fill_u32(c->overflow, 0u, 1, c->stream);
+  PK_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < iters; ++i)
+    if ((rc = trace_rays_launch(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, 0, (uint32_t)i))) return rc;
+  PK_HIP(hipEventRecord(c->ev1, c->stream));
+  PK_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  PK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_out = ms / (float)iters;
+  return PK_OK;
+}
+
+// Kernel-level timing of whole frames: events around each stage on the
+// context stream; ms5 receives total milliseconds of raygen, trace_closest,
+// shade, trace_shadow and accumulate over num_frames frames.
+int pk_render_profile(pk_frame* f, pk_scene* s, const pk_render_params* p, double* ms5) {
+  if (!f || !s || !p || !ms5) {
+    pk_set_error("pk_render_profile: null argument");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_render_profile: no camera set");
+    return PK_E_STATE;
+  }
+  hipStream_t st = f->ctx->stream;
+  std::vector<hipEvent_t> ev;
+  const uint32_t per_frame = 2 + 3 * p->max_depth;  // launches timed per frame
+  const size_t nev = (size_t)p->num_frames * per_frame * 2;
+  ev.resize(nev);
+  for (auto& e : ev) PK_HIP(hipEventCreate(&e));
+  for (int k = 0; k < 5; ++k) ms5[k] = 0;
+  pk::launch_begin(f->dev, p->first_sample, p->env_theta, f->cam, st);
+  f->begun = true;
+  size_t ei = 0;
+  std::vector<int> kind;
+  for (uint32_t fr = 0; fr < p->num_frames; ++fr) {
+    auto timed = [&](int k, auto fn) {
+      (void)hipEventRecord(ev[ei++], st);
+      fn();
+      (void)hipEventRecord(ev[ei++], st);
+      kind.push_back(k);
+    };
+    timed(0, [&]() { pk::launch_raygen(f->dev, st); });
+    for (uint32_t b = 0; b < p->max_depth; ++b) {
+      timed(1, [&]() { pk::launch_trace_closest(s->dev, f->dev, (int)b, f->ctx->grid, f->count_nodes, st); });
+      timed(2, [&]() { pk::launch_shade(s->dev, f->dev, (int)b, st); });
+      timed(3, [&]() { pk::launch_trace_shadow(s->dev, f->dev, f->ctx->grid, f->count_nodes, st); });
+    }
+    timed(4, [&]() { pk::launch_accumulate(f->dev, st); });
+    pk::launch_advance(f->dev, st);
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  for (size_t i = 0; i < kind.size() && e == hipSuccess; ++i) {
+    float ms = 0;
+    e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+    ms5[kind[i]] += ms;
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("pk_render_profile: ") + hipGetErrorString(e));
+    return PK_E_HIP;
+  }
+  return PK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""Closest-hit and any-hit parity of the gfx950 traversal (pk_trace_rays)
+against the oracle's canonical closest hit: argmin (t, original triangle id)
+over every triangle with the reference's f32 Möller–Trumbore
+(trace.wgsl:97-114).  Bit-exact ids, t bits and bary masks."""
+import numpy as np
+import pytest
+
+import pike_amd
+from pike_amd import scenes
+from oracle.oracle import OracleScene
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ctx, data):
+    bvh = pike_amd.BVH(data.positions, data.triangles)
+    scene = pike_amd.Scene(ctx, data, bvh)
+    orc = OracleScene(data)
+    orc.build_bvh()
+    return bvh, scene, orc
+
+
+def _camera_rays(n, seed):
+    """Rays from outside the scene towards it (camera-like)."""
+    rng = scenes.Rng(seed)
+    o = np.stack([rng.uniform(n, -0.3, 0.3), rng.uniform(n, 0.2, 0.8), np.full(n, 1.8)], 1)
+    tgt = np.stack([rng.uniform(n, -0.8, 0.8), rng.uniform(n, -0.6, 0.2), rng.uniform(n, -0.8, 0.8)], 1)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 4:7] = d
+    return r
+
+
+def _assert_same(ids_g, t_g, b_g, ids_o, t_o, b_o):
+    bad = np.flatnonzero((ids_g != ids_o) | (t_g.view(np.uint32) != t_o.view(np.uint32)) | (b_g != b_o))
+    assert bad.size == 0, f"{bad.size} rays differ, first {bad[:5]}: gpu {ids_g[bad[:5]]} {t_g[bad[:5]]} " \
+                          f"oracle {ids_o[bad[:5]]} {t_o[bad[:5]]}"
+
+
+@pytest.mark.parametrize("seed", [1, 7])
+def test_closest_hit_config1_incoherent(ctx, seed):
+    data = scenes.config1(seed=seed)
+    bvh, scene, orc = _setup(ctx, data)
+    rays = scenes.incoherent_rays(50_000, bvh.normalizer(), seed=seed + 100)
+    ids, t, bary = ctx.trace_rays(scene, rays)
+    ids_o, t_o, b_o, _ = orc.trace(rays, "canonical")
+    assert (ids >= 0).mean() > 0.3
+    _assert_same(ids, t, bary, ids_o, t_o, b_o)
+
+
+def test_closest_hit_camera_rays_vs_brute_force(ctx):
+    data = scenes.config1(seed=3, num_spheres=12)
+    bvh, scene, orc = _setup(ctx, data)
+    rays = _camera_rays(4000, 9)
+    ids, t, bary = ctx.trace_rays(scene, rays)
+    ids_b, t_b, b_b, _ = orc.trace(rays, "brute")
+    _assert_same(ids, t, bary, ids_b, t_b, b_b)
+
+
+def test_any_hit_matches_occlusion(ctx):
+    data = scenes.config1(seed=2)
+    bvh, scene, orc = _setup(ctx, data)
+    rays = scenes.incoherent_rays(50_000, bvh.normalizer(), seed=77)
+    ids, t, _ = ctx.trace_rays(scene, rays, any_hit=True)
+    ids_o, _, _, _ = orc.trace(rays, "canonical", any_hit=True)
+    np.testing.assert_array_equal(ids >= 0, ids_o >= 0)
+    # a reported occluder really is hit (epsilon 1e-7 semantics)
+    hit = np.flatnonzero(ids >= 0)
+    assert np.all(t[hit] > 1e-7) and np.all(t[hit] < 1e5)
+
+
+def test_triangle_soup_against_canonical(ctx):
+    data = scenes.config3(seed=11, num_tris=200_000)
+    bvh, scene, orc = _setup(ctx, data)
+    rays = scenes.incoherent_rays(100_000, bvh.normalizer(), seed=12)
+    ids, t, bary = ctx.trace_rays(scene, rays)
+    ids_o, t_o, b_o, _ = orc.trace(rays, "canonical")
+    _assert_same(ids, t, bary, ids_o, t_o, b_o)
+    occ, _, _ = ctx.trace_rays(scene, rays, any_hit=True)
+    occ_o, _, _, _ = orc.trace(rays, "canonical", any_hit=True)
+    np.testing.assert_array_equal(occ >= 0, occ_o >= 0)
+
+
+def test_edge_cases(ctx):
+    data = scenes.config1(seed=4, num_spheres=4)
+    bvh, scene, orc = _setup(ctx, data)
+    rays = np.zeros((6, 8), np.float32)
+    rays[0, 4:7] = (0, 0, 1)          # axis-aligned, zero components
+    rays[1, 4:7] = (0, -1, 0)         # straight down at the ground
+    rays[2, 0:3] = (0, 5, 0)          # far outside, pointing away
+    rays[2, 4:7] = (0, 1, 0)
+    rays[3, 4:7] = (1e-30, 1, 0)      # tiny component
+    rays[4, 0:3] = (0.1, 0.1, 0.1)
+    rays[4, 4:7] = (-0.577, -0.577, -0.577)
+    rays[5, 4:7] = (0, 0, -1)
+    ids, t, bary = ctx.trace_rays(scene, rays)
+    ids_b, t_b, b_b, _ = orc.trace(rays, "brute")
+    _assert_same(ids, t, bary, ids_b, t_b, b_b)
+    # empty input is a no-op
+    e = ctx.trace_rays(scene, np.zeros((0, 8), np.float32))
+    assert e[0].size == 0
+
+
+def test_node_counters_match_between_runs(ctx):
+    """Counting kernels report deterministic totals (the traversal is
+    deterministic per ray)."""
+    data = scenes.config1(seed=5)
+    bvh, scene, orc = _setup(ctx, data)
+    host_rays = scenes.incoherent_rays(20_000, bvh.normalizer(), seed=5)
+    n = host_rays.shape[0]
+    rays = pike_amd.DeviceBuffer.from_array(ctx, host_rays)
+    ids = pike_amd.DeviceBuffer(ctx, 4 * n)
+    t = pike_amd.DeviceBuffer(ctx, 4 * n)
+    b = pike_amd.DeviceBuffer(ctx, 4 * n)
+    stats = []
+    for _ in range(2):
+        ctx.trace_rays_device(scene, rays.ptr, n, False, ids.ptr, t.ptr, b.ptr, True)
+        stats.append(ctx.trace_stats())
+    assert stats[0] == stats[1] and stats[0][0] > n
+    ids_o, t_o, b_o, _ = orc.trace(host_rays, "canonical")
+    _assert_same(ids.read(np.int32, n), t.read(np.float32, n), b.read(np.uint32, n), ids_o, t_o, b_o)
+    # the timed (uncounted) kernel gives the same answers
+    ms = ctx.time_trace_rays(scene, rays.ptr, n, False, ids.ptr, t.ptr, b.ptr, 2)
+    assert ms > 0
+    _assert_same(ids.read(np.int32, n), t.read(np.float32, n), b.read(np.uint32, n), ids_o, t_o, b_o)
