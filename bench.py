@@ -70,6 +70,12 @@ def run_render(args, pike_amd, scenes):
         f"overflow={cnt['stack_overflow']}")
     # kernel-level profile: counted run for algorithmic bytes, uncounted for time
     pf = args.profile_frames
+    if pf <= 0:
+        return dict(spp_s=spp_s, ms_per_step=dt / args.steps * 1e3, rays_per_s=rays_total / dt, counters=cnt,
+                    kernel_ms=None, profile_frames=0, profile_counters=None,
+                    roofline=dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
+                                  traffic=None, bytes_per_launch=None, launch_ms=None),
+                    triangles=data.num_triangles), data, bvh, cam
     frame.reset()
     frame.count_nodes(True)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=pf, use_graph=False)
@@ -101,18 +107,20 @@ def run_cpu_baseline(args, data, cam):
     orc = OracleScene(data, threads=threads)
     orc.build_bvh()
     log(f"oracle BVH ({time.time() - t0:.1f}s)")
-    rows = args.cpu_rows
-    y0 = args.height // 2 - rows // 2
-    win = (0, args.width, y0, y0 + rows)
+    # one whole frame to size the sample, then whole frames up to ~cpu_seconds
     t0 = time.perf_counter()
-    orc.render(cam, args.width, args.height, args.depth, 0, 1, precision=32, window=win)
+    orc.render(cam, args.width, args.height, args.depth, 0, 1, precision=32)
+    t1 = time.perf_counter() - t0
+    frames = int(max(1, min(args.steps, round(args.cpu_seconds / max(t1, 1e-3)))))
+    t0 = time.perf_counter()
+    orc.render(cam, args.width, args.height, args.depth, 1, frames, precision=32)
     dt = time.perf_counter() - t0
-    frac = (args.width * rows) / (args.width * args.height)
-    v = frac / dt
-    log(f"cpu baseline: {args.width}x{rows} band, 1 frame in {dt:.2f}s on {threads} threads -> {v:.4f} spp/s")
+    v = frames / dt
+    log(f"cpu baseline: {frames} frames of {args.width}x{args.height} in {dt:.2f}s on {threads} threads "
+        f"-> {v:.3f} spp/s")
     return dict(value=v, unit="spp/s", cores=threads, kind="port",
-                sample=f"oracle FP32 restatement, {args.width}x{rows} central band of the 1920x1080 frame, "
-                       f"1 sample per pixel, depth {args.depth}, scaled to the full frame")
+                sample=f"oracle FP32 restatement (C++, std::thread), {frames} whole {args.width}x{args.height} "
+                       f"frames (1 sample per pixel each, depth {args.depth}) of the same scene and camera")
 
 
 def run_traversal(args, pike_amd, scenes):
@@ -154,12 +162,15 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--profile-frames", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=24)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
     ap.add_argument("--trace-iters", type=int, default=5)
+    ap.add_argument("--mode", choices=["full", "render-only", "copy"], default="full",
+                    help="render-only: the timed frames alone (for rocprofv3 runs); copy: a 1 GiB device copy "
+                         "through the library's streaming copy kernel (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -167,6 +178,20 @@ def main():
         sys.exit(2)
     import pike_amd
     from pike_amd import scenes
+
+    if args.mode == "copy":
+        ctx = pike_amd.Context(0)
+        nbytes = 1 << 30
+        a, b = pike_amd.DeviceBuffer(ctx, nbytes), pike_amd.DeviceBuffer(ctx, nbytes)
+        for _ in range(3):
+            pike_amd.lib().pk_buffer_copy(ctx.h, b.ptr, a.ptr, nbytes)
+        ctx.sync()
+        print(json.dumps({"mode": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes}))
+        return
+    if args.mode == "render-only":
+        args.no_cpu = True
+        args.no_traversal = True
+        args.profile_frames = 0
 
     res, data, bvh, cam = run_render(args, pike_amd, scenes)
     extra = dict(mrays_per_s_render=res["rays_per_s"] / 1e6, msamples_per_s=res["spp_s"] * args.width * args.height / 1e6,

@@ -128,6 +128,7 @@ int pk_buffer_create(pk_ctx* ctx, size_t bytes, void** device_ptr);
 int pk_buffer_destroy(pk_ctx* ctx, void* device_ptr);
 int pk_buffer_write(pk_ctx* ctx, void* device_dst, const void* host_src, size_t bytes);  /* synchronous */
 int pk_buffer_read(pk_ctx* ctx, void* host_dst, const void* device_src, size_t bytes);   /* synchronous */
+int pk_buffer_copy(pk_ctx* ctx, void* device_dst, const void* device_src, size_t bytes);  /* async on the stream */
 
 /* ---- BVH (render/bvh.js) : host-side, no device needed ----
  * Builds the SAH binary BVH with bvh.js's decisions (full sweep, float64,

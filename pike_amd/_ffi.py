@@ -106,6 +106,7 @@ SIGNATURES = {
     "pk_buffer_destroy": (c_int, [c_void_p, c_void_p]),
     "pk_buffer_write": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "pk_buffer_read": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "pk_buffer_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "pk_bvh_build":(c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_int, ctypes.POINTER(c_void_p)]),
     "pk_bvh_destroy": (c_int, [c_void_p]),
     "pk_bvh_info": (c_int, [c_void_p, c_void_p]),

@@ -65,6 +65,7 @@ void launch_trace_shadow(const DevScene& S, const FrameDev& F, uint32_t grid, bo
 void launch_accumulate(const FrameDev& F, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
+void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s);
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats, uint32_t* overflow,
                        uint32_t grid, bool count, hipStream_t s);

@@ -529,6 +529,9 @@ __global__ void k_fill_f4(float4* p, float4 v, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void __launch_bounds__(256) k_copy_f4(float4* dst, const float4* src, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
 
 // ---------------------------------------------------------------------------
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
@@ -606,6 +609,9 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
 }
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_fill_f4, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_copy_f4, dim3(min(cdiv(n, 256), 8192u)), dim3(256), 0, s, dst, src, n);
 }
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,

@@ -226,6 +226,21 @@ int pk_buffer_read(pk_ctx* c, void* dst, const void* src, size_t bytes) {
   return PK_OK;
 }
 
+int pk_buffer_copy(pk_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (bytes && (!dst || !src))) {
+    pk_set_error("pk_buffer_copy: null argument");
+    return PK_E_INVALID;
+  }
+  if (bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0 && bytes / 16 < (1ull << 32)) {
+    // a plain streaming copy kernel (also the reference point for HBM byte counters)
+    pk::launch_copy_f4((float4*)dst, (const float4*)src, (uint32_t)(bytes / 16), c->stream);
+    PK_HIP(hipGetLastError());
+  } else if (bytes) {
+    PK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return PK_OK;
+}
+
 // ---------------------------------------------------------------------------
 // scene
 // ---------------------------------------------------------------------------

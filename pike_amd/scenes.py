@@ -388,6 +388,9 @@ def config1(seed=1, num_spheres=64, subdiv=2, env_size=(64, 32)):
                      camera=camera(C1_CAMERA["origin"], C1_CAMERA["target"], C1_CAMERA["fov"]), name="c1")
 
 
+C2_CAMERA = dict(origin=(0.5, 0.42, 1.2), target=(0.5, 0.02, 0.4), fov=1.0)
+
+
 def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024), env_sampler=None):
     """Terrain of cells^2 quads (fBm, 6 octaves, amplitude 0.2) plus
     num_spheres icospheres (subdivision 4); log-normal (sigma 1.5) HDR
@@ -415,7 +418,7 @@ def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024)
     return SceneData(positions=pos, attributes=attr, triangles=tris, materials=np.array(mats, dtype=np.int32),
                      tex=packer.atlas(), env=env,
                      env_sampler=_ffi.PK_ENV_SAMPLER_BINS if env_sampler is None else env_sampler,
-                     camera=camera((0.5, 0.45, 1.45), (0.5, 0.05, 0.45), 1.0), name="c2")
+                     camera=camera(C2_CAMERA["origin"], C2_CAMERA["target"], C2_CAMERA["fov"]), name="c2")
 
 
 def config3(seed=3, num_tris=10_000_000):
