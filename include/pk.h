@@ -146,6 +146,10 @@ int pk_bvh_info(const pk_bvh* bvh, int64_t* info6);
  * and the normalizer (center xyz, scale) it applied. */
 int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, float* verts4,
                             double* normalizer4);
+/* The wide layout itself (no reference counterpart; for inspection and the
+ * CPU traversal cost model): wide_nodes * 128 bytes, leaf_triangles * 48
+ * bytes of triangle records, and leaf_triangles * 4 words {i0,i1,i2,mat}. */
+int pk_bvh_export_wide(const pk_bvh* bvh, void* nodes128, void* tris48, int32_t* tri_index4);
 
 /* ---- scene (TracePass / ShadeHitPass / ShadeMissPass resources) ---- */
 int pk_scene_upload(pk_ctx* ctx, const pk_scene_desc* desc, const pk_bvh* bvh, pk_scene** out);

@@ -161,6 +161,22 @@ class BVH:
                "pk_bvh_export_reference")
         return nodes, tris, verts, norm
 
+    WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "u1", 3), ("nchild", "u1"), ("child", "<u4", 8),
+                                ("meta", "u1", 8), ("qlox", "u1", 8), ("qhix", "u1", 8), ("qloy", "u1", 8),
+                                ("qhiy", "u1", 8), ("qloz", "u1", 8), ("qhiz", "u1", 8), ("pad", "<u4", 6)])
+    TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("id", "<u4"), ("e1", "<f4", 3), ("pad0", "<u4"), ("e2", "<f4", 3),
+                          ("pad1", "<u4")])
+
+    def export_wide(self):
+        """(nodes, triangle records, triangle {i0,i1,i2,mat}) of the 128-byte layout."""
+        info = self.info()
+        nodes = np.zeros(info["wide_nodes"], self.WIDE_NODE_DTYPE)
+        tris = np.zeros(info["leaf_triangles"], self.TRI_DTYPE)
+        tidx = np.zeros((info["leaf_triangles"], 4), np.int32)
+        assert nodes.itemsize == 128 and tris.itemsize == 48
+        _check(lib().pk_bvh_export_wide(self.h, _ptr(nodes), _ptr(tris), _ptr(tidx)), "pk_bvh_export_wide")
+        return nodes, tris, tidx
+
     def normalizer(self):
         norm = np.zeros(4, np.float64)
         _check(lib().pk_bvh_export_reference(self.h, None, None, None, _ptr(norm)), "pk_bvh_export_reference")

@@ -111,6 +111,7 @@ SIGNATURES = {
     "pk_bvh_destroy": (c_int, [c_void_p]),
     "pk_bvh_info": (c_int, [c_void_p, c_void_p]),
     "pk_bvh_export_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pk_bvh_export_wide": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "pk_scene_upload": (c_int, [c_void_p, ctypes.POINTER(SceneDesc), c_void_p, ctypes.POINTER(c_void_p)]),
     "pk_scene_destroy": (c_int, [c_void_p]),
     "pk_scene_env_info": (c_int, [c_void_p, c_void_p]),

@@ -439,6 +439,17 @@ int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, 
   return PK_OK;
 }
 
+int pk_bvh_export_wide(const pk_bvh* bvh, void* nodes, void* tris, int32_t* tri_index) {
+  if (!bvh) {
+    pk_set_error("pk_bvh_export_wide: null bvh");
+    return PK_E_INVALID;
+  }
+  if (nodes) std::memcpy(nodes, bvh->wnodes.data(), bvh->wnodes.size() * sizeof(pk::WNode));
+  if (tris) std::memcpy(tris, bvh->wtris.data(), bvh->wtris.size() * sizeof(pk::TriRecord));
+  if (tri_index) std::memcpy(tri_index, bvh->wtri_index.data(), bvh->wtri_index.size() * sizeof(int32_t));
+  return PK_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,106 @@
+"""Environment importance-sampling tables (render/env_sampler.js:31-157):
+the product's tables equal the oracle's restatement bit for bit, and satisfy
+the invariants the construction fixes (bins partition the sorted texels,
+probabilities sum to one, equal-radiance bins)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import pike_amd
+from pike_amd import _ffi, scenes
+from oracle.oracle import OracleScene
+
+
+def product_tables(data):
+    d = data.desc()
+    info = np.zeros(4, np.int64)
+    lib = pike_amd.lib()
+    assert lib.pk_env_tables(ctypes.byref(d), info.ctypes.data, None, None, None) == 0
+    bins = np.zeros((int(info[0]), 2), np.int32)
+    coords = np.zeros((int(info[1]), 2), np.int32)
+    pdf = np.zeros((int(info[3]), int(info[2])), np.float32)
+    assert lib.pk_env_tables(ctypes.byref(d), info.ctypes.data, bins.ctypes.data, coords.ctypes.data,
+                             pdf.ctypes.data) == 0
+    return bins, coords, pdf
+
+
+def radiance(env):
+    """_getRadiance (env_sampler.js:43-48) / luma of a float map, float64."""
+    if env.dtype == np.uint8:
+        c = env.astype(np.float64)
+        return np.power(2.0, c[..., 3] - 128) * (0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]) / 255.0
+    c = env.astype(np.float64)
+    return 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
+
+
+def _scene_with_env(env, sampler=_ffi.PK_ENV_SAMPLER_BINS):
+    s = scenes.config1(seed=1, num_spheres=2, subdiv=0)
+    s.env = env
+    s.env_sampler = sampler
+    return s
+
+
+ENVS = {
+    "constant_rgbe": lambda: scenes.constant_env_rgbe(64, 32),
+    "lognormal_float": lambda: scenes.lognormal_sun_env(128, 64, seed=4, sigma=1.5, sun_deg=5.0, sun_scale=1e4),
+    "random_rgbe": lambda: np.random.default_rng(3).integers(0, 256, (48, 96, 4)).astype(np.uint8),
+}
+
+
+@pytest.mark.parametrize("name", list(ENVS))
+def test_tables_match_oracle(name):
+    data = _scene_with_env(ENVS[name]())
+    b, c, p = product_tables(data)
+    ob, oc, op = OracleScene(data).env_tables()
+    np.testing.assert_array_equal(b, ob)
+    np.testing.assert_array_equal(c, oc)
+    np.testing.assert_array_equal(p.view(np.uint32), op.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", list(ENVS))
+def test_bin_invariants(name):
+    env = ENVS[name]()
+    b, c, p = product_tables(_scene_with_env(env))
+    H, W = env.shape[:2]
+    T = W * H
+    # bins are contiguous, non-empty, and cover [0, T)
+    assert b[0, 0] == 0 and b[-1, 1] == T
+    assert np.all(b[1:, 0] == b[:-1, 1]) and np.all(b[:, 1] > b[:, 0])
+    # numBins = min(256, max(32, 2 ceil(total/brightest))) (env_sampler.js:77)
+    r = radiance(env).ravel()
+    nb_formula = min(256, max(32, 2 * math.ceil(r.sum() / r.max())))
+    assert len(b) <= nb_formula + 1
+    # coords are every texel once, in non-increasing radiance order
+    idx = c[:, 0] + W * c[:, 1]
+    assert np.array_equal(np.sort(idx), np.arange(T))
+    assert np.all(np.diff(r[idx]) <= 0)
+    # probabilities: sum over texels of pdf * 2 pi^2 / T equals 1
+    total = float((p.astype(np.float64) * 2 * math.pi ** 2 / T).sum())
+    assert abs(total - 1.0) < 1e-5
+    # every bin except the last closes once it holds >= total/numBins radiance
+    nb = len(b)
+    sums = np.array([r[idx[h0:h1]].sum() for h0, h1 in b])
+    assert np.all(sums[:-1] >= r.sum() / nb_formula * (1 - 1e-12))
+
+
+def test_constant_env_gives_256_equal_bins():
+    b, c, p = product_tables(_scene_with_env(scenes.constant_env_rgbe(64, 32)))
+    assert len(b) == 256
+    assert np.all(b[:, 1] - b[:, 0] == 8)
+    np.testing.assert_allclose(p, 1.0 / (2 * math.pi ** 2), rtol=1e-6)
+
+
+def test_cdf_sampler_pdf_normalized():
+    env = ENVS["lognormal_float"]()
+    b, c, p = product_tables(_scene_with_env(env, _ffi.PK_ENV_SAMPLER_CDF))
+    H, W = env.shape[:2]
+    total = float((p.astype(np.float64) * 2 * math.pi ** 2 / (W * H)).sum())
+    assert abs(total - 1.0) < 1e-4
+    # proportional to luma * sin(theta), up to the f32 resolution of the
+    # CDF steps the device search sees (the pdf is defined by those steps)
+    r = radiance(env) * np.sin(math.pi * (np.arange(H) + 0.5) / H)[:, None]
+    pn, rn = p / p.sum(), r / r.sum()
+    assert np.abs(pn - rn).sum() < 1e-4
+    assert np.abs(pn - rn).max() < 2 * 2.0 ** -24
