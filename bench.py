@@ -52,6 +52,16 @@ def run_render(args, pike_amd, scenes):
     W, H, depth = args.width, args.height, args.depth
     frame = pike_amd.Frame(ctx, W, H)
     frame.set_camera(cam)
+    tuned = None
+    if args.tuning:
+        for cls, shape in json.loads(args.tuning).items():
+            ctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+        tuned = dict(shapes=ctx.tuning(), source="--tuning")
+    elif not args.no_autotune:
+        t0 = time.time()
+        before, after = frame.autotune(scene, max_depth=depth, frames=2)
+        tuned = dict(ms_before=before, ms_after=after, shapes=ctx.tuning(), seconds=time.time() - t0)
+        log(f"autotune: traversal {before:.2f} -> {after:.2f} ms/frame, {ctx.tuning()} ({tuned['seconds']:.1f}s)")
     # warmup (also instantiates the graph)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=max(1, args.warmup), use_graph=True)
     ctx.sync()
@@ -75,7 +85,7 @@ def run_render(args, pike_amd, scenes):
                     kernel_ms=None, profile_frames=0, profile_counters=None,
                     roofline=dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
                                   traffic=None, bytes_per_launch=None, launch_ms=None),
-                    triangles=data.num_triangles), data, bvh, cam
+                    triangles=data.num_triangles, autotune=tuned), data, bvh, cam
     frame.reset()
     frame.count_nodes(True)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=pf, use_graph=False)
@@ -95,7 +105,7 @@ def run_render(args, pike_amd, scenes):
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                       traffic=None, kernel="k_trace_closest", bytes_per_launch=bytes_launch,
                       launch_ms=t_launch * 1e3),
-        triangles=data.num_triangles)
+        triangles=data.num_triangles, autotune=tuned)
     return result, data, bvh, cam
 
 
@@ -141,6 +151,9 @@ def run_traversal(args, pike_amd, scenes):
     b = pike_amd.DeviceBuffer(ctx, 4 * n)
     out = {}
     for name, anyh in (("closest", False), ("any", True)):
+        if not args.no_autotune:
+            before, after = ctx.autotune_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr)
+            log(f"autotune {name}: {before:.2f} -> {after:.2f} ms, {ctx.tuning()['rays_any' if anyh else 'rays_closest']}")
         ctx.trace_rays_device(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr, True)
         nodes, tris = ctx.trace_stats()
         ms = ctx.time_trace_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr, args.trace_iters)
@@ -165,6 +178,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traversal", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", help="JSON {class: {waves_per_cu, refill_min}} to apply instead of "
+                                                  "autotuning (e.g. the 'extra.autotune.shapes' of a previous run)")
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
     ap.add_argument("--trace-iters", type=int, default=5)
@@ -197,7 +213,8 @@ def main():
     extra = dict(mrays_per_s_render=res["rays_per_s"] / 1e6, msamples_per_s=res["spp_s"] * args.width * args.height / 1e6,
                  kernel_ms_per_profile=res["kernel_ms"], profile_frames=res["profile_frames"],
                  trace_closest_bytes_per_launch=res["roofline"]["bytes_per_launch"],
-                 trace_closest_launch_ms=res["roofline"]["launch_ms"], counters=res["counters"])
+                 trace_closest_launch_ms=res["roofline"]["launch_ms"], counters=res["counters"],
+                 autotune=res["autotune"])
     cpu = None if args.no_cpu else run_cpu_baseline(args, data, cam)
     if not args.no_traversal:
         extra["traversal_10M_soup"] = run_traversal(args, pike_amd, scenes)

@@ -121,6 +121,29 @@ const char* pk_version(void);
 int pk_ctx_create(int device, pk_ctx** out);
 int pk_ctx_destroy(pk_ctx* ctx);
 int pk_sync(pk_ctx* ctx);
+/* Launch shape of the persistent traversal kernels, per kernel class (no
+ * reference counterpart; chosen by the autotuner): waves_per_cu in [1, 24]
+ * one-wave blocks per CU, refill_min in [1, 64] idle lanes that trigger a
+ * wave's ray refill (64 = refill only when the whole wave is idle, which
+ * keeps coherent rays together). */
+#define PK_TUNE_PRIMARY 0      /* closest hit, bounce 0 (camera rays) */
+#define PK_TUNE_SECONDARY 1    /* closest hit, bounces >= 1 */
+#define PK_TUNE_SHADOW 2       /* any hit, NEE shadow rays */
+#define PK_TUNE_RAYS_CLOSEST 3 /* pk_trace_rays*, closest hit */
+#define PK_TUNE_RAYS_ANY 4     /* pk_trace_rays*, any hit */
+#define PK_TUNE_CLASSES 5
+int pk_ctx_set_tuning(pk_ctx* ctx, int kernel_class, uint32_t waves_per_cu, uint32_t refill_min);
+int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_cu, uint32_t* refill_min,
+                      uint32_t* num_cus);
+/* Autotuner: coordinate search over (waves_per_cu, refill_min) for each
+ * render kernel class, timing that class's kernels with HIP events over
+ * `frames` profiled frames of this scene, camera and resolution; keeps the
+ * fastest shape per class in the context.  ms2 receives the traversal
+ * milliseconds per frame before and after. */
+int pk_autotune_render(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t frames, float* ms2);
+/* The same for the standalone ray queries on device-resident rays. */
+int pk_autotune_rays(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
+                     float* d_t, uint32_t* d_bary, float* ms2);
 
 /* ---- device buffers (GPUDevice.createBuffer / queue.writeBuffer /
  * mapAsync in the reference's WebGPU use, e.g. structs.js:148-157) ---- */

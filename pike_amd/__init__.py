@@ -52,6 +52,27 @@ class Context:
     def sync(self):
         _check(lib().pk_sync(self.h), "pk_sync")
 
+    TUNE_CLASSES = {"primary": 0, "secondary": 1, "shadow": 2, "rays_closest": 3, "rays_any": 4}
+
+    def set_tuning(self, kernel_class, waves_per_cu, refill_min):
+        cls = self.TUNE_CLASSES[kernel_class] if isinstance(kernel_class, str) else kernel_class
+        _check(lib().pk_ctx_set_tuning(self.h, cls, waves_per_cu, refill_min), "pk_ctx_set_tuning")
+
+    def tuning(self):
+        out = {}
+        for name, cls in self.TUNE_CLASSES.items():
+            w, r, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+            _check(lib().pk_ctx_get_tuning(self.h, cls, ctypes.byref(w), ctypes.byref(r), ctypes.byref(n)),
+                   "pk_ctx_get_tuning")
+            out[name] = dict(waves_per_cu=w.value, refill_min=r.value)
+        return out
+
+    def autotune_rays(self, scene, rays_ptr, n, any_hit, ids_ptr, t_ptr, bary_ptr):
+        ms = (ctypes.c_float * 2)()
+        _check(lib().pk_autotune_rays(self.h, scene.h, rays_ptr, n, int(any_hit), ids_ptr, t_ptr, bary_ptr, ms),
+               "pk_autotune_rays")
+        return float(ms[0]), float(ms[1])
+
     def close(self):
         if self.h:
             lib().pk_ctx_destroy(self.h)
@@ -267,6 +288,13 @@ class Frame:
     def render(self, scene, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0, use_graph=True):
         p = RenderParams(self.width, self.height, max_depth, first_sample, num_frames, env_theta, int(use_graph))
         _check(lib().pk_render(self.h, scene.h, ctypes.byref(p)), "pk_render")
+
+    def autotune(self, scene, max_depth=4, frames=2):
+        """Choose traversal launch shapes for this scene/camera/resolution
+        (resets the accumulation).  Returns (ms/frame before, after)."""
+        ms = (ctypes.c_float * 2)()
+        _check(lib().pk_autotune_render(self.h, scene.h, max_depth, frames, ms), "pk_autotune_render")
+        return float(ms[0]), float(ms[1])
 
     def profile(self, scene, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0):
         p = RenderParams(self.width, self.height, max_depth, first_sample, num_frames, env_theta, 0)

@@ -102,6 +102,12 @@ SIGNATURES = {
     "pk_ctx_create": (c_int, [c_int, ctypes.POINTER(c_void_p)]),
     "pk_ctx_destroy": (c_int, [c_void_p]),
     "pk_sync": (c_int, [c_void_p]),
+    "pk_ctx_set_tuning": (c_int, [c_void_p, c_int, c_uint32, c_uint32]),
+    "pk_ctx_get_tuning": (c_int, [c_void_p, c_int, ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
+                                  ctypes.POINTER(c_uint32)]),
+    "pk_autotune_render": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
+    "pk_autotune_rays": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
     "pk_buffer_create": (c_int, [c_void_p, c_size_t, ctypes.POINTER(c_void_p)]),
     "pk_buffer_destroy": (c_int, [c_void_p, c_void_p]),
     "pk_buffer_write": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),

@@ -63,9 +63,17 @@ __device__ inline float intersect_tri(const float4* tris, uint32_t k, v3 o, v3 d
   return dist > eps ? dist : kMaxT;
 }
 
-constexpr int kStackLds = 16;     // LDS entries per lane
-constexpr int kStackMax = 64;     // total entries per lane (LDS + global spill)
-constexpr uint32_t kLeafBit = 0x80000000u;
+// Per-lane traversal stack: kStackLds entries in LDS ([entry][lane], so the
+// 64 lanes of a wave hit 64 distinct banks), the rest spilled to a per-wave
+// region in global memory.  An entry is a node index and the entry distance
+// of its box (for culling at pop time).
+constexpr int kStackLds = 12;
+constexpr int kStackMax = 64;
+constexpr int kSpillPerLane = kStackMax - kStackLds;
+// Occupancy target of the traversal kernels: 6 waves per SIMD lets the
+// compiler keep the traversal state in 80 VGPRs without scratch spills; 24
+// one-wave blocks per CU then use 24 * 6 KB = 144 KB of the 160 KB LDS.
+constexpr int kTraceWavesPerSimd = 6;
 
 // Culling bound: a box entered beyond the best hit by more than the
 // intersector's rounding cannot hold a better (t, id) hit.
@@ -75,167 +83,207 @@ struct TraverseStats {
   uint32_t nodes, tris;
 };
 
-// Wide-BVH traversal for one ray.  any_hit: stop at the first accepted
-// triangle (trace_shadow.wgsl semantics); otherwise the canonical closest hit
-// argmin (t, id).  The per-lane short stack lives in LDS (lds_node/lds_t,
-// [kStackLds][64] per wave) and spills to spill[] (kStackMax - kStackLds
-// entries per lane) when deeper.  Returns false on stack overflow.
+struct Stack {
+  uint32_t* lds_node;   // [kStackLds][64]
+  float* lds_t;         // [kStackLds][64]
+  uint2* spill;         // [kSpillPerLane][64] for this wave
+  int lane;
+};
+
+// State of one ray's traversal, kept in registers between steps.
+struct RayState {
+  v3 o, d;
+  float invx, invy, invz;
+  float tbest, cull;
+  uint32_t best_id, best_leaf, best_bary;
+  uint32_t node;
+  int sp;
+  bool have;       // `node` is to be visited
+  bool overflow;
+};
+
+template <bool kAnyHit>
+__device__ inline void ray_init(RayState& r, v3 o, v3 d) {
+  r.o = o;
+  r.d = d;
+  r.invx = 1.0f / (pkm::abs_(d.x) < 1e-30f ? pkm::copysign_(1e-30f, d.x) : d.x);
+  r.invy = 1.0f / (pkm::abs_(d.y) < 1e-30f ? pkm::copysign_(1e-30f, d.y) : d.y);
+  r.invz = 1.0f / (pkm::abs_(d.z) < 1e-30f ? pkm::copysign_(1e-30f, d.z) : d.z);
+  r.tbest = kMaxT;
+  r.cull = kAnyHit ? kMaxT * 2.0f : cull_bound(kMaxT);
+  r.best_id = 0xffffffffu;
+  r.best_leaf = 0xffffffffu;
+  r.best_bary = 0;
+  r.node = 0;
+  r.sp = 0;
+  r.have = true;
+  r.overflow = false;
+}
+
+__device__ inline void cswap(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = min(a, b), hi = max(a, b);
+  a = lo;
+  b = hi;
+}
+
+// One traversal step: visit the current node (box-test its 8 children,
+// intersect the hit leaf children, continue with the nearest hit internal
+// child and push the others far-to-near), or pop the next live stack entry.
+// Returns true when the ray is finished.
 template <bool kAnyHit, bool kCount>
-__device__ inline bool traverse(const DevScene& S, v3 o, v3 d, Hit* out, uint32_t* lds_node, float* lds_t,
-                                uint2* spill, int lane, TraverseStats* st) {
+__device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
+  const int lane = stk.lane;
+  if (!r.have) {
+    while (r.sp > 0) {
+      --r.sp;
+      uint32_t n;
+      float tn;
+      if (r.sp < kStackLds) {
+        n = stk.lds_node[r.sp * 64 + lane];
+        tn = stk.lds_t[r.sp * 64 + lane];
+      } else {
+        const uint2 e = stk.spill[(r.sp - kStackLds) * 64 + lane];
+        n = e.x;
+        tn = __uint_as_float(e.y);
+      }
+      if (tn <= r.cull) {
+        r.node = n;
+        r.have = true;
+        break;
+      }
+    }
+    if (!r.have) return true;
+  }
+  r.have = false;
+  if (kCount) st->nodes++;
   const float eps = kAnyHit ? kEpsShadow : kEpsTrace;
-  float invx = 1.0f / (pkm::abs_(d.x) < 1e-30f ? pkm::copysign_(1e-30f, d.x) : d.x);
-  float invy = 1.0f / (pkm::abs_(d.y) < 1e-30f ? pkm::copysign_(1e-30f, d.y) : d.y);
-  float invz = 1.0f / (pkm::abs_(d.z) < 1e-30f ? pkm::copysign_(1e-30f, d.z) : d.z);
-  const bool negx = invx < 0.0f, negy = invy < 0.0f, negz = invz < 0.0f;
-  float tbest = kMaxT;
-  uint32_t best_id = 0xffffffffu, best_leaf = 0xffffffffu, best_bary = 0;
-  float cull = kAnyHit ? kMaxT * 2.0f : cull_bound(kMaxT);
-  int sp = 0;
-  bool overflow = false;
-  uint32_t node = 0;
-  bool have = true;
-  while (true) {
-    if (!have) {
-      // pop, skipping entries whose box starts beyond the culling bound
-      while (sp > 0) {
-        --sp;
-        uint32_t n;
-        float tn;
-        if (sp < kStackLds) {
-          n = lds_node[sp * 64 + lane];
-          tn = lds_t[sp * 64 + lane];
-        } else {
-          uint2 e = spill[(sp - kStackLds) * 64 + lane];
-          n = e.x;
-          tn = __uint_as_float(e.y);
-        }
-        if (tn <= cull) {
-          node = n;
-          have = true;
-          break;
-        }
-      }
-      if (!have) break;
-    }
-    have = false;
-    if (kCount) st->nodes++;
-    const uint4* np = reinterpret_cast<const uint4*>(S.nodes + node);
-    uint4 L0 = np[0], L1 = np[1], L2 = np[2], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
-    const float ox = __uint_as_float(L0.x), oy = __uint_as_float(L0.y), oz = __uint_as_float(L0.z);
-    const uint32_t ew = L0.w;
-    const int nchild = (int)(ew >> 24);
-    const float sx = __uint_as_float((ew & 0xffu) << 23);
-    const float sy = __uint_as_float(((ew >> 8) & 0xffu) << 23);
-    const float sz = __uint_as_float(((ew >> 16) & 0xffu) << 23);
-    const float ax = sx * invx, ay = sy * invy, az = sz * invz;
-    const float bx = (ox - o.x) * invx, by = (oy - o.y) * invy, bz = (oz - o.z) * invz;
-    // near/far plane bytes per axis by ray direction sign
-    const uint32_t nx0 = negx ? L4.x : L3.z, nx1 = negx ? L4.y : L3.w;
-    const uint32_t fx0 = negx ? L3.z : L4.x, fx1 = negx ? L3.w : L4.y;
-    const uint32_t ny0 = negy ? L5.x : L4.z, ny1 = negy ? L5.y : L4.w;
-    const uint32_t fy0 = negy ? L4.z : L5.x, fy1 = negy ? L4.w : L5.y;
-    const uint32_t nz0 = negz ? L6.x : L5.z, nz1 = negz ? L6.y : L5.w;
-    const uint32_t fz0 = negz ? L5.z : L6.x, fz1 = negz ? L5.w : L6.y;
-    const uint32_t child[8] = {L1.x, L1.y, L1.z, L1.w, L2.x, L2.y, L2.z, L2.w};
-    const uint32_t meta0 = L3.x, meta1 = L3.y;
-    float tn[8];
-    uint32_t hitmask = 0;
+  const uint4* np = reinterpret_cast<const uint4*>(S.nodes + r.node);
+  const uint4 L0 = np[0], L1 = np[1], L2 = np[2], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
+  const float ox = __uint_as_float(L0.x), oy = __uint_as_float(L0.y), oz = __uint_as_float(L0.z);
+  const uint32_t ew = L0.w;
+  const int nchild = (int)(ew >> 24);
+  const float sx = __uint_as_float((ew & 0xffu) << 23);
+  const float sy = __uint_as_float(((ew >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((ew >> 16) & 0xffu) << 23);
+  const float ax = sx * r.invx, ay = sy * r.invy, az = sz * r.invz;
+  const float bx = (ox - r.o.x) * r.invx, by = (oy - r.o.y) * r.invy, bz = (oz - r.o.z) * r.invz;
+  const bool negx = r.invx < 0.0f, negy = r.invy < 0.0f, negz = r.invz < 0.0f;
+  // near/far plane bytes per axis by ray direction sign
+  const uint32_t nx0 = negx ? L4.x : L3.z, nx1 = negx ? L4.y : L3.w;
+  const uint32_t fx0 = negx ? L3.z : L4.x, fx1 = negx ? L3.w : L4.y;
+  const uint32_t ny0 = negy ? L5.x : L4.z, ny1 = negy ? L5.y : L4.w;
+  const uint32_t fy0 = negy ? L4.z : L5.x, fy1 = negy ? L4.w : L5.y;
+  const uint32_t nz0 = negz ? L6.x : L5.z, nz1 = negz ? L6.y : L5.w;
+  const uint32_t fz0 = negz ? L5.z : L6.x, fz1 = negz ? L5.w : L6.y;
+  const uint32_t child[8] = {L1.x, L1.y, L1.z, L1.w, L2.x, L2.y, L2.z, L2.w};
+  const uint32_t meta0 = L3.x, meta1 = L3.y;
+  float tn[8];
+  uint32_t hitmask = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int sh = (c & 3) * 8;
-      const uint32_t wnx = c < 4 ? nx0 : nx1, wfx = c < 4 ? fx0 : fx1;
-      const uint32_t wny = c < 4 ? ny0 : ny1, wfy = c < 4 ? fy0 : fy1;
-      const uint32_t wnz = c < 4 ? nz0 : nz1, wfz = c < 4 ? fz0 : fz1;
-      const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bx);
-      const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, by);
-      const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bz);
-      const float tfx = fmaf((float)((wfx >> sh) & 0xffu), ax, bx);
-      const float tfy = fmaf((float)((wfy >> sh) & 0xffu), ay, by);
-      const float tfz = fmaf((float)((wfz >> sh) & 0xffu), az, bz);
-      const float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
-      const float tfar = fminf(fminf(tfx, tfy), tfz);
-      tn[c] = tnear;
-      const bool h = (c < nchild) && tnear <= tfar && tnear <= cull;
-      hitmask |= h ? (1u << c) : 0u;
-    }
-    // leaf children first
-    uint32_t leafmask = 0;
+  for (int c = 0; c < 8; ++c) {
+    const int sh = (c & 3) * 8;
+    const uint32_t wnx = c < 4 ? nx0 : nx1, wfx = c < 4 ? fx0 : fx1;
+    const uint32_t wny = c < 4 ? ny0 : ny1, wfy = c < 4 ? fy0 : fy1;
+    const uint32_t wnz = c < 4 ? nz0 : nz1, wfz = c < 4 ? fz0 : fz1;
+    const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bx);
+    const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, by);
+    const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bz);
+    const float tfx = fmaf((float)((wfx >> sh) & 0xffu), ax, bx);
+    const float tfy = fmaf((float)((wfy >> sh) & 0xffu), ay, by);
+    const float tfz = fmaf((float)((wfz >> sh) & 0xffu), az, bz);
+    const float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+    const float tfar = fminf(fminf(tfx, tfy), tfz);
+    tn[c] = tnear;
+    const bool h = (c < nchild) && tnear <= tfar && tnear <= r.cull;
+    hitmask |= h ? (1u << c) : 0u;
+  }
+  uint32_t leafmask = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
-      if (m) leafmask |= 1u << c;
-    }
-    uint32_t lm = hitmask & leafmask;
-    while (lm) {
-      const int c = __builtin_ctz(lm);
-      lm &= lm - 1;
-      const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
-      uint32_t first = 0;
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
+    if (m) leafmask |= 1u << c;
+  }
+  // leaf children first: their hits tighten the bound for the internal ones
+  uint32_t lm = hitmask & leafmask;
+  while (lm) {
+    const int c = __builtin_ctz(lm);
+    lm &= lm - 1;
+    const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
+    uint32_t first = 0;
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (j == c) first = child[j];
-      for (uint32_t k = first; k < first + m; ++k) {
-        if (kCount) st->tris++;
-        uint32_t id, bary = 0;
-        const float t = intersect_tri(S.tris, k, o, d, eps, &id, &bary);
-        if (t < kMaxT && (t < tbest || (t == tbest && id < best_id))) {
-          tbest = t;
-          best_id = id;
-          best_leaf = k;
-          best_bary = bary;
-          if (kAnyHit) break;
-          cull = cull_bound(tbest);
-        }
-      }
-      if (kAnyHit && best_id != 0xffffffffu) break;
-    }
-    if (kAnyHit && best_id != 0xffffffffu) break;
-    // internal children: nearest next, the rest onto the stack
-    uint32_t im = hitmask & ~leafmask;
-    // re-check against a cull bound tightened by the leaves just tested
-    float near_t = 3.0e38f;
-    int near_c = -1;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      if ((im >> c) & 1u) {
-        if (tn[c] > cull) {
-          im &= ~(1u << c);
-        } else if (tn[c] < near_t) {
-          near_t = tn[c];
-          near_c = c;
-        }
-      }
-    }
-    if (near_c >= 0) {
-      im &= ~(1u << near_c);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (c == near_c) node = child[c];
-      }
-      have = true;
-#pragma unroll
-      for (int c = 7; c >= 0; --c) {
-        if ((im >> c) & 1u) {
-          if (sp < kStackLds) {
-            lds_node[sp * 64 + lane] = child[c];
-            lds_t[sp * 64 + lane] = tn[c];
-          } else if (sp < kStackMax) {
-            spill[(sp - kStackLds) * 64 + lane] = make_uint2(child[c], __float_as_uint(tn[c]));
-          } else {
-            overflow = true;
-            --sp;
-          }
-          ++sp;
-        }
+    for (int j = 0; j < 8; ++j)
+      if (j == c) first = child[j];
+    for (uint32_t k = first; k < first + m; ++k) {
+      if (kCount) st->tris++;
+      uint32_t id, bary = 0;
+      const float t = intersect_tri(S.tris, k, r.o, r.d, eps, &id, &bary);
+      if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
+        r.tbest = t;
+        r.best_id = id;
+        r.best_leaf = k;
+        r.best_bary = bary;
+        if (kAnyHit) return true;
+        r.cull = cull_bound(t);
       }
     }
   }
-  out->t = tbest;
-  out->id = best_id;
-  out->leaf = best_leaf;
-  out->bary = best_bary;
-  return !overflow;
+  // internal children, sorted near-to-far: key = entry distance bits (>= 0,
+  // so order-preserving) with the slot in the low 3 bits
+  uint32_t k[8];
+  int m = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const bool live = ((hitmask & ~leafmask) >> c) & 1u && tn[c] <= r.cull;
+    k[c] = live ? ((__float_as_uint(tn[c]) & ~7u) | (uint32_t)c) : 0xffffffffu;
+    m += live ? 1 : 0;
+  }
+  if (m == 0) return r.sp == 0;
+  cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
+  cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
+  cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
+  cswap(k[2], k[4]); cswap(k[3], k[5]);
+  cswap(k[1], k[4]); cswap(k[3], k[6]);
+  cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
+  {
+    const uint32_t s = k[0] & 7u;
+    uint32_t nx = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if ((uint32_t)j == s) nx = child[j];
+    r.node = nx;
+    r.have = true;
+  }
+#pragma unroll
+  for (int i = 7; i >= 1; --i) {
+    if (i < m) {
+      const uint32_t s = k[i] & 7u;
+      uint32_t nx = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if ((uint32_t)j == s) nx = child[j];
+      const float t = __uint_as_float(k[i] & ~7u);  // rounded down: conservative
+      if (r.sp < kStackLds) {
+        stk.lds_node[r.sp * 64 + lane] = nx;
+        stk.lds_t[r.sp * 64 + lane] = t;
+        ++r.sp;
+      } else if (r.sp < kStackMax) {
+        stk.spill[(r.sp - kStackLds) * 64 + lane] = make_uint2(nx, __float_as_uint(t));
+        ++r.sp;
+      } else {
+        r.overflow = true;
+      }
+    }
+  }
+  return false;
+}
+
+__device__ inline Hit ray_result(const RayState& r) {
+  Hit h;
+  h.t = r.tbest;
+  h.id = r.best_id;
+  h.leaf = r.best_leaf;
+  h.bary = r.best_bary;
+  return h;
 }
 
 // ---------------------------------------------------------------------------

@@ -59,15 +59,23 @@ struct FrameDev {
 void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s);
 void launch_advance(const FrameDev& F, hipStream_t s);
 void launch_raygen(const FrameDev& F, hipStream_t s);
-void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, uint32_t grid, bool count, hipStream_t s);
+// persistent trace launch shape: blocks of one wave, lane refill threshold
+struct TraceLaunch {
+  uint32_t grid;
+  uint32_t refill;
+  bool count;
+  uint2* spill;  // grid * kSpillPerLane * 64 entries
+};
+
+void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, const TraceLaunch& L, hipStream_t s);
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s);
-void launch_trace_shadow(const DevScene& S, const FrameDev& F, uint32_t grid, bool count, hipStream_t s);
+void launch_trace_shadow(const DevScene& S, const FrameDev& F, const TraceLaunch& L, hipStream_t s);
 void launch_accumulate(const FrameDev& F, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s);
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats, uint32_t* overflow,
-                       uint32_t grid, bool count, hipStream_t s);
+                       const TraceLaunch& L, hipStream_t s);
 
 }  // namespace pk

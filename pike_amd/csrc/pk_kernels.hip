@@ -101,8 +101,57 @@ __device__ inline void add_color(const FrameDev& F, uint32_t pid, v3 L) {
 // ---------------------------------------------------------------------------
 // trace.wgsl main (:191-213) + shade_miss.wgsl (:45-57) for the misses
 // ---------------------------------------------------------------------------
+// Persistent traversal loop shared by the three trace kernels.  Each lane
+// owns one ray at a time; a lane that finishes stays idle until at least
+// `refill_min` lanes of the wave are idle, then the wave compacts the
+// finished results (flush) and pulls that many new rays from the queue with
+// one atomic.  This keeps lanes busy on incoherent rays without paying an
+// atomic per ray.  Every wave leaves the loop once the queue is exhausted
+// and its own lanes are done.
+template <bool kAnyHit, bool kCount, class Load, class Finish, class Flush>
+__device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* cursor, uint32_t refill_min,
+                                  const Stack& stk, TraverseStats* st, uint32_t* overflow, Load load,
+                                  Finish finish, Flush flush) {
+  const int lane = stk.lane;
+  RayState r;
+  r.best_id = 0xffffffffu;
+  bool active = false, done = false, exhausted = false;
+  uint32_t slot = 0;
+  while (true) {
+    if (active && ray_step<kAnyHit, kCount>(S, r, stk, st)) {
+      active = false;
+      done = true;
+      if (r.overflow) *overflow = 1;
+      finish(slot, r);
+    }
+    const unsigned long long idle = __ballot(!active);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (nidle < refill_min && nidle < 64u) continue;
+    flush(done, slot, r);
+    done = false;
+    if (exhausted) {
+      if (nidle == 64u) break;
+      continue;
+    }
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(cursor, nidle);
+    base = __shfl(base, 0);
+    if (!active) {
+      slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+      if (slot < n) {
+        load(slot, r);
+        active = true;
+      }
+    }
+    if (base + nidle >= n) {
+      exhausted = true;
+      if (__ballot(active) == 0ull) break;
+    }
+  }
+}
+
 template <bool kCount>
-__global__ void __launch_bounds__(64) k_trace_closest(DevScene S, FrameDev F, int bounce) {
+__global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevScene S, FrameDev F, int bounce, uint32_t refill_min) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
@@ -113,38 +162,33 @@ __global__ void __launch_bounds__(64) k_trace_closest(DevScene S, FrameDev F, in
   }
   const uint32_t n = F.cnt[kCntRays0 + in];
   const float theta = F.params->env_theta;
-  uint2* spill = F.spill + (size_t)blockIdx.x * (kStackMax - kStackLds) * 64;
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
-  uint32_t traced = 0, hits = 0, overflow = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&F.cnt[kCntWorkTrace], 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t slot = base + (uint32_t)lane;
-    const bool active = slot < n;
-    Hit h;
-    h.id = 0xffffffffu;
-    float4 o4 = make_float4(0, 0, 0, 0), d4 = make_float4(0, 0, 1, 0);
-    if (active) {
-      o4 = F.qo[in][slot];
-      d4 = F.qd[in][slot];
-      const v3 o = pkm::mk(o4.x, o4.y, o4.z), d = pkm::mk(d4.x, d4.y, d4.z);
-      if (!traverse<false, kCount>(S, o, d, &h, lds_node, lds_t, spill, lane, &st)) overflow = 1;
-      traced++;
-    }
-    const bool is_hit = active && h.id != 0xffffffffu;
-    const uint32_t hs = wave_append(&F.cnt[kCntHits], is_hit);
-    if (is_hit) {
-      F.hits[hs] = make_uint4(slot, h.leaf, __float_as_uint(h.t), h.bary);
-      hits++;
-    } else if (active) {
-      const uint32_t pid = __float_as_uint(o4.w);
-      const float4 t4 = F.thr[pid];
-      const v3 L = pkm::mul(env_color(S, theta, pkm::mk(d4.x, d4.y, d4.z)), pkm::mk(t4.x, t4.y, t4.z));
-      add_color(F, pid, L);
-    }
-  }
+  uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
+  const float4* qo = F.qo[in];
+  const float4* qd = F.qd[in];
+  trace_loop<false, kCount>(
+      S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
+      [&](uint32_t slot, RayState& r) {
+        const float4 o4 = qo[slot], d4 = qd[slot];
+        pid = __float_as_uint(o4.w);
+        ray_init<false>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
+      },
+      [&](uint32_t, const RayState& r) {
+        traced++;
+        if (r.best_id == 0xffffffffu) {  // miss: shade_miss.wgsl envColor x throughput
+          const float4 t4 = F.thr[pid];
+          add_color(F, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
+        }
+      },
+      [&](bool done, uint32_t slot, const RayState& r) {
+        const bool is_hit = done && r.best_id != 0xffffffffu;
+        const uint32_t hs = wave_append(&F.cnt[kCntHits], is_hit);
+        if (is_hit) {
+          F.hits[hs] = make_uint4(slot, r.best_leaf, __float_as_uint(r.tbest), r.best_bary);
+          hits++;
+        }
+      });
   traced = wave_sum(traced);
   hits = wave_sum(hits);
   if (kCount) {
@@ -444,7 +488,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
 // trace_shadow.wgsl main (:192-207) + shade_miss for unoccluded shadow rays
 // ---------------------------------------------------------------------------
 template <bool kCount>
-__global__ void __launch_bounds__(64) k_trace_shadow(DevScene S, FrameDev F) {
+__global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScene S, FrameDev F, uint32_t refill_min) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
@@ -454,28 +498,24 @@ __global__ void __launch_bounds__(64) k_trace_shadow(DevScene S, FrameDev F) {
   }
   const uint32_t n = F.cnt[kCntShadow];
   const float theta = F.params->env_theta;
-  uint2* spill = F.spill + (size_t)blockIdx.x * (kStackMax - kStackLds) * 64;
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
-  uint32_t traced = 0, overflow = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&F.cnt[kCntWorkShadow], 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t slot = base + (uint32_t)lane;
-    if (slot < n) {
-      const float4 o4 = F.so[slot], d4 = F.sd[slot];
-      const v3 o = pkm::mk(o4.x, o4.y, o4.z), d = pkm::mk(d4.x, d4.y, d4.z);
-      Hit h;
-      if (!traverse<true, kCount>(S, o, d, &h, lds_node, lds_t, spill, lane, &st)) overflow = 1;
-      traced++;
-      if (h.id == 0xffffffffu) {
-        const float4 t4 = F.sthr[slot];
-        const v3 L = pkm::mul(env_color(S, theta, d), pkm::mk(t4.x, t4.y, t4.z));
-        add_color(F, __float_as_uint(o4.w), L);
-      }
-    }
-  }
+  uint32_t traced = 0, overflow = 0, pid = 0;
+  trace_loop<true, kCount>(
+      S, n, &F.cnt[kCntWorkShadow], refill_min, stk, &st, &overflow,
+      [&](uint32_t slot, RayState& r) {
+        const float4 o4 = F.so[slot], d4 = F.sd[slot];
+        pid = __float_as_uint(o4.w);
+        ray_init<true>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
+      },
+      [&](uint32_t slot, const RayState& r) {
+        traced++;
+        if (r.best_id == 0xffffffffu) {  // unoccluded: env radiance x NEE throughput
+          const float4 t4 = F.sthr[slot];
+          add_color(F, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
+        }
+      },
+      [&](bool, uint32_t, const RayState&) {});
   traced = wave_sum(traced);
   if (kCount) {
     st.nodes = wave_sum(st.nodes);
@@ -537,32 +577,28 @@ __global__ void __launch_bounds__(256) k_copy_f4(float4* dst, const float4* src,
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
 // ---------------------------------------------------------------------------
 template <bool kAnyHit, bool kCount>
-__global__ void __launch_bounds__(64) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int32_t* ids,
-                                                    float* tout, uint32_t* bary, uint32_t* cursor, uint2* spill_all,
-                                                    unsigned long long* stats, uint32_t* overflow_flag) {
+__global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int32_t* ids,
+                                                       float* tout, uint32_t* bary, uint32_t* cursor,
+                                                       uint2* spill_all, unsigned long long* stats,
+                                                       uint32_t* overflow_flag, uint32_t refill_min) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  uint2* spill = spill_all + (size_t)blockIdx.x * (kStackMax - kStackLds) * 64;
+  const Stack stk{lds_node, lds_t, spill_all + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(cursor, 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t i = base + (uint32_t)lane;
-    if (i < n) {
-      const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
-      Hit h;
-      if (!traverse<kAnyHit, kCount>(S, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), &h, lds_node, lds_t,
-                                     spill, lane, &st))
-        overflow = 1;
-      ids[i] = (int32_t)h.id;
-      tout[i] = h.t;
-      bary[i] = h.bary;
-    }
-  }
+  trace_loop<kAnyHit, kCount>(
+      S, n, cursor, refill_min, stk, &st, &overflow,
+      [&](uint32_t i, RayState& r) {
+        const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
+        ray_init<kAnyHit>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
+      },
+      [&](uint32_t i, const RayState& r) {
+        ids[i] = (int32_t)r.best_id;
+        tout[i] = r.tbest;
+        bary[i] = r.best_bary;
+      },
+      [&](bool, uint32_t, const RayState&) {});
   if (kCount) {
     st.nodes = wave_sum(st.nodes);
     st.tris = wave_sum(st.tris);
@@ -589,17 +625,20 @@ void launch_advance(const FrameDev& F, hipStream_t s) { hipLaunchKernelGGL(k_adv
 void launch_raygen(const FrameDev& F, hipStream_t s) {
   hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
 }
-void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, uint32_t grid, bool count,
-                          hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(64), 0, s, S, F, bounce);
-  else hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(64), 0, s, S, F, bounce);
+void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
+  FrameDev F = F0;
+  F.spill = L.spill;
+  if (L.count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else hipLaunchKernelGGL(k_trace_closest<false>, dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
   hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, S, F, bounce);
 }
-void launch_trace_shadow(const DevScene& S, const FrameDev& F, uint32_t grid, bool count, hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid), dim3(64), 0, s, S, F);
-  else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid), dim3(64), 0, s, S, F);
+void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
+  FrameDev F = F0;
+  F.spill = L.spill;
+  if (L.count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
 }
 void launch_accumulate(const FrameDev& F, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
@@ -615,22 +654,18 @@ void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s) {
 }
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,
-                       uint32_t* overflow, uint32_t grid, bool count, hipStream_t s) {
+                       uint32_t* overflow, const TraceLaunch& L, hipStream_t s) {
+#define PK_LAUNCH_RAYS(A, C)                                                                                    \
+  hipLaunchKernelGGL((k_trace_rays<A, C>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor, spill, \
+                     stats, overflow, L.refill)
   if (any_hit) {
-    if (count)
-      hipLaunchKernelGGL((k_trace_rays<true, true>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
-                         spill, stats, overflow);
-    else
-      hipLaunchKernelGGL((k_trace_rays<true, false>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
-                         spill, stats, overflow);
+    if (L.count) PK_LAUNCH_RAYS(true, true);
+    else PK_LAUNCH_RAYS(true, false);
   } else {
-    if (count)
-      hipLaunchKernelGGL((k_trace_rays<false, true>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
-                         spill, stats, overflow);
-    else
-      hipLaunchKernelGGL((k_trace_rays<false, false>), dim3(grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor,
-                         spill, stats, overflow);
+    if (L.count) PK_LAUNCH_RAYS(false, true);
+    else PK_LAUNCH_RAYS(false, false);
   }
+#undef PK_LAUNCH_RAYS
 }
 
 }  // namespace pk

@@ -30,7 +30,11 @@ struct pk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
-  uint32_t grid = 0;  // persistent trace grid (one wave per block)
+  struct Shape {
+    uint32_t waves;   // one-wave blocks per CU
+    uint32_t refill;  // idle lanes that trigger a wave-level ray refill
+  } shape[PK_TUNE_CLASSES];
+  uint32_t tuning_gen = 0;  // bumped by pk_ctx_set_tuning; captured graphs keep the old shape
   uint32_t* cursors = nullptr;  // per-launch work cursors for ray queries
   uint32_t ncursors = 0;
   uint2* spill = nullptr;
@@ -59,10 +63,21 @@ struct pk_frame {
   hipGraphExec_t exec = nullptr;
   const pk_scene* graph_scene = nullptr;
   uint32_t graph_depth = 0;
+  uint32_t graph_gen = 0;
   bool count_nodes = false;
 };
 
 namespace {
+
+// One-wave traversal blocks per CU at most: kTraceWavesPerSimd waves on each
+// of the 4 SIMDs (the register and LDS budget of the traversal kernels).
+constexpr uint32_t kMaxWavesPerCu = 4 * pk::kTraceWavesPerSimd;
+
+pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
+  return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill};
+}
+pk::TraceLaunch trace_launch(const pk_frame* f, int cls) { return shape_launch(f->ctx, cls, f->count_nodes); }
+int closest_class(uint32_t bounce) { return bounce == 0 ? PK_TUNE_PRIMARY : PK_TUNE_SECONDARY; }
 
 template <class T> int dev_alloc(std::vector<void*>& list, T** p, size_t count) {
   *p = nullptr;
@@ -130,9 +145,16 @@ int pk_ctx_create(int device, pk_ctx** out) {
   auto* c = new pk_ctx();
   c->device = device;
   c->num_cus = prop.multiProcessorCount;
-  c->grid = (uint32_t)c->num_cus * 16u;
+  // Defaults measured on MI355X: coherent camera and shadow rays keep whole
+  // waves together (refill 64); incoherent ray queries refill early.
+  c->shape[PK_TUNE_PRIMARY] = {16, 64};
+  c->shape[PK_TUNE_SECONDARY] = {16, 64};
+  c->shape[PK_TUNE_SHADOW] = {8, 64};
+  c->shape[PK_TUNE_RAYS_CLOSEST] = {kMaxWavesPerCu, 16};
+  c->shape[PK_TUNE_RAYS_ANY] = {kMaxWavesPerCu, 24};
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->spill, (size_t)c->grid * (pk::kStackMax - pk::kStackLds) * 64 * sizeof(uint2));
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&c->spill, (size_t)c->num_cus * kMaxWavesPerCu * pk::kSpillPerLane * 64 * sizeof(uint2));
   if (e == hipSuccess) e = hipMalloc((void**)&c->stats, 8 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc((void**)&c->overflow, 4 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -172,6 +194,32 @@ int pk_sync(pk_ctx* c) {
     return PK_E_INVALID;
   }
   PK_HIP(hipStreamSynchronize(c->stream));
+  return PK_OK;
+}
+
+int pk_ctx_set_tuning(pk_ctx* c, int cls, uint32_t waves_per_cu, uint32_t refill_min) {
+  if (!c || cls < 0 || cls >= PK_TUNE_CLASSES) {
+    pk_set_error("pk_ctx_set_tuning: null context or unknown kernel class");
+    return PK_E_INVALID;
+  }
+  if (waves_per_cu < 1 || waves_per_cu > kMaxWavesPerCu || refill_min < 1 || refill_min > 64) {
+    pk_set_error("pk_ctx_set_tuning: waves_per_cu must be in [1, " + std::to_string(kMaxWavesPerCu) +
+                 "] and refill_min in [1, 64]");
+    return PK_E_LIMIT;
+  }
+  c->shape[cls] = {waves_per_cu, refill_min};
+  c->tuning_gen++;
+  return PK_OK;
+}
+
+int pk_ctx_get_tuning(const pk_ctx* c, int cls, uint32_t* waves_per_cu, uint32_t* refill_min, uint32_t* num_cus) {
+  if (!c || cls < 0 || cls >= PK_TUNE_CLASSES) {
+    pk_set_error("pk_ctx_get_tuning: null context or unknown kernel class");
+    return PK_E_INVALID;
+  }
+  if (waves_per_cu) *waves_per_cu = c->shape[cls].waves;
+  if (refill_min) *refill_min = c->shape[cls].refill;
+  if (num_cus) *num_cus = (uint32_t)c->num_cus;
   return PK_OK;
 }
 
@@ -553,7 +601,7 @@ int pk_raygen(pk_frame* f) {
 int pk_trace_closest(pk_frame* f, pk_scene* s, uint32_t bounce) {
   int rc = check_pass(f, s, "pk_trace_closest");
   if (rc) return rc;
-  pk::launch_trace_closest(s->dev, f->dev, (int)bounce, f->ctx->grid, f->count_nodes, f->ctx->stream);
+  pk::launch_trace_closest(s->dev, f->dev, (int)bounce, trace_launch(f, closest_class(bounce)), f->ctx->stream);
   PK_HIP(hipGetLastError());
   return PK_OK;
 }
@@ -569,7 +617,7 @@ int pk_shade(pk_frame* f, pk_scene* s, uint32_t bounce) {
 int pk_trace_shadow(pk_frame* f, pk_scene* s) {
   int rc = check_pass(f, s, "pk_trace_shadow");
   if (rc) return rc;
-  pk::launch_trace_shadow(s->dev, f->dev, f->ctx->grid, f->count_nodes, f->ctx->stream);
+  pk::launch_trace_shadow(s->dev, f->dev, trace_launch(f, PK_TUNE_SHADOW), f->ctx->stream);
   PK_HIP(hipGetLastError());
   return PK_OK;
 }
@@ -585,9 +633,9 @@ int pk_accumulate(pk_frame* f) {
 static void enqueue_frame(pk_frame* f, pk_scene* s, uint32_t depth, hipStream_t st) {
   pk::launch_raygen(f->dev, st);
   for (uint32_t b = 0; b < depth; ++b) {
-    pk::launch_trace_closest(s->dev, f->dev, (int)b, f->ctx->grid, f->count_nodes, st);
+    pk::launch_trace_closest(s->dev, f->dev, (int)b, trace_launch(f, closest_class(b)), st);
     pk::launch_shade(s->dev, f->dev, (int)b, st);
-    pk::launch_trace_shadow(s->dev, f->dev, f->ctx->grid, f->count_nodes, st);
+    pk::launch_trace_shadow(s->dev, f->dev, trace_launch(f, PK_TUNE_SHADOW), st);
   }
   pk::launch_accumulate(f->dev, st);
   pk::launch_advance(f->dev, st);
@@ -619,7 +667,7 @@ int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
     PK_HIP(hipGetLastError());
     return PK_OK;
   }
-  if (!f->exec || f->graph_scene != s || f->graph_depth != p->max_depth) {
+  if (!f->exec || f->graph_scene != s || f->graph_depth != p->max_depth || f->graph_gen != f->ctx->tuning_gen) {
     drop_graph(f);
     PK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     enqueue_frame(f, s, p->max_depth, st);
@@ -631,6 +679,7 @@ int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
     PK_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
     f->graph_scene = s;
     f->graph_depth = p->max_depth;
+    f->graph_gen = f->ctx->tuning_gen;
   }
   for (uint32_t i = 0; i < p->num_frames; ++i) PK_HIP(hipGraphLaunch(f->exec, st));
   return PK_OK;
@@ -694,7 +743,8 @@ int pk_frame_count_nodes(pk_frame* f, int enable) {
 static int trace_rays_launch(pk_ctx* c, pk_scene* s, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
                              float* d_t, uint32_t* d_bary, int count, uint32_t cursor_index) {
   pk::launch_trace_rays(s->dev, (const float4*)d_rays, (uint32_t)n, any_hit, d_ids, d_t, d_bary,
-                        c->cursors + cursor_index, c->spill, c->stats, c->overflow, c->grid, count != 0, c->stream);
+                        c->cursors + cursor_index, c->spill, c->stats, c->overflow,
+                        shape_launch(c, any_hit ? PK_TUNE_RAYS_ANY : PK_TUNE_RAYS_CLOSEST, count != 0), c->stream);
   PK_HIP(hipGetLastError());
   return PK_OK;
 }
@@ -846,9 +896,9 @@ int pk_render_profile(pk_frame* f, pk_scene* s, const pk_render_params* p, doubl
     };
     timed(0, [&]() { pk::launch_raygen(f->dev, st); });
     for (uint32_t b = 0; b < p->max_depth; ++b) {
-      timed(1, [&]() { pk::launch_trace_closest(s->dev, f->dev, (int)b, f->ctx->grid, f->count_nodes, st); });
+      timed(1, [&]() { pk::launch_trace_closest(s->dev, f->dev, (int)b, trace_launch(f, closest_class(b)), st); });
       timed(2, [&]() { pk::launch_shade(s->dev, f->dev, (int)b, st); });
-      timed(3, [&]() { pk::launch_trace_shadow(s->dev, f->dev, f->ctx->grid, f->count_nodes, st); });
+      timed(3, [&]() { pk::launch_trace_shadow(s->dev, f->dev, trace_launch(f, PK_TUNE_SHADOW), st); });
     }
     timed(4, [&]() { pk::launch_accumulate(f->dev, st); });
     pk::launch_advance(f->dev, st);
@@ -868,3 +918,127 @@ int pk_render_profile(pk_frame* f, pk_scene* s, const pk_render_params* p, doubl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner.  The launch shape that wins depends on ray coherence: camera and
+// shadow rays toward a sun are coherent and want whole waves refilled
+// together, incoherent bounces and ray soups want early per-lane refills and
+// more waves to hide memory latency.  A coordinate search times each class's
+// kernels in isolation (HIP events on the context stream) and keeps the best.
+// ---------------------------------------------------------------------------
+namespace {
+
+const uint32_t kTuneWaves[] = {4, 8, 12, 16, 20, 24};
+const uint32_t kTuneRefill[] = {16, 32, 48, 64};
+
+// Milliseconds per frame spent in each render traversal class.
+int time_render_classes(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t frames, double ms[3]) {
+  hipStream_t st = f->ctx->stream;
+  const size_t per_frame = 2 * (size_t)depth;
+  std::vector<hipEvent_t> ev(2 * per_frame * frames);
+  for (auto& e : ev) PK_HIP(hipEventCreate(&e));
+  std::vector<int> cls;
+  size_t ei = 0;
+  pk::launch_begin(f->dev, 0, f->theta, f->cam, st);
+  for (uint32_t fr = 0; fr < frames; ++fr) {
+    pk::launch_raygen(f->dev, st);
+    for (uint32_t b = 0; b < depth; ++b) {
+      (void)hipEventRecord(ev[ei++], st);
+      pk::launch_trace_closest(s->dev, f->dev, (int)b, trace_launch(f, closest_class(b)), st);
+      (void)hipEventRecord(ev[ei++], st);
+      cls.push_back(closest_class(b));
+      pk::launch_shade(s->dev, f->dev, (int)b, st);
+      (void)hipEventRecord(ev[ei++], st);
+      pk::launch_trace_shadow(s->dev, f->dev, trace_launch(f, PK_TUNE_SHADOW), st);
+      (void)hipEventRecord(ev[ei++], st);
+      cls.push_back(PK_TUNE_SHADOW);
+    }
+    pk::launch_accumulate(f->dev, st);
+    pk::launch_advance(f->dev, st);
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  for (int k = 0; k < 3; ++k) ms[k] = 0;
+  for (size_t i = 0; i < cls.size() && e == hipSuccess; ++i) {
+    float t = 0;
+    e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+    ms[cls[i]] += t / frames;
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("autotune: ") + hipGetErrorString(e));
+    return PK_E_HIP;
+  }
+  return PK_OK;
+}
+
+}  // namespace
+
+extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t frames, float* ms2) {
+  if (!f || !s || depth == 0 || depth > 64 || frames == 0) {
+    pk_set_error("pk_autotune_render: null argument, depth outside [1, 64] or zero frames");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_autotune_render: no camera set");
+    return PK_E_STATE;
+  }
+  pk_ctx* c = f->ctx;
+  const bool count = f->count_nodes;
+  f->count_nodes = false;
+  double base[3];
+  int rc = time_render_classes(f, s, depth, frames, base);
+  if (rc) return rc;
+  const double before = base[0] + base[1] + base[2];
+  for (int cls = PK_TUNE_PRIMARY; cls <= PK_TUNE_SHADOW; ++cls) {
+    pk_ctx::Shape best = c->shape[cls];
+    double best_ms = base[cls];
+    for (uint32_t w : kTuneWaves)
+      for (uint32_t r : kTuneRefill) {
+        c->shape[cls] = {w, r};
+        double ms[3];
+        if ((rc = time_render_classes(f, s, depth, frames, ms))) return rc;
+        if (ms[cls] < best_ms) {
+          best_ms = ms[cls];
+          best = c->shape[cls];
+        }
+      }
+    c->shape[cls] = best;
+  }
+  double after[3];
+  if ((rc = time_render_classes(f, s, depth, frames, after))) return rc;
+  c->tuning_gen++;
+  f->count_nodes = count;
+  if (ms2) {
+    ms2[0] = (float)before;
+    ms2[1] = (float)(after[0] + after[1] + after[2]);
+  }
+  return pk_frame_reset(f);
+}
+
+extern "C" int pk_autotune_rays(pk_ctx* c, pk_scene* s, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
+                                float* d_t, uint32_t* d_bary, float* ms2) {
+  int rc = check_rays(c, s, n, "pk_autotune_rays");
+  if (rc) return rc;
+  const int cls = any_hit ? PK_TUNE_RAYS_ANY : PK_TUNE_RAYS_CLOSEST;
+  float before = 0;
+  if ((rc = pk_time_trace_rays(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, 2, &before))) return rc;
+  pk_ctx::Shape best = c->shape[cls];
+  float best_ms = before;
+  for (uint32_t w : kTuneWaves)
+    for (uint32_t r : {1u, 8u, 16u, 24u, 32u, 48u, 64u}) {
+      c->shape[cls] = {w, r};
+      float ms = 0;
+      if ((rc = pk_time_trace_rays(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, 2, &ms))) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c->shape[cls];
+      }
+    }
+  c->shape[cls] = best;
+  c->tuning_gen++;
+  if (ms2) {
+    ms2[0] = before;
+    ms2[1] = best_ms;
+  }
+  return PK_OK;
+}
