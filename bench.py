@@ -37,6 +37,21 @@ def traversal_bytes(rays, nodes, tris):
     return rays * RAY_BYTES + nodes * NODE_BYTES + tris * TRI_BYTES
 
 
+def measured_traffic(kernel="k_trace_closest"):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known
+    size); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "render_c2_1080p_hbm_traffic.md")
+    try:
+        for line in open(path):
+            cells = [c.strip().strip("`") for c in line.split("|")]
+            if len(cells) > 7 and cells[1] == kernel:
+                return float(cells[7]) * 1e6
+    except OSError:
+        return None
+    return None
+
+
 def run_render(args, pike_amd, scenes):
     import numpy as np
 
@@ -103,7 +118,7 @@ def run_render(args, pike_amd, scenes):
         spp_s=spp_s, ms_per_step=dt / args.steps * 1e3, rays_per_s=rays_total / dt, counters=cnt,
         kernel_ms=ms, profile_frames=pf, profile_counters=cc,
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, kernel="k_trace_closest", bytes_per_launch=bytes_launch,
+                      traffic=measured_traffic(), kernel="k_trace_closest", bytes_per_launch=bytes_launch,
                       launch_ms=t_launch * 1e3),
         triangles=data.num_triangles, autotune=tuned)
     return result, data, bvh, cam

@@ -1,0 +1,39 @@
+'use strict';
+// BVH (render/bvh.js:7-108): same SAH split decisions, built natively
+// (pk_bvh_build) and collapsed into the 128-byte wide layout the kernels
+// traverse.  exportReference() returns the reference's flattened buffers
+// (trace_pass.js:220-263: BFS nodes of 5 words, leaf-ordered triangles,
+// normalized float32 vertices).
+
+class BVH {
+  constructor(native, desc) {
+    this.native = native;
+    this.numVertices = desc.positions.length / 3;
+    this.numTriangles = desc.triangles.length / 4;
+    this.handle = native.bvhBuild(desc.positions, desc.triangles, 0);
+    const info = native.bvhInfo(this.handle);
+    this.info = info;
+    this.depth = info.binaryDepth;
+    this._ref = null;
+  }
+
+  getNumLeafTris() {
+    return this.info.leafTriangles;
+  }
+
+  exportReference() {
+    if (!this._ref) this._ref = this.native.bvhExportReference(this.handle, this.numVertices, this.numTriangles);
+    return this._ref;
+  }
+
+  normalizer() {
+    return this.exportReference().normalizer;
+  }
+
+  destroy() {
+    if (this.handle) this.native.bvhDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+module.exports = { BVH };

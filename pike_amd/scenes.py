@@ -226,6 +226,39 @@ class SceneData:
     def num_triangles(self):
         return int(self.triangles.shape[0])
 
+    def save(self, directory):
+        """Raw arrays + scene.json manifest (read by pike_amd/js/raw_scene.js)."""
+        import json
+        import os
+
+        os.makedirs(directory, exist_ok=True)
+
+        def put(name, arr, dtype):
+            arr = np.ascontiguousarray(arr, dtype=dtype)
+            arr.tofile(os.path.join(directory, name))
+            code = {np.float64: "f64", np.float32: "f32", np.int32: "i32", np.uint32: "u32", np.uint8: "u8"}
+            return {"file": name, "dtype": code[np.dtype(dtype).type]}
+
+        tex = np.ascontiguousarray(self.tex)
+        m = {
+            "name": self.name,
+            "positions": put("positions.bin", self.positions, np.float64),
+            "attributes": put("attributes.bin", self.attributes, np.uint32),
+            "triangles": put("triangles.bin", self.triangles, np.int32),
+            "materials": put("materials.bin", self.materials, np.int32),
+            "atlas": {"width": int(tex.shape[2]), "height": int(tex.shape[1]), "layers": int(tex.shape[0]),
+                      "format": _ffi.PK_TEX_RGBA8 if tex.dtype == np.uint8 else _ffi.PK_TEX_RGBA32F,
+                      "data": put("atlas.bin", tex, tex.dtype)},
+            "camera": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in self.camera.items()},
+        }
+        if self.env is not None:
+            env = np.ascontiguousarray(self.env)
+            m["env"] = {"width": int(env.shape[1]), "height": int(env.shape[0]),
+                        "format": _ffi.PK_ENV_RGBE8 if env.dtype == np.uint8 else _ffi.PK_ENV_RGB32F,
+                        "sampler": int(self.env_sampler), "data": put("env.bin", env, env.dtype)}
+        with open(os.path.join(directory, "scene.json"), "w") as f:
+            json.dump(m, f)
+
     def desc(self):
         """ctypes pk_scene_desc over this scene's arrays (kept alive by self)."""
         pos = np.ascontiguousarray(self.positions, dtype=np.float64)
