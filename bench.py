@@ -124,6 +124,60 @@ def run_render(args, pike_amd, scenes):
     return result, data, bvh, cam
 
 
+def run_batching(args, pike_amd, scenes, ctx):
+    """Config 4: 32 scenes x 64 views (aggregate samples/s), plain per-view
+    wavefront renders vs one batched graph launch; the images are identical
+    (checked on every view), so the comparison is at equal error."""
+    import numpy as np
+
+    t0 = time.time()
+    sc, views = scenes.config4(seed=4)
+    threads = min(16, os.cpu_count() or 1)
+    bvhs = [pike_amd.BVH(s.positions, s.triangles, threads=threads) for s in sc]
+    per_scene = [pike_amd.Scene(ctx, s, b) for s, b in zip(sc, bvhs)]
+    merged = pike_amd.BatchScene(ctx, sc, bvhs)
+    cams = [scenes.normalized_camera(c, bvhs[k].normalizer()) for k, _, _, _, c in views]
+    samples = sum(w * h * spp for _, w, h, spp, _ in views)
+    log(f"config4: {len(sc)} scenes ({sum(s.num_triangles for s in sc)} triangles), {len(views)} views, "
+        f"{samples / 1e6:.1f} M samples ({time.time() - t0:.1f}s setup)")
+    depth = args.batch_depth
+    frames = []
+    for i, (k, w, h, spp, _) in enumerate(views):
+        f = pike_amd.Frame(ctx, w, h)
+        f.set_camera(cams[i])
+        frames.append(f)
+
+    def plain():
+        for i, (k, w, h, spp, _) in enumerate(views):
+            frames[i].reset()
+            frames[i].render(per_scene[k], max_depth=depth, first_sample=0, num_frames=spp, use_graph=True)
+        ctx.sync()
+
+    batch = pike_amd.Batch(ctx, merged, [(k, w, h, spp, cams[i]) for i, (k, w, h, spp, _) in enumerate(views)])
+
+    def batched():
+        batch.reset()
+        batch.render(max_depth=depth, use_graph=True)
+        ctx.sync()
+
+    plain()
+    batched()  # warm-up: instantiates every graph
+    t0 = time.perf_counter()
+    plain()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched()
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(batch.read_view(i).view(np.uint32), frames[i].read_image().view(np.uint32))
+               for i in range(len(views)))
+    out = dict(samples=samples, plain_s=t_plain, batched_s=t_batch, plain_msamples_s=samples / t_plain / 1e6,
+               batched_msamples_s=samples / t_batch / 1e6, speedup=t_plain / t_batch, images_identical=same,
+               depth=depth, scenes=len(sc), views=len(views))
+    log(f"batching: plain {out['plain_msamples_s']:.0f} Msamples/s, batched {out['batched_msamples_s']:.0f} "
+        f"Msamples/s ({out['speedup']:.2f}x), identical images: {same}")
+    return out
+
+
 def run_cpu_baseline(args, data, cam):
     from oracle.oracle import OracleScene
 
@@ -194,6 +248,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-depth", type=int, default=4)
     ap.add_argument("--tuning", default="", help="JSON {class: {waves_per_cu, refill_min}} to apply instead of "
                                                   "autotuning (e.g. the 'extra.autotune.shapes' of a previous run)")
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
@@ -222,6 +278,7 @@ def main():
     if args.mode == "render-only":
         args.no_cpu = True
         args.no_traversal = True
+        args.no_batch = True
         args.profile_frames = 0
 
     res, data, bvh, cam = run_render(args, pike_amd, scenes)
@@ -233,6 +290,12 @@ def main():
     cpu = None if args.no_cpu else run_cpu_baseline(args, data, cam)
     if not args.no_traversal:
         extra["traversal_10M_soup"] = run_traversal(args, pike_amd, scenes)
+    if not args.no_batch:
+        bctx = pike_amd.Context(0)
+        if res["autotune"]:
+            for cls, shape in res["autotune"]["shapes"].items():
+                bctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+        extra["batching_32_scenes_64_views"] = run_batching(args, pike_amd, scenes, bctx)
     roof = {k: res["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
     line = {
         "metric": "spp/s at 1920x1080 (plain wavefront, depth 8)",

@@ -213,6 +213,30 @@ int pk_frame_count_nodes(pk_frame* frame, int enable);
  * accumulate.  Renders exactly what pk_render renders. */
 int pk_render_profile(pk_frame* frame, pk_scene* scene, const pk_render_params* params, double* ms5);
 
+/* ---- batches: many scenes and camera views in one launch (no reference
+ * counterpart: Pike renders one scene and one camera).  The scenes are merged
+ * into one device scene (every scene keeps its own normalized space and BVH;
+ * they must share the atlas layer size/format, and the environment of
+ * descs[0] lights all of them).  A batch frame lays every view's pixels out
+ * in one global wavefront; pk_batch_render traces all sample rounds of all
+ * views (round r covers the views with spp > r) and, with use_graph,
+ * replays them as one captured hipGraph launch.  Per-pixel results equal
+ * pk_render of that view alone (seeds depend on the view-local pixel, the
+ * sample and the bounce only). ---- */
+typedef struct pk_view {
+  uint32_t scene;  /* index into the merged scenes */
+  uint32_t width, height, spp;
+  pk_camera camera; /* in that scene's normalized space */
+} pk_view;
+int pk_scene_upload_batch(pk_ctx* ctx, const pk_scene_desc* descs, const pk_bvh* const* bvhs, uint32_t count,
+                          pk_scene** out);
+/* Root wide node of each merged scene; returns the number of scenes. */
+int pk_scene_roots(const pk_scene* scene, uint32_t* roots, uint32_t count);
+int pk_batch_create(pk_ctx* ctx, pk_scene* merged, const pk_view* views, uint32_t nviews, pk_frame** out);
+int pk_batch_render(pk_frame* batch, uint32_t max_depth, float env_theta, int use_graph);
+/* Running mean of one view, width*height RGBA float. */
+int pk_batch_read_view(pk_frame* batch, uint32_t view, float* rgba, size_t bytes);
+
 /* ---- standalone ray queries (traversal microbenchmark) ----
  * rays: [n][8] floats {ox, oy, oz, 0, dx, dy, dz, 0} in normalized space.
  * Outputs per ray: original triangle id (-1 on miss), t, bary mask

@@ -18,7 +18,7 @@ import numpy as np
 from . import _ffi
 from ._ffi import Camera, Counters, RenderParams  # noqa: F401
 
-__all__ = ["PikeError", "Context", "BVH", "Scene", "Frame", "lib"]
+__all__ = ["PikeError", "Context", "BVH", "Scene", "BatchScene", "Batch", "Frame", "DeviceBuffer", "lib"]
 
 
 class PikeError(RuntimeError):
@@ -234,6 +234,68 @@ class Scene:
     def close(self):
         if self.h:
             lib().pk_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchScene(Scene):
+    """Several scenes merged into one device scene (pk_scene_upload_batch)."""
+
+    def __init__(self, ctx, datas, bvhs):  # noqa: super().__init__ is not used: one upload for all
+        self.ctx = ctx
+        self._datas = list(datas)
+        descs = (_ffi.SceneDesc * len(datas))()
+        for i, d in enumerate(datas):
+            descs[i] = d.desc()
+        self.descs = descs
+        handles = (ctypes.c_void_p * len(bvhs))(*[b.h.value for b in bvhs])
+        h = ctypes.c_void_p()
+        _check(lib().pk_scene_upload_batch(ctx.h, descs, handles, len(datas), ctypes.byref(h)),
+               "pk_scene_upload_batch")
+        self.h = h
+        self.num_scenes = len(datas)
+
+
+class Batch:
+    """Many views over a BatchScene, traced through one global wavefront."""
+
+    def __init__(self, ctx, scene, views):
+        """views: list of (scene index, width, height, spp, Camera)."""
+        self.ctx = ctx
+        self.scene = scene
+        arr = (_ffi.View * len(views))()
+        for i, (si, w, hh, spp, cam) in enumerate(views):
+            arr[i] = _ffi.View(si, w, hh, spp, cam)
+        self.views = [(si, w, hh, spp) for si, w, hh, spp, _ in views]
+        h = ctypes.c_void_p()
+        _check(lib().pk_batch_create(ctx.h, scene.h, arr, len(views), ctypes.byref(h)), "pk_batch_create")
+        self.h = h
+
+    def render(self, max_depth=4, env_theta=0.0, use_graph=True):
+        _check(lib().pk_batch_render(self.h, max_depth, float(env_theta), int(use_graph)), "pk_batch_render")
+
+    def read_view(self, i):
+        _, w, hh, _ = self.views[i]
+        out = np.empty((hh, w, 4), np.float32)
+        _check(lib().pk_batch_read_view(self.h, i, _ptr(out), out.nbytes), "pk_batch_read_view")
+        return out
+
+    def reset(self):
+        _check(lib().pk_frame_reset(self.h), "pk_frame_reset")
+
+    def counters(self):
+        c = Counters()
+        _check(lib().pk_read_counters(self.h, ctypes.byref(c)), "pk_read_counters")
+        return {k: int(getattr(c, k)) for k, _ in Counters._fields_}
+
+    def close(self):
+        if self.h:
+            lib().pk_frame_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -69,6 +69,16 @@ class Camera(ctypes.Structure):
     ]
 
 
+class View(ctypes.Structure):
+    _fields_ = [
+        ("scene", c_uint32),
+        ("width", c_uint32),
+        ("height", c_uint32),
+        ("spp", c_uint32),
+        ("camera", Camera),
+    ]
+
+
 class RenderParams(ctypes.Structure):
     _fields_ = [
         ("width", c_uint32),
@@ -139,6 +149,12 @@ SIGNATURES = {
     "pk_read_counters": (c_int, [c_void_p, ctypes.POINTER(Counters)]),
     "pk_frame_count_nodes": (c_int, [c_void_p, c_int]),
     "pk_render_profile": (c_int, [c_void_p, c_void_p, ctypes.POINTER(RenderParams), c_void_p]),
+    "pk_scene_upload_batch": (c_int, [c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(c_void_p), c_uint32,
+                                      ctypes.POINTER(c_void_p)]),
+    "pk_scene_roots": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "pk_batch_create": (c_int, [c_void_p, c_void_p, ctypes.POINTER(View), c_uint32, ctypes.POINTER(c_void_p)]),
+    "pk_batch_render": (c_int, [c_void_p, c_uint32, c_float, c_int]),
+    "pk_batch_read_view": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t]),
     "pk_trace_rays": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]),
     "pk_trace_rays_device": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p,
                                      c_int]),

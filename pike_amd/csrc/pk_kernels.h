@@ -39,6 +39,15 @@ struct FrameParams {
   pk_camera cam;
 };
 
+// One camera view of a batch: its pixels occupy [offset, offset + W*H) of
+// the batch's global path index space; rays of this view start at `root`,
+// the wide-BVH root of its scene inside the merged scene.
+struct ViewDev {
+  uint32_t W, H, offset, spp;
+  uint32_t root, scene, pad0, pad1;
+  pk_camera cam;
+};
+
 struct FrameDev {
   float4* qo[2];   // ray queues: origin.xyz, pixel id bits
   float4* qd[2];   //             direction.xyz
@@ -53,8 +62,21 @@ struct FrameDev {
   unsigned long long* stats;
   FrameParams* params;
   uint2* spill;    // traversal stack spill, per wave
-  uint32_t W, H;
+  uint32_t W, H;   // single view: the resolution; batch: W = total paths, H = 1
+  const ViewDev* views;  // batch views sorted by offset (nullptr: single view)
+  uint32_t nviews;
 };
+
+// view of global path index p (batch mode): views are sorted by offset
+__device__ inline uint32_t view_of(const FrameDev& F, uint32_t p) {
+  uint32_t lo = 0, hi = F.nviews;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (F.views[mid].offset <= p) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s);
 void launch_advance(const FrameDev& F, hipStream_t s);

@@ -45,20 +45,11 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
 // ---------------------------------------------------------------------------
 // camera.wgsl main (:75-92) / createPrimaryRay (:58-73)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
-  const uint32_t W = F.W, H = F.H, B = W * H;
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const FrameParams P = *F.params;
-  if (idx == 0) {
-    F.cnt[kCntRays0] = B;
-    F.cnt[kCntHits] = 0;
-    F.cnt[kCntWorkTrace] = 0;
-  }
-  if (idx >= B) return;
-  const uint32_t px = idx % W, py = idx / W;
-  uint32_t seed = (px * 1973u + py * 9277u + P.sample * 26699u) | 1u;
+__device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t sample,
+                                  const pk_camera& cam, v3* o_out, v3* d_out) {
+  uint32_t seed = (px * 1973u + py * 9277u + sample * 26699u) | 1u;
   seed = pkm::pcg_next(&seed);
-  const float k = P.cam.distortion;
+  const float k = cam.distortion;
   const float resx = (float)W, resy = (float)H;
   const float jx = pkm::pcg_rand(&seed);
   const float jy = pkm::pcg_rand(&seed);
@@ -70,23 +61,73 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   ux = ru * (ux / l2);
   uy = ru * (uy / l2);
   const v3 up = pkm::mk(0.0f, 1.0f, 0.0f);
-  const v3 dir = pkm::mk(P.cam.dir[0], P.cam.dir[1], P.cam.dir[2]);
-  const v3 org = pkm::mk(P.cam.origin[0], P.cam.origin[1], P.cam.origin[2]);
-  const float tf = pkm::tanf_(P.cam.fov * 0.5f);
+  const v3 dir = pkm::mk(cam.dir[0], cam.dir[1], cam.dir[2]);
+  const v3 org = pkm::mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+  const float tf = pkm::tanf_(cam.fov * 0.5f);
   const v3 bx = pkm::scale(pkm::normalize(pkm::cross(dir, up)), tf);
   const v3 by = pkm::scale(pkm::normalize(pkm::cross(bx, dir)), tf);
   const float theta = pkm::pcg_rand(&seed) * pkm::kTau;
-  const float pw = pkm::powf_(pkm::pcg_rand(&seed), P.cam.bokeh);
+  const float pw = pkm::powf_(pkm::pcg_rand(&seed), cam.bokeh);
   float st, ct;
   pkm::sincosf_(theta, &st, &ct);
-  const v3 dof = pkm::scale(pkm::scale(pkm::add(pkm::scale(bx, ct), pkm::scale(by, st)), P.cam.aperture), pw);
+  const v3 dof = pkm::scale(pkm::scale(pkm::add(pkm::scale(bx, ct), pkm::scale(by, st)), cam.aperture), pw);
   const v3 screen = pkm::add(pkm::add(pkm::add(pkm::scale(bx, ux), pkm::scale(by, uy)), dir), org);
-  const v3 d = pkm::normalize(pkm::sub(pkm::add(screen, pkm::scale(dof, P.cam.focal_depth)), pkm::add(org, dof)));
-  const v3 o = pkm::add(dof, org);
+  *d_out = pkm::normalize(pkm::sub(pkm::add(screen, pkm::scale(dof, cam.focal_depth)), pkm::add(org, dof)));
+  *o_out = pkm::add(dof, org);
+}
+
+__global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
+  const uint32_t W = F.W, H = F.H, B = W * H;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const FrameParams P = *F.params;
+  if (idx == 0) {
+    F.cnt[kCntRays0] = B;
+    F.cnt[kCntHits] = 0;
+    F.cnt[kCntWorkTrace] = 0;
+  }
+  if (idx >= B) return;
+  v3 o, d;
+  camera_ray(idx % W, idx / W, W, H, P.sample, P.cam, &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
   F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Batch ray generation: one thread per global path; pixels of views whose
+// sample budget is spent this round emit nothing, the rest are compacted
+// into ray queue 0 (d.w carries the view).  The queue counter is zeroed by
+// the previous round's trace_shadow / by pk_batch_render before round 0.
+__global__ void __launch_bounds__(256) k_raygen_batch(FrameDev F) {
+  const uint32_t B = F.W;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const FrameParams P = *F.params;
+  bool emit = false;
+  uint32_t view = 0;
+  v3 o = pkm::mk(0.0f, 0.0f, 0.0f), d = o;
+  if (idx < B) {
+    view = view_of(F, idx);
+    const ViewDev& V = F.views[view];
+    if (P.sample < V.spp) {
+      const uint32_t lp = idx - V.offset;
+      camera_ray(lp % V.W, lp / V.W, V.W, V.H, P.sample, V.cam, &o, &d);
+      F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+      F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      emit = true;
+    }
+  }
+  const uint32_t slot = wave_append(&F.cnt[kCntRays0], emit);
+  if (emit) {
+    F.qo[0][slot] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
+    F.qd[0][slot] = make_float4(d.x, d.y, d.z, __uint_as_float(view));
+  }
+}
+
+// Zeroes the queue counters a batch round's raygen and first trace use.
+__global__ void k_batch_round_begin(FrameDev F) {
+  F.cnt[kCntRays0] = 0;
+  F.cnt[kCntHits] = 0;
+  F.cnt[kCntWorkTrace] = 0;
 }
 
 __device__ inline void add_color(const FrameDev& F, uint32_t pid, v3 L) {
@@ -173,6 +214,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
         const float4 o4 = qo[slot], d4 = qd[slot];
         pid = __float_as_uint(o4.w);
         ray_init<false>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
+        if (F.views) r.node = F.views[__float_as_uint(d4.w)].root;
       },
       [&](uint32_t, const RayState& r) {
         traced++;
@@ -347,18 +389,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
   const bool active = idx < nh;
   bool emit_bounce = false, emit_shadow = false;
   v3 b_o, b_d, b_thr, s_o, s_d, s_thr;
-  uint32_t pid = 0;
+  uint32_t pid = 0, view_bits = 0;
   if (active) {
     const uint4 hr = F.hits[idx];
     const uint32_t slot = hr.x, leaf = hr.y, bary_mask = hr.w;
     const float t = __uint_as_float(hr.z);
     const float4 o4 = F.qo[in][slot], d4 = F.qd[in][slot];
     pid = __float_as_uint(o4.w);
+    view_bits = __float_as_uint(d4.w);
+    // the pixel index the seeds use is local to the view
+    const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid;
     const v3 ro = pkm::mk(o4.x, o4.y, o4.z), rd = pkm::mk(d4.x, d4.y, d4.z);
     const float4 t4 = F.thr[pid];
     const v3 thr = pkm::mk(t4.x, t4.y, t4.z);
     // deterministic seed: bounce index in place of the compaction slot
-    uint32_t seed = ((uint32_t)bounce * 1973u + pid * 9277u + P.sample * 26699u) | 1u;
+    uint32_t seed = ((uint32_t)bounce * 1973u + pixel * 9277u + P.sample * 26699u) | 1u;
     seed = pkm::pcg_next(&seed);
     // getSurfaceInteraction (:179-200), interpolateVertexAttribute (:363-394)
     const int4 tri = S.tri_index[leaf];
@@ -473,13 +518,13 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
   const uint32_t bs = wave_append(&F.cnt[kCntRays0 + (in ^ 1)], emit_bounce);
   if (emit_bounce) {
     F.qo[in ^ 1][bs] = make_float4(b_o.x, b_o.y, b_o.z, __uint_as_float(pid));
-    F.qd[in ^ 1][bs] = make_float4(b_d.x, b_d.y, b_d.z, 0.0f);
+    F.qd[in ^ 1][bs] = make_float4(b_d.x, b_d.y, b_d.z, __uint_as_float(view_bits));
     F.thr[pid] = make_float4(b_thr.x, b_thr.y, b_thr.z, 0.0f);
   }
   const uint32_t ss = wave_append(&F.cnt[kCntShadow], emit_shadow);
   if (emit_shadow) {
     F.so[ss] = make_float4(s_o.x, s_o.y, s_o.z, __uint_as_float(pid));
-    F.sd[ss] = make_float4(s_d.x, s_d.y, s_d.z, 0.0f);
+    F.sd[ss] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(view_bits));
     F.sthr[ss] = make_float4(s_thr.x, s_thr.y, s_thr.z, 0.0f);
   }
 }
@@ -507,6 +552,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
         const float4 o4 = F.so[slot], d4 = F.sd[slot];
         pid = __float_as_uint(o4.w);
         ray_init<true>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
+        if (F.views) r.node = F.views[__float_as_uint(d4.w)].root;
       },
       [&](uint32_t slot, const RayState& r) {
         traced++;
@@ -539,6 +585,8 @@ __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
   const uint32_t B = F.W * F.H;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= B) return;
+  // batch: views whose sample budget is spent this round keep their mean
+  if (F.views && F.params->sample >= F.views[view_of(F, idx)].spp) return;
   const float s = (float)F.params->sample;
   const float s1 = (float)(F.params->sample + 1u);
   const float4 c = F.color[idx];
@@ -623,7 +671,12 @@ void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_came
 }
 void launch_advance(const FrameDev& F, hipStream_t s) { hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, F); }
 void launch_raygen(const FrameDev& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+  if (F.views) {
+    hipLaunchKernelGGL(k_batch_round_begin, dim3(1), dim3(1), 0, s, F);
+    hipLaunchKernelGGL(k_raygen_batch, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+  } else {
+    hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+  }
 }
 void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
   FrameDev F = F0;

@@ -48,6 +48,7 @@ struct pk_scene {
   pk::DevScene dev{};
   std::vector<void*> allocs;
   int64_t env_info[4] = {0, 0, 0, 0};
+  std::vector<uint32_t> roots{0};  // wide-BVH root of each merged sub-scene
 };
 
 struct pk_frame {
@@ -64,7 +65,11 @@ struct pk_frame {
   const pk_scene* graph_scene = nullptr;
   uint32_t graph_depth = 0;
   uint32_t graph_gen = 0;
+  float graph_theta = 0;
   bool count_nodes = false;
+  // batch frames: views over one merged scene, paths laid out view after view
+  std::vector<pk::ViewDev> views;
+  pk_scene* batch_scene = nullptr;
 };
 
 namespace {
@@ -914,6 +919,210 @@ int pk_render_profile(pk_frame* f, pk_scene* s, const pk_render_params* p, doubl
     pk_set_error(std::string("pk_render_profile: ") + hipGetErrorString(e));
     return PK_E_HIP;
   }
+  return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many scenes and camera views traced through one global wavefront.
+// The scenes are merged into one device scene whose indices are global
+// (child and leaf references, vertex ids, material ids, atlas layers), so
+// the kernels need only a per-ray root node; each view's pixels own a range
+// of the global path index space.
+// ---------------------------------------------------------------------------
+int pk_scene_upload_batch(pk_ctx* c, const pk_scene_desc* descs, const pk_bvh* const* bvhs, uint32_t count,
+                          pk_scene** out) {
+  if (!c || !descs || !bvhs || !out || count == 0) {
+    pk_set_error("pk_scene_upload_batch: null argument or no scenes");
+    return PK_E_INVALID;
+  }
+  *out = nullptr;
+  const pk_scene_desc& d0 = descs[0];
+  pk_bvh merged;
+  pk_scene_desc md = d0;
+  std::vector<uint32_t> attrs, roots;
+  std::vector<int32_t> tris, mats;
+  std::vector<uint8_t> tex8;
+  std::vector<float> tex32;
+  const size_t texel_bytes = d0.tex_format == PK_TEX_RGBA8 ? 4 : 16;
+  uint32_t voff = 0, moff = 0, loff = 0;
+  for (uint32_t k = 0; k < count; ++k) {
+    const pk_scene_desc& d = descs[k];
+    const pk_bvh* b = bvhs[k];
+    if (!b || d.num_triangles != b->nt || d.num_vertices != b->nv || !d.materials || !d.tex_data) {
+      pk_set_error("pk_scene_upload_batch: scene " + std::to_string(k) +
+                   " lacks materials/atlas or its BVH was built for other geometry");
+      return PK_E_INVALID;
+    }
+    if (d.tex_width != d0.tex_width || d.tex_height != d0.tex_height || d.tex_format != d0.tex_format) {
+      pk_set_error("pk_scene_upload_batch: every scene's atlas needs the same layer size and format");
+      return PK_E_INVALID;
+    }
+    const uint32_t noff = (uint32_t)merged.wnodes.size(), toff = (uint32_t)merged.wtris.size();
+    roots.push_back(noff);
+    for (pk::WNode w : b->wnodes) {
+      for (int i = 0; i < w.nchild; ++i) w.child[i] += w.meta[i] ? toff : noff;
+      merged.wnodes.push_back(w);
+    }
+    merged.wtris.insert(merged.wtris.end(), b->wtris.begin(), b->wtris.end());
+    for (size_t i = 0; i < b->wtri_index.size(); i += 4) {
+      for (int v = 0; v < 3; ++v) merged.wtri_index.push_back(b->wtri_index[i + v] + (int32_t)voff);
+      merged.wtri_index.push_back(b->wtri_index[i + 3] + (int32_t)moff);
+    }
+    for (size_t t = 0; t < (size_t)d.num_triangles; ++t) {
+      for (int v = 0; v < 3; ++v) tris.push_back(d.triangles[t * 4 + v] + (int32_t)voff);
+      tris.push_back(d.triangles[t * 4 + 3] + (int32_t)moff);
+    }
+    if (d.attributes) attrs.insert(attrs.end(), d.attributes, d.attributes + (size_t)d.num_vertices * 4);
+    else attrs.insert(attrs.end(), (size_t)d.num_vertices * 4, 0u);
+    for (size_t i = 0; i < (size_t)d.num_materials * 4; ++i) mats.push_back(d.materials[i] + (int32_t)loff);
+    const size_t layer_bytes = (size_t)d.tex_width * d.tex_height * texel_bytes;
+    if (texel_bytes == 4)
+      tex8.insert(tex8.end(), (const uint8_t*)d.tex_data, (const uint8_t*)d.tex_data + layer_bytes * d.tex_layers);
+    else
+      tex32.insert(tex32.end(), (const float*)d.tex_data,
+                   (const float*)d.tex_data + layer_bytes / 4 * d.tex_layers);
+    voff += d.num_vertices;
+    moff += d.num_materials;
+    loff += d.tex_layers;
+  }
+  merged.nt = (uint32_t)merged.wtris.size();
+  merged.nv = voff;
+  md.num_vertices = voff;
+  md.num_triangles = merged.nt;
+  md.num_materials = moff;
+  md.tex_layers = loff;
+  md.attributes = attrs.data();
+  md.triangles = tris.data();
+  md.materials = mats.data();
+  md.tex_data = texel_bytes == 4 ? (const void*)tex8.data() : (const void*)tex32.data();
+  // the upload reads no positions: the wide layout already holds every
+  // scene's normalized geometry
+  md.positions = nullptr;
+  int rc = pk_scene_upload(c, &md, &merged, out);
+  if (rc) return rc;
+  (*out)->roots = roots;
+  return PK_OK;
+}
+
+int pk_scene_roots(const pk_scene* s, uint32_t* roots, uint32_t count) {
+  if (!s || (count && !roots)) {
+    pk_set_error("pk_scene_roots: null argument");
+    return PK_E_INVALID;
+  }
+  for (uint32_t i = 0; i < count && i < s->roots.size(); ++i) roots[i] = s->roots[i];
+  return (int)s->roots.size();
+}
+
+int pk_batch_create(pk_ctx* c, pk_scene* s, const pk_view* views, uint32_t nviews, pk_frame** out) {
+  if (!c || !s || !views || !out || nviews == 0) {
+    pk_set_error("pk_batch_create: null argument or no views");
+    return PK_E_INVALID;
+  }
+  *out = nullptr;
+  std::vector<pk::ViewDev> vd(nviews);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < nviews; ++i) {
+    const pk_view& v = views[i];
+    if (v.scene >= s->roots.size() || v.width == 0 || v.height == 0 || v.spp == 0) {
+      pk_set_error("pk_batch_create: view " + std::to_string(i) + " has a bad scene index, size or sample count");
+      return PK_E_INVALID;
+    }
+    vd[i] = pk::ViewDev{v.width, v.height, (uint32_t)total, v.spp, s->roots[v.scene], v.scene, 0, 0, v.camera};
+    total += (uint64_t)v.width * v.height;
+  }
+  if (total >= (1ull << 28)) {
+    pk_set_error("pk_batch_create: more than 2^28 pixels in one batch");
+    return PK_E_LIMIT;
+  }
+  pk_frame* f = nullptr;
+  int rc = pk_frame_create(c, (uint32_t)total, 1, &f);
+  if (rc) return rc;
+  pk::ViewDev* dv = nullptr;
+  if ((rc = dev_alloc(f->allocs, &dv, nviews))) {
+    pk_frame_destroy(f);
+    return rc;
+  }
+  hipError_t e = hipMemcpy(dv, vd.data(), nviews * sizeof(pk::ViewDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("pk_batch_create: ") + hipGetErrorString(e));
+    pk_frame_destroy(f);
+    return PK_E_HIP;
+  }
+  f->dev.views = dv;
+  f->dev.nviews = nviews;
+  f->views = vd;
+  f->batch_scene = s;
+  f->cam = views[0].camera;
+  f->has_camera = true;
+  *out = f;
+  return PK_OK;
+}
+
+static void enqueue_batch(pk_frame* f, pk_scene* s, uint32_t depth, float theta, hipStream_t st) {
+  uint32_t rounds = 0;
+  for (const auto& v : f->views) rounds = std::max(rounds, v.spp);
+  for (uint32_t r = 0; r < rounds; ++r) {
+    pk::launch_begin(f->dev, r, theta, f->cam, st);
+    pk::launch_raygen(f->dev, st);
+    for (uint32_t b = 0; b < depth; ++b) {
+      pk::launch_trace_closest(s->dev, f->dev, (int)b, trace_launch(f, closest_class(b)), st);
+      pk::launch_shade(s->dev, f->dev, (int)b, st);
+      pk::launch_trace_shadow(s->dev, f->dev, trace_launch(f, PK_TUNE_SHADOW), st);
+    }
+    pk::launch_accumulate(f->dev, st);
+  }
+}
+
+int pk_batch_render(pk_frame* f, uint32_t depth, float theta, int use_graph) {
+  if (!f || !f->batch_scene) {
+    pk_set_error("pk_batch_render: not a batch (pk_batch_create)");
+    return PK_E_INVALID;
+  }
+  if (depth == 0 || depth > 64) {
+    pk_set_error("pk_batch_render: depth outside [1, 64]");
+    return PK_E_LIMIT;
+  }
+  pk_scene* s = f->batch_scene;
+  hipStream_t st = f->ctx->stream;
+  f->theta = theta;
+  f->begun = true;
+  if (!use_graph) {
+    enqueue_batch(f, s, depth, theta, st);
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+  }
+  if (!f->exec || f->graph_depth != depth || f->graph_gen != f->ctx->tuning_gen || f->graph_theta != theta) {
+    drop_graph(f);
+    PK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_batch(f, s, depth, theta, st);
+    hipError_t e = hipStreamEndCapture(st, &f->graph);
+    if (e != hipSuccess) {
+      pk_set_error(std::string("pk_batch_render: graph capture failed: ") + hipGetErrorString(e));
+      return PK_E_HIP;
+    }
+    PK_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
+    f->graph_scene = s;
+    f->graph_depth = depth;
+    f->graph_gen = f->ctx->tuning_gen;
+    f->graph_theta = theta;
+  }
+  PK_HIP(hipGraphLaunch(f->exec, st));
+  return PK_OK;
+}
+
+int pk_batch_read_view(pk_frame* f, uint32_t view, float* rgba, size_t bytes) {
+  if (!f || !f->batch_scene || view >= f->views.size() || !rgba) {
+    pk_set_error("pk_batch_read_view: not a batch, view out of range or null buffer");
+    return PK_E_INVALID;
+  }
+  const pk::ViewDev& v = f->views[view];
+  const size_t n = (size_t)v.W * v.H * 16;
+  if (bytes < n) {
+    pk_set_error("pk_batch_read_view: buffer smaller than width*height*16");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipStreamSynchronize(f->ctx->stream));
+  PK_HIP(hipMemcpy(rgba, f->dev.accum + v.offset, n, hipMemcpyDeviceToHost));
   return PK_OK;
 }
 

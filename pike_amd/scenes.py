@@ -518,6 +518,58 @@ def config5(seed=5, cells=300, num_spheres=50, tex_res=512, env_size=(512, 256))
                      tex=tex, env=env, camera=camera((0.5, 0.5, 1.4), (0.5, 0.05, 0.45), 1.0), name="c5")
 
 
+C4_RESOLUTIONS = ((128, 128), (256, 256), (512, 384), (1024, 1024))
+
+
+def mixed_scene(seed, target_tris, env=None):
+    """Terrain + subdivision-2 icospheres with about target_tris triangles
+    (a quarter to two thirds in the terrain, the rest in spheres)."""
+    rng = Rng(seed)
+    frac = rng.uniform(1, 0.25, 0.67)[0]
+    cells = max(4, int(math.sqrt(target_tris * frac / 2)))
+    tp, ta, tt = terrain(cells, seed, 0.2)
+    packer = AtlasPacker()
+    mats = [packer.material(rng.uniform(3, 0.3, 0.6), rng.uniform(1, 0.6, 1.0)[0], 0.0)]
+    parts = [(tp, ta, tt, 0)]
+    uv_, f = icosphere(2)
+    sa = _sphere_attrs(uv_)
+    n = max(1, (target_tris - len(tt)) // len(f))
+    cx, cz = rng.uniform(n, 0.05, 0.95), rng.uniform(n, 0.05, 0.95)
+    r = rng.uniform(n, 0.01, 0.05)
+    cy = fbm_height(cx, cz, seed, 6, 0.2) + 0.8 * r
+    albedo = rng.uniform(n * 3, 0.2, 0.8).reshape(n, 3)
+    rough = rng.uniform(n, 0.05, 1.0)
+    metal = rng.uniform(n) < 0.3
+    for i in range(n):
+        mats.append(packer.material(albedo[i], rough[i], 1.0 if metal[i] else 0.0))
+        parts.append((np.array([cx[i], cy[i], cz[i]]) + r[i] * uv_, sa, f, i + 1))
+    pos, attr, tris = _merge(parts)
+    return SceneData(positions=pos, attributes=attr, triangles=tris, materials=np.array(mats, dtype=np.int32),
+                     tex=packer.atlas(), env=env, name=f"mixed{seed}")
+
+
+def config4(seed=4, num_scenes=32, num_views=64, tri_range=(1_000, 500_000), spp_range=(8, 64), env_size=(512, 256)):
+    """Batching: scenes with triangle counts log-uniform in tri_range and
+    views with resolutions from C4_RESOLUTIONS and spp in spp_range; one
+    shared log-normal + sun environment.  Returns (scenes, views) with views
+    as (scene index, width, height, spp, world-space camera dict)."""
+    rng = Rng(seed)
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    counts = np.exp(rng.uniform(num_scenes, lo, hi)).astype(np.int64)
+    env = lognormal_sun_env(env_size[0], env_size[1], seed + 17, sigma=1.0, sun_deg=2.0, sun_scale=100.0)
+    sc = [mixed_scene(seed * 1000 + k, int(counts[k]), env=env) for k in range(num_scenes)]
+    views = []
+    res_idx = rng.u32(num_views) % len(C4_RESOLUTIONS)
+    spp = spp_range[0] + (rng.u32(num_views) % (spp_range[1] - spp_range[0] + 1))
+    az = rng.uniform(num_views, 0.0, 2 * math.pi)
+    for v in range(num_views):
+        k = v % num_scenes
+        w, h = C4_RESOLUTIONS[int(res_idx[v])]
+        o = (0.5 + 0.75 * math.cos(az[v]), 0.45, 0.5 + 0.75 * math.sin(az[v]))
+        views.append((k, w, h, int(spp[v]), camera(o, (0.5, 0.05, 0.5), 1.0)))
+    return sc, views
+
+
 def random_scene(seed, num_spheres=6, subdiv=1, env=True):
     """Small scene for parity tests: a few spheres, a ground quad, the
     constant environment."""

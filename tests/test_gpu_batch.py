@@ -1,0 +1,51 @@
+"""Batched multi-scene, multi-view tracing (one global wavefront, one graph
+launch) renders exactly what each view renders alone through pk_render:
+seeds depend only on the view-local pixel, the sample and the bounce, and
+the traversal result is the canonical closest hit in each scene's BVH."""
+import numpy as np
+import pytest
+
+import pike_amd
+from pike_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch():
+    env = scenes.lognormal_sun_env(64, 32, seed=5, sigma=1.0, sun_deg=6.0, sun_scale=30.0)
+    sc = [scenes.mixed_scene(100 + k, n, env=env) for k, n in enumerate([1200, 5000, 20000])]
+    views = [(0, 40, 30, 3), (1, 64, 48, 1), (2, 32, 32, 4), (1, 24, 40, 2), (0, 16, 16, 5)]
+    return sc, views
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_equals_per_view_render(ctx, use_graph):
+    sc, views = _small_batch()
+    bvhs = [pike_amd.BVH(s.positions, s.triangles) for s in sc]
+    merged = pike_amd.BatchScene(ctx, sc, bvhs)
+    cams = []
+    for i, (k, w, h, spp) in enumerate(views):
+        az = 0.7 * i
+        cam = scenes.camera((0.5 + 0.8 * np.cos(az), 0.4, 0.5 + 0.8 * np.sin(az)), (0.5, 0.05, 0.5), 1.0)
+        cams.append(scenes.normalized_camera(cam, bvhs[k].normalizer()))
+    batch = pike_amd.Batch(ctx, merged, [(k, w, h, spp, cams[i]) for i, (k, w, h, spp) in enumerate(views)])
+    batch.render(max_depth=4, use_graph=use_graph)
+    assert batch.counters()["stack_overflow"] == 0
+    for i, (k, w, h, spp) in enumerate(views):
+        scene = pike_amd.Scene(ctx, sc[k], bvhs[k])
+        fr = pike_amd.Frame(ctx, w, h)
+        fr.set_camera(cams[i])
+        fr.render(scene, max_depth=4, first_sample=0, num_frames=spp)
+        np.testing.assert_array_equal(batch.read_view(i).view(np.uint32), fr.read_image().view(np.uint32),
+                                      err_msg=f"view {i}")
+
+
+def test_batch_rejects_bad_views(ctx):
+    sc, _ = _small_batch()
+    bvhs = [pike_amd.BVH(s.positions, s.triangles) for s in sc]
+    merged = pike_amd.BatchScene(ctx, sc, bvhs)
+    cam = scenes.normalized_camera(scenes.camera((0.5, 0.5, 1.5), (0.5, 0.0, 0.5)), bvhs[0].normalizer())
+    with pytest.raises(pike_amd.PikeError):
+        pike_amd.Batch(ctx, merged, [(7, 8, 8, 1, cam)])
+    with pytest.raises(pike_amd.PikeError):
+        pike_amd.Batch(ctx, merged, [(0, 8, 8, 0, cam)])
