@@ -201,6 +201,8 @@ int pk_accumulate(pk_frame* frame);                        /* AccumulatePass */
 int pk_render(pk_frame* frame, pk_scene* scene, const pk_render_params* params);
 /* Running-mean image, float RGBA, width*height*16 bytes. */
 int pk_read_image(pk_frame* frame, float* rgba, size_t bytes);
+/* The same image copied to device memory (asynchronous, context stream). */
+int pk_read_image_device(pk_frame* frame, float* d_rgba, size_t bytes);
 /* The current frame's color buffer (RenderState.colorBuffer) before accumulation. */
 int pk_read_color(pk_frame* frame, float* rgba, size_t bytes);
 int pk_read_counters(pk_frame* frame, pk_counters* out);
@@ -257,6 +259,56 @@ int pk_ctx_trace_stats(pk_ctx* ctx, uint64_t* stats2);
  * the context stream and returns the mean kernel milliseconds. */
 int pk_time_trace_rays(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit,
                        int32_t* d_ids, float* d_t, uint32_t* d_bary, int iters, float* ms_out);
+
+/* ---- differentiation (no reference counterpart: Pike renders only; the
+ * parameters are the ones shade_hit.wgsl:179-200 reads from its atlas and
+ * shade_miss.wgsl:37-43 from its environment) ----
+ * Parameters: PK_PARAM_ATLAS is the texture atlas in its own layout,
+ * [layers][height][width] RGBA f32 (base color = RGB of a material's diffuse
+ * layer, roughness = G of its metal-roughness layer, alpha = roughness^2);
+ * PK_PARAM_ENV is the environment radiance, [height][width] RGBA f32 (A
+ * unused).  Sampling is detached: sampling decisions and pdfs (opacity,
+ * metalness, the GGX sampling roughness, the environment's importance tables)
+ * keep the values given at upload, while BSDF values and environment radiance
+ * read the current parameters.  A frame rendered from the current parameters
+ * is therefore a smooth function of them along fixed paths, which central
+ * finite differences over common seeds and the adjoint both differentiate. */
+#define PK_PARAM_ATLAS 0
+#define PK_PARAM_ENV 1
+/* info6 = {tex width, tex height, tex layers, tex format, env width, env height} (env 0x0 if none). */
+int pk_scene_param_info(const pk_scene* scene, uint32_t* info6);
+/* Writes count floats of a parameter set starting at float `offset`
+ * (host or device source).  Atlas writes need an RGBA32F atlas; the first one
+ * with also_sampling == 0 gives the scene its own evaluation atlas (a copy of
+ * the sampling atlas), after which the two diverge.  also_sampling != 0 also
+ * writes the sampling atlas (sampling follows the parameters).  Environment
+ * writes change radiance only (the importance tables stay frozen). */
+int pk_scene_write_params(pk_scene* scene, int which, size_t offset, const float* src, size_t count,
+                          int src_on_device, int also_sampling);
+/* Reads the current (evaluation) parameters to host memory. */
+int pk_scene_read_params(pk_scene* scene, int which, size_t offset, float* dst, size_t count);
+/* L2 image loss of the frame's running-mean image against a device target
+ * (width*height RGBA f32): writes d_grad = 2 (image - target) (A = 0, device)
+ * and, if loss != NULL, blocks and returns sum over pixels and RGB of
+ * (image - target)^2. */
+int pk_loss_l2(pk_frame* frame, const float* d_target, float* d_grad, double* loss);
+typedef struct pk_grad_buffers {
+  float* atlas;    /* PK_PARAM_ATLAS layout, or NULL */
+  float* env;      /* PK_PARAM_ENV layout, or NULL */
+  int on_device;   /* 1: device pointers (asynchronous); 0: host pointers (the call blocks) */
+  int accumulate;  /* 1: add to the buffers' contents; 0: overwrite them */
+} pk_grad_buffers;
+/* Path-replay adjoint: gradients of sum_p sum_c d_image[p][c] * I[p][c],
+ * where I is the mean over the samples first_sample .. first_sample +
+ * num_frames - 1 of the frame's paths (with first_sample 0, the image
+ * pk_render produces on a reset frame with the same params), with respect to
+ * both parameter sets.  Replaying other samples than the forward's gives an
+ * independent gradient estimate.  d_image is width*height RGBA f32
+ * (A ignored), host or device.  Each path is recomputed from its seeds
+ * (nothing is stored by the forward render); device gradients are summed
+ * with atomics, so their low bits vary run to run. */
+int pk_adjoint(pk_frame* frame, pk_scene* scene, const pk_render_params* params, const float* d_image,
+               int d_image_on_device, const pk_grad_buffers* grads);
 
 #ifdef __cplusplus
 }

@@ -265,6 +265,7 @@ struct or_scene {
   uint32_t tw = 0, th = 0, tl = 0, tfmt = 0;
   std::vector<uint8_t> tex8;
   std::vector<float> tex32;
+  std::vector<float> tex32_eval;  // evaluation atlas (empty: the sampling atlas)
   uint32_t ew = 0, eh = 0, efmt = 0, esampler = 0;
   std::vector<uint8_t> env8;
   std::vector<float> env32;
@@ -755,31 +756,117 @@ void build_env(or_scene* s) {
   }
 }
 
+// Forward-mode dual number for the reference gradients: a value and its
+// derivatives with respect to up to kDualN seeded parameters.  S = double is
+// the FP64 reference; S = float keeps the value arithmetic of the FP32
+// restatement (so its paths are bit-identical to the product's).
+constexpr int kDualN = 16;
+template <class S> struct DualT {
+  S v = 0;
+  S g[kDualN] = {};
+  DualT() = default;
+  DualT(S x) : v(x) {}  // NOLINT: constants and real values convert implicitly
+};
+template <class S> inline DualT<S> operator+(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r(a.v + b.v);
+  for (int i = 0; i < kDualN; ++i) r.g[i] = a.g[i] + b.g[i];
+  return r;
+}
+template <class S> inline DualT<S> operator-(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r(a.v - b.v);
+  for (int i = 0; i < kDualN; ++i) r.g[i] = a.g[i] - b.g[i];
+  return r;
+}
+template <class S> inline DualT<S> operator-(const DualT<S>& a) {
+  DualT<S> r(-a.v);
+  for (int i = 0; i < kDualN; ++i) r.g[i] = -a.g[i];
+  return r;
+}
+template <class S> inline DualT<S> operator*(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r(a.v * b.v);
+  for (int i = 0; i < kDualN; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+  return r;
+}
+template <class S> inline DualT<S> operator/(const DualT<S>& a, const DualT<S>& b) {
+  DualT<S> r(a.v / b.v);
+  for (int i = 0; i < kDualN; ++i) r.g[i] = (a.g[i] - r.v * b.g[i]) / b.v;
+  return r;
+}
+// mixed real / dual operands (template deduction does not convert)
+template <class S> inline DualT<S> operator+(S a, const DualT<S>& b) { return DualT<S>(a) + b; }
+template <class S> inline DualT<S> operator+(const DualT<S>& a, S b) { return a + DualT<S>(b); }
+template <class S> inline DualT<S> operator-(S a, const DualT<S>& b) { return DualT<S>(a) - b; }
+template <class S> inline DualT<S> operator-(const DualT<S>& a, S b) { return a - DualT<S>(b); }
+template <class S> inline DualT<S> operator*(S a, const DualT<S>& b) { return DualT<S>(a) * b; }
+template <class S> inline DualT<S> operator*(const DualT<S>& a, S b) { return a * DualT<S>(b); }
+template <class S> inline DualT<S> operator/(S a, const DualT<S>& b) { return DualT<S>(a) / b; }
+template <class S> inline DualT<S> operator/(const DualT<S>& a, S b) { return a / DualT<S>(b); }
+template <class S> inline DualT<S>& operator+=(DualT<S>& a, const DualT<S>& b) { return a = a + b; }
+template <class S> inline bool operator<(const DualT<S>& a, const DualT<S>& b) { return a.v < b.v; }
+template <class S> inline bool operator<=(const DualT<S>& a, S b) { return a.v <= b; }
+template <class S> inline bool operator>(const DualT<S>& a, const DualT<S>& b) { return a.v > b.v; }
+// square root of a value type: the f32 contract, libm, or the dual rule
+inline float usqrt(float x) { return M<float>::sqrt(x); }
+inline double usqrt(double x) { return std::sqrt(x); }
+template <class S> inline DualT<S> usqrt(const DualT<S>& a) {
+  DualT<S> r(usqrt(a.v));
+  const S k = r.v > S(0) ? S(0.5) / r.v : S(0);
+  for (int i = 0; i < kDualN; ++i) r.g[i] = a.g[i] * k;
+  return r;
+}
+// mixed real / dual vector arithmetic of the path loop
+template <class S> inline V3<DualT<S>> operator*(V3<S> a, V3<DualT<S>> b) {
+  return v3(DualT<S>(a.x) * b.x, DualT<S>(a.y) * b.y, DualT<S>(a.z) * b.z);
+}
+template <class S> inline V3<DualT<S>> operator*(V3<DualT<S>> a, S s) {
+  return v3(a.x * DualT<S>(s), a.y * DualT<S>(s), a.z * DualT<S>(s));
+}
+
+// Parameters whose derivatives a dual evaluation carries: float offsets into
+// the PK_PARAM_ATLAS / PK_PARAM_ENV layouts mapped to dual slots.
+struct Seeds {
+  std::vector<std::pair<int64_t, int>> atlas, env;  // sorted by offset
+  static int find(const std::vector<std::pair<int64_t, int>>& v, int64_t off) {
+    auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(off, -1));
+    return it != v.end() && it->first == off ? it->second : -1;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Shading (shade_hit.wgsl), camera (camera.wgsl), miss (shade_miss.wgsl)
+//
+// T is the real type of geometry and sampling; U is the type of the values
+// that depend on the differentiable parameters (base color, GGX evaluation
+// roughness, environment radiance) and of everything they multiply (BSDF
+// weights, throughput, radiance).  U = T renders; U = Dual differentiates.
+// Sampling reads the sampling atlas, evaluation the evaluation atlas
+// (detached sampling, as in the product).
 // ---------------------------------------------------------------------------
 template <class T> struct Ctx {
   const or_scene* s;
   const std::vector<CNode>* nodes;
   T env_theta;
   bool sample_env;
+  const Seeds* seeds;
 };
 
-template <class T> inline T texel(const or_scene* s, int layer, int x, int y, int c) {
+template <class T> inline T texel(const or_scene* s, const std::vector<float>& t32, int layer, int x, int y, int c) {
   size_t i = (((size_t)layer * s->th + (size_t)y) * s->tw + (size_t)x) * 4 + c;
   if (s->tfmt == PK_TEX_RGBA8) return (T)((float)s->tex8[i] / 255.0f);
-  return (T)s->tex32[i];
+  return (T)t32[i];
 }
-template <> inline double texel<double>(const or_scene* s, int layer, int x, int y, int c) {
+template <>
+inline double texel<double>(const or_scene* s, const std::vector<float>& t32, int layer, int x, int y, int c) {
   size_t i = (((size_t)layer * s->th + (size_t)y) * s->tw + (size_t)x) * 4 + c;
   if (s->tfmt == PK_TEX_RGBA8) return (double)s->tex8[i] / 255.0;
-  return (double)s->tex32[i];
+  return (double)t32[i];
 }
 inline int wrap(int i, int n) {
   int r = i % n;
   return r < 0 ? r + n : r;
 }
-// textureSampleLevel with a linear filter and repeat addressing, level 0
+// textureSampleLevel with a linear filter and repeat addressing, level 0,
+// of the sampling atlas
 template <class T> void sample_tex(const or_scene* s, int layer, T u, T v, T out[4]) {
   T x = u * (T)s->tw - T(0.5), y = v * (T)s->th - T(0.5);
   T fx0 = M<T>::floor(x), fy0 = M<T>::floor(y);
@@ -787,34 +874,91 @@ template <class T> void sample_tex(const or_scene* s, int layer, T u, T v, T out
   int x0 = wrap((int)fx0, (int)s->tw), x1 = wrap((int)fx0 + 1, (int)s->tw);
   int y0 = wrap((int)fy0, (int)s->th), y1 = wrap((int)fy0 + 1, (int)s->th);
   for (int c = 0; c < 4; ++c) {
-    T a = texel<T>(s, layer, x0, y0, c), b = texel<T>(s, layer, x1, y0, c);
-    T cc = texel<T>(s, layer, x0, y1, c), dd = texel<T>(s, layer, x1, y1, c);
+    T a = texel<T>(s, s->tex32, layer, x0, y0, c), b = texel<T>(s, s->tex32, layer, x1, y0, c);
+    T cc = texel<T>(s, s->tex32, layer, x0, y1, c), dd = texel<T>(s, s->tex32, layer, x1, y1, c);
     T top = a + (b - a) * fx;
     T bot = cc + (dd - cc) * fx;
     out[c] = top + (bot - top) * fy;
   }
 }
+// evaluation texel as a value type (seeded when differentiating)
+template <class T, class U> U eval_texel(const Ctx<T>& c, int layer, int x, int y, int ch);
+template <> float eval_texel<float, float>(const Ctx<float>& c, int layer, int x, int y, int ch) {
+  const or_scene* s = c.s;
+  return texel<float>(s, s->tex32_eval.empty() ? s->tex32 : s->tex32_eval, layer, x, y, ch);
+}
+template <> double eval_texel<double, double>(const Ctx<double>& c, int layer, int x, int y, int ch) {
+  const or_scene* s = c.s;
+  return texel<double>(s, s->tex32_eval.empty() ? s->tex32 : s->tex32_eval, layer, x, y, ch);
+}
+template <class T>
+DualT<T> seeded_texel(const Ctx<T>& c, int layer, int x, int y, int ch) {
+  DualT<T> r(eval_texel<T, T>(c, layer, x, y, ch));
+  const int64_t off = ((((int64_t)layer * c.s->th + y) * c.s->tw + x) * 4 + ch);
+  const int slot = c.seeds ? Seeds::find(c.seeds->atlas, off) : -1;
+  if (slot >= 0) r.g[slot] = T(1);
+  return r;
+}
+template <> DualT<float> eval_texel<float, DualT<float>>(const Ctx<float>& c, int layer, int x, int y, int ch) {
+  return seeded_texel<float>(c, layer, x, y, ch);
+}
+template <> DualT<double> eval_texel<double, DualT<double>>(const Ctx<double>& c, int layer, int x, int y, int ch) {
+  return seeded_texel<double>(c, layer, x, y, ch);
+}
+// the same lookup of the evaluation atlas
+template <class T, class U> void sample_tex_eval(const Ctx<T>& c, int layer, T u, T v, U out[4]) {
+  const or_scene* s = c.s;
+  T x = u * (T)s->tw - T(0.5), y = v * (T)s->th - T(0.5);
+  T fx0 = M<T>::floor(x), fy0 = M<T>::floor(y);
+  T fx = x - fx0, fy = y - fy0;
+  int x0 = wrap((int)fx0, (int)s->tw), x1 = wrap((int)fx0 + 1, (int)s->tw);
+  int y0 = wrap((int)fy0, (int)s->th), y1 = wrap((int)fy0 + 1, (int)s->th);
+  for (int ch = 0; ch < 4; ++ch) {
+    U a = eval_texel<T, U>(c, layer, x0, y0, ch), b = eval_texel<T, U>(c, layer, x1, y0, ch);
+    U cc = eval_texel<T, U>(c, layer, x0, y1, ch), dd = eval_texel<T, U>(c, layer, x1, y1, ch);
+    U top = a + (b - a) * fx;
+    U bot = cc + (dd - cc) * fx;
+    out[ch] = top + (bot - top) * fy;
+  }
+}
 
-template <class T> V3<T> env_radiance(const or_scene* s, size_t idx);
-template <> V3<float> env_radiance<float>(const or_scene* s, size_t i) {
+template <class T, class U> V3<U> env_radiance(const Ctx<T>& c, size_t idx);
+template <> V3<float> env_radiance<float, float>(const Ctx<float>& c, size_t i) {
+  const or_scene* s = c.s;
   return v3(s->erad32[i * 3], s->erad32[i * 3 + 1], s->erad32[i * 3 + 2]);
 }
-template <> V3<double> env_radiance<double>(const or_scene* s, size_t i) {
+template <> V3<double> env_radiance<double, double>(const Ctx<double>& c, size_t i) {
+  const or_scene* s = c.s;
   return v3(s->erad64[i * 3], s->erad64[i * 3 + 1], s->erad64[i * 3 + 2]);
+}
+template <class T> V3<DualT<T>> seeded_env(const Ctx<T>& c, size_t i) {
+  V3<T> e = env_radiance<T, T>(c, i);
+  DualT<T> r[3] = {DualT<T>(e.x), DualT<T>(e.y), DualT<T>(e.z)};
+  for (int k = 0; k < 3; ++k) {
+    const int slot = c.seeds ? Seeds::find(c.seeds->env, (int64_t)i * 4 + k) : -1;
+    if (slot >= 0) r[k].g[slot] = T(1);
+  }
+  return v3(r[0], r[1], r[2]);
+}
+template <> V3<DualT<float>> env_radiance<float, DualT<float>>(const Ctx<float>& c, size_t i) {
+  return seeded_env<float>(c, i);
+}
+template <> V3<DualT<double>> env_radiance<double, DualT<double>>(const Ctx<double>& c, size_t i) {
+  return seeded_env<double>(c, i);
 }
 template <class T> T env_pdf_tex(const or_scene* s, size_t i);
 template <> float env_pdf_tex<float>(const or_scene* s, size_t i) { return s->epdf32[i]; }
 template <> double env_pdf_tex<double>(const or_scene* s, size_t i) { return s->epdf64[i]; }
 
 // shade_miss.wgsl envColor (:37-43): nearest sampler, repeat on u and v
-template <class T> V3<T> env_color(const Ctx<T>& c, V3<T> dir) {
+template <class T, class U> V3<U> env_color(const Ctx<T>& c, V3<T> dir) {
   const or_scene* s = c.s;
-  if (s->efmt == PK_ENV_NONE) return v3(T(0), T(0), T(0));
+  if (s->efmt == PK_ENV_NONE) return v3(U(0), U(0), U(0));
   T u = c.env_theta + M<T>::atan2(dir.z, dir.x) / M<T>::tau();
   T v = M<T>::acos(dir.y) * M<T>::inv_pi();
   int ix = wrap((int)M<T>::floor(u * (T)s->ew), (int)s->ew);
   int iy = wrap((int)M<T>::floor(v * (T)s->eh), (int)s->eh);
-  return env_radiance<T>(s, (size_t)iy * s->ew + (size_t)ix);
+  return env_radiance<T, U>(c, (size_t)iy * s->ew + (size_t)ix);
 }
 // shade_hit.wgsl envPdf (:202-210); the texel index is clamped into the map
 template <class T> T env_pdf(const Ctx<T>& c, V3<T> dir) {
@@ -833,10 +977,16 @@ template <class T> struct Sample {
   V3<T> wi;
   T pdf;
 };
+// sampling side of the surface interaction
 template <class T> struct SI {
   M3<T> basis;
-  V3<T> wo, origin, normal, base, spec, emission;
+  V3<T> wo, origin, normal, emission;
   T metallic, opacity, alpha;
+};
+// evaluation side (base color, specular color, GGX alpha)
+template <class U> struct EV {
+  V3<U> base, spec;
+  U alpha;
 };
 // branchlessONB (:166-177), Duff et al. 2017
 template <class T> M3<T> onb(V3<T> n) {
@@ -847,34 +997,35 @@ template <class T> M3<T> onb(V3<T> n) {
   V3<T> b2 = v3(b, sign + n.y * n.y * a, -n.y);
   return M3<T>{b1, b2, n};
 }
-template <class T> T ggx_d(V3<T> m, T au, T av) {
-  T auv = au * av;
-  T tx = m.x / au, ty = m.y / av;
-  T e = tx * tx + ty * ty + m.z * m.z;
-  return T(1) / (M<T>::pi() * auv * e * e);
+template <class T, class U> U ggx_d(V3<T> m, U au, U av) {
+  U auv = au * av;
+  U tx = m.x / au, ty = m.y / av;
+  U e = tx * tx + ty * ty + m.z * m.z;
+  return U(1) / (M<T>::pi() * auv * e * e);
 }
-template <class T> T ggx_g1(V3<T> w, V3<T> m, T au, T av) {
-  T ax = w.x * au, ay = w.y * av;
-  T axy2 = ax * ax + ay * ay;
-  T tan_theta = axy2 / (w.z * w.z);
-  T r = T(2) / (T(1) + M<T>::sqrt(T(1) + tan_theta));
-  return dot(w, m) * dot(w, v3(T(0), T(0), T(1))) <= T(0) ? T(0) : r;
+template <class T, class U> U ggx_g1(V3<T> w, V3<T> m, U au, U av) {
+  U ax = w.x * au, ay = w.y * av;
+  U axy2 = ax * ax + ay * ay;
+  U tan_theta = axy2 / (w.z * w.z);
+  U r = U(2) / (U(1) + usqrt(U(1) + tan_theta));
+  return dot(w, m) * dot(w, v3(T(0), T(0), T(1))) <= T(0) ? U(0) : r;
 }
 template <class T> T lambert_pdf(V3<T> wi) { return mx(wi.z, T(0.0000001f)) * M<T>::inv_pi(); }
 template <class T> T specular_pdf(const SI<T>& si, V3<T> m) {
-  return mx(T(0.0000001f), ggx_d(m, si.alpha, si.alpha) * ggx_g1(si.wo, m, si.alpha, si.alpha) / (T(4) * si.wo.z));
+  return mx(T(0.0000001f), ggx_d<T, T>(m, si.alpha, si.alpha) * ggx_g1<T, T>(si.wo, m, si.alpha, si.alpha) /
+                               (T(4) * si.wo.z));
 }
-template <class T> V3<T> eval_lambert(const SI<T>& si, const Sample<T>& s) {
+template <class T, class U> V3<U> eval_lambert(const EV<U>& e, const Sample<T>& s) {
   T k = mx(T(0.0000001f), s.wi.z);
-  return v3(si.base.x * M<T>::inv_pi() * k / s.pdf, si.base.y * M<T>::inv_pi() * k / s.pdf,
-            si.base.z * M<T>::inv_pi() * k / s.pdf);
+  return v3(e.base.x * M<T>::inv_pi() * k / s.pdf, e.base.y * M<T>::inv_pi() * k / s.pdf,
+            e.base.z * M<T>::inv_pi() * k / s.pdf);
 }
-template <class T> V3<T> eval_specular(const SI<T>& si, const Sample<T>& s) {
+template <class T, class U> V3<U> eval_specular(const SI<T>& si, const EV<U>& e, const Sample<T>& s) {
   V3<T> h = normalize(si.wo + s.wi);
-  T d = ggx_d(h, si.alpha, si.alpha);
-  T g = ggx_g1(s.wi, h, si.alpha, si.alpha) * ggx_g1(si.wo, h, si.alpha, si.alpha);
-  T k = mx(d * g / (T(4) * si.wo.z * s.pdf), T(0));
-  return si.spec * k;
+  U d = ggx_d<T, U>(h, e.alpha, e.alpha);
+  U g = ggx_g1<T, U>(s.wi, h, e.alpha, e.alpha) * ggx_g1<T, U>(si.wo, h, e.alpha, e.alpha);
+  U k = mx(d * g / (T(4) * si.wo.z * s.pdf), U(0));
+  return e.spec * k;
 }
 template <class T> T power_heuristic(T a, T b) {
   T a2 = a * a;
@@ -951,7 +1102,7 @@ template <> double attr_uv<double>(const or_scene* s, uint32_t vi, int comp) {
 }
 
 // getSurfaceInteraction (:179-200) with interpolateVertexAttribute (:363-394)
-template <class T> SI<T> surface(const Ctx<T>& c, V3<T> ro, V3<T> rd, const HitT<T>& h) {
+template <class T, class U> SI<T> surface(const Ctx<T>& c, V3<T> ro, V3<T> rd, const HitT<T>& h, EV<U>* ev) {
   const or_scene* s = c.s;
   const int32_t* tr = &s->tri[(size_t)h.id * 4];
   T bu = (T)(float)(h.bary & 0xffff) / T(65535), bv = (T)(float)(h.bary >> 16) / T(65535);
@@ -974,14 +1125,19 @@ template <class T> SI<T> surface(const Ctx<T>& c, V3<T> ro, V3<T> rd, const HitT
   si.basis = onb(si.normal);
   si.wo = vec_mat(-rd, si.basis);
   sample_tex<T>(s, mi[0], uvx, uvy, dc);
-  si.base = v3(dc[0], dc[1], dc[2]);
   si.opacity = dc[3];
   sample_tex<T>(s, mi[1], uvx, uvy, mr);
   si.metallic = mr[2];
   si.alpha = mr[1] * mr[1];
   sample_tex<T>(s, mi[3], uvx, uvy, em);
   si.emission = v3(em[0], em[1], em[2]);
-  si.spec = v3(mix(T(1), si.base.x, si.metallic), mix(T(1), si.base.y, si.metallic), mix(T(1), si.base.z, si.metallic));
+  U dce[4], mre[4];
+  sample_tex_eval<T, U>(c, mi[0], uvx, uvy, dce);
+  sample_tex_eval<T, U>(c, mi[1], uvx, uvy, mre);
+  ev->base = v3(dce[0], dce[1], dce[2]);
+  ev->alpha = mre[1] * mre[1];
+  auto mixu = [&](U b) { return T(1) * (T(1) - si.metallic) + b * si.metallic; };
+  ev->spec = v3(mixu(ev->base.x), mixu(ev->base.y), mixu(ev->base.z));
   return si;
 }
 
@@ -1015,35 +1171,35 @@ template <class T> Ray<T> primary_ray(const pk_camera& cam, uint32_t px, uint32_
   return Ray<T>{dof + org, d};
 }
 
-template <class T> struct PixelOut {
-  T c[4] = {0, 0, 0, 0};
+template <class U> struct PixelOut {
+  U c[4] = {U(0), U(0), U(0), U(0)};
   uint64_t rays = 0, shadow = 0, hits = 0;
 };
 
 template <class T>
 HitT<T> closest(const Ctx<T>& c, V3<T> o, V3<T> d) { return canonical_trace<T>(c.s, *c.nodes, o, d, false); }
 
-template <class T>
+template <class T, class U>
 void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
-                  uint32_t sample, uint32_t depth, PixelOut<T>& out) {
+                  uint32_t sample, uint32_t depth, PixelOut<U>& out) {
   const uint32_t pixel = py * W + px;
   Rng rng;
   rng.seed = (px * 1973u + py * 9277u + sample * 26699u) | 1u;  // camera.wgsl:86-87
   rng.seed = rng.hash();
   Ray<T> ray = primary_ray<T>(cam, px, py, W, H, rng);
-  V3<T> thr = v3(T(1), T(1), T(1));
-  auto add = [&](V3<T> L) {
+  V3<U> thr = v3(U(1), U(1), U(1));
+  auto add = [&](V3<U> L) {
     out.c[0] += L.x;
     out.c[1] += L.y;
     out.c[2] += L.z;
-    out.c[3] += T(1);
+    out.c[3] += U(1);
   };
   const T eps = T(0.0000001f);
   for (uint32_t b = 0; b < depth; ++b) {
     out.rays++;
     HitT<T> h = closest<T>(c, ray.o, ray.d);
     if (h.id < 0) {  // shade_miss.wgsl:53-57
-      add(env_color<T>(c, ray.d) * thr);
+      add(env_color<T, U>(c, ray.d) * thr);
       return;
     }
     out.hits++;
@@ -1052,7 +1208,8 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
     Rng sr;
     sr.seed = (b * 1973u + pixel * 9277u + sample * 26699u) | 1u;
     sr.seed = sr.hash();
-    SI<T> si = surface<T>(c, ray.o, ray.d, h);
+    EV<U> ev;
+    SI<T> si = surface<T, U>(c, ray.o, ray.d, h, &ev);
     if (sr.rand<T>() > si.opacity) {
       V3<T> o2 = ray.o + ray.d * (h.t + eps * T(40));
       ray = Ray<T>{o2, ray.d};
@@ -1066,18 +1223,18 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
     V3<T> m = sample_ggx<T>(si, sr);
     T f = mix(schlick(mx(dot(si.wo, m), T(0)), T(1.5)), T(1), si.metallic);
     Sample<T> bs;
-    V3<T> bsdf;
+    V3<U> bsdf;
     if (sr.rand<T>() > f) {
       bs = sample_lambert<T>(sr);
-      bsdf = eval_lambert(si, bs);
+      bsdf = eval_lambert<T, U>(ev, bs);
     } else {
       V3<T> wi = reflect(-si.wo, m);
       bs = Sample<T>{wi, specular_pdf(si, m)};
-      bsdf = eval_specular(si, bs);
+      bsdf = eval_specular<T, U>(si, ev, bs);
     }
     bool bounce = false;
     Ray<T> next;
-    V3<T> next_thr;
+    V3<U> next_thr;
     if (bs.wi.z > T(0)) {
       V3<T> dir = mat_vec(si.basis, bs.wi);
       T weight = c.sample_env ? power_heuristic(bs.pdf, env_pdf<T>(c, dir)) : T(1);
@@ -1090,17 +1247,17 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
       if (dot(es.wi, m) > T(0) && es.wi.z > T(0)) {
         // createMaterialSampleRay (:339-348)
         T lw = power_heuristic(es.pdf, lambert_pdf(es.wi));
-        V3<T> el = eval_lambert(si, es);
-        V3<T> scale = v3((T(1) - f) * el.x * lw, (T(1) - f) * el.y * lw, (T(1) - f) * el.z * lw);
+        V3<U> el = eval_lambert<T, U>(ev, es);
+        V3<U> scale = v3((T(1) - f) * el.x * lw, (T(1) - f) * el.y * lw, (T(1) - f) * el.z * lw);
         V3<T> hh = normalize(si.wo + es.wi);
         T sw = power_heuristic(es.pdf, specular_pdf(si, hh));
-        V3<T> ev = eval_specular(si, es);
-        scale = scale + v3(f * ev.x * sw, f * ev.y * sw, f * ev.z * sw);
+        V3<U> evs = eval_specular<T, U>(si, ev, es);
+        scale = scale + v3(f * evs.x * sw, f * evs.y * sw, f * evs.z * sw);
         V3<T> sdir = mat_vec(si.basis, es.wi);
-        V3<T> sthr = scale * thr;
+        V3<U> sthr = scale * thr;
         out.shadow++;
         HitT<T> oh = canonical_trace<T>(c.s, *c.nodes, si.origin, sdir, true);
-        if (oh.id < 0) add(env_color<T>(c, sdir) * sthr);
+        if (oh.id < 0) add(env_color<T, U>(c, sdir) * sthr);
       }
     }
     if (!bounce) return;
@@ -1117,6 +1274,7 @@ int render_t(const or_scene* s, const pk_camera* cam, const or_render_params* p,
   c.nodes = sizeof(T) == 4 ? &s->cnodes32 : &s->cnodes64;
   c.env_theta = (T)p->env_theta;
   c.sample_env = s->efmt != PK_ENV_NONE;
+  c.seeds = nullptr;
   const uint32_t W = p->width, H = p->height;
   uint32_t x0 = p->x0, x1 = p->x1, y0 = p->y0, y1 = p->y1;
   if (x1 == 0 && y1 == 0) {
@@ -1132,7 +1290,7 @@ int render_t(const or_scene* s, const pk_camera* cam, const or_render_params* p,
     parallel_for((int64_t)ww * wh, p->num_threads, [&](int64_t i) {
       uint32_t px = x0 + (uint32_t)(i % ww), py = y0 + (uint32_t)(i / ww);
       PixelOut<T> o;
-      render_pixel<T>(c, *cam, px, py, W, H, sample, p->max_depth, o);
+      render_pixel<T, T>(c, *cam, px, py, W, H, sample, p->max_depth, o);
       rays[i] += o.rays;
       shadow[i] += o.shadow;
       hits[i] += o.hits;
@@ -1160,6 +1318,59 @@ int render_t(const or_scene* s, const pk_camera* cam, const or_render_params* p,
     counters[0] = std::accumulate(rays.begin(), rays.end(), (uint64_t)0);
     counters[1] = std::accumulate(shadow.begin(), shadow.end(), (uint64_t)0);
     counters[2] = std::accumulate(hits.begin(), hits.end(), (uint64_t)0);
+  }
+  return 0;
+}
+
+// Reference gradient: the frame's estimator evaluated with dual numbers
+// (real type T for geometry, sampling and values), kDualN parameters per
+// pass over the image.  Derivatives are summed in f64.
+template <class T>
+int gradient_t(const or_scene* s, const pk_camera* cam, const or_render_params* p, const float* d_image,
+               const int32_t* which, const int64_t* index, int64_t n, double* out, double* value) {
+  using D = DualT<T>;
+  const uint32_t W = p->width, H = p->height;
+  const int64_t npx = (int64_t)W * H;
+  const double inv_n = 1.0 / (double)p->num_frames;
+  // one pass at least, for the objective value
+  for (int64_t base = 0; base < std::max<int64_t>(n, 1); base += kDualN) {
+    const int cnt = (int)std::min<int64_t>(kDualN, n - base);
+    Seeds seeds;
+    for (int k = 0; k < cnt; ++k) {
+      auto& v = which[base + k] == PK_PARAM_ENV ? seeds.env : seeds.atlas;
+      v.push_back({index[base + k], k});
+    }
+    std::sort(seeds.atlas.begin(), seeds.atlas.end());
+    std::sort(seeds.env.begin(), seeds.env.end());
+    Ctx<T> c;
+    c.s = s;
+    c.nodes = sizeof(T) == 4 ? &s->cnodes32 : &s->cnodes64;
+    c.env_theta = (T)p->env_theta;
+    c.sample_env = s->efmt != PK_ENV_NONE;
+    c.seeds = &seeds;
+    std::vector<double> part((size_t)npx * std::max(cnt, 1), 0.0), val((size_t)npx, 0.0);
+    parallel_for(npx, p->num_threads, [&](int64_t i) {
+      const uint32_t px = (uint32_t)(i % W), py = (uint32_t)(i / W);
+      const float* g = &d_image[(size_t)i * 4];
+      if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f) return;
+      for (uint32_t f = 0; f < p->num_frames; ++f) {
+        PixelOut<D> o;
+        render_pixel<T, D>(c, *cam, px, py, W, H, p->first_sample + f, p->max_depth, o);
+        for (int k = 0; k < cnt; ++k)
+          part[(size_t)i * cnt + k] += ((double)g[0] * (double)o.c[0].g[k] + (double)g[1] * (double)o.c[1].g[k] +
+                                        (double)g[2] * (double)o.c[2].g[k]) *
+                                       inv_n;
+        val[i] += ((double)g[0] * (double)o.c[0].v + (double)g[1] * (double)o.c[1].v +
+                   (double)g[2] * (double)o.c[2].v) *
+                  inv_n;
+      }
+    });
+    for (int k = 0; k < cnt; ++k) {
+      double acc = 0;
+      for (int64_t i = 0; i < npx; ++i) acc += part[(size_t)i * cnt + k];
+      out[base + k] = acc;
+    }
+    if (value && base == 0) *value = std::accumulate(val.begin(), val.end(), 0.0);
   }
   return 0;
 }
@@ -1344,6 +1555,41 @@ int or_render(const or_scene* s, const pk_camera* cam, const or_render_params* p
   if (!s->root) return -1;
   if (p->precision == 64) return render_t<double>(s, cam, p, accum, color_out, counters);
   return render_t<float>(s, cam, p, accum, color_out, counters);
+}
+
+int or_scene_write_params(or_scene* s, int which, int64_t offset, const float* src, int64_t count,
+                          int also_sampling) {
+  if (which == PK_PARAM_ATLAS) {
+    if (s->tfmt != PK_TEX_RGBA32F) return -1;
+    const int64_t n = (int64_t)s->tex32.size();
+    if (offset < 0 || count < 0 || offset + count > n) return -1;
+    if (!also_sampling && s->tex32_eval.empty()) s->tex32_eval = s->tex32;
+    for (int64_t i = 0; i < count; ++i) {
+      if (!s->tex32_eval.empty()) s->tex32_eval[offset + i] = src[i];
+      if (also_sampling) s->tex32[offset + i] = src[i];
+    }
+    return 0;
+  }
+  if (which == PK_PARAM_ENV) {
+    if (s->efmt == PK_ENV_NONE) return -1;
+    const int64_t n = (int64_t)s->ew * s->eh * 4;
+    if (offset < 0 || count < 0 || offset + count > n) return -1;
+    for (int64_t i = 0; i < count; ++i) {
+      const int64_t o = offset + i, texel = o / 4, c = o % 4;
+      if (c == 3) continue;
+      s->erad32[(size_t)texel * 3 + c] = src[i];
+      s->erad64[(size_t)texel * 3 + c] = (double)src[i];
+    }
+    return 0;
+  }
+  return -1;
+}
+
+int or_gradient(const or_scene* s, const pk_camera* cam, const or_render_params* p, const float* d_image,
+                const int32_t* which, const int64_t* index, int64_t n, double* out, double* value) {
+  if (s->cnodes64.empty() || p->num_frames == 0) return -1;
+  if (p->precision == 32) return gradient_t<float>(s, cam, p, d_image, which, index, n, out, value);
+  return gradient_t<double>(s, cam, p, d_image, which, index, n, out, value);
 }
 
 void or_math_eval(int fn, const float* x, const float* y, float* out, int64_t n) {

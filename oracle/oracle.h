@@ -65,6 +65,23 @@ typedef struct or_render_params {
 int or_render(const or_scene* s, const pk_camera* cam, const or_render_params* p, float* accum_inout,
               float* color_out, uint64_t* counters3);
 
+/* Parameters, as pk_scene_write_params: PK_PARAM_ATLAS writes the evaluation
+ * atlas (RGBA32F scenes; with also_sampling the sampling atlas too),
+ * PK_PARAM_ENV the environment radiance ([height][width] RGBA, A ignored).
+ * Returns 0 or -1 on a bad range. */
+int or_scene_write_params(or_scene* s, int which, int64_t offset, const float* src, int64_t count,
+                          int also_sampling);
+/* Reference gradient of sum_p sum_c d_image[p][c] * I[p][c] (I = the mean
+ * image over the rendered frames, d_image [H][W] RGBA with A ignored) with
+ * respect to n parameters given as float offsets index[i] into the
+ * PK_PARAM_* layout which[i]: the same detached-sampling estimator evaluated
+ * with forward-mode dual numbers (no hand-derived derivative), in FP64
+ * (p->precision 64) or with the FP32 restatement's value arithmetic (32: the
+ * product's paths bit for bit).  If value is not NULL it receives the
+ * objective.  The pixel window is ignored. */
+int or_gradient(const or_scene* s, const pk_camera* cam, const or_render_params* p, const float* d_image,
+                const int32_t* which, const int64_t* index, int64_t n, double* out, double* value);
+
 /* pk_math contract probes for tests */
 void or_math_eval(int fn, const float* x, const float* y, float* out, int64_t n);
 

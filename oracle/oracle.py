@@ -44,6 +44,10 @@ def load():
     lib.or_render.restype = i32
     lib.or_render.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.or_math_eval.argtypes = [i32, vp, vp, vp, i64]
+    lib.or_scene_write_params.restype = i32
+    lib.or_scene_write_params.argtypes = [vp, i32, i64, vp, i64, i32]
+    lib.or_gradient.restype = i32
+    lib.or_gradient.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp]
     _lib = lib
     return lib
 
@@ -136,6 +140,26 @@ class OracleScene:
                          x0, x1, y0, y1)
         assert self.lib.or_render(self.h, ctypes.byref(cam), ctypes.byref(p), _p(accum), _p(color), _p(cnt)) == 0
         return accum, color, cnt
+
+    def write_params(self, which, values, offset=0, also_sampling=False):
+        """Writes float parameters (PK_PARAM_ATLAS = 0 / PK_PARAM_ENV = 1 layouts)."""
+        v = np.ascontiguousarray(values, np.float32).ravel()
+        assert self.lib.or_scene_write_params(self.h, which, offset, _p(v), v.size, int(also_sampling)) == 0
+
+    def gradient(self, cam, width, height, max_depth, d_image, params, first_sample=0, num_frames=1,
+                 env_theta=0.0, precision=64):
+        """FP64 gradient of J = sum(d_image * image) w.r.t. params = [(which, float offset), ...];
+        returns (gradient, J)."""
+        d = np.ascontiguousarray(d_image, np.float32).reshape(height, width, 4)
+        which = np.ascontiguousarray([w for w, _ in params] or [0], np.int32)
+        index = np.ascontiguousarray([i for _, i in params] or [0], np.int64)
+        out = np.zeros(max(len(params), 1), np.float64)
+        value = np.zeros(1, np.float64)
+        p = RenderParams(width, height, max_depth, first_sample, num_frames, env_theta, precision, self.threads,
+                         0, 0, 0, 0)
+        assert self.lib.or_gradient(self.h, ctypes.byref(cam), ctypes.byref(p), _p(d), _p(which), _p(index),
+                                    len(params), _p(out), _p(value)) == 0
+        return out[:len(params)], float(value[0])
 
     def close(self):
         if self.h:

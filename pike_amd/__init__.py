@@ -12,6 +12,7 @@ Frame (RenderState + CameraPass state) whose methods are the passes'
 generateCommands.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -41,6 +42,13 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
 
+def _dev(x):
+    """Device address of a DeviceBuffer, an integer pointer, or None."""
+    if x is None:
+        return None
+    return ctypes.c_void_p(x.ptr if isinstance(x, DeviceBuffer) else int(x))
+
+
 class Context:
     """A HIP device context (one stream)."""
 
@@ -48,6 +56,10 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().pk_ctx_create(device, ctypes.byref(h)), "pk_ctx_create")
         self.h = h
+        self._children = weakref.WeakSet()  # scenes, frames, buffers: destroyed before the context
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def sync(self):
         _check(lib().pk_sync(self.h), "pk_sync")
@@ -75,6 +87,11 @@ class Context:
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                try:
+                    child.close()
+                except AttributeError:  # construction failed before the handle existed
+                    pass
             lib().pk_ctx_destroy(self.h)
             self.h = None
 
@@ -117,6 +134,7 @@ class DeviceBuffer:
 
     def __init__(self, ctx, nbytes):
         self.ctx = ctx
+        ctx._adopt(self)
         self.nbytes = int(nbytes)
         p = ctypes.c_void_p()
         _check(lib().pk_buffer_create(ctx.h, self.nbytes, ctypes.byref(p)), "pk_buffer_create")
@@ -220,6 +238,7 @@ class Scene:
 
     def __init__(self, ctx, data, bvh):
         self.ctx = ctx
+        ctx._adopt(self)
         self.desc = data.desc()
         self._data = data
         h = ctypes.c_void_p()
@@ -230,6 +249,38 @@ class Scene:
         a = np.zeros(4, np.int64)
         _check(lib().pk_scene_env_info(self.h, _ptr(a)), "pk_scene_env_info")
         return dict(bins=int(a[0]), coords=int(a[1]), width=int(a[2]), height=int(a[3]))
+
+    # differentiable parameters (pk_scene_write_params): the atlas as
+    # [layers][height][width][4] f32 and the environment as [height][width][4]
+    def param_info(self):
+        a = np.zeros(6, np.uint32)
+        _check(lib().pk_scene_param_info(self.h, _ptr(a)), "pk_scene_param_info")
+        return dict(tex_width=int(a[0]), tex_height=int(a[1]), tex_layers=int(a[2]), tex_format=int(a[3]),
+                    env_width=int(a[4]), env_height=int(a[5]))
+
+    def param_count(self, which):
+        i = self.param_info()
+        if which == _ffi.PK_PARAM_ATLAS:
+            return i["tex_width"] * i["tex_height"] * i["tex_layers"] * 4
+        return i["env_width"] * i["env_height"] * 4
+
+    def write_params(self, which, values=None, offset=0, also_sampling=False, device_ptr=None, count=None):
+        """Writes floats of a parameter set from host `values` or from a device
+        pointer (`device_ptr`, `count` floats; the caller has synchronized the
+        producer of that memory)."""
+        if device_ptr is not None:
+            src, n, dev = ctypes.c_void_p(int(device_ptr)), int(count), 1
+        else:
+            v = np.ascontiguousarray(values, np.float32).ravel()
+            src, n, dev, keep = _ptr(v), v.size, 0, v  # noqa: F841 (keep v alive through the call)
+        _check(lib().pk_scene_write_params(self.h, which, offset, src, n, dev, int(also_sampling)),
+               "pk_scene_write_params")
+
+    def read_params(self, which, count=None, offset=0):
+        n = self.param_count(which) - offset if count is None else count
+        out = np.empty(n, np.float32)
+        _check(lib().pk_scene_read_params(self.h, which, offset, _ptr(out), n), "pk_scene_read_params")
+        return out
 
     def close(self):
         if self.h:
@@ -248,6 +299,7 @@ class BatchScene(Scene):
 
     def __init__(self, ctx, datas, bvhs):  # noqa: super().__init__ is not used: one upload for all
         self.ctx = ctx
+        ctx._adopt(self)
         self._datas = list(datas)
         descs = (_ffi.SceneDesc * len(datas))()
         for i, d in enumerate(datas):
@@ -267,6 +319,7 @@ class Batch:
     def __init__(self, ctx, scene, views):
         """views: list of (scene index, width, height, spp, Camera)."""
         self.ctx = ctx
+        ctx._adopt(self)
         self.scene = scene
         arr = (_ffi.View * len(views))()
         for i, (si, w, hh, spp, cam) in enumerate(views):
@@ -310,6 +363,7 @@ class Frame:
 
     def __init__(self, ctx, width, height):
         self.ctx = ctx
+        ctx._adopt(self)
         self.width, self.height = width, height
         h = ctypes.c_void_p()
         _check(lib().pk_frame_create(ctx.h, width, height, ctypes.byref(h)), "pk_frame_create")
@@ -368,6 +422,41 @@ class Frame:
         out = np.empty((self.height, self.width, 4), np.float32)
         _check(lib().pk_read_image(self.h, _ptr(out), out.nbytes), "pk_read_image")
         return out
+
+    def read_image_device(self, dst):
+        """Copies the running-mean image to device memory (asynchronous)."""
+        _check(lib().pk_read_image_device(self.h, _dev(dst), self.width * self.height * 16), "pk_read_image_device")
+
+    # differentiation
+    def loss_l2(self, target, grad, want_loss=True):
+        """L2 loss of the running-mean image against a device target; writes
+        2 (image - target) to the device buffer `grad`.  target/grad are
+        DeviceBuffers or device pointers.  Returns the loss (blocking) or None."""
+        loss = ctypes.c_double()
+        _check(lib().pk_loss_l2(self.h, _dev(target), _dev(grad), ctypes.byref(loss) if want_loss else None),
+               "pk_loss_l2")
+        return loss.value if want_loss else None
+
+    def adjoint(self, scene, d_image, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0):
+        """Path-replay adjoint with host arrays: d_image (H, W, 4) f32; returns
+        (atlas gradient [layers][h][w][4], environment gradient [h][w][4])."""
+        info = scene.param_info()
+        d = np.ascontiguousarray(d_image, np.float32)
+        assert d.shape == (self.height, self.width, 4)
+        ga = np.empty((info["tex_layers"], info["tex_height"], info["tex_width"], 4), np.float32)
+        ge = np.empty((info["env_height"], info["env_width"], 4), np.float32)
+        g = _ffi.GradBuffers(ga.ctypes.data, ge.ctypes.data if ge.size else None, 0, 0)
+        p = RenderParams(self.width, self.height, max_depth, first_sample, num_frames, env_theta, 0)
+        _check(lib().pk_adjoint(self.h, scene.h, ctypes.byref(p), _ptr(d), 0, ctypes.byref(g)), "pk_adjoint")
+        return ga, ge
+
+    def adjoint_device(self, scene, d_image, atlas_grad, env_grad, max_depth=4, first_sample=0, num_frames=1,
+                       env_theta=0.0, accumulate=False):
+        """Asynchronous adjoint on device memory (DeviceBuffers or pointers)."""
+        g = _ffi.GradBuffers(_dev(atlas_grad), _dev(env_grad), 1, int(accumulate))
+        p = RenderParams(self.width, self.height, max_depth, first_sample, num_frames, env_theta, 0)
+        _check(lib().pk_adjoint(self.h, scene.h, ctypes.byref(p), _dev(d_image), 1, ctypes.byref(g)),
+               "pk_adjoint")
 
     def read_color(self):
         out = np.empty((self.height, self.width, 4), np.float32)

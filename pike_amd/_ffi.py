@@ -26,6 +26,8 @@ PK_ENV_RGBE8 = 1
 PK_ENV_RGB32F = 2
 PK_ENV_SAMPLER_BINS = 0
 PK_ENV_SAMPLER_CDF = 1
+PK_PARAM_ATLAS = 0
+PK_PARAM_ENV = 1
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -106,6 +108,15 @@ class Counters(ctypes.Structure):
 
 
 # name -> (restype, argtypes); every entry point declared in include/pk.h
+class GradBuffers(ctypes.Structure):
+    _fields_ = [
+        ("atlas", c_void_p),
+        ("env", c_void_p),
+        ("on_device", c_int),
+        ("accumulate", c_int),
+    ]
+
+
 SIGNATURES = {
     "pk_last_error": (ctypes.c_char_p, []),
     "pk_version": (ctypes.c_char_p, []),
@@ -161,6 +172,13 @@ SIGNATURES = {
     "pk_ctx_trace_stats": (c_int, [c_void_p, c_void_p]),
     "pk_time_trace_rays": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p,
                                    c_int, ctypes.POINTER(c_float)]),
+    "pk_read_image_device": (c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
+    "pk_scene_param_info": (c_int, [c_void_p, c_void_p]),
+    "pk_scene_write_params": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, ctypes.c_size_t, c_int, c_int]),
+    "pk_scene_read_params": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, ctypes.c_size_t]),
+    "pk_loss_l2": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "pk_adjoint": (c_int, [c_void_p, c_void_p, ctypes.POINTER(RenderParams), c_void_p, c_int,
+                           ctypes.POINTER(GradBuffers)]),
 }
 
 _lib = None

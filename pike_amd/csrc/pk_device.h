@@ -22,6 +22,13 @@ struct DevScene {
   const int4* materials;     // {diffMap, metRoughMap, normMap, emitMap}
   const void* tex;
   uint32_t tw, th, tl, tfmt;
+  // Evaluation copy of the atlas for differentiation with detached sampling:
+  // BSDF values (base color; GGX D and G roughness) read it, while sampling
+  // decisions and pdfs (opacity, metalness, sampling roughness) read `tex`.
+  // It aliases `tex` unless parameters were written.  Environment radiance
+  // is only ever evaluated (sampling reads the frozen importance tables), so
+  // it needs no second copy.
+  const void* tex_eval;
   const float4* env_rad;     // decoded radiance (rgb, 0)
   const int2* env_bins;
   const int2* env_coords;
@@ -294,14 +301,18 @@ __device__ inline int wrapi(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
-// shade_miss.wgsl envColor (:37-43): nearest sampler, repeat addressing
-__device__ inline v3 env_color(const DevScene& S, float theta, v3 dir) {
-  if (S.env_mode == 0) return pkm::mk(0.0f, 0.0f, 0.0f);
+// shade_miss.wgsl envColor (:37-43): nearest sampler, repeat addressing.
+// Returns the texel index through *texel (for gradients).
+__device__ inline uint32_t env_texel(const DevScene& S, float theta, v3 dir) {
   float u = theta + pkm::atan2f_(dir.z, dir.x) / pkm::kTau;
   float v = pkm::acosf_(dir.y) * pkm::kInvPi;
   int ix = wrapi((int)floorf(u * (float)S.ew), (int)S.ew);
   int iy = wrapi((int)floorf(v * (float)S.eh), (int)S.eh);
-  float4 L = S.env_rad[(size_t)iy * S.ew + ix];
+  return (uint32_t)iy * S.ew + (uint32_t)ix;
+}
+__device__ inline v3 env_color(const DevScene& S, float theta, v3 dir) {
+  if (S.env_mode == 0) return pkm::mk(0.0f, 0.0f, 0.0f);
+  float4 L = S.env_rad[env_texel(S, theta, dir)];
   return pkm::mk(L.x, L.y, L.z);
 }
 
@@ -317,13 +328,39 @@ __device__ inline float env_pdf(const DevScene& S, float theta, v3 dir) {
   return S.env_pdf[(size_t)iy * S.ew + ix] / sin_phi;
 }
 
-__device__ inline float texel(const DevScene& S, int layer, int x, int y, int c) {
+__device__ inline float texel(const DevScene& S, const void* tex, int layer, int x, int y, int c) {
   size_t i = (((size_t)layer * S.th + (size_t)y) * S.tw + (size_t)x) * 4 + c;
-  if (S.tfmt == 0) return (float)((const uint8_t*)S.tex)[i] / 255.0f;
-  return ((const float*)S.tex)[i];
+  if (S.tfmt == 0) return (float)((const uint8_t*)tex)[i] / 255.0f;
+  return ((const float*)tex)[i];
 }
-// textureSampleLevel, linear filter, repeat, level 0
-__device__ inline float4 sample_tex(const DevScene& S, int layer, float u, float v) {
+
+// Bilinear footprint of one texture lookup (the same index arithmetic as
+// sample_tex): texel indices within a layer, corners (x0,y0) (x1,y0)
+// (x0,y1) (x1,y1), and the interpolation weights of those corners.
+struct Footprint {
+  uint32_t idx[4];
+  float w[4];
+};
+__device__ inline Footprint tex_footprint(const DevScene& S, float u, float v) {
+  const float x = u * (float)S.tw - 0.5f, y = v * (float)S.th - 0.5f;
+  const float fx0 = floorf(x), fy0 = floorf(y);
+  const float fx = x - fx0, fy = y - fy0;
+  const int x0 = wrapi((int)fx0, (int)S.tw), x1 = wrapi((int)fx0 + 1, (int)S.tw);
+  const int y0 = wrapi((int)fy0, (int)S.th), y1 = wrapi((int)fy0 + 1, (int)S.th);
+  Footprint fp;
+  fp.idx[0] = (uint32_t)(y0 * (int)S.tw + x0);
+  fp.idx[1] = (uint32_t)(y0 * (int)S.tw + x1);
+  fp.idx[2] = (uint32_t)(y1 * (int)S.tw + x0);
+  fp.idx[3] = (uint32_t)(y1 * (int)S.tw + x1);
+  fp.w[0] = (1.0f - fx) * (1.0f - fy);
+  fp.w[1] = fx * (1.0f - fy);
+  fp.w[2] = (1.0f - fx) * fy;
+  fp.w[3] = fx * fy;
+  return fp;
+}
+
+// textureSampleLevel, linear filter, repeat, level 0, from atlas `tex`
+__device__ inline float4 sample_tex(const DevScene& S, const void* tex, int layer, float u, float v) {
   float x = u * (float)S.tw - 0.5f, y = v * (float)S.th - 0.5f;
   float fx0 = floorf(x), fy0 = floorf(y);
   float fx = x - fx0, fy = y - fy0;
@@ -332,8 +369,8 @@ __device__ inline float4 sample_tex(const DevScene& S, int layer, float u, float
   float r[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    float a = texel(S, layer, x0, y0, c), b = texel(S, layer, x1, y0, c);
-    float cc = texel(S, layer, x0, y1, c), dd = texel(S, layer, x1, y1, c);
+    float a = texel(S, tex, layer, x0, y0, c), b = texel(S, tex, layer, x1, y0, c);
+    float cc = texel(S, tex, layer, x0, y1, c), dd = texel(S, tex, layer, x1, y1, c);
     float top = a + (b - a) * fx;
     float bot = cc + (dd - cc) * fx;
     r[c] = top + (bot - top) * fy;

@@ -96,6 +96,20 @@ void launch_accumulate(const FrameDev& F, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s);
+// Path-replay adjoint of a frame's image (k_adjoint).
+struct AdjointArgs {
+  const float4* d_image;  // dLoss/dImage per pixel (rgb)
+  float* g_atlas;         // gradient in the atlas layout [layer][y][x][4]
+  float* g_env;           // gradient in the environment layout [y][x][4]
+  uint32_t W, H, first_sample, num_frames, depth;
+  float inv_frames, theta;
+  pk_camera cam;
+  uint32_t* cursor;       // zeroed before the launch
+  uint2* spill;           // grid * kSpillPerLane * 64 entries
+  uint32_t* overflow;
+};
+void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, hipStream_t s);
+void launch_loss_l2(const float4* img, const float4* target, float4* grad, double* loss, uint32_t n, hipStream_t s);
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats, uint32_t* overflow,
                        const TraceLaunch& L, hipStream_t s);

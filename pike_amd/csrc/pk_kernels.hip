@@ -252,10 +252,16 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
 // ---------------------------------------------------------------------------
 // shade_hit.wgsl main (:407-483)
 // ---------------------------------------------------------------------------
+// Surface interaction.  Sampling decisions and pdfs read metallic, opacity
+// and alpha from the sampling atlas; BSDF values read base_e, spec_e and
+// alpha_e from the evaluation atlas (the same texels unless parameters were
+// written), so derivatives flow only through evaluation (detached sampling).
 struct SI {
   pkm::m3 basis;
-  v3 wo, origin, normal, base, spec, emission;
+  v3 wo, origin, normal, emission;
+  v3 base_e, spec_e;
   float metallic, opacity, alpha;
+  float alpha_e, rough_e;  // alpha_e = rough_e^2
 };
 
 __device__ inline float ggx_d(v3 m, float au, float av) {
@@ -277,15 +283,44 @@ __device__ inline float specular_pdf(const SI& si, v3 m) {
 }
 __device__ inline v3 eval_lambert(const SI& si, v3 wi, float pdf) {
   float k = pkm::max_(kEpsShadow, wi.z);
-  return pkm::mk(si.base.x * pkm::kInvPi * k / pdf, si.base.y * pkm::kInvPi * k / pdf,
-                 si.base.z * pkm::kInvPi * k / pdf);
+  return pkm::mk(si.base_e.x * pkm::kInvPi * k / pdf, si.base_e.y * pkm::kInvPi * k / pdf,
+                 si.base_e.z * pkm::kInvPi * k / pdf);
 }
 __device__ inline v3 eval_specular(const SI& si, v3 wi, float pdf) {
   v3 h = pkm::normalize(pkm::add(si.wo, wi));
-  float d = ggx_d(h, si.alpha, si.alpha);
-  float g = ggx_g1(wi, h, si.alpha, si.alpha) * ggx_g1(si.wo, h, si.alpha, si.alpha);
+  float d = ggx_d(h, si.alpha_e, si.alpha_e);
+  float g = ggx_g1(wi, h, si.alpha_e, si.alpha_e) * ggx_g1(si.wo, h, si.alpha_e, si.alpha_e);
   float k = pkm::max_(d * g / (4.0f * si.wo.z * pdf), 0.0f);
-  return pkm::scale(si.spec, k);
+  return pkm::scale(si.spec_e, k);
+}
+
+// Derivatives with respect to (isotropic) alpha of the GGX terms:
+//   d ln D / d alpha = 4 s / (alpha^3 e) - 2 / alpha,
+//     s = m.x^2 + m.y^2, e = s / alpha^2 + m.z^2;
+//   dG1 / d alpha = -G1^2 alpha q / (2 sqrt(1 + alpha^2 q)),
+//     q = (w.x^2 + w.y^2) / w.z^2.
+__device__ inline float ggx_d_dalpha(v3 m, float a, float d) {
+  const float s = m.x * m.x + m.y * m.y;
+  const float e = s / (a * a) + m.z * m.z;
+  return d * (4.0f * s / (a * a * a * e) - 2.0f / a);
+}
+__device__ inline float ggx_g1_dalpha(v3 w, float a, float g1) {
+  if (g1 == 0.0f) return 0.0f;
+  const float q = (w.x * w.x + w.y * w.y) / (w.z * w.z);
+  return -0.5f * g1 * g1 * a * q / pkm::sqrt_(1.0f + a * a * q);
+}
+// eval_specular = spec_e * k; returns k and dk / d alpha_e
+__device__ inline void specular_k_grad(const SI& si, v3 wi, float pdf, float* k, float* dk) {
+  const v3 h = pkm::normalize(pkm::add(si.wo, wi));
+  const float a = si.alpha_e;
+  const float d = ggx_d(h, a, a);
+  const float gi = ggx_g1(wi, h, a, a), go = ggx_g1(si.wo, h, a, a);
+  const float den = 4.0f * si.wo.z * pdf;
+  const float kk = d * (gi * go) / den;
+  *k = pkm::max_(kk, 0.0f);
+  *dk = kk > 0.0f ? (ggx_d_dalpha(h, a, d) * (gi * go) +
+                     d * (ggx_g1_dalpha(wi, a, gi) * go + gi * ggx_g1_dalpha(si.wo, a, go))) / den
+                  : 0.0f;
 }
 __device__ inline float power_heuristic(float a, float b) {
   float a2 = a * a;
@@ -380,152 +415,227 @@ __device__ inline void sample_env(const DevScene& S, float theta, const pkm::m3&
   *wi = pkm::vec_mul(dir, basis);
 }
 
+// Result of shading one closest hit (shade_hit.wgsl main :411-483).
+struct ShadeOut {
+  bool emit;     // emissive surface: L is the contribution and the path ends
+  bool scatter;  // the BSDF branch ran (the vertex depends on the parameters)
+  bool bounce;   // continuation ray (also the alpha pass-through)
+  bool shadow;   // NEE shadow ray
+  v3 L;
+  v3 b_o, b_d, b_thr;
+  v3 s_o, s_d, s_thr;
+};
+
+// What the adjoint needs about a scattering vertex: where the evaluation
+// parameters were read and the derivatives of the two weights it applied,
+// `bsdf` (the continuation's BSDF value, before the MIS weight) and the NEE
+// weight (before the path throughput).  Base-color derivatives are per
+// channel (d w_c / d base_c), roughness derivatives are taken with respect
+// to alpha_e.
+struct ShadeGrad {
+  float u, v;
+  int layer_base, layer_rough;
+  float rough;
+  v3 bsdf;
+  float db_base;
+  v3 db_alpha;
+  float dn_base;
+  v3 dn_alpha;
+};
+
+// deterministic shading seed: the bounce index in place of the reference's
+// compaction slot, the pixel index local to its view
+__device__ inline uint32_t shade_seed(uint32_t bounce, uint32_t pixel, uint32_t sample) {
+  uint32_t seed = (bounce * 1973u + pixel * 9277u + sample * 26699u) | 1u;
+  return pkm::pcg_next(&seed);
+}
+
+template <bool kGrad>
+__device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, float t, uint32_t leaf,
+                                 uint32_t bary_mask, uint32_t seed, v3 thr, ShadeOut& o, ShadeGrad* g) {
+  o.emit = o.scatter = o.bounce = o.shadow = false;
+  // getSurfaceInteraction (:179-200), interpolateVertexAttribute (:363-394)
+  const int4 tri = S.tri_index[leaf];
+  const float bu = (float)(bary_mask & 0xffffu) / 65535.0f, bv = (float)(bary_mask >> 16) / 65535.0f;
+  const v3 bary = pkm::mk(1.0f - bu - bv, bu, bv);
+  v3 tng, btg, nrm;
+  {
+    pkm::m3 m;
+    m.c0 = attr_vec(S, tri.x, 0);
+    m.c1 = attr_vec(S, tri.y, 0);
+    m.c2 = attr_vec(S, tri.z, 0);
+    tng = pkm::normalize(pkm::mat_mul(m, bary));
+    m.c0 = attr_vec(S, tri.x, 1);
+    m.c1 = attr_vec(S, tri.y, 1);
+    m.c2 = attr_vec(S, tri.z, 1);
+    btg = pkm::normalize(pkm::mat_mul(m, bary));
+    m.c0 = attr_vec(S, tri.x, 2);
+    m.c1 = attr_vec(S, tri.y, 2);
+    m.c2 = attr_vec(S, tri.z, 2);
+    nrm = pkm::normalize(pkm::mat_mul(m, bary));
+  }
+  const uint32_t w0 = S.attrs[tri.x].w, w1 = S.attrs[tri.y].w, w2 = S.attrs[tri.z].w;
+  const float uvx = pkm::unpack_snorm16(w0 & 0xffffu) * bary.x + pkm::unpack_snorm16(w1 & 0xffffu) * bary.y +
+                    pkm::unpack_snorm16(w2 & 0xffffu) * bary.z;
+  const float uvy = pkm::unpack_snorm16(w0 >> 16) * bary.x + pkm::unpack_snorm16(w1 >> 16) * bary.y +
+                    pkm::unpack_snorm16(w2 >> 16) * bary.z;
+  const int4 mi = S.materials[tri.w];
+  const float4 nm = sample_tex(S, S.tex, mi.z, uvx, uvy);
+  const v3 map_normal = pkm::mk((nm.x - 0.5f) * 2.0f, (nm.y - 0.5f) * 2.0f, (nm.z - 0.0f) * 1.0f);
+  SI si;
+  {
+    pkm::m3 tbn;
+    tbn.c0 = tng;
+    tbn.c1 = btg;
+    tbn.c2 = nrm;
+    si.normal = pkm::normalize(pkm::mat_mul(tbn, map_normal));
+  }
+  const float eps40 = kEpsShadow * 40.0f;
+  si.origin = pkm::add(ro, pkm::scale(rd, t - eps40));
+  si.basis = onb(si.normal);
+  si.wo = pkm::vec_mul(pkm::neg(rd), si.basis);
+  const float4 dc = sample_tex(S, S.tex, mi.x, uvx, uvy);
+  si.opacity = dc.w;
+  const float4 mr = sample_tex(S, S.tex, mi.y, uvx, uvy);
+  si.metallic = mr.z;
+  si.alpha = mr.y * mr.y;
+  // evaluation parameters: the same texels unless an evaluation copy is set
+  const bool split = S.tex_eval != S.tex;
+  const float4 dce = split ? sample_tex(S, S.tex_eval, mi.x, uvx, uvy) : dc;
+  const float4 mre = split ? sample_tex(S, S.tex_eval, mi.y, uvx, uvy) : mr;
+  si.base_e = pkm::mk(dce.x, dce.y, dce.z);
+  si.rough_e = mre.y;
+  si.alpha_e = mre.y * mre.y;
+  const float4 em = sample_tex(S, S.tex, mi.w, uvx, uvy);
+  si.emission = pkm::mk(em.x, em.y, em.z);
+  si.spec_e = pkm::mix3(pkm::mk(1.0f, 1.0f, 1.0f), si.base_e, si.metallic);
+  if (pkm::pcg_rand(&seed) > si.opacity) {
+    // alpha pass-through (:426-431)
+    o.b_o = pkm::add(ro, pkm::scale(rd, t + kEpsShadow * 40.0f));
+    o.b_d = rd;
+    o.b_thr = thr;
+    o.bounce = true;
+    return;
+  }
+  if (si.emission.x * 1.0f + si.emission.y * 1.0f + si.emission.z * 1.0f > 0.0f) {
+    // emission (:434-438)
+    const float att = pkm::max_(pkm::dot(si.normal, pkm::mat_mul(si.basis, si.wo)), 0.0f);
+    o.L = pkm::scale(pkm::mul(pkm::mul(pkm::mk(50.0f, 50.0f, 50.0f), si.emission), thr), att);
+    o.emit = true;
+    return;
+  }
+  o.scatter = true;
+  if (kGrad) {
+    g->u = uvx;
+    g->v = uvy;
+    g->layer_base = mi.x;
+    g->layer_rough = mi.y;
+    g->rough = si.rough_e;
+  }
+  const v3 m = sample_ggx(si, &seed);
+  const float f = pkm::mix(schlick(pkm::max_(pkm::dot(si.wo, m), 0.0f), 1.5f), 1.0f, si.metallic);
+  v3 wi, bsdf;
+  float pdf;
+  if (pkm::pcg_rand(&seed) > f) {
+    // sampleLambert (:243-253)
+    const float r = pkm::sqrt_(pkm::pcg_rand(&seed));
+    const float phi = pkm::kTau * pkm::pcg_rand(&seed);
+    float sp, cp;
+    pkm::sincosf_(phi, &sp, &cp);
+    const float x = r * cp, y = r * sp;
+    const float z = pkm::sqrt_(pkm::max_(0.0f, 1.0f - x * x - y * y));
+    wi = pkm::mk(x, y, z);
+    pdf = lambert_pdf(wi);
+    bsdf = eval_lambert(si, wi, pdf);
+    if (kGrad) {
+      g->db_base = pkm::kInvPi * pkm::max_(kEpsShadow, wi.z) / pdf;
+      g->db_alpha = pkm::mk(0.0f, 0.0f, 0.0f);
+    }
+  } else {
+    wi = pkm::reflect(pkm::neg(si.wo), m);
+    pdf = specular_pdf(si, m);
+    bsdf = eval_specular(si, wi, pdf);
+    if (kGrad) {
+      float k, dk;
+      specular_k_grad(si, wi, pdf, &k, &dk);
+      g->db_base = si.metallic * k;
+      g->db_alpha = pkm::scale(si.spec_e, dk);
+    }
+  }
+  if (kGrad) {
+    g->bsdf = bsdf;
+    g->dn_base = 0.0f;
+    g->dn_alpha = pkm::mk(0.0f, 0.0f, 0.0f);
+  }
+  if (wi.z > 0.0f) {
+    const v3 dir = pkm::mat_mul(si.basis, wi);
+    const float weight = S.env_mode ? power_heuristic(pdf, env_pdf(S, theta, dir)) : 1.0f;
+    o.b_o = si.origin;
+    o.b_d = dir;
+    o.b_thr = pkm::scale(pkm::mul(bsdf, thr), weight);
+    o.bounce = true;
+  }
+  if (S.env_mode) {
+    v3 ewi;
+    float epdf;
+    sample_env(S, theta, si.basis, &seed, &ewi, &epdf);
+    if (pkm::dot(ewi, m) > 0.0f && ewi.z > 0.0f) {
+      // createMaterialSampleRay (:339-348)
+      const float lw = power_heuristic(epdf, lambert_pdf(ewi));
+      const v3 el = eval_lambert(si, ewi, epdf);
+      v3 sc = pkm::mk((1.0f - f) * el.x * lw, (1.0f - f) * el.y * lw, (1.0f - f) * el.z * lw);
+      const v3 hh = pkm::normalize(pkm::add(si.wo, ewi));
+      const float sw = power_heuristic(epdf, specular_pdf(si, hh));
+      const v3 ev = eval_specular(si, ewi, epdf);
+      sc = pkm::add(sc, pkm::mk(f * ev.x * sw, f * ev.y * sw, f * ev.z * sw));
+      o.s_o = si.origin;
+      o.s_d = pkm::mat_mul(si.basis, ewi);
+      o.s_thr = pkm::mul(sc, thr);
+      o.shadow = true;
+      if (kGrad) {
+        float ks, dks;
+        specular_k_grad(si, ewi, epdf, &ks, &dks);
+        g->dn_base = (1.0f - f) * lw * (pkm::kInvPi * pkm::max_(kEpsShadow, ewi.z) / epdf) + f * sw * si.metallic * ks;
+        g->dn_alpha = pkm::scale(si.spec_e, f * sw * dks);
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounce) {
   const int in = bounce & 1;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
   const uint32_t nh = F.cnt[kCntHits];
   const FrameParams P = *F.params;
-  const bool active = idx < nh;
-  bool emit_bounce = false, emit_shadow = false;
-  v3 b_o, b_d, b_thr, s_o, s_d, s_thr;
+  ShadeOut o;
+  o.emit = o.scatter = o.bounce = o.shadow = false;
   uint32_t pid = 0, view_bits = 0;
-  if (active) {
+  if (idx < nh) {
     const uint4 hr = F.hits[idx];
-    const uint32_t slot = hr.x, leaf = hr.y, bary_mask = hr.w;
-    const float t = __uint_as_float(hr.z);
+    const uint32_t slot = hr.x;
     const float4 o4 = F.qo[in][slot], d4 = F.qd[in][slot];
     pid = __float_as_uint(o4.w);
     view_bits = __float_as_uint(d4.w);
     // the pixel index the seeds use is local to the view
     const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid;
-    const v3 ro = pkm::mk(o4.x, o4.y, o4.z), rd = pkm::mk(d4.x, d4.y, d4.z);
     const float4 t4 = F.thr[pid];
-    const v3 thr = pkm::mk(t4.x, t4.y, t4.z);
-    // deterministic seed: bounce index in place of the compaction slot
-    uint32_t seed = ((uint32_t)bounce * 1973u + pixel * 9277u + P.sample * 26699u) | 1u;
-    seed = pkm::pcg_next(&seed);
-    // getSurfaceInteraction (:179-200), interpolateVertexAttribute (:363-394)
-    const int4 tri = S.tri_index[leaf];
-    const float bu = (float)(bary_mask & 0xffffu) / 65535.0f, bv = (float)(bary_mask >> 16) / 65535.0f;
-    const v3 bary = pkm::mk(1.0f - bu - bv, bu, bv);
-    v3 tng, btg, nrm;
-    {
-      pkm::m3 m;
-      m.c0 = attr_vec(S, tri.x, 0);
-      m.c1 = attr_vec(S, tri.y, 0);
-      m.c2 = attr_vec(S, tri.z, 0);
-      tng = pkm::normalize(pkm::mat_mul(m, bary));
-      m.c0 = attr_vec(S, tri.x, 1);
-      m.c1 = attr_vec(S, tri.y, 1);
-      m.c2 = attr_vec(S, tri.z, 1);
-      btg = pkm::normalize(pkm::mat_mul(m, bary));
-      m.c0 = attr_vec(S, tri.x, 2);
-      m.c1 = attr_vec(S, tri.y, 2);
-      m.c2 = attr_vec(S, tri.z, 2);
-      nrm = pkm::normalize(pkm::mat_mul(m, bary));
-    }
-    const uint32_t w0 = S.attrs[tri.x].w, w1 = S.attrs[tri.y].w, w2 = S.attrs[tri.z].w;
-    const float uvx = pkm::unpack_snorm16(w0 & 0xffffu) * bary.x + pkm::unpack_snorm16(w1 & 0xffffu) * bary.y +
-                      pkm::unpack_snorm16(w2 & 0xffffu) * bary.z;
-    const float uvy = pkm::unpack_snorm16(w0 >> 16) * bary.x + pkm::unpack_snorm16(w1 >> 16) * bary.y +
-                      pkm::unpack_snorm16(w2 >> 16) * bary.z;
-    const int4 mi = S.materials[tri.w];
-    const float4 nm = sample_tex(S, mi.z, uvx, uvy);
-    const v3 map_normal = pkm::mk((nm.x - 0.5f) * 2.0f, (nm.y - 0.5f) * 2.0f, (nm.z - 0.0f) * 1.0f);
-    SI si;
-    {
-      pkm::m3 tbn;
-      tbn.c0 = tng;
-      tbn.c1 = btg;
-      tbn.c2 = nrm;
-      si.normal = pkm::normalize(pkm::mat_mul(tbn, map_normal));
-    }
-    const float eps40 = kEpsShadow * 40.0f;
-    si.origin = pkm::add(ro, pkm::scale(rd, t - eps40));
-    si.basis = onb(si.normal);
-    si.wo = pkm::vec_mul(pkm::neg(rd), si.basis);
-    const float4 dc = sample_tex(S, mi.x, uvx, uvy);
-    si.base = pkm::mk(dc.x, dc.y, dc.z);
-    si.opacity = dc.w;
-    const float4 mr = sample_tex(S, mi.y, uvx, uvy);
-    si.metallic = mr.z;
-    si.alpha = mr.y * mr.y;
-    const float4 em = sample_tex(S, mi.w, uvx, uvy);
-    si.emission = pkm::mk(em.x, em.y, em.z);
-    si.spec = pkm::mix3(pkm::mk(1.0f, 1.0f, 1.0f), si.base, si.metallic);
-    if (pkm::pcg_rand(&seed) > si.opacity) {
-      // alpha pass-through (:426-431)
-      b_o = pkm::add(ro, pkm::scale(rd, t + kEpsShadow * 40.0f));
-      b_d = rd;
-      b_thr = thr;
-      emit_bounce = true;
-    } else if (si.emission.x * 1.0f + si.emission.y * 1.0f + si.emission.z * 1.0f > 0.0f) {
-      // emission (:434-438)
-      const float att = pkm::max_(pkm::dot(si.normal, pkm::mat_mul(si.basis, si.wo)), 0.0f);
-      const v3 L = pkm::scale(pkm::mul(pkm::mul(pkm::mk(50.0f, 50.0f, 50.0f), si.emission), thr), att);
-      add_color(F, pid, L);
-    } else {
-      const v3 m = sample_ggx(si, &seed);
-      const float f = pkm::mix(schlick(pkm::max_(pkm::dot(si.wo, m), 0.0f), 1.5f), 1.0f, si.metallic);
-      v3 wi, bsdf;
-      float pdf;
-      if (pkm::pcg_rand(&seed) > f) {
-        // sampleLambert (:243-253)
-        const float r = pkm::sqrt_(pkm::pcg_rand(&seed));
-        const float phi = pkm::kTau * pkm::pcg_rand(&seed);
-        float sp, cp;
-        pkm::sincosf_(phi, &sp, &cp);
-        const float x = r * cp, y = r * sp;
-        const float z = pkm::sqrt_(pkm::max_(0.0f, 1.0f - x * x - y * y));
-        wi = pkm::mk(x, y, z);
-        pdf = lambert_pdf(wi);
-        bsdf = eval_lambert(si, wi, pdf);
-      } else {
-        wi = pkm::reflect(pkm::neg(si.wo), m);
-        pdf = specular_pdf(si, m);
-        bsdf = eval_specular(si, wi, pdf);
-      }
-      if (wi.z > 0.0f) {
-        const v3 dir = pkm::mat_mul(si.basis, wi);
-        const float weight = S.env_mode ? power_heuristic(pdf, env_pdf(S, P.env_theta, dir)) : 1.0f;
-        b_o = si.origin;
-        b_d = dir;
-        b_thr = pkm::scale(pkm::mul(bsdf, thr), weight);
-        emit_bounce = true;
-      }
-      if (S.env_mode) {
-        v3 ewi;
-        float epdf;
-        sample_env(S, P.env_theta, si.basis, &seed, &ewi, &epdf);
-        if (pkm::dot(ewi, m) > 0.0f && ewi.z > 0.0f) {
-          // createMaterialSampleRay (:339-348)
-          const float lw = power_heuristic(epdf, lambert_pdf(ewi));
-          const v3 el = eval_lambert(si, ewi, epdf);
-          v3 sc = pkm::mk((1.0f - f) * el.x * lw, (1.0f - f) * el.y * lw, (1.0f - f) * el.z * lw);
-          const v3 hh = pkm::normalize(pkm::add(si.wo, ewi));
-          const float sw = power_heuristic(epdf, specular_pdf(si, hh));
-          const v3 ev = eval_specular(si, ewi, epdf);
-          sc = pkm::add(sc, pkm::mk(f * ev.x * sw, f * ev.y * sw, f * ev.z * sw));
-          s_o = si.origin;
-          s_d = pkm::mat_mul(si.basis, ewi);
-          s_thr = pkm::mul(sc, thr);
-          emit_shadow = true;
-        }
-      }
-    }
+    shade_hit<false>(S, P.env_theta, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), __uint_as_float(hr.z),
+                     hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample), pkm::mk(t4.x, t4.y, t4.z), o,
+                     nullptr);
+    if (o.emit) add_color(F, pid, o.L);
   }
-  const uint32_t bs = wave_append(&F.cnt[kCntRays0 + (in ^ 1)], emit_bounce);
-  if (emit_bounce) {
-    F.qo[in ^ 1][bs] = make_float4(b_o.x, b_o.y, b_o.z, __uint_as_float(pid));
-    F.qd[in ^ 1][bs] = make_float4(b_d.x, b_d.y, b_d.z, __uint_as_float(view_bits));
-    F.thr[pid] = make_float4(b_thr.x, b_thr.y, b_thr.z, 0.0f);
+  const uint32_t bs = wave_append(&F.cnt[kCntRays0 + (in ^ 1)], o.bounce);
+  if (o.bounce) {
+    F.qo[in ^ 1][bs] = make_float4(o.b_o.x, o.b_o.y, o.b_o.z, __uint_as_float(pid));
+    F.qd[in ^ 1][bs] = make_float4(o.b_d.x, o.b_d.y, o.b_d.z, __uint_as_float(view_bits));
+    F.thr[pid] = make_float4(o.b_thr.x, o.b_thr.y, o.b_thr.z, 0.0f);
   }
-  const uint32_t ss = wave_append(&F.cnt[kCntShadow], emit_shadow);
-  if (emit_shadow) {
-    F.so[ss] = make_float4(s_o.x, s_o.y, s_o.z, __uint_as_float(pid));
-    F.sd[ss] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(view_bits));
-    F.sthr[ss] = make_float4(s_thr.x, s_thr.y, s_thr.z, 0.0f);
+  const uint32_t ss = wave_append(&F.cnt[kCntShadow], o.shadow);
+  if (o.shadow) {
+    F.so[ss] = make_float4(o.s_o.x, o.s_o.y, o.s_o.z, __uint_as_float(pid));
+    F.sd[ss] = make_float4(o.s_d.x, o.s_d.y, o.s_d.z, __uint_as_float(view_bits));
+    F.sthr[ss] = make_float4(o.s_thr.x, o.s_thr.y, o.s_thr.z, 0.0f);
   }
 }
 
@@ -622,6 +732,238 @@ __global__ void __launch_bounds__(256) k_copy_f4(float4* dst, const float4* src,
 }
 
 // ---------------------------------------------------------------------------
+// Path-replay adjoint (differentiation of the frame's image)
+// ---------------------------------------------------------------------------
+__device__ inline float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Rounds of key peeling before the remaining lanes add directly.
+constexpr int kScatterPeel = 4;
+
+// Adds v[0..C) to dst[key * 4 + c0 + c] for every lane where `valid` holds.
+// Lanes sharing a key are summed across the wave first: a segmented
+// reduction that peels one key per round (the key of the lowest pending
+// lane), so a texel hit by many lanes (coherent camera paths, the bright
+// environment texels most NEE rays pick) costs one atomic per wave.  After
+// kScatterPeel rounds the lanes still pending add on their own.  Called by
+// the whole wave.
+template <int C>
+__device__ inline void wave_scatter_add(float* dst, uint32_t key, bool valid, const float* v, int c0) {
+  const int lane = (int)__lane_id();
+  unsigned long long pending = __ballot(valid);
+  for (int round = 0; pending; ++round) {
+    const bool mine_pending = (pending >> lane) & 1ull;
+    if (round == kScatterPeel) {
+      if (mine_pending)
+#pragma unroll
+        for (int c = 0; c < C; ++c) unsafeAtomicAdd(dst + (size_t)key * 4 + c0 + c, v[c]);
+      break;
+    }
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const bool mine = mine_pending && key == k;
+    const unsigned long long grp = __ballot(mine);
+    if (__popcll(grp) == 1) {
+      if (lane == leader)
+#pragma unroll
+        for (int c = 0; c < C; ++c) unsafeAtomicAdd(dst + (size_t)k * 4 + c0 + c, v[c]);
+    } else {
+      float s[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) s[c] = wave_sum_f(mine ? v[c] : 0.0f);
+      if (lane == leader)
+#pragma unroll
+        for (int c = 0; c < C; ++c) unsafeAtomicAdd(dst + (size_t)k * 4 + c0 + c, s[c]);
+    }
+    pending &= ~grp;
+  }
+}
+
+// One thread per path (pixel, sample); persistent one-wave blocks pull
+// 64-path batches from A.cursor.  Each path is replayed twice from its
+// seeds with the same arithmetic as the wavefront frame:
+//   pass 0 sums its radiance L (equal to the frame's color for that sample);
+//   pass 1 walks it again with rem = L, subtracting each contribution as it
+//   is met, so that at a scattering vertex rem is the radiance that flows
+//   through that vertex's continuation weight.  With delta = dLoss/dI / N:
+//     environment texel e of a miss or an unoccluded NEE ray:
+//       dLoss/dE_e += delta * (throughput of that contribution);
+//     base color / alpha at a vertex with throughput thr:
+//       NEE:   delta * E * thr * d(nee weight)/d(theta)
+//       bounce: delta * rem * d(bsdf)/d(theta) / bsdf
+//     and d alpha / d roughness = 2 * roughness, scattered bilinearly to the
+//     4 texels of the lookup.
+// Nothing per path is stored between the passes.
+__global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
+  __shared__ uint32_t lds_node[kStackLds * 64];
+  __shared__ float lds_t[kStackLds * 64];
+  const int lane = (int)threadIdx.x;
+  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const uint32_t B = A.W * A.H;
+  const uint32_t total = B * A.num_frames;
+  const uint32_t plane = S.tw * S.th;
+  TraverseStats st = {0, 0};
+  uint32_t overflow = 0;
+  const v3 zero = pkm::mk(0.0f, 0.0f, 0.0f);
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(A.cursor, 64u);
+    base = __shfl(base, 0);
+    if (base >= total) break;
+    const uint32_t i = base + (uint32_t)lane;
+    uint32_t pixel = 0, sample = 0;
+    v3 delta = zero;
+    bool live = i < total;
+    if (live) {
+      pixel = i % B;
+      sample = A.first_sample + i / B;
+      const float4 g4 = A.d_image[pixel];
+      delta = pkm::mk(g4.x * A.inv_frames, g4.y * A.inv_frames, g4.z * A.inv_frames);
+      live = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f;
+    }
+    v3 sum = zero, rem = zero;
+    for (int pass = 0; pass < 2; ++pass) {
+      // contributions in the frame's order: pass 0 adds, pass 1 subtracts
+      auto contribute = [&](v3 c) {
+        if (pass == 0) {
+          sum.x += c.x;
+          sum.y += c.y;
+          sum.z += c.z;
+        } else {
+          rem.x -= c.x;
+          rem.y -= c.y;
+          rem.z -= c.z;
+        }
+      };
+      bool alive = live;
+      v3 o = zero, d = zero, thr = pkm::mk(1.0f, 1.0f, 1.0f);
+      if (alive) camera_ray(pixel % A.W, pixel / A.W, A.W, A.H, sample, A.cam, &o, &d);
+      for (uint32_t b = 0; b < A.depth; ++b) {
+        if (__ballot(alive) == 0ull) break;
+        // closest hit (trace.wgsl) and miss shading (shade_miss.wgsl)
+        RayState r;
+        r.best_id = 0xffffffffu;
+        r.tbest = kMaxT;
+        r.best_leaf = 0;
+        r.best_bary = 0;
+        if (alive) {
+          ray_init<false>(r, o, d);
+          while (!ray_step<false, false>(S, r, stk, &st)) {
+          }
+          if (r.overflow) overflow = 1;
+        }
+        const bool miss = alive && r.best_id == 0xffffffffu;
+        if (miss) {
+          contribute(pkm::mul(env_color(S, A.theta, d), thr));
+          alive = false;
+        }
+        if (pass == 1) {
+          const bool valid = miss && S.env_mode != 0;
+          const float ev[3] = {delta.x * thr.x, delta.y * thr.y, delta.z * thr.z};
+          wave_scatter_add<3>(A.g_env, valid ? env_texel(S, A.theta, d) : 0u, valid, ev, 0);
+        }
+        // shading (shade_hit.wgsl)
+        ShadeOut so;
+        so.emit = so.scatter = so.bounce = so.shadow = false;
+        ShadeGrad sg;
+        sg.u = sg.v = sg.rough = 0.0f;
+        sg.layer_base = sg.layer_rough = 0;
+        const bool hit = alive;
+        if (hit) {
+          shade_hit<true>(S, A.theta, o, d, r.tbest, r.best_leaf, r.best_bary, shade_seed(b, pixel, sample), thr,
+                          so, &sg);
+          if (so.emit) {
+            contribute(so.L);
+            alive = false;
+          }
+        }
+        // NEE shadow ray (trace_shadow.wgsl) and its environment term
+        bool vis = false;
+        if (hit && so.shadow) {
+          RayState q;
+          ray_init<true>(q, so.s_o, so.s_d);
+          while (!ray_step<true, false>(S, q, stk, &st)) {
+          }
+          if (q.overflow) overflow = 1;
+          vis = q.best_id == 0xffffffffu;
+        }
+        v3 E = zero;
+        if (vis) {
+          E = env_color(S, A.theta, so.s_d);
+          contribute(pkm::mul(E, so.s_thr));
+        }
+        if (pass == 1) {
+          const float ev[3] = {delta.x * so.s_thr.x, delta.y * so.s_thr.y, delta.z * so.s_thr.z};
+          wave_scatter_add<3>(A.g_env, vis ? env_texel(S, A.theta, so.s_d) : 0u, vis, ev, 0);
+          const bool scat = hit && so.scatter;
+          v3 ga = zero;
+          float gr = 0.0f;
+          if (scat) {
+            if (vis) {
+              const v3 w = pkm::mul(pkm::mul(delta, E), thr);
+              ga = pkm::scale(w, sg.dn_base);
+              gr = pkm::dot(w, sg.dn_alpha);
+            }
+            // the last vertex's continuation is never traced
+            if (so.bounce && b + 1 < A.depth) {
+              const float q0 = sg.bsdf.x != 0.0f ? delta.x * rem.x / sg.bsdf.x : 0.0f;
+              const float q1 = sg.bsdf.y != 0.0f ? delta.y * rem.y / sg.bsdf.y : 0.0f;
+              const float q2 = sg.bsdf.z != 0.0f ? delta.z * rem.z / sg.bsdf.z : 0.0f;
+              ga = pkm::add(ga, pkm::mk(q0 * sg.db_base, q1 * sg.db_base, q2 * sg.db_base));
+              gr += q0 * sg.db_alpha.x + q1 * sg.db_alpha.y + q2 * sg.db_alpha.z;
+            }
+            gr *= 2.0f * sg.rough;
+          }
+          const Footprint fp = tex_footprint(S, sg.u, sg.v);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float va[3] = {ga.x * fp.w[j], ga.y * fp.w[j], ga.z * fp.w[j]};
+            wave_scatter_add<3>(A.g_atlas, (uint32_t)sg.layer_base * plane + fp.idx[j], scat, va, 0);
+            const float vr[1] = {gr * fp.w[j]};
+            wave_scatter_add<1>(A.g_atlas, (uint32_t)sg.layer_rough * plane + fp.idx[j], scat, vr, 1);
+          }
+        }
+        if (hit && !so.emit) {
+          if (so.bounce) {
+            o = so.b_o;
+            d = so.b_d;
+            thr = so.b_thr;
+          } else {
+            alive = false;
+          }
+        }
+      }
+      if (pass == 0) rem = sum;
+    }
+  }
+  overflow = wave_sum(overflow);
+  if (lane == 0 && overflow) atomicOr(A.overflow, 1u);
+}
+
+// L2 image loss: loss += sum over pixels and rgb of (img - target)^2 (in
+// f64), grad = 2 (img - target).
+__global__ void __launch_bounds__(256) k_loss_l2(const float4* img, const float4* target, float4* grad,
+                                                 double* loss, uint32_t n) {
+  __shared__ double part[4];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (i < n) {
+    const float4 a = img[i], t = target[i];
+    const float dx = a.x - t.x, dy = a.y - t.y, dz = a.z - t.z;
+    grad[i] = make_float4(2.0f * dx, 2.0f * dy, 2.0f * dz, 0.0f);
+    l = (double)dx * dx + (double)dy * dy + (double)dz * dz;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
+}
+
+// ---------------------------------------------------------------------------
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
 // ---------------------------------------------------------------------------
 template <bool kAnyHit, bool kCount>
@@ -704,6 +1046,12 @@ void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s) {
 }
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy_f4, dim3(min(cdiv(n, 256), 8192u)), dim3(256), 0, s, dst, src, n);
+}
+void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(64), 0, s, S, A);
+}
+void launch_loss_l2(const float4* img, const float4* target, float4* grad, double* loss, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_loss_l2, dim3(cdiv(n, 256)), dim3(256), 0, s, img, target, grad, loss, n);
 }
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,

@@ -49,6 +49,8 @@ struct pk_scene {
   std::vector<void*> allocs;
   int64_t env_info[4] = {0, 0, 0, 0};
   std::vector<uint32_t> roots{0};  // wide-BVH root of each merged sub-scene
+  float* tex_eval = nullptr;       // evaluation atlas, allocated by the first parameter write
+  uint32_t gen = 0;                // bumped when `dev` changes; captured graphs hold a copy of it
 };
 
 struct pk_frame {
@@ -63,6 +65,7 @@ struct pk_frame {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const pk_scene* graph_scene = nullptr;
+  uint32_t graph_scene_gen = 0;
   uint32_t graph_depth = 0;
   uint32_t graph_gen = 0;
   float graph_theta = 0;
@@ -70,6 +73,14 @@ struct pk_frame {
   // batch frames: views over one merged scene, paths laid out view after view
   std::vector<pk::ViewDev> views;
   pk_scene* batch_scene = nullptr;
+  // adjoint scratch, allocated on first use
+  float4* adj_dimage = nullptr;
+  float* adj_atlas = nullptr;
+  size_t adj_atlas_n = 0;
+  float* adj_env = nullptr;
+  size_t adj_env_n = 0;
+  uint32_t* adj_cursor = nullptr;
+  double* loss = nullptr;
 };
 
 namespace {
@@ -77,6 +88,8 @@ namespace {
 // One-wave traversal blocks per CU at most: kTraceWavesPerSimd waves on each
 // of the 4 SIMDs (the register and LDS budget of the traversal kernels).
 constexpr uint32_t kMaxWavesPerCu = 4 * pk::kTraceWavesPerSimd;
+// One-wave adjoint blocks per CU (its registers allow 2 waves per SIMD).
+constexpr uint32_t kAdjointWavesPerCu = 8;
 
 pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
   return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill};
@@ -394,6 +407,7 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     D.tw = D.th = D.tl = 1;
     D.tfmt = PK_TEX_RGBA8;
   }
+  D.tex_eval = D.tex;  // parameters not yet written: evaluation reads the sampling atlas
   D.env_mode = 0;
   if (d->env_format != PK_ENV_NONE) {
     const size_t T = (size_t)env.width * env.height;
@@ -672,7 +686,8 @@ int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
     PK_HIP(hipGetLastError());
     return PK_OK;
   }
-  if (!f->exec || f->graph_scene != s || f->graph_depth != p->max_depth || f->graph_gen != f->ctx->tuning_gen) {
+  if (!f->exec || f->graph_scene != s || f->graph_scene_gen != s->gen || f->graph_depth != p->max_depth ||
+      f->graph_gen != f->ctx->tuning_gen) {
     drop_graph(f);
     PK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     enqueue_frame(f, s, p->max_depth, st);
@@ -683,6 +698,7 @@ int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
     }
     PK_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
     f->graph_scene = s;
+    f->graph_scene_gen = s->gen;
     f->graph_depth = p->max_depth;
     f->graph_gen = f->ctx->tuning_gen;
   }
@@ -697,6 +713,16 @@ int pk_read_image(pk_frame* f, float* rgba, size_t bytes) {
   }
   PK_HIP(hipStreamSynchronize(f->ctx->stream));
   PK_HIP(hipMemcpy(rgba, f->dev.accum, (size_t)f->W * f->H * 16, hipMemcpyDeviceToHost));
+  return PK_OK;
+}
+
+int pk_read_image_device(pk_frame* f, float* d_rgba, size_t bytes) {
+  if (!f || !d_rgba || bytes < (size_t)f->W * f->H * 16) {
+    pk_set_error("pk_read_image_device: null argument or buffer smaller than width*height*16");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipSetDevice(f->ctx->device));
+  PK_HIP(hipMemcpyAsync(d_rgba, f->dev.accum, (size_t)f->W * f->H * 16, hipMemcpyDeviceToDevice, f->ctx->stream));
   return PK_OK;
 }
 
@@ -1091,7 +1117,8 @@ int pk_batch_render(pk_frame* f, uint32_t depth, float theta, int use_graph) {
     PK_HIP(hipGetLastError());
     return PK_OK;
   }
-  if (!f->exec || f->graph_depth != depth || f->graph_gen != f->ctx->tuning_gen || f->graph_theta != theta) {
+  if (!f->exec || f->graph_depth != depth || f->graph_gen != f->ctx->tuning_gen || f->graph_theta != theta ||
+      f->graph_scene_gen != s->gen) {
     drop_graph(f);
     PK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     enqueue_batch(f, s, depth, theta, st);
@@ -1102,6 +1129,7 @@ int pk_batch_render(pk_frame* f, uint32_t depth, float theta, int use_graph) {
     }
     PK_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
     f->graph_scene = s;
+    f->graph_scene_gen = s->gen;
     f->graph_depth = depth;
     f->graph_gen = f->ctx->tuning_gen;
     f->graph_theta = theta;
@@ -1123,6 +1151,202 @@ int pk_batch_read_view(pk_frame* f, uint32_t view, float* rgba, size_t bytes) {
   }
   PK_HIP(hipStreamSynchronize(f->ctx->stream));
   PK_HIP(hipMemcpy(rgba, f->dev.accum + v.offset, n, hipMemcpyDeviceToHost));
+  return PK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable parameters and the path-replay adjoint
+// ---------------------------------------------------------------------------
+static size_t param_floats(const pk_scene* s, int which) {
+  const pk::DevScene& D = s->dev;
+  if (which == PK_PARAM_ATLAS) return (size_t)D.tw * D.th * D.tl * 4;
+  if (which == PK_PARAM_ENV) return D.env_mode ? (size_t)D.ew * D.eh * 4 : 0;
+  return 0;
+}
+
+int pk_scene_param_info(const pk_scene* s, uint32_t* info6) {
+  if (!s || !info6) {
+    pk_set_error("pk_scene_param_info: null argument");
+    return PK_E_INVALID;
+  }
+  const pk::DevScene& D = s->dev;
+  info6[0] = D.tw;
+  info6[1] = D.th;
+  info6[2] = D.tl;
+  info6[3] = D.tfmt;
+  info6[4] = D.env_mode ? D.ew : 0;
+  info6[5] = D.env_mode ? D.eh : 0;
+  return PK_OK;
+}
+
+int pk_scene_write_params(pk_scene* s, int which, size_t offset, const float* src, size_t count, int src_on_device,
+                          int also_sampling) {
+  if (!s || (count && !src) || (which != PK_PARAM_ATLAS && which != PK_PARAM_ENV)) {
+    pk_set_error("pk_scene_write_params: null argument or unknown parameter set");
+    return PK_E_INVALID;
+  }
+  if (which == PK_PARAM_ATLAS && s->dev.tfmt != PK_TEX_RGBA32F) {
+    pk_set_error("pk_scene_write_params: atlas parameters need a scene uploaded with an RGBA32F atlas");
+    return PK_E_INVALID;
+  }
+  const size_t n = param_floats(s, which);
+  if (n == 0 || offset > n || count > n - offset) {
+    pk_set_error("pk_scene_write_params: range [" + std::to_string(offset) + ", +" + std::to_string(count) +
+                 ") outside the " + std::to_string(n) + " parameter floats (or the scene has no environment)");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const hipMemcpyKind kind = src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (which == PK_PARAM_ATLAS) {
+    float* samp = (float*)s->dev.tex;
+    if (!also_sampling && !s->tex_eval) {
+      int rc = dev_alloc(s->allocs, &s->tex_eval, n);
+      if (rc) return rc;
+      PK_HIP(hipMemcpyAsync(s->tex_eval, samp, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      s->dev.tex_eval = s->tex_eval;
+      s->gen++;
+    }
+    if (count && s->tex_eval)
+      PK_HIP(hipMemcpyAsync(s->tex_eval + offset, src, count * sizeof(float), kind, st));
+    if (count && also_sampling) PK_HIP(hipMemcpyAsync(samp + offset, src, count * sizeof(float), kind, st));
+  } else if (count) {
+    PK_HIP(hipMemcpyAsync((float*)s->dev.env_rad + offset, src, count * sizeof(float), kind, st));
+  }
+  if (!src_on_device) PK_HIP(hipStreamSynchronize(st));
+  return PK_OK;
+}
+
+int pk_scene_read_params(pk_scene* s, int which, size_t offset, float* dst, size_t count) {
+  if (!s || (count && !dst) || (which != PK_PARAM_ATLAS && which != PK_PARAM_ENV)) {
+    pk_set_error("pk_scene_read_params: null argument or unknown parameter set");
+    return PK_E_INVALID;
+  }
+  if (which == PK_PARAM_ATLAS && s->dev.tfmt != PK_TEX_RGBA32F) {
+    pk_set_error("pk_scene_read_params: atlas parameters need a scene uploaded with an RGBA32F atlas");
+    return PK_E_INVALID;
+  }
+  const size_t n = param_floats(s, which);
+  if (n == 0 || offset > n || count > n - offset) {
+    pk_set_error("pk_scene_read_params: range outside the parameter floats (or the scene has no environment)");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipSetDevice(s->ctx->device));
+  PK_HIP(hipStreamSynchronize(s->ctx->stream));
+  const float* base = which == PK_PARAM_ENV ? (const float*)s->dev.env_rad : (const float*)s->dev.tex_eval;
+  if (count) PK_HIP(hipMemcpy(dst, base + offset, count * sizeof(float), hipMemcpyDeviceToHost));
+  return PK_OK;
+}
+
+int pk_loss_l2(pk_frame* f, const float* d_target, float* d_grad, double* loss) {
+  if (!f || !d_target || !d_grad || f->batch_scene) {
+    pk_set_error("pk_loss_l2: null argument or a batch frame");
+    return PK_E_INVALID;
+  }
+  PK_HIP(hipSetDevice(f->ctx->device));
+  hipStream_t st = f->ctx->stream;
+  if (!f->loss) {
+    int rc = dev_alloc(f->allocs, &f->loss, 1);
+    if (rc) return rc;
+  }
+  pk::launch_fill_u32((uint32_t*)f->loss, 0u, 2, st);
+  pk::launch_loss_l2(f->dev.accum, (const float4*)d_target, (float4*)d_grad, f->loss, f->W * f->H, st);
+  PK_HIP(hipGetLastError());
+  if (loss) {
+    PK_HIP(hipStreamSynchronize(st));
+    PK_HIP(hipMemcpy(loss, f->loss, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return PK_OK;
+}
+
+int pk_adjoint(pk_frame* f, pk_scene* s, const pk_render_params* p, const float* d_image, int d_image_on_device,
+               const pk_grad_buffers* g) {
+  if (!f || !s || !p || !d_image || !g) {
+    pk_set_error("pk_adjoint: null argument");
+    return PK_E_INVALID;
+  }
+  if (f->batch_scene || s->roots.size() != 1) {
+    pk_set_error("pk_adjoint: batch frames and merged scenes are not differentiable");
+    return PK_E_INVALID;
+  }
+  if (p->width != f->W || p->height != f->H) {
+    pk_set_error("pk_adjoint: params resolution differs from the frame's");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_adjoint: no camera set (pk_frame_set_camera)");
+    return PK_E_STATE;
+  }
+  if (p->max_depth == 0 || p->max_depth > 64 || p->num_frames == 0) {
+    pk_set_error("pk_adjoint: max_depth outside [1, 64] or no frames");
+    return PK_E_LIMIT;
+  }
+  const uint64_t paths = (uint64_t)f->W * f->H * p->num_frames;
+  if (paths >= (1ull << 32) - 64) {
+    pk_set_error("pk_adjoint: width*height*num_frames must stay below 2^32");
+    return PK_E_LIMIT;
+  }
+  PK_HIP(hipSetDevice(f->ctx->device));
+  hipStream_t st = f->ctx->stream;
+  const size_t na = param_floats(s, PK_PARAM_ATLAS), ne = param_floats(s, PK_PARAM_ENV);
+  int rc;
+  if (!f->adj_cursor && (rc = dev_alloc(f->allocs, &f->adj_cursor, 1))) return rc;
+  const float4* dimg = (const float4*)d_image;
+  if (!d_image_on_device) {
+    if (!f->adj_dimage && (rc = dev_alloc(f->allocs, &f->adj_dimage, (size_t)f->W * f->H))) return rc;
+    PK_HIP(hipMemcpyAsync(f->adj_dimage, d_image, (size_t)f->W * f->H * 16, hipMemcpyHostToDevice, st));
+    dimg = f->adj_dimage;
+  }
+  // gradient targets: the caller's device buffers, or frame scratch
+  float* ga = g->on_device ? g->atlas : nullptr;
+  float* ge = g->on_device ? g->env : nullptr;
+  if (!ga) {
+    if (f->adj_atlas_n < na) {
+      if ((rc = dev_alloc(f->allocs, &f->adj_atlas, na))) return rc;
+      f->adj_atlas_n = na;
+    }
+    ga = f->adj_atlas;
+  }
+  if (!ge) {
+    if (f->adj_env_n < ne || !f->adj_env) {
+      if ((rc = dev_alloc(f->allocs, &f->adj_env, ne ? ne : 4))) return rc;
+      f->adj_env_n = ne ? ne : 4;
+    }
+    ge = f->adj_env;
+  }
+  // start values: zero, or the caller's gradients when accumulating
+  const bool host_acc = !g->on_device && g->accumulate;
+  if (host_acc && g->atlas) PK_HIP(hipMemcpyAsync(ga, g->atlas, na * 4, hipMemcpyHostToDevice, st));
+  if (host_acc && g->env && ne) PK_HIP(hipMemcpyAsync(ge, g->env, ne * 4, hipMemcpyHostToDevice, st));
+  const bool keep_a = g->accumulate && g->atlas, keep_e = g->accumulate && g->env;
+  if (!keep_a) pk::launch_fill_u32((uint32_t*)ga, 0u, (uint32_t)na, st);
+  if (ne && !keep_e) pk::launch_fill_u32((uint32_t*)ge, 0u, (uint32_t)ne, st);
+  pk::launch_fill_u32(f->adj_cursor, 0u, 1, st);
+  pk::AdjointArgs A;
+  A.d_image = dimg;
+  A.g_atlas = ga;
+  A.g_env = ge;
+  A.W = f->W;
+  A.H = f->H;
+  A.first_sample = p->first_sample;
+  A.num_frames = p->num_frames;
+  A.depth = p->max_depth;
+  A.inv_frames = 1.0f / (float)p->num_frames;
+  A.theta = p->env_theta;
+  A.cam = f->cam;
+  A.cursor = f->adj_cursor;
+  A.spill = f->ctx->spill;
+  A.overflow = f->dev.cnt + pk::kCntOverflow;
+  // 2 waves per SIMD fit the adjoint's registers; the spill area holds
+  // kMaxWavesPerCu waves per CU
+  const uint32_t grid = (uint32_t)f->ctx->num_cus * kAdjointWavesPerCu;
+  pk::launch_adjoint(s->dev, A, grid, st);
+  PK_HIP(hipGetLastError());
+  if (!g->on_device) {
+    PK_HIP(hipStreamSynchronize(st));
+    if (g->atlas) PK_HIP(hipMemcpy(g->atlas, ga, na * 4, hipMemcpyDeviceToHost));
+    if (g->env && ne) PK_HIP(hipMemcpy(g->env, ge, ne * 4, hipMemcpyDeviceToHost));
+  }
   return PK_OK;
 }
 

@@ -1,0 +1,224 @@
+"""Differentiation on the GPU (path-replay adjoint, detached sampling).
+
+* Forward: with evaluation parameters that differ from the sampling ones
+  (perturbed base color, roughness and environment), the HIP frame equals the
+  FP32 oracle bit for bit, also after a captured graph was replayed before
+  the parameters were written.
+* Adjoint vs the oracle's FP32 dual-number gradient (same paths bit for bit:
+  differences are derivative rounding, the replay's subtraction and atomic
+  order) per parameter, and vs the FP64 dual-number gradient over the set.
+* Adjoint vs central finite differences of the GPU forward (eps = 1e-3,
+  common seeds) within 2%.
+* Device-resident API: loss/gradient kernel, device gradients equal host
+  ones, accumulate adds.
+"""
+import numpy as np
+import pytest
+
+import pike_amd
+from pike_amd import _ffi, scenes
+from oracle.oracle import OracleScene
+
+pytestmark = pytest.mark.gpu
+
+ATLAS, ENV = _ffi.PK_PARAM_ATLAS, _ffi.PK_PARAM_ENV
+W, H, DEPTH, SPP = 48, 40, 3, 4
+TEX, EW, EH = 64, 64, 32
+
+
+def _aoff(layer, x, y, c):
+    return ((layer * TEX + y) * TEX + x) * 4 + c
+
+
+def _eoff(x, y, c):
+    return (y * EW + x) * 4 + c
+
+
+def _perturbed(data, seed):
+    rng = np.random.default_rng(seed)
+    tex = data.tex.copy()
+    tex[0, ..., :3] = np.clip(tex[0, ..., :3] + rng.normal(0, 0.1, tex[0, ..., :3].shape), 0.02, 1.0)
+    tex[1, ..., 1] = np.clip(tex[1, ..., 1] + rng.normal(0, 0.1, tex[1, ..., 1].shape), 0.05, 1.0)
+    env = np.zeros((EH, EW, 4), np.float32)
+    env[..., :3] = data.env.reshape(EH, EW, 3) * rng.uniform(0.7, 1.3, (EH, EW, 3))
+    return tex.astype(np.float32), env
+
+
+@pytest.fixture(scope="module")
+def c5(ctx):
+    data = scenes.config5(cells=40, num_spheres=8, tex_res=TEX, env_size=(EW, EH))
+    bvh = pike_amd.BVH(data.positions, data.triangles)
+    cam = scenes.normalized_camera(data.camera, bvh.normalizer())
+    tex, env = _perturbed(data, 7)
+    o = OracleScene(data, threads=16)
+    o.build_bvh()
+    o.write_params(ATLAS, tex)
+    o.write_params(ENV, env)
+    d_image = np.random.default_rng(3).uniform(-1.0, 1.0, (H, W, 4)).astype(np.float32)
+    return data, bvh, cam, tex, env, o, d_image
+
+
+def _scene(ctx, c5):
+    data, bvh, cam, tex, env, _, _ = c5
+    s = pike_amd.Scene(ctx, data, bvh)
+    s.write_params(ATLAS, tex)
+    s.write_params(ENV, env)
+    return s
+
+
+def _frame(ctx, cam):
+    f = pike_amd.Frame(ctx, W, H)
+    f.set_camera(cam)
+    return f
+
+
+def test_forward_with_evaluation_params_matches_fp32_oracle(ctx, c5):
+    data, bvh, cam, tex, env, o, _ = c5
+    s = pike_amd.Scene(ctx, data, bvh)
+    f = _frame(ctx, cam)
+    f.render(s, max_depth=DEPTH, num_frames=1)        # captures a graph with the sampling atlas only
+    s.write_params(ATLAS, tex)
+    s.write_params(ENV, env)
+    np.testing.assert_array_equal(s.read_params(ATLAS), tex.ravel())
+    np.testing.assert_array_equal(s.read_params(ENV), env.ravel())
+    f.reset()
+    f.render(s, max_depth=DEPTH, num_frames=SPP)
+    gpu = f.read_image()
+    ref, _, _ = o.render(cam, W, H, DEPTH, 0, SPP, precision=32)
+    np.testing.assert_array_equal(gpu.view(np.uint32), ref.view(np.uint32))
+    assert f.counters()["stack_overflow"] == 0
+
+
+def _pick(ga, ge, rng):
+    """64 parameters with nonzero adjoint gradient: 24 base color, 16
+    roughness, 24 environment (the 8 largest and 16 random)."""
+    out = []
+    base = np.flatnonzero(ga[0, ..., :3].ravel())
+    for k in rng.choice(base, 24, replace=False):
+        t, c = divmod(int(k), 3)
+        out.append((ATLAS, t * 4 + c))
+    rough = np.flatnonzero(ga[1, ..., 1].ravel())
+    for t in rng.choice(rough, 16, replace=False):
+        out.append((ATLAS, (TEX * TEX + int(t)) * 4 + 1))
+    e = np.abs(ge[..., :3]).ravel()
+    order = np.argsort(e)[::-1]
+    nz = np.flatnonzero(e)
+    chosen = list(order[:8]) + list(rng.choice(np.setdiff1d(nz, order[:8]), 16, replace=False))
+    for k in chosen:
+        t, c = divmod(int(k), 3)
+        out.append((ENV, t * 4 + c))
+    return out
+
+
+def _lookup(ga, ge, params):
+    fa, fe = ga.ravel(), ge.ravel()
+    return np.array([fa[i] if w == ATLAS else fe[i] for w, i in params], np.float64)
+
+
+def test_adjoint_matches_dual_number_gradients(ctx, c5):
+    data, bvh, cam, tex, env, o, d_image = c5
+    s = _scene(ctx, c5)
+    f = _frame(ctx, cam)
+    ga, ge = f.adjoint(s, d_image, max_depth=DEPTH, num_frames=SPP)
+    assert f.counters()["stack_overflow"] == 0
+    params = _pick(ga, ge, np.random.default_rng(19))
+    g = _lookup(ga, ge, params)
+    g32, j32 = o.gradient(cam, W, H, DEPTH, d_image, params, num_frames=SPP, precision=32)
+    g64, _ = o.gradient(cam, W, H, DEPTH, d_image, params, num_frames=SPP, precision=64)
+    scale = np.abs(g32).max()
+    err = np.abs(g - g32)
+    print("adjoint vs fp32 dual: max rel", (err / np.maximum(np.abs(g32), 1e-30)).max(),
+          "vs fp64: rel L1", np.abs(g - g64).sum() / np.abs(g64).sum())
+    assert np.all(err <= 2e-3 * np.abs(g32) + 1e-5 * scale), np.column_stack([g, g32])[err > 2e-3 * np.abs(g32)]
+    assert np.abs(g - g64).sum() <= 2e-3 * np.abs(g64).sum()
+    # the objective the adjoint differentiates is the rendered image
+    f.reset()
+    f.render(s, max_depth=DEPTH, num_frames=SPP)
+    img = f.read_image()
+    J = float((d_image[..., :3].astype(np.float64) * img[..., :3]).sum())
+    assert abs(J - j32) <= 1e-5 * np.abs(d_image[..., :3] * img[..., :3]).sum()
+
+
+def test_adjoint_matches_central_differences(ctx, c5):
+    """Central differences of J = sum(d_image * image) on the GPU forward
+    (eps = 1e-3, common seeds): |fd - adjoint| <= 2% |fd| + the FD's own
+    rounding bound (the pixels a perturbation changes carry f32 running-mean
+    rounding of up to 2 ulp per accumulated frame; pixels it does not change
+    are bit-identical and cancel exactly)."""
+    data, bvh, cam, tex, env, o, d_image = c5
+    s = _scene(ctx, c5)
+    f = _frame(ctx, cam)
+    ga, ge = f.adjoint(s, d_image, max_depth=DEPTH, num_frames=SPP)
+    params = _pick(ga, ge, np.random.default_rng(23))
+    g = _lookup(ga, ge, params)
+    eps = 1e-3
+    d3 = d_image[..., :3].astype(np.float64)
+
+    def image():
+        f.reset()
+        f.render(s, max_depth=DEPTH, num_frames=SPP)
+        return f.read_image()[..., :3]
+
+    fd, noise = [], []
+    for which, off in params:
+        v = float((tex.ravel() if which == ATLAS else env.ravel())[off])
+        hi, lo = np.float32(v + eps), np.float32(v - eps)
+        s.write_params(which, [hi], off)
+        ip = image()
+        s.write_params(which, [lo], off)
+        im = image()
+        s.write_params(which, [v], off)
+        den = float(hi) - float(lo)
+        fd.append(float((d3 * ip).sum() - (d3 * im).sum()) / den)
+        ulps = 2 * SPP * (np.spacing(np.abs(ip)) + np.spacing(np.abs(im)))
+        noise.append(float((np.abs(d3) * ulps * (ip != im)).sum()) / den)
+    fd, noise = np.array(fd), np.array(noise)
+    err = np.abs(fd - g)
+    rel = err / np.abs(fd)
+    print("adjoint vs central FD: median rel", np.median(rel), "within 2%:", int((rel <= 0.02).sum()), "of",
+          len(rel), "max err / rounding bound", (err / noise).max())
+    bad = err > 0.02 * np.abs(fd) + noise
+    assert not bad.any(), np.column_stack([g, fd, noise])[bad]
+    assert (rel <= 0.02).sum() >= 48, np.column_stack([g, fd, noise])[rel > 0.02]
+
+
+def test_device_api_loss_and_accumulate(ctx, c5):
+    data, bvh, cam, tex, env, o, d_image = c5
+    s = _scene(ctx, c5)
+    f = _frame(ctx, cam)
+    f.render(s, max_depth=DEPTH, num_frames=SPP)
+    img = f.read_image()
+    target = np.random.default_rng(4).uniform(0, 1, (H, W, 4)).astype(np.float32)
+    tb = pike_amd.DeviceBuffer.from_array(ctx, target)
+    gb = pike_amd.DeviceBuffer(ctx, target.nbytes)
+    loss = f.loss_l2(tb, gb)
+    ref = ((img[..., :3].astype(np.float64) - target[..., :3]) ** 2).sum()
+    assert abs(loss - ref) <= 1e-6 * ref
+    dimg = gb.read(np.float32, H * W * 4).reshape(H, W, 4)
+    np.testing.assert_allclose(dimg[..., :3], 2 * (img[..., :3] - target[..., :3]), rtol=0, atol=1e-6)
+    assert np.all(dimg[..., 3] == 0)
+    na, ne = s.param_count(ATLAS), s.param_count(ENV)
+    ab, eb = pike_amd.DeviceBuffer(ctx, na * 4), pike_amd.DeviceBuffer(ctx, ne * 4)
+    f.adjoint_device(s, gb, ab, eb, max_depth=DEPTH, num_frames=SPP)
+    a1, e1 = ab.read(np.float32, na), eb.read(np.float32, ne)
+    ha, he = f.adjoint(s, dimg, max_depth=DEPTH, num_frames=SPP)
+    tol = 1e-5 * max(np.abs(a1).max(), np.abs(e1).max())
+    np.testing.assert_allclose(a1, ha.ravel(), rtol=1e-4, atol=tol)
+    np.testing.assert_allclose(e1, he.ravel(), rtol=1e-4, atol=tol)
+    f.adjoint_device(s, gb, ab, eb, max_depth=DEPTH, num_frames=SPP, accumulate=True)
+    np.testing.assert_allclose(ab.read(np.float32, na), 2 * a1, rtol=1e-4, atol=2 * tol)
+    np.testing.assert_allclose(eb.read(np.float32, ne), 2 * e1, rtol=1e-4, atol=2 * tol)
+
+
+def test_param_writes_are_validated(ctx, c5):
+    data, bvh, cam, tex, env, o, d_image = c5
+    s = pike_amd.Scene(ctx, data, bvh)
+    with pytest.raises(pike_amd.PikeError):
+        s.write_params(ATLAS, np.zeros(8, np.float32), offset=s.param_count(ATLAS) - 4)
+    with pytest.raises(pike_amd.PikeError):
+        s.write_params(ENV, np.zeros(8, np.float32), offset=s.param_count(ENV))
+    rgba8 = scenes.config1(seed=1)
+    b1 = pike_amd.BVH(rgba8.positions, rgba8.triangles)
+    s8 = pike_amd.Scene(ctx, rgba8, b1)
+    with pytest.raises(pike_amd.PikeError):
+        s8.write_params(ATLAS, np.zeros(4, np.float32))
