@@ -67,6 +67,8 @@ def run_render(args, pike_amd, scenes):
     W, H, depth = args.width, args.height, args.depth
     frame = pike_amd.Frame(ctx, W, H)
     frame.set_camera(cam)
+    if args.spp_per_pass > 1:
+        frame.set_samples_per_pass(args.spp_per_pass)
     tuned = None
     if args.tuning:
         for cls, shape in json.loads(args.tuning).items():
@@ -108,7 +110,7 @@ def run_render(args, pike_amd, scenes):
     frame.count_nodes(False)
     frame.reset()
     ms = frame.profile(scene, max_depth=depth, first_sample=0, num_frames=pf)
-    launches = pf * depth
+    launches = -(-pf // max(1, args.spp_per_pass)) * depth
     bytes_launch = traversal_bytes(cc["rays_traced"], cc["nodes_visited"], cc["tris_tested"]) / launches
     t_launch = ms["trace_closest"] / launches * 1e-3
     achieved = bytes_launch / t_launch / 1e9
@@ -175,6 +177,146 @@ def run_batching(args, pike_amd, scenes, ctx):
                depth=depth, scenes=len(sc), views=len(views))
     log(f"batching: plain {out['plain_msamples_s']:.0f} Msamples/s, batched {out['batched_msamples_s']:.0f} "
         f"Msamples/s ({out['speedup']:.2f}x), identical images: {same}")
+    return out
+
+
+def run_diff(args, pike_amd, scenes):
+    """Config 5 (differentiation): 196,000 triangles (terrain + 50 icospheres),
+    512x512 RGBA32F atlas whose base color (RGB) and roughness (G) layers and
+    the 512x256 environment are the parameters; target = ground truth at 256
+    spp; parameters start at ground truth + N(0, 0.1), clamped.  One iteration
+    = 16-spp forward render (graph) + L2 loss/gradient kernel + path-replay
+    adjoint over 16 independent samples, all device-resident.  Checked on 64
+    random texels (base color, roughness, environment) against central
+    differences of the GPU forward (eps 1e-3, common seeds) and against the
+    oracle's FP64 dual-number gradient of the same estimator."""
+    import numpy as np
+    from pike_amd import _ffi
+
+    ATLAS, ENV = _ffi.PK_PARAM_ATLAS, _ffi.PK_PARAM_ENV
+    t0 = time.time()
+    data = scenes.config5(seed=5)
+    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1))
+    cam = scenes.normalized_camera(data.camera, bvh.normalizer())
+    ctx = pike_amd.Context(0)
+    scene = pike_amd.Scene(ctx, data, bvh)
+    W = H = args.diff_res
+    spp, depth = args.diff_spp, args.diff_depth
+    frame = pike_amd.Frame(ctx, W, H)
+    frame.set_camera(cam)
+    frame.set_samples_per_pass(spp)  # the 16 samples of an iteration are one wavefront
+    frame.render(scene, max_depth=depth, num_frames=args.diff_target_spp)
+    target = pike_amd.DeviceBuffer(ctx, W * H * 16)
+    frame.read_image_device(target)
+    ctx.sync()
+    log(f"config5: {data.num_triangles} triangles, target at {args.diff_target_spp} spp ({time.time() - t0:.1f}s)")
+    rng = np.random.default_rng(55)
+    tex = data.tex.copy()
+    tex[0, ..., :3] = np.clip(tex[0, ..., :3] + rng.normal(0, 0.1, tex[0, ..., :3].shape), 0.02, 1.0)
+    tex[1, ..., 1] = np.clip(tex[1, ..., 1] + rng.normal(0, 0.1, tex[1, ..., 1].shape), 0.05, 1.0)
+    info = scene.param_info()
+    eh, ew = info["env_height"], info["env_width"]
+    env = np.zeros((eh, ew, 4), np.float32)
+    env[..., :3] = np.maximum(data.env.reshape(eh, ew, 3) * (1.0 + rng.normal(0, 0.1, (eh, ew, 3))), 0.0)
+    scene.write_params(ATLAS, tex.astype(np.float32), also_sampling=True)
+    scene.write_params(ENV, env)
+    na, ne = scene.param_count(ATLAS), scene.param_count(ENV)
+    dimg = pike_amd.DeviceBuffer(ctx, W * H * 16)
+    ga, ge = pike_amd.DeviceBuffer(ctx, na * 4), pike_amd.DeviceBuffer(ctx, ne * 4)
+
+    def forward():
+        frame.reset()
+        frame.render(scene, max_depth=depth, num_frames=spp)
+
+    def iteration():
+        forward()
+        frame.loss_l2(target, dimg, want_loss=False)
+        frame.adjoint_device(scene, dimg, ga, ge, max_depth=depth, first_sample=spp, num_frames=spp)
+
+    for _ in range(2):
+        iteration()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.diff_iters):
+        iteration()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(args.diff_iters):
+        forward()
+    ctx.sync()
+    t_fwd = (time.perf_counter() - t0) / args.diff_iters
+    t0 = time.perf_counter()
+    for _ in range(args.diff_iters):
+        frame.adjoint_device(scene, dimg, ga, ge, max_depth=depth, first_sample=spp, num_frames=spp)
+    ctx.sync()
+    t_adj = (time.perf_counter() - t0) / args.diff_iters
+    out = dict(iters_per_s=args.diff_iters / dt, ms_per_iter=dt / args.diff_iters * 1e3, forward_ms=t_fwd * 1e3,
+               adjoint_ms=t_adj * 1e3, adjoint_mpaths_s=W * H * spp / t_adj / 1e6, resolution=[W, H], spp=spp,
+               depth=depth, triangles=data.num_triangles,
+               params=int(info["tex_width"] * info["tex_height"] * 4 + eh * ew * 3))
+    log(f"differentiation: {out['iters_per_s']:.1f} iters/s (forward {out['forward_ms']:.2f} ms, adjoint "
+        f"{out['adjoint_ms']:.2f} ms)")
+    if args.no_diff_checks:
+        return out
+    # gradient checks: J = sum(d * image) with d = dLoss/dImage at the start,
+    # differentiated on the forward's own samples
+    forward()
+    frame.loss_l2(target, dimg, want_loss=False)
+    d = dimg.read(np.float32, W * H * 4).reshape(H, W, 4)
+    gatl, genv = frame.adjoint(scene, d, max_depth=depth, num_frames=spp)
+    prng = np.random.default_rng(66)
+    params = []
+    for _ in range(24):
+        y, x = prng.integers(0, info["tex_height"]), prng.integers(0, info["tex_width"])
+        params.append((ATLAS, int(((0 * info["tex_height"] + y) * info["tex_width"] + x) * 4 + prng.integers(0, 3))))
+    for _ in range(16):
+        y, x = prng.integers(0, info["tex_height"]), prng.integers(0, info["tex_width"])
+        params.append((ATLAS, int(((1 * info["tex_height"] + y) * info["tex_width"] + x) * 4 + 1)))
+    for _ in range(24):
+        y, x = prng.integers(0, eh), prng.integers(0, ew)
+        params.append((ENV, int((y * ew + x) * 4 + prng.integers(0, 3))))
+    g = np.array([(gatl.ravel() if w == ATLAS else genv.ravel())[i] for w, i in params], np.float64)
+    d3 = d[..., :3].astype(np.float64)
+    fd, noise = [], []
+    flat = {ATLAS: tex.ravel(), ENV: env.ravel()}
+    for which, off in params:
+        v = float(flat[which][off])
+        hi, lo = np.float32(v + 1e-3), np.float32(v - 1e-3)
+        imgs = []
+        for val in (hi, lo):
+            scene.write_params(which, [val], off)
+            forward()
+            imgs.append(frame.read_image()[..., :3])
+        scene.write_params(which, [np.float32(v)], off)
+        den = float(hi) - float(lo)
+        fd.append(float((d3 * imgs[0]).sum() - (d3 * imgs[1]).sum()) / den)
+        ulps = 2 * spp * (np.spacing(np.abs(imgs[0])) + np.spacing(np.abs(imgs[1])))
+        noise.append(float((np.abs(d3) * ulps * (imgs[0] != imgs[1])).sum()) / den)
+    fd, noise = np.array(fd), np.array(noise)
+    nz = fd != 0
+    rel = np.abs(g - fd)[nz] / np.abs(fd[nz])
+    out["fd_check"] = dict(texels=len(params), nonzero=int(nz.sum()), zero_agree=bool(np.all(g[~nz] == 0)),
+                           within_2pct=int((rel <= 0.02).sum()), median_rel=float(np.median(rel)) if rel.size else None,
+                           within_2pct_plus_rounding=int((np.abs(g - fd) <= 0.02 * np.abs(fd) + noise).sum()))
+    log(f"FD check: {out['fd_check']}")
+    if not args.no_cpu:
+        from oracle.oracle import OracleScene
+
+        t0 = time.time()
+        orc = OracleScene(data, threads=min(16, os.cpu_count() or 1))
+        orc.build_bvh()
+        orc.write_params(ATLAS, tex.astype(np.float32), also_sampling=True)
+        orc.write_params(ENV, env)
+        for prec in (64, 32):
+            t0 = time.time()
+            gr, _ = orc.gradient(cam, W, H, depth, d, params, num_frames=spp, precision=prec)
+            nzr = gr != 0
+            out[f"fp{prec}_check"] = dict(rel_l1=float(np.abs(g - gr).sum() / np.abs(gr).sum()),
+                                          max_rel=float((np.abs(g - gr)[nzr] / np.abs(gr[nzr])).max()),
+                                          median_rel=float(np.median(np.abs(g - gr)[nzr] / np.abs(gr[nzr]))),
+                                          zero_agree=bool(np.all(g[~nzr] == 0)), cpu_s=time.time() - t0)
+            log(f"FP{prec} dual-number check: {out[f'fp{prec}_check']}")
     return out
 
 
@@ -255,7 +397,16 @@ def main():
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
     ap.add_argument("--trace-iters", type=int, default=5)
-    ap.add_argument("--mode", choices=["full", "render-only", "copy"], default="full",
+    ap.add_argument("--spp-per-pass", type=int, default=1,
+                    help="samples per pixel traced per wavefront pass of the 1080p render")
+    ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-diff-checks", action="store_true")
+    ap.add_argument("--diff-res", type=int, default=512)
+    ap.add_argument("--diff-spp", type=int, default=16)
+    ap.add_argument("--diff-depth", type=int, default=4)
+    ap.add_argument("--diff-target-spp", type=int, default=256)
+    ap.add_argument("--diff-iters", type=int, default=20)
+    ap.add_argument("--mode", choices=["full", "render-only", "copy", "diff-only"], default="full",
                     help="render-only: the timed frames alone (for rocprofv3 runs); copy: a 1 GiB device copy "
                          "through the library's streaming copy kernel (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -275,10 +426,14 @@ def main():
         ctx.sync()
         print(json.dumps({"mode": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes}))
         return
+    if args.mode == "diff-only":
+        print(json.dumps({"mode": "diff-only", "differentiation_c5": run_diff(args, pike_amd, scenes)}))
+        return
     if args.mode == "render-only":
         args.no_cpu = True
         args.no_traversal = True
         args.no_batch = True
+        args.no_diff = True
         args.profile_frames = 0
 
     res, data, bvh, cam = run_render(args, pike_amd, scenes)
@@ -296,6 +451,8 @@ def main():
             for cls, shape in res["autotune"]["shapes"].items():
                 bctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
         extra["batching_32_scenes_64_views"] = run_batching(args, pike_amd, scenes, bctx)
+    if not args.no_diff:
+        extra["differentiation_c5"] = run_diff(args, pike_amd, scenes)
     roof = {k: res["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
     line = {
         "metric": "spp/s at 1920x1080 (plain wavefront, depth 8)",

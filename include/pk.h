@@ -199,6 +199,12 @@ int pk_trace_shadow(pk_frame* frame, pk_scene* scene);     /* TraceShadowPass + 
 int pk_accumulate(pk_frame* frame);                        /* AccumulatePass */
 /* A whole run of frames (tick() x num_frames), optionally graph-replayed. */
 int pk_render(pk_frame* frame, pk_scene* scene, const pk_render_params* params);
+/* Samples per pass (no reference counterpart; default 1): each pass of
+ * pk_render and of the per-pass entry points traces `samples` consecutive
+ * samples of every pixel as one wavefront (width*height*samples paths, below
+ * 2^28), so small frames fill the GPU.  The accumulated image equals that of
+ * one-sample frames bit for bit.  Reallocates the frame's queues. */
+int pk_frame_set_samples_per_pass(pk_frame* frame, uint32_t samples);
 /* Running-mean image, float RGBA, width*height*16 bytes. */
 int pk_read_image(pk_frame* frame, float* rgba, size_t bytes);
 /* The same image copied to device memory (asynchronous, context stream). */

@@ -376,6 +376,10 @@ class Frame:
     def set_env_theta(self, theta):
         _check(lib().pk_frame_set_env_theta(self.h, float(theta)), "pk_frame_set_env_theta")
 
+    def set_samples_per_pass(self, samples):
+        """Trace `samples` samples of every pixel per pass (one wavefront)."""
+        _check(lib().pk_frame_set_samples_per_pass(self.h, int(samples)), "pk_frame_set_samples_per_pass")
+
     def reset(self):
         _check(lib().pk_frame_reset(self.h), "pk_frame_reset")
 

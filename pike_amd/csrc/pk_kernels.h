@@ -34,9 +34,10 @@ enum : int {
 };
 
 struct FrameParams {
-  uint32_t sample;
+  uint32_t sample;  // first sample of the current pass (RenderState.samples)
   float env_theta;
   pk_camera cam;
+  uint32_t limit;   // samples >= limit are not traced (end of a pk_render run)
 };
 
 // One camera view of a batch: its pixels occupy [offset, offset + W*H) of
@@ -65,7 +66,15 @@ struct FrameDev {
   uint32_t W, H;   // single view: the resolution; batch: W = total paths, H = 1
   const ViewDev* views;  // batch views sorted by offset (nullptr: single view)
   uint32_t nviews;
+  // Samples per pass: a single-view pass traces samples sample .. sample+S-1
+  // of every pixel as one wavefront, path id = s * W * H + pixel (batches use 1).
+  uint32_t S;
 };
+
+// samples the current pass traces (fewer than S at the end of a run)
+__device__ inline uint32_t pass_samples(const FrameDev& F, const FrameParams& P) {
+  return P.limit > P.sample ? min(F.S, P.limit - P.sample) : 0u;
+}
 
 // view of global path index p (batch mode): views are sorted by offset
 __device__ inline uint32_t view_of(const FrameDev& F, uint32_t p) {
@@ -78,7 +87,8 @@ __device__ inline uint32_t view_of(const FrameDev& F, uint32_t p) {
   return lo;
 }
 
-void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s);
+void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s,
+                  uint32_t limit = 0xffffffffu);
 void launch_advance(const FrameDev& F, hipStream_t s);
 void launch_raygen(const FrameDev& F, hipStream_t s);
 // persistent trace launch shape: blocks of one wave, lane refill threshold

@@ -76,18 +76,23 @@ __device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t
   *o_out = pkm::add(dof, org);
 }
 
+// One thread per path of the pass: path idx = s * W * H + pixel traces
+// sample P.sample + s of that pixel; the paths of samples at or beyond the
+// run's limit are the tail of the index range and are not emitted.
 __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   const uint32_t W = F.W, H = F.H, B = W * H;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   const FrameParams P = *F.params;
+  const uint32_t n = B * pass_samples(F, P);
   if (idx == 0) {
-    F.cnt[kCntRays0] = B;
+    F.cnt[kCntRays0] = n;
     F.cnt[kCntHits] = 0;
     F.cnt[kCntWorkTrace] = 0;
   }
-  if (idx >= B) return;
+  if (idx >= n) return;
+  const uint32_t s = idx / B, pixel = idx - s * B;
   v3 o, d;
-  camera_ray(idx % W, idx / W, W, H, P.sample, P.cam, &o, &d);
+  camera_ray(pixel % W, pixel / W, W, H, P.sample + s, P.cam, &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
   F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -617,11 +622,14 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
     const float4 o4 = F.qo[in][slot], d4 = F.qd[in][slot];
     pid = __float_as_uint(o4.w);
     view_bits = __float_as_uint(d4.w);
-    // the pixel index the seeds use is local to the view
-    const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid;
+    // the pixel index the seeds use is local to the view; a single-view
+    // pass holds samples P.sample .. P.sample + S - 1, one per W*H paths
+    const uint32_t B = F.W * F.H;
+    const uint32_t s = F.views ? 0u : pid / B;
+    const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid - s * B;
     const float4 t4 = F.thr[pid];
     shade_hit<false>(S, P.env_theta, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), __uint_as_float(hr.z),
-                     hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample), pkm::mk(t4.x, t4.y, t4.z), o,
+                     hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample + s), pkm::mk(t4.x, t4.y, t4.z), o,
                      nullptr);
     if (o.emit) add_color(F, pid, o.L);
   }
@@ -689,35 +697,43 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
 }
 
 // ---------------------------------------------------------------------------
-// accumulate.wgsl main (:20-34): running mean over frames
+// accumulate.wgsl main (:20-34): running mean over frames.  A pass of S
+// samples folds them in sample order with the same per-sample update, so the
+// mean equals S one-sample frames bit for bit.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
   const uint32_t B = F.W * F.H;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= B) return;
+  const FrameParams P = *F.params;
   // batch: views whose sample budget is spent this round keep their mean
-  if (F.views && F.params->sample >= F.views[view_of(F, idx)].spp) return;
-  const float s = (float)F.params->sample;
-  const float s1 = (float)(F.params->sample + 1u);
-  const float4 c = F.color[idx];
+  if (F.views && P.sample >= F.views[view_of(F, idx)].spp) return;
+  const uint32_t ns = pass_samples(F, P);
   float4 a = F.accum[idx];
-  a.x = (pkm::max_(c.x, 0.0f) + a.x * s) / s1;
-  a.y = (pkm::max_(c.y, 0.0f) + a.y * s) / s1;
-  a.z = (pkm::max_(c.z, 0.0f) + a.z * s) / s1;
+  for (uint32_t k = 0; k < ns; ++k) {
+    const float s = (float)(P.sample + k);
+    const float s1 = (float)(P.sample + k + 1u);
+    const float4 c = F.color[(size_t)k * B + idx];
+    a.x = (pkm::max_(c.x, 0.0f) + a.x * s) / s1;
+    a.y = (pkm::max_(c.y, 0.0f) + a.y * s) / s1;
+    a.z = (pkm::max_(c.z, 0.0f) + a.z * s) / s1;
+  }
   a.w = 1.0f;
   F.accum[idx] = a;
 }
 
 // Frame bookkeeping kernels (no memset/memcpy nodes, so frames can be
 // captured and replayed as one graph).
-__global__ void k_begin(FrameDev F, uint32_t sample, float theta, pk_camera cam) {
+__global__ void k_begin(FrameDev F, uint32_t sample, float theta, pk_camera cam, uint32_t limit) {
   F.params->sample = sample;
   F.params->env_theta = theta;
   F.params->cam = cam;
+  F.params->limit = limit;
 }
 __global__ void k_advance(FrameDev F) {
-  F.params->sample += 1u;
-  F.stats[kStatFrames] += 1ull;
+  const FrameParams P = *F.params;
+  F.stats[kStatFrames] += (unsigned long long)(F.views ? 1u : pass_samples(F, P));
+  F.params->sample = P.sample + F.S;
 }
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1008,8 +1024,9 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_rays(DevScene 
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s) {
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, s, F, sample, theta, cam);
+void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_camera& cam, hipStream_t s,
+                  uint32_t limit) {
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, s, F, sample, theta, cam, limit);
 }
 void launch_advance(const FrameDev& F, hipStream_t s) { hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, F); }
 void launch_raygen(const FrameDev& F, hipStream_t s) {
@@ -1017,7 +1034,7 @@ void launch_raygen(const FrameDev& F, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_round_begin, dim3(1), dim3(1), 0, s, F);
     hipLaunchKernelGGL(k_raygen_batch, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
   } else {
-    hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+    hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, F);
   }
 }
 void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
@@ -1027,7 +1044,7 @@ void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, con
   else hipLaunchKernelGGL(k_trace_closest<false>, dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, S, F, bounce);
+  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, S, F, bounce);
 }
 void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
   FrameDev F = F0;

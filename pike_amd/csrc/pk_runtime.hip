@@ -58,6 +58,7 @@ struct pk_frame {
   uint32_t W = 0, H = 0;
   pk::FrameDev dev{};
   std::vector<void*> allocs;
+  std::vector<void*> path_allocs;  // per-path queues, sized W * H * samples per pass
   pk_camera cam{};
   bool has_camera = false;
   float theta = 0;
@@ -474,6 +475,25 @@ int pk_scene_env_info(const pk_scene* s, int64_t* info4) {
 // ---------------------------------------------------------------------------
 // frame
 // ---------------------------------------------------------------------------
+// Per-path queue buffers for `paths` paths (replacing any previous ones).
+static int alloc_paths(pk_frame* f, size_t paths) {
+  free_all(f->path_allocs);
+  pk::FrameDev& F = f->dev;
+  int rc;
+  for (int q = 0; q < 2; ++q) {
+    if ((rc = dev_alloc(f->path_allocs, &F.qo[q], paths))) return rc;
+    if ((rc = dev_alloc(f->path_allocs, &F.qd[q], paths))) return rc;
+  }
+  if ((rc = dev_alloc(f->path_allocs, &F.thr, paths))) return rc;
+  if ((rc = dev_alloc(f->path_allocs, &F.hits, paths))) return rc;
+  if ((rc = dev_alloc(f->path_allocs, &F.so, paths))) return rc;
+  if ((rc = dev_alloc(f->path_allocs, &F.sd, paths))) return rc;
+  if ((rc = dev_alloc(f->path_allocs, &F.sthr, paths))) return rc;
+  if ((rc = dev_alloc(f->path_allocs, &F.color, paths))) return rc;
+  pk::launch_fill_f4(F.color, make_float4(0, 0, 0, 0), (uint32_t)paths, f->ctx->stream);
+  return PK_OK;
+}
+
 int pk_frame_create(pk_ctx* c, uint32_t W, uint32_t H, pk_frame** out) {
   if (!c || !out) {
     pk_set_error("pk_frame_create: null argument");
@@ -495,26 +515,18 @@ int pk_frame_create(pk_ctx* c, uint32_t W, uint32_t H, pk_frame** out) {
   F.H = H;
   int rc = 0;
   auto fail = [&](int code) {
+    free_all(f->path_allocs);
     free_all(f->allocs);
     delete f;
     return code;
   };
-  for (int q = 0; q < 2; ++q) {
-    if ((rc = dev_alloc(f->allocs, &F.qo[q], B))) return fail(rc);
-    if ((rc = dev_alloc(f->allocs, &F.qd[q], B))) return fail(rc);
-  }
-  if ((rc = dev_alloc(f->allocs, &F.thr, B))) return fail(rc);
-  if ((rc = dev_alloc(f->allocs, &F.hits, B))) return fail(rc);
-  if ((rc = dev_alloc(f->allocs, &F.so, B))) return fail(rc);
-  if ((rc = dev_alloc(f->allocs, &F.sd, B))) return fail(rc);
-  if ((rc = dev_alloc(f->allocs, &F.sthr, B))) return fail(rc);
-  if ((rc = dev_alloc(f->allocs, &F.color, B))) return fail(rc);
+  F.S = 1;
+  if ((rc = alloc_paths(f, B))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.accum, B))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.cnt, pk::kNumCounters))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.stats, pk::kNumStats))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.params, 1))) return fail(rc);
   F.spill = c->spill;
-  pk::launch_fill_f4(F.color, make_float4(0, 0, 0, 0), (uint32_t)B, c->stream);
   pk::launch_fill_f4(F.accum, make_float4(0, 0, 0, 0), (uint32_t)B, c->stream);
   pk::launch_fill_u32(F.cnt, 0u, pk::kNumCounters, c->stream);
   pk::launch_fill_u32((uint32_t*)F.stats, 0u, pk::kNumStats * 2, c->stream);
@@ -540,8 +552,35 @@ int pk_frame_destroy(pk_frame* f) {
   (void)hipSetDevice(f->ctx->device);
   (void)hipStreamSynchronize(f->ctx->stream);
   drop_graph(f);
+  free_all(f->path_allocs);
   free_all(f->allocs);
   delete f;
+  return PK_OK;
+}
+
+int pk_frame_set_samples_per_pass(pk_frame* f, uint32_t samples) {
+  if (!f || samples == 0 || f->batch_scene) {
+    pk_set_error("pk_frame_set_samples_per_pass: null frame, zero samples or a batch frame");
+    return PK_E_INVALID;
+  }
+  const uint64_t paths = (uint64_t)f->W * f->H * samples;
+  if (paths >= (1ull << 28)) {
+    pk_set_error("pk_frame_set_samples_per_pass: width*height*samples must stay below 2^28 paths");
+    return PK_E_LIMIT;
+  }
+  if (samples == f->dev.S) return PK_OK;
+  PK_HIP(hipSetDevice(f->ctx->device));
+  PK_HIP(hipStreamSynchronize(f->ctx->stream));
+  drop_graph(f);
+  f->dev.S = samples;
+  int rc = alloc_paths(f, (size_t)paths);
+  if (rc) {
+    // keep the frame usable: fall back to one sample per pass
+    f->dev.S = 1;
+    int rc1 = alloc_paths(f, (size_t)f->W * f->H);
+    return rc1 ? rc1 : rc;
+  }
+  PK_HIP(hipStreamSynchronize(f->ctx->stream));
   return PK_OK;
 }
 
@@ -552,7 +591,7 @@ int pk_frame_reset(pk_frame* f) {
   }
   const uint32_t B = f->W * f->H;
   pk::launch_fill_f4(f->dev.accum, make_float4(0, 0, 0, 0), B, f->ctx->stream);
-  pk::launch_fill_f4(f->dev.color, make_float4(0, 0, 0, 0), B, f->ctx->stream);
+  pk::launch_fill_f4(f->dev.color, make_float4(0, 0, 0, 0), B * (f->batch_scene ? 1u : f->dev.S), f->ctx->stream);
   pk::launch_fill_u32((uint32_t*)f->dev.stats, 0u, pk::kNumStats * 2, f->ctx->stream);
   pk::launch_fill_u32(f->dev.cnt, 0u, pk::kNumCounters, f->ctx->stream);
   PK_HIP(hipGetLastError());
@@ -677,12 +716,18 @@ int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
     pk_set_error("pk_render: max_depth above 64");
     return PK_E_LIMIT;
   }
+  if (p->first_sample + p->num_frames < p->first_sample) {
+    pk_set_error("pk_render: first_sample + num_frames overflows");
+    return PK_E_LIMIT;
+  }
   hipStream_t st = f->ctx->stream;
   f->theta = p->env_theta;
-  pk::launch_begin(f->dev, p->first_sample, p->env_theta, f->cam, st);
+  // one pass traces S samples per pixel; the last pass stops at the limit
+  const uint32_t passes = (p->num_frames + f->dev.S - 1) / f->dev.S;
+  pk::launch_begin(f->dev, p->first_sample, p->env_theta, f->cam, st, p->first_sample + p->num_frames);
   f->begun = true;
   if (!p->use_graph) {
-    for (uint32_t i = 0; i < p->num_frames; ++i) enqueue_frame(f, s, p->max_depth, st);
+    for (uint32_t i = 0; i < passes; ++i) enqueue_frame(f, s, p->max_depth, st);
     PK_HIP(hipGetLastError());
     return PK_OK;
   }
@@ -702,7 +747,7 @@ int pk_render(pk_frame* f, pk_scene* s, const pk_render_params* p) {
     f->graph_depth = p->max_depth;
     f->graph_gen = f->ctx->tuning_gen;
   }
-  for (uint32_t i = 0; i < p->num_frames; ++i) PK_HIP(hipGraphLaunch(f->exec, st));
+  for (uint32_t i = 0; i < passes; ++i) PK_HIP(hipGraphLaunch(f->exec, st));
   return PK_OK;
 }
 
@@ -909,16 +954,17 @@ int pk_render_profile(pk_frame* f, pk_scene* s, const pk_render_params* p, doubl
   }
   hipStream_t st = f->ctx->stream;
   std::vector<hipEvent_t> ev;
-  const uint32_t per_frame = 2 + 3 * p->max_depth;  // launches timed per frame
-  const size_t nev = (size_t)p->num_frames * per_frame * 2;
+  const uint32_t per_pass = 2 + 3 * p->max_depth;  // launches timed per pass
+  const uint32_t passes = (p->num_frames + f->dev.S - 1) / f->dev.S;
+  const size_t nev = (size_t)passes * per_pass * 2;
   ev.resize(nev);
   for (auto& e : ev) PK_HIP(hipEventCreate(&e));
   for (int k = 0; k < 5; ++k) ms5[k] = 0;
-  pk::launch_begin(f->dev, p->first_sample, p->env_theta, f->cam, st);
+  pk::launch_begin(f->dev, p->first_sample, p->env_theta, f->cam, st, p->first_sample + p->num_frames);
   f->begun = true;
   size_t ei = 0;
   std::vector<int> kind;
-  for (uint32_t fr = 0; fr < p->num_frames; ++fr) {
+  for (uint32_t fr = 0; fr < passes; ++fr) {
     auto timed = [&](int k, auto fn) {
       (void)hipEventRecord(ev[ei++], st);
       fn();

@@ -104,3 +104,30 @@ def test_empty_scene_region_is_env(ctx):
     fr.render(scene, max_depth=2, num_frames=1)
     img = fr.read_image()
     np.testing.assert_array_equal(img[..., :3], np.ones_like(img[..., :3]))
+
+
+@pytest.mark.parametrize("samples,frames", [(4, 8), (3, 7), (16, 5)])
+def test_samples_per_pass_equals_one_sample_frames(ctx, samples, frames):
+    """A pass of S samples per pixel (one wavefront of W*H*S paths) folds its
+    samples in order: the image equals one-sample frames bit for bit, also
+    when the last pass is partial, and equals the FP32 oracle."""
+    env = scenes.lognormal_sun_env(64, 32, seed=5, sigma=1.0, sun_deg=6.0, sun_scale=30.0)
+    data = scenes.mixed_scene(7, 4000, env=env)
+    data.camera = scenes.camera((0.5, 0.5, 1.4), (0.5, 0.05, 0.45), 1.0)
+    bvh, scene, orc, cam = _setup(ctx, data)
+    W, H, depth = 40, 24, 4
+    ref = pike_amd.Frame(ctx, W, H)
+    ref.set_camera(cam)
+    ref.render(scene, max_depth=depth, first_sample=2, num_frames=frames)
+    fr = pike_amd.Frame(ctx, W, H)
+    fr.set_camera(cam)
+    fr.set_samples_per_pass(samples)
+    for use_graph in (True, False):
+        fr.reset()
+        fr.render(scene, max_depth=depth, first_sample=2, num_frames=frames, use_graph=use_graph)
+        np.testing.assert_array_equal(fr.read_image().view(np.uint32), ref.read_image().view(np.uint32))
+        c = fr.counters()
+        assert c["frames"] == frames and c["stack_overflow"] == 0
+    acc_o, _, _ = orc.render(cam, W, H, depth, 2, frames, precision=32, accum=np.zeros((H, W, 4), np.float32))
+    # the run started at sample 2 on a cleared accumulator, as the frames did
+    np.testing.assert_array_equal(fr.read_image().view(np.uint32), acc_o.view(np.uint32))
