@@ -2,11 +2,13 @@
 
 Workload (the plain method at scale): 2,097,152-triangle fBm terrain plus 200
 subdivision-4 icospheres (3,121,152 triangles), 2048x1024 log-normal HDR
-environment with a 0.5-degree sun, importance-sampled; 1920x1080, depth 8,
-FP32.  One step = one frame = one sample per pixel through
+environment with a 0.5-degree sun, importance-sampled via the 2D
+(marginal/conditional) CDF; 1920x1080, depth 8, FP32.  One step = one sample per pixel of the frame through
 raygen -> {closest hit -> shade -> shadow any-hit} x 8 -> accumulate,
-replayed from one captured hipGraph.  value = samples per pixel per second
-(frames/s at 1080p); scene, BVH and queues are resident in HBM before timing.
+replayed from one captured hipGraph; the autotuner may trace several samples
+per pixel in one wavefront pass (the image equals one-sample frames bit for
+bit).  value = samples per pixel per second (frames/s at 1080p); scene, BVH
+and queues are resident in HBM before timing.
 
 Also reported: traversal microbenchmark (10M-triangle soup, 16M incoherent
 rays, closest and any hit, Mrays/s), the roofline of the dominant kernel
@@ -71,14 +73,23 @@ def run_render(args, pike_amd, scenes):
         frame.set_samples_per_pass(args.spp_per_pass)
     tuned = None
     if args.tuning:
-        for cls, shape in json.loads(args.tuning).items():
-            ctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
-        tuned = dict(shapes=ctx.tuning(), source="--tuning")
+        cfg = json.loads(args.tuning)
+        if "samples_per_pass" in cfg:
+            args.spp_per_pass = int(cfg["samples_per_pass"])
+            frame.set_samples_per_pass(args.spp_per_pass)
+        for cls, shape in cfg.get("shapes", cfg).items():
+            if isinstance(shape, dict):
+                ctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+        tuned = dict(shapes=ctx.tuning(), samples_per_pass=frame.samples_per_pass, source="--tuning")
     elif not args.no_autotune:
         t0 = time.time()
+        s_before, s_after = frame.autotune_samples(scene, max_depth=depth, max_samples=args.max_spp_per_pass)
+        args.spp_per_pass = frame.samples_per_pass
+        log(f"autotune samples per pass: {s_before:.2f} -> {s_after:.2f} ms/sample at {args.spp_per_pass}")
         before, after = frame.autotune(scene, max_depth=depth, frames=2)
-        tuned = dict(ms_before=before, ms_after=after, shapes=ctx.tuning(), seconds=time.time() - t0)
-        log(f"autotune: traversal {before:.2f} -> {after:.2f} ms/frame, {ctx.tuning()} ({tuned['seconds']:.1f}s)")
+        tuned = dict(ms_before=before, ms_after=after, shapes=ctx.tuning(), samples_per_pass=args.spp_per_pass,
+                     ms_per_sample_s1=s_before, ms_per_sample_chosen=s_after, seconds=time.time() - t0)
+        log(f"autotune: traversal {before:.2f} -> {after:.2f} ms/pass, {ctx.tuning()} ({tuned['seconds']:.1f}s)")
     # warmup (also instantiates the graph)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=max(1, args.warmup), use_graph=True)
     ctx.sync()
@@ -95,8 +106,12 @@ def run_render(args, pike_amd, scenes):
     rays_total = cnt["rays_traced"] + cnt["shadow_rays"]
     log(f"render: {args.steps} frames in {dt:.3f}s = {spp_s:.2f} spp/s, {rays_total / dt / 1e6:.0f} Mrays/s, "
         f"overflow={cnt['stack_overflow']}")
-    # kernel-level profile: counted run for algorithmic bytes, uncounted for time
+    # kernel-level profile: counted run for algorithmic bytes, uncounted for
+    # time, over whole passes
     pf = args.profile_frames
+    if pf > 0:
+        sp = max(1, args.spp_per_pass)
+        pf = -(-pf // sp) * sp
     if pf <= 0:
         return dict(spp_s=spp_s, ms_per_step=dt / args.steps * 1e3, rays_per_s=rays_total / dt, counters=cnt,
                     kernel_ms=None, profile_frames=0, profile_counters=None,
@@ -155,6 +170,21 @@ def run_batching(args, pike_amd, scenes, ctx):
             frames[i].render(per_scene[k], max_depth=depth, first_sample=0, num_frames=spp, use_graph=True)
         ctx.sync()
 
+    # the plain method with multi-sample passes: each view traces up to
+    # 2^22 paths per pass
+    frames_ms = []
+    for i, (k, w, h, spp, _) in enumerate(views):
+        f = pike_amd.Frame(ctx, w, h)
+        f.set_camera(cams[i])
+        f.set_samples_per_pass(max(1, min(spp, (1 << 22) // (w * h))))
+        frames_ms.append(f)
+
+    def plain_ms():
+        for i, (k, w, h, spp, _) in enumerate(views):
+            frames_ms[i].reset()
+            frames_ms[i].render(per_scene[k], max_depth=depth, first_sample=0, num_frames=spp, use_graph=True)
+        ctx.sync()
+
     batch = pike_amd.Batch(ctx, merged, [(k, w, h, spp, cams[i]) for i, (k, w, h, spp, _) in enumerate(views)])
 
     def batched():
@@ -163,20 +193,28 @@ def run_batching(args, pike_amd, scenes, ctx):
         ctx.sync()
 
     plain()
+    plain_ms()
     batched()  # warm-up: instantiates every graph
     t0 = time.perf_counter()
     plain()
     t_plain = time.perf_counter() - t0
     t0 = time.perf_counter()
+    plain_ms()
+    t_plain_ms = time.perf_counter() - t0
+    t0 = time.perf_counter()
     batched()
     t_batch = time.perf_counter() - t0
-    same = all(np.array_equal(batch.read_view(i).view(np.uint32), frames[i].read_image().view(np.uint32))
+    same = all(np.array_equal(batch.read_view(i).view(np.uint32), frames[i].read_image().view(np.uint32)) and
+               np.array_equal(frames_ms[i].read_image().view(np.uint32), frames[i].read_image().view(np.uint32))
                for i in range(len(views)))
-    out = dict(samples=samples, plain_s=t_plain, batched_s=t_batch, plain_msamples_s=samples / t_plain / 1e6,
-               batched_msamples_s=samples / t_batch / 1e6, speedup=t_plain / t_batch, images_identical=same,
+    out = dict(samples=samples, plain_s=t_plain, plain_multisample_s=t_plain_ms, batched_s=t_batch,
+               plain_msamples_s=samples / t_plain / 1e6, plain_multisample_msamples_s=samples / t_plain_ms / 1e6,
+               batched_msamples_s=samples / t_batch / 1e6, speedup=t_plain / t_batch,
+               speedup_vs_plain_multisample=t_plain_ms / t_batch, images_identical=same,
                depth=depth, scenes=len(sc), views=len(views))
-    log(f"batching: plain {out['plain_msamples_s']:.0f} Msamples/s, batched {out['batched_msamples_s']:.0f} "
-        f"Msamples/s ({out['speedup']:.2f}x), identical images: {same}")
+    log(f"batching: plain {out['plain_msamples_s']:.0f} Msamples/s, plain multi-sample "
+        f"{out['plain_multisample_msamples_s']:.0f}, batched {out['batched_msamples_s']:.0f} Msamples/s "
+        f"({out['speedup']:.2f}x / {out['speedup_vs_plain_multisample']:.2f}x), identical images: {same}")
     return out
 
 
@@ -398,7 +436,8 @@ def main():
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
     ap.add_argument("--trace-iters", type=int, default=5)
     ap.add_argument("--spp-per-pass", type=int, default=1,
-                    help="samples per pixel traced per wavefront pass of the 1080p render")
+                    help="samples per pixel traced per wavefront pass of the 1080p render (without autotuning)")
+    ap.add_argument("--max-spp-per-pass", type=int, default=8, help="autotuner's largest samples per pass")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-diff-checks", action="store_true")
     ap.add_argument("--diff-res", type=int, default=512)
@@ -467,8 +506,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural terrain + icospheres, log-normal HDR env with sun; seeded PCG)",
-        "config": {"workload": "plain wavefront path tracing, 3,121,152 triangles, 2048x1024 env, 1920x1080, "
-                               "depth 8, 1 spp per step", "width": args.width, "height": args.height,
+        "config": {"workload": "plain wavefront path tracing, 3,121,152 triangles, 2048x1024 env (2D CDF "
+                               "importance sampling), 1920x1080, depth 8, 1 spp per step", "width": args.width, "height": args.height,
                    "depth": args.depth, "triangles": res["triangles"]},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -141,6 +141,14 @@ int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_c
  * fastest shape per class in the context.  ms2 receives the traversal
  * milliseconds per frame before and after. */
 int pk_autotune_render(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t frames, float* ms2);
+/* Queue capacity: chooses the frame's samples per pass (1, 2, 4, ... up to
+ * max_samples, below 2^24 paths) by timing max_samples samples of
+ * graph-replayed passes of this scene, camera and resolution; leaves the
+ * fastest set and the frame reset.  ms2 = milliseconds per sample at the
+ * frame's previous setting (0 if it was not among the candidates) and at the
+ * chosen one. */
+int pk_autotune_samples_per_pass(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t max_samples,
+                                 float* ms2);
 /* The same for the standalone ray queries on device-resident rays. */
 int pk_autotune_rays(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
                      float* d_t, uint32_t* d_bary, float* ms2);
@@ -205,6 +213,7 @@ int pk_render(pk_frame* frame, pk_scene* scene, const pk_render_params* params);
  * 2^28), so small frames fill the GPU.  The accumulated image equals that of
  * one-sample frames bit for bit.  Reallocates the frame's queues. */
 int pk_frame_set_samples_per_pass(pk_frame* frame, uint32_t samples);
+int pk_frame_get_samples_per_pass(const pk_frame* frame, uint32_t* samples);
 /* Running-mean image, float RGBA, width*height*16 bytes. */
 int pk_read_image(pk_frame* frame, float* rgba, size_t bytes);
 /* The same image copied to device memory (asynchronous, context stream). */

@@ -289,6 +289,8 @@ struct or_scene {
   std::vector<double> epdf64;
   std::vector<float> erad32;    // decoded radiance rgb
   std::vector<double> erad64;
+  // marginal (rows) / conditional (columns) CDF tables of the CDF sampler
+  std::vector<float> row_cdf, col_cdf;
 };
 
 namespace {
@@ -754,6 +756,46 @@ void build_env(or_scene* s) {
       s->epdf32[idx] = (float)pdf;
     }
   }
+  if (s->esampler != PK_ENV_SAMPLER_CDF) return;
+  // CDF sampler (the product's device binary-search table; no reference
+  // counterpart): texel weight = luminance * sin(theta at the row center);
+  // rows by their sums, columns within a row by the weights; a texel's pdf is
+  // the product of its two f32 CDF steps times T / (2 pi^2) (solid angle).
+  const double pi = 3.141592653589793;
+  std::vector<double> wgt(T), rows(H, 0.0);
+  for (uint32_t y = 0; y < H; ++y) {
+    const double sy = std::sin(pi * ((double)y + 0.5) / (double)H);
+    for (uint32_t x = 0; x < W; ++x) {
+      wgt[(size_t)y * W + x] = rad[(size_t)y * W + x] * sy;
+      rows[y] += wgt[(size_t)y * W + x];
+    }
+  }
+  const double all = std::accumulate(rows.begin(), rows.end(), 0.0);
+  s->row_cdf.assign(H + 1, 0.0f);
+  s->col_cdf.assign((size_t)H * (W + 1), 0.0f);
+  double below = 0;
+  for (uint32_t y = 0; y < H; ++y) {
+    s->row_cdf[y] = (float)(below / all);
+    below += rows[y];
+    float* cc = &s->col_cdf[(size_t)y * (W + 1)];
+    double left = 0;
+    for (uint32_t x = 0; x < W; ++x) {
+      cc[x] = rows[y] > 0 ? (float)(left / rows[y]) : (float)x / (float)W;
+      left += wgt[(size_t)y * W + x];
+    }
+    cc[W] = 1.0f;
+  }
+  s->row_cdf[H] = 1.0f;
+  for (uint32_t y = 0; y < H; ++y) {
+    const double py = (double)s->row_cdf[y + 1] - (double)s->row_cdf[y];
+    const float* cc = &s->col_cdf[(size_t)y * (W + 1)];
+    for (uint32_t x = 0; x < W; ++x) {
+      const double px = (double)cc[x + 1] - (double)cc[x];
+      const double pdf = py * px * (double)T / (2 * pi * pi);
+      s->epdf64[(size_t)y * W + x] = pdf;
+      s->epdf32[(size_t)y * W + x] = (float)pdf;
+    }
+  }
 }
 
 // Forward-mode dual number for the reference gradients: a value and its
@@ -1063,13 +1105,34 @@ template <class T> Sample<T> sample_lambert(Rng& rng) {
   V3<T> d = v3(x, y, z);
   return Sample<T>{d, lambert_pdf(d)};
 }
-// sampleEnv (:213-227)
+// sampleEnv (:213-227); the CDF sampler draws its two uniforms from the same
+// two hashes and binary-searches the f32 tables (largest index whose CDF
+// value is <= the uniform)
 template <class T> Sample<T> sample_env(const Ctx<T>& c, const M3<T>& basis, Rng& rng) {
   const or_scene* s = c.s;
-  uint32_t bi = rng.hash() % (uint32_t)s->ebins.size();
-  const auto& bin = s->ebins[bi];
-  int32_t ci = (int32_t)(rng.hash() % (uint32_t)(bin[1] - bin[0])) + bin[0];
-  const auto& coord = s->ecoords[ci];
+  int32_t cx, cy;
+  if (s->esampler == PK_ENV_SAMPLER_CDF) {
+    const T u1 = mn(rng.rand<T>(), (T)0x1.fffffep-1f);
+    const T u2 = mn(rng.rand<T>(), (T)0x1.fffffep-1f);
+    auto search = [](const float* cdf, int n, T u) {
+      int lo = 0, hi = n;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((T)cdf[mid] <= u) lo = mid;
+        else hi = mid;
+      }
+      return lo;
+    };
+    cy = search(s->row_cdf.data(), (int)s->eh, u1);
+    cx = search(&s->col_cdf[(size_t)cy * (s->ew + 1)], (int)s->ew, u2);
+  } else {
+    uint32_t bi = rng.hash() % (uint32_t)s->ebins.size();
+    const auto& bin = s->ebins[bi];
+    int32_t ci = (int32_t)(rng.hash() % (uint32_t)(bin[1] - bin[0])) + bin[0];
+    cx = s->ecoords[ci][0];
+    cy = s->ecoords[ci][1];
+  }
+  const std::array<int32_t, 2> coord = {cx, cy};
   T u = -c.env_theta + ((T(0.5) + (T)coord[0]) / (T)s->ew);
   T v = (T(0.5) + (T)coord[1]) / (T)s->eh;
   T theta = u * M<T>::tau();

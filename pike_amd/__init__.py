@@ -380,6 +380,12 @@ class Frame:
         """Trace `samples` samples of every pixel per pass (one wavefront)."""
         _check(lib().pk_frame_set_samples_per_pass(self.h, int(samples)), "pk_frame_set_samples_per_pass")
 
+    @property
+    def samples_per_pass(self):
+        v = ctypes.c_uint32()
+        _check(lib().pk_frame_get_samples_per_pass(self.h, ctypes.byref(v)), "pk_frame_get_samples_per_pass")
+        return v.value
+
     def reset(self):
         _check(lib().pk_frame_reset(self.h), "pk_frame_reset")
 
@@ -414,6 +420,14 @@ class Frame:
         (resets the accumulation).  Returns (ms/frame before, after)."""
         ms = (ctypes.c_float * 2)()
         _check(lib().pk_autotune_render(self.h, scene.h, max_depth, frames, ms), "pk_autotune_render")
+        return float(ms[0]), float(ms[1])
+
+    def autotune_samples(self, scene, max_depth=4, max_samples=8):
+        """Choose this frame's samples per pass (queue capacity) by timing;
+        returns (ms/sample before, after, chosen samples per pass)."""
+        ms = (ctypes.c_float * 2)()
+        _check(lib().pk_autotune_samples_per_pass(self.h, scene.h, max_depth, max_samples, ms),
+               "pk_autotune_samples_per_pass")
         return float(ms[0]), float(ms[1])
 
     def profile(self, scene, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0):

@@ -174,6 +174,9 @@ SIGNATURES = {
                                    c_int, ctypes.POINTER(c_float)]),
     "pk_read_image_device": (c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
     "pk_frame_set_samples_per_pass": (c_int, [c_void_p, ctypes.c_uint32]),
+    "pk_frame_get_samples_per_pass": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "pk_autotune_samples_per_pass": (c_int, [c_void_p, c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.POINTER(c_float)]),
     "pk_scene_param_info": (c_int, [c_void_p, c_void_p]),
     "pk_scene_write_params": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, ctypes.c_size_t, c_int, c_int]),
     "pk_scene_read_params": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, ctypes.c_size_t]),

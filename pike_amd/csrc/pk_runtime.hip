@@ -558,6 +558,15 @@ int pk_frame_destroy(pk_frame* f) {
   return PK_OK;
 }
 
+int pk_frame_get_samples_per_pass(const pk_frame* f, uint32_t* samples) {
+  if (!f || !samples) {
+    pk_set_error("pk_frame_get_samples_per_pass: null argument");
+    return PK_E_INVALID;
+  }
+  *samples = f->dev.S;
+  return PK_OK;
+}
+
 int pk_frame_set_samples_per_pass(pk_frame* f, uint32_t samples) {
   if (!f || samples == 0 || f->batch_scene) {
     pk_set_error("pk_frame_set_samples_per_pass: null frame, zero samples or a batch frame");
@@ -1490,6 +1499,56 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
   if (ms2) {
     ms2[0] = (float)before;
     ms2[1] = (float)(after[0] + after[1] + after[2]);
+  }
+  return pk_frame_reset(f);
+}
+
+// Queue capacity: samples per pass in 1, 2, 4, ... up to max_samples (and
+// below kTunePathCap paths), each timed over max_samples samples of
+// graph-replayed passes; keeps the fastest per sample.
+constexpr uint64_t kTunePathCap = 1ull << 24;
+
+extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t max_samples,
+                                            float* ms2) {
+  if (!f || !s || depth == 0 || depth > 64 || max_samples == 0 || f->batch_scene) {
+    pk_set_error("pk_autotune_samples_per_pass: null argument, depth outside [1, 64], zero samples or a batch");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_autotune_samples_per_pass: no camera set");
+    return PK_E_STATE;
+  }
+  pk_ctx* c = f->ctx;
+  hipStream_t st = c->stream;
+  const uint32_t start = f->dev.S;
+  uint32_t best_S = start;
+  float best = 0, at_start = 0;
+  pk_render_params p{f->W, f->H, depth, 0, max_samples, f->theta, 1};
+  int rc = PK_OK;
+  for (uint32_t S = 1; S <= max_samples; S *= 2) {
+    if ((uint64_t)f->W * f->H * S >= kTunePathCap) break;
+    if ((rc = pk_frame_set_samples_per_pass(f, S))) break;
+    pk_render_params warm = p;
+    warm.num_frames = S;
+    if ((rc = pk_render(f, s, &warm))) break;
+    PK_HIP(hipEventRecord(c->ev0, st));
+    if ((rc = pk_render(f, s, &p))) break;
+    PK_HIP(hipEventRecord(c->ev1, st));
+    PK_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    PK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    ms /= (float)max_samples;
+    if (S == start) at_start = ms;
+    if (best == 0 || ms < best) {
+      best = ms;
+      best_S = S;
+    }
+  }
+  int rc2 = pk_frame_set_samples_per_pass(f, best_S);
+  if (rc || rc2) return rc ? rc : rc2;
+  if (ms2) {
+    ms2[0] = at_start;
+    ms2[1] = best;
   }
   return pk_frame_reset(f);
 }

@@ -4,6 +4,9 @@
 //   node cli.js env <scene_dir> <out_dir>      environment tables (host only)
 //   node cli.js render <scene_dir> <out.f32> <width> <height> <frames> <depth>
 //        PikeRenderer tick loop; writes the running mean (RGBA float32)
+//   node cli.js adjoint <scene_dir> <out_dir> <d_image.f32> <width> <height> <spp> <depth>
+//        path-replay adjoint of the frame's image (pk_adjoint): writes the
+//        atlas and environment gradients (float32, PK_PARAM_* layouts)
 const fs = require('fs');
 const path = require('path');
 const { loadRawScene } = require('./raw_scene');
@@ -49,6 +52,24 @@ async function main() {
     for (let f = 0; f < frames; f++) r.tick();
     await r.device.queue.onSubmittedWorkDone();
     write(out, r.readImage());
+    r.destroy();
+  } else if (cmd === 'adjoint') {
+    const dImage = process.argv[5];
+    const [w, h, spp, depth] = process.argv.slice(6).map(Number);
+    const r = new PikeRenderer({ desc: scene.desc, env: null }, [w, h], { numBounces: depth });
+    await r.initDevice();
+    const n = native.load();
+    const cam = scene.camera;
+    const frame = r.cameraPass.getRenderState().frame;
+    n.frameSetCamera(frame, { pos: r.tracePass.normalize(cam.origin), dir: cam.dir, fov: cam.fov,
+      focalDepth: cam.focal_depth, apertureSize: cam.aperture, distortion: cam.distortion, bokeh: cam.bokeh });
+    const buf = fs.readFileSync(dImage);
+    const d = new Float32Array(buf.buffer, buf.byteOffset, buf.byteLength / 4);
+    const g = n.adjoint(frame, r.tracePass.getSceneHandle(),
+      { width: w, height: h, maxDepth: depth, numFrames: spp }, d);
+    fs.mkdirSync(out, { recursive: true });
+    write(path.join(out, 'atlas.f32'), g.atlas);
+    write(path.join(out, 'env.f32'), g.env);
     r.destroy();
   } else {
     throw new Error(`unknown command ${cmd}`);

@@ -608,6 +608,108 @@ napi_value TraceRays(napi_env env, napi_callback_info info) {
   return o;
 }
 
+napi_value FrameSetSamplesPerPass(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 2)) return nullptr;
+  FRAME_ARG(0);
+  uint32_t samples = 1;
+  napi_get_value_uint32(env, a.argv[1], &samples);
+  PK_CALL(pk_frame_set_samples_per_pass(f, samples), "pk_frame_set_samples_per_pass");
+  return undefined(env);
+}
+
+napi_value AutotuneSamples(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 3)) return nullptr;
+  FRAME_ARG(0);
+  SCENE_ARG(1);
+  uint32_t depth = 4, max_samples = 8;
+  napi_get_value_uint32(env, a.argv[2], &depth);
+  if (a.argc > 3) napi_get_value_uint32(env, a.argv[3], &max_samples);
+  float ms[2];
+  PK_CALL(pk_autotune_samples_per_pass(f, s, depth, max_samples, ms), "pk_autotune_samples_per_pass");
+  uint32_t chosen = 1;
+  PK_CALL(pk_frame_get_samples_per_pass(f, &chosen), "pk_frame_get_samples_per_pass");
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "msPerSampleBefore", ms[0]);
+  set_num(env, o, "msPerSampleAfter", ms[1]);
+  set_num(env, o, "samplesPerPass", chosen);
+  return o;
+}
+
+// sceneWriteParams(scene, which, offset, Float32Array, alsoSampling)
+napi_value SceneWriteParams(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 4)) return nullptr;
+  SCENE_ARG(0);
+  uint32_t which = 0, offset = 0;
+  napi_get_value_uint32(env, a.argv[1], &which);
+  napi_get_value_uint32(env, a.argv[2], &offset);
+  void* data;
+  size_t n;
+  if (!typed(env, a.argv[3], napi_float32_array, &data, &n, "params")) return nullptr;
+  bool also = false;
+  if (a.argc > 4) napi_get_value_bool(env, a.argv[4], &also);
+  PK_CALL(pk_scene_write_params(s, (int)which, offset, (const float*)data, n, 0, also ? 1 : 0),
+          "pk_scene_write_params");
+  return undefined(env);
+}
+
+// sceneReadParams(scene, which) -> Float32Array of the whole set
+napi_value SceneReadParams(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 2)) return nullptr;
+  SCENE_ARG(0);
+  uint32_t which = 0;
+  napi_get_value_uint32(env, a.argv[1], &which);
+  uint32_t pi[6];
+  PK_CALL(pk_scene_param_info(s, pi), "pk_scene_param_info");
+  const size_t n = which == PK_PARAM_ATLAS ? (size_t)pi[0] * pi[1] * pi[2] * 4 : (size_t)pi[4] * pi[5] * 4;
+  void* data;
+  napi_value t = make_typed(env, napi_float32_array, n, 4, &data);
+  if (!t) return nullptr;
+  PK_CALL(pk_scene_read_params(s, (int)which, 0, (float*)data, n), "pk_scene_read_params");
+  return t;
+}
+
+// adjoint(frame, scene, params, dImage Float32Array) -> {atlas, env} gradients
+napi_value Adjoint(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 4)) return nullptr;
+  FRAME_ARG(0);
+  SCENE_ARG(1);
+  pk_render_params p;
+  napi_value o = a.argv[2];
+  get_u32(env, o, "width", &p.width, 0);
+  get_u32(env, o, "height", &p.height, 0);
+  get_u32(env, o, "maxDepth", &p.max_depth, 4);
+  get_u32(env, o, "firstSample", &p.first_sample, 0);
+  get_u32(env, o, "numFrames", &p.num_frames, 1);
+  get_f32(env, o, "envTheta", &p.env_theta, 0.0f);
+  p.use_graph = 0;
+  void* dimg;
+  size_t n;
+  if (!typed(env, a.argv[3], napi_float32_array, &dimg, &n, "dImage")) return nullptr;
+  if (n < (size_t)p.width * p.height * 4) {
+    napi_throw_range_error(env, nullptr, "dImage is smaller than width*height*4");
+    return nullptr;
+  }
+  uint32_t pi[6];
+  PK_CALL(pk_scene_param_info(s, pi), "pk_scene_param_info");
+  void *ga, *ge;
+  napi_value ta = make_typed(env, napi_float32_array, (size_t)pi[0] * pi[1] * pi[2] * 4, 4, &ga);
+  napi_value te = make_typed(env, napi_float32_array, (size_t)pi[4] * pi[5] * 4, 4, &ge);
+  if (!ta || !te) return nullptr;
+  pk_grad_buffers g{(float*)ga, pi[4] ? (float*)ge : nullptr, 0, 0};
+  PK_CALL(pk_adjoint(f, s, &p, (const float*)dimg, 0, &g), "pk_adjoint");
+  napi_value r;
+  napi_create_object(env, &r);
+  napi_set_named_property(env, r, "atlas", ta);
+  napi_set_named_property(env, r, "env", te);
+  return r;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct Fn {
     const char* name;
@@ -627,6 +729,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"render", Render},             {"readImage", ReadImage},
       {"counters", Counters},         {"autotune", Autotune},
       {"traceRays", TraceRays},
+      {"frameSetSamplesPerPass", FrameSetSamplesPerPass}, {"autotuneSamples", AutotuneSamples},
+      {"sceneWriteParams", SceneWriteParams}, {"sceneReadParams", SceneReadParams},
+      {"adjoint", Adjoint},
   };
   for (const Fn& f : fns) {
     napi_value v;

@@ -427,7 +427,9 @@ C2_CAMERA = dict(origin=(0.5, 0.42, 1.2), target=(0.5, 0.02, 0.4), fov=1.0)
 def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024), env_sampler=None):
     """Terrain of cells^2 quads (fBm, 6 octaves, amplitude 0.2) plus
     num_spheres icospheres (subdivision 4); log-normal (sigma 1.5) HDR
-    environment with a 0.5-degree sun at 1e4 x mean radiance."""
+    environment with a 0.5-degree sun at 1e4 x mean radiance, importance
+    sampled through the marginal/conditional CDF (pass
+    env_sampler=PK_ENV_SAMPLER_BINS for env_sampler.js's bins)."""
     rng = Rng(seed)
     tp, ta, tt = terrain(cells, seed, 0.2)
     packer = AtlasPacker()
@@ -450,7 +452,7 @@ def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024)
     env = lognormal_sun_env(env_size[0], env_size[1], seed + 17)
     return SceneData(positions=pos, attributes=attr, triangles=tris, materials=np.array(mats, dtype=np.int32),
                      tex=packer.atlas(), env=env,
-                     env_sampler=_ffi.PK_ENV_SAMPLER_BINS if env_sampler is None else env_sampler,
+                     env_sampler=_ffi.PK_ENV_SAMPLER_CDF if env_sampler is None else env_sampler,
                      camera=camera(C2_CAMERA["origin"], C2_CAMERA["target"], C2_CAMERA["fov"]), name="c2")
 
 

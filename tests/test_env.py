@@ -49,9 +49,10 @@ ENVS = {
 }
 
 
+@pytest.mark.parametrize("sampler", [_ffi.PK_ENV_SAMPLER_BINS, _ffi.PK_ENV_SAMPLER_CDF])
 @pytest.mark.parametrize("name", list(ENVS))
-def test_tables_match_oracle(name):
-    data = _scene_with_env(ENVS[name]())
+def test_tables_match_oracle(name, sampler):
+    data = _scene_with_env(ENVS[name](), sampler)
     b, c, p = product_tables(data)
     ob, oc, op = OracleScene(data).env_tables()
     np.testing.assert_array_equal(b, ob)

@@ -131,3 +131,45 @@ def test_samples_per_pass_equals_one_sample_frames(ctx, samples, frames):
     acc_o, _, _ = orc.render(cam, W, H, depth, 2, frames, precision=32, accum=np.zeros((H, W, 4), np.float32))
     # the run started at sample 2 on a cleared accumulator, as the frames did
     np.testing.assert_array_equal(fr.read_image().view(np.uint32), acc_o.view(np.uint32))
+
+
+def _sun_scene(sampler):
+    env = scenes.lognormal_sun_env(128, 64, seed=8, sigma=1.5, sun_deg=4.0, sun_scale=1e3)
+    data = scenes.mixed_scene(11, 6000, env=env)
+    data.env_sampler = sampler
+    data.camera = scenes.camera((0.5, 0.45, 1.3), (0.5, 0.05, 0.45), 1.0)
+    return data
+
+
+def test_render_cdf_sampler_matches_fp32_oracle(ctx):
+    """The marginal/conditional CDF sampler (device binary search) against the
+    oracle's restatement: bit-identical frames."""
+    data = _sun_scene(pike_amd._ffi.PK_ENV_SAMPLER_CDF)
+    bvh, scene, orc, cam = _setup(ctx, data)
+    W, H, depth, frames = 48, 32, 5, 3
+    fr = pike_amd.Frame(ctx, W, H)
+    fr.set_camera(cam)
+    fr.render(scene, max_depth=depth, first_sample=0, num_frames=frames)
+    acc_o, _, _ = orc.render(cam, W, H, depth, 0, frames, precision=32)
+    np.testing.assert_array_equal(fr.read_image().view(np.uint32), acc_o.view(np.uint32))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("sampler", [pike_amd._ffi.PK_ENV_SAMPLER_BINS, pike_amd._ffi.PK_ENV_SAMPLER_CDF])
+def test_converged_image_matches_fp64_oracle(ctx, sampler):
+    """4096 samples per pixel on the GPU against the FP64 oracle of the same
+    estimator (common random numbers): mean relative L1 < 1e-3."""
+    data = _sun_scene(sampler)
+    bvh, scene, orc, cam = _setup(ctx, data)
+    W, H, depth, spp = 24, 16, 4, 4096
+    fr = pike_amd.Frame(ctx, W, H)
+    fr.set_camera(cam)
+    fr.set_samples_per_pass(256)
+    fr.render(scene, max_depth=depth, first_sample=0, num_frames=spp)
+    gpu = fr.read_image()[..., :3].astype(np.float64)
+    print("fp64 oracle render", flush=True)
+    ref, _, _ = orc.render(cam, W, H, depth, 0, spp, precision=64)
+    ref = ref[..., :3].astype(np.float64)
+    rel = np.abs(gpu - ref).sum() / np.abs(ref).sum()
+    print("converged image mean relative L1 vs FP64:", rel)
+    assert rel < 1e-3, rel

@@ -74,3 +74,27 @@ def test_js_render_equals_python_render(tmp_path, ctx):
     fr.set_camera(scenes.normalized_camera(data.camera, bvh.normalizer()))
     fr.render(scene, max_depth=depth, first_sample=0, num_frames=frames, use_graph=False)
     np.testing.assert_array_equal(img_js.view(np.uint32), fr.read_image().view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_js_adjoint_equals_python_adjoint(tmp_path, ctx):
+    """pk_adjoint through the N-API addon (node cli.js adjoint) gives the
+    Python harness's gradients (up to the atomics' summation order)."""
+    data = scenes.config5(cells=24, num_spheres=6, tex_res=32, env_size=(32, 16))
+    data.save(tmp_path / "scene")
+    W, H, spp, depth = 32, 24, 2, 3
+    d = np.random.default_rng(2).uniform(-1, 1, (H, W, 4)).astype(np.float32)
+    d.tofile(tmp_path / "d.f32")
+    node(CLI, "adjoint", str(tmp_path / "scene"), str(tmp_path / "g"), str(tmp_path / "d.f32"), str(W), str(H),
+         str(spp), str(depth))
+    ga_js = np.fromfile(tmp_path / "g" / "atlas.f32", np.float32)
+    ge_js = np.fromfile(tmp_path / "g" / "env.f32", np.float32)
+    bvh = pike_amd.BVH(data.positions, data.triangles)
+    scene = pike_amd.Scene(ctx, data, bvh)
+    fr = pike_amd.Frame(ctx, W, H)
+    fr.set_camera(scenes.normalized_camera(data.camera, bvh.normalizer()))
+    ga, ge = fr.adjoint(scene, d, max_depth=depth, num_frames=spp)
+    scale = max(np.abs(ga).max(), np.abs(ge).max())
+    assert scale > 0
+    np.testing.assert_allclose(ga_js, ga.ravel(), rtol=1e-4, atol=1e-6 * scale)
+    np.testing.assert_allclose(ge_js, ge.ravel(), rtol=1e-4, atol=1e-6 * scale)
