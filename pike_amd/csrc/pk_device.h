@@ -35,6 +35,8 @@ struct DevScene {
   const float* env_pdf;
   const float* row_cdf;
   const float* col_cdf;
+  const uint32_t* row_guide;  // cutpoint guides of the two CDFs
+  const uint32_t* col_guide;
   uint32_t ew, eh, nbins;
   int env_mode;              // 0 none, 1 bins, 2 cdf
 };

@@ -103,6 +103,8 @@ int pk_build_env_tables(const pk_scene_desc* d, EnvTables* e, std::string* err) 
     }
     e->row_cdf.clear();
     e->col_cdf.clear();
+    e->row_guide.clear();
+    e->col_guide.clear();
     return PK_OK;
   }
   // ---- marginal / conditional CDF -----------------------------------------
@@ -135,6 +137,24 @@ int pk_build_env_tables(const pk_scene_desc* d, EnvTables* e, std::string* err) 
     e->col_cdf[(size_t)y * (W + 1) + W] = 1.0f;
   }
   e->row_cdf[H] = 1.0f;
+  // Cutpoint guides (Chen & Asau 1974): for cell k of [0, 1) the answer of
+  // "largest index with cdf <= u" at u = k/n.  A lookup computes k = (int)(u*n)
+  // and searches only between the guides of cells k-1 and k+2, which bracket
+  // the answer for every u that can round into cell k; the result equals the
+  // full binary search's.
+  auto guide = [](const float* cdf, uint32_t n, uint32_t* out) {
+    uint32_t y = 0;
+    for (uint32_t k = 0; k <= n; ++k) {
+      const float t = (float)k / (float)n;
+      while (y + 1 < n && cdf[y + 1] <= t) ++y;
+      out[k] = y;
+    }
+  };
+  e->row_guide.assign(H + 1, 0u);
+  guide(e->row_cdf.data(), H, e->row_guide.data());
+  e->col_guide.assign((size_t)H * (W + 1), 0u);
+  for (uint32_t y = 0; y < H; ++y)
+    guide(&e->col_cdf[(size_t)y * (W + 1)], W, &e->col_guide[(size_t)y * (W + 1)]);
   // The sampled texel's probability is the product of the two CDF steps the
   // device search will see (f32 table values), so pdf and sampling agree.
   for (uint32_t y = 0; y < H; ++y) {

@@ -47,6 +47,10 @@ struct EnvTables {
   // marginal/conditional CDF sampler
   std::vector<float> row_cdf;       // [h+1]
   std::vector<float> col_cdf;       // [h][w+1]
+  // cutpoint guides: row_guide[k] = largest y with row_cdf[y] <= k/h,
+  // col_guide[y][k] = largest x with col_cdf[y][x] <= k/w (k = 0..h / 0..w)
+  std::vector<uint32_t> row_guide;  // [h+1]
+  std::vector<uint32_t> col_guide;  // [h][w+1]
 };
 
 int pk_build_env_tables(const pk_scene_desc* d, EnvTables* out, std::string* err);

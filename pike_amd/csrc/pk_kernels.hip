@@ -373,6 +373,22 @@ __device__ inline v3 attr_vec(const DevScene& S, int vi, int word) {
   return pkm::unpack4x8snorm_xyz(w);
 }
 
+// Largest i in [0, n) with cdf[i] <= u (cdf[0] = 0 <= u < 1), searched
+// between the cutpoint guides of the cells around k = (int)(u * n): the
+// guides of cells k-1 and k+2 bracket the answer even when u * n rounds
+// across a cell boundary, so the result is the full binary search's.
+__device__ inline int guided_search(const float* cdf, const uint32_t* guide, uint32_t n, float u) {
+  const int k = (int)(u * (float)n);
+  int lo = (int)guide[max(k - 1, 0)];
+  int hi = (int)guide[min(k + 2, (int)n)] + 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // sampleEnv (shade_hit.wgsl:213-227) for the bins sampler; the CDF sampler
 // draws two uniform numbers from the same two hashes and binary-searches the
 // marginal and conditional tables.
@@ -391,22 +407,9 @@ __device__ inline void sample_env(const DevScene& S, float theta, const pkm::m3&
   } else {
     const float u1 = pkm::min_(pkm::pcg_rand(seed), 0x1.fffffep-1f);
     const float u2 = pkm::min_(pkm::pcg_rand(seed), 0x1.fffffep-1f);
-    int lo = 0, hi = (int)S.eh;  // largest y with row_cdf[y] <= u1
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (S.row_cdf[mid] <= u1) lo = mid;
-      else hi = mid;
-    }
-    cy = lo;
-    const float* cc = S.col_cdf + (size_t)cy * (S.ew + 1);
-    lo = 0;
-    hi = (int)S.ew;
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (cc[mid] <= u2) lo = mid;
-      else hi = mid;
-    }
-    cx = lo;
+    cy = guided_search(S.row_cdf, S.row_guide, S.eh, u1);
+    const size_t row = (size_t)cy * (S.ew + 1);
+    cx = guided_search(S.col_cdf + row, S.col_guide + row, S.ew, u2);
   }
   float u = -theta + ((0.5f + (float)cx) / (float)S.ew);
   float v = (0.5f + (float)cy) / (float)S.eh;

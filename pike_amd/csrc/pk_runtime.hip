@@ -422,6 +422,7 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     float4* er;
     int2 *eb, *ec;
     float *ep, *rc_cdf = nullptr, *cc_cdf = nullptr;
+    uint32_t *rguide = nullptr, *cguide = nullptr;
     if ((rc = upload(s->allocs, &er, rad4.data(), T, st))) return fail(rc);
     if ((rc = upload(s->allocs, &eb, env.bins.data(), env.bins.size() / 2, st))) return fail(rc);
     if ((rc = upload(s->allocs, &ec, env.coords.data(), env.coords.size() / 2, st))) return fail(rc);
@@ -429,6 +430,8 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     if (d->env_sampler == PK_ENV_SAMPLER_CDF) {
       if ((rc = upload(s->allocs, &rc_cdf, env.row_cdf.data(), env.row_cdf.size(), st))) return fail(rc);
       if ((rc = upload(s->allocs, &cc_cdf, env.col_cdf.data(), env.col_cdf.size(), st))) return fail(rc);
+      if ((rc = upload(s->allocs, &rguide, env.row_guide.data(), env.row_guide.size(), st))) return fail(rc);
+      if ((rc = upload(s->allocs, &cguide, env.col_guide.data(), env.col_guide.size(), st))) return fail(rc);
     }
     D.env_rad = er;
     D.env_bins = eb;
@@ -436,6 +439,8 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     D.env_pdf = ep;
     D.row_cdf = rc_cdf;
     D.col_cdf = cc_cdf;
+    D.row_guide = rguide;
+    D.col_guide = cguide;
     D.ew = env.width;
     D.eh = env.height;
     D.nbins = (uint32_t)(env.bins.size() / 2);

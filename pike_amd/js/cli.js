@@ -7,9 +7,20 @@
 //   node cli.js adjoint <scene_dir> <out_dir> <d_image.f32> <width> <height> <spp> <depth>
 //        path-replay adjoint of the frame's image (pk_adjoint): writes the
 //        atlas and environment gradients (float32, PK_PARAM_* layouts)
+//   node cli.js load-scene <scene.json> <out_dir>
+//        Scene.load (scene JSON + glTF props + environment PNG) baked into
+//        the raw scene format above
+//   node cli.js render-scene <scene.json> <out.png> <width> <height> <frames> <depth> [out.f32]
+//        loads, renders (PikeRenderer), tonemaps (PostProcessPass) and writes
+//        a PNG (and optionally the linear running mean).  The camera is the
+//        scene JSON's "camera" {origin, target, fov} or a view of the whole
+//        scene from +z.
 const fs = require('fs');
 const path = require('path');
-const { loadRawScene } = require('./raw_scene');
+const { loadRawScene, saveRawScene } = require('./raw_scene');
+const { Scene } = require('./scene');
+const { sceneDesc } = require('./scene_desc');
+const png = require('./png');
 const { BVH } = require('./bvh');
 const { EnvironmentGenerator } = require('./env_sampler');
 const { PikeRenderer } = require('./pike');
@@ -19,8 +30,49 @@ function write(file, arr) {
   fs.writeFileSync(file, Buffer.from(arr.buffer, arr.byteOffset, arr.byteLength));
 }
 
+// camera of a loaded scene JSON: its "camera" entry, else the bounding box
+// seen from +z
+function sceneCamera(scene) {
+  const c = scene.json.camera;
+  const pos = scene.desc.positions;
+  const lo = [Infinity, Infinity, Infinity], hi = [-Infinity, -Infinity, -Infinity];
+  for (let i = 0; i < pos.length; i += 3)
+    for (let k = 0; k < 3; k++) {
+      lo[k] = Math.min(lo[k], pos[i + k]);
+      hi[k] = Math.max(hi[k], pos[i + k]);
+    }
+  const center = [0, 1, 2].map((k) => (lo[k] + hi[k]) / 2);
+  const extent = Math.max(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
+  const origin = c && c.origin ? c.origin : [center[0], center[1], center[2] + 1.5 * extent];
+  const target = c && c.target ? c.target : center;
+  const d = [0, 1, 2].map((k) => target[k] - origin[k]);
+  const l = Math.hypot(d[0], d[1], d[2]);
+  return { origin, dir: d.map((x) => x / l), fov: c && c.fov ? c.fov : 1.0, focal_depth: 0.5, aperture: 0,
+    distortion: 0, bokeh: 0.5 };
+}
+
 async function main() {
   const [cmd, dir, out] = process.argv.slice(2);
+  if (cmd === 'load-scene' || cmd === 'render-scene') {
+    const scene = await new Scene().load(dir);
+    sceneDesc(scene);
+    const cam = sceneCamera(scene);
+    if (cmd === 'load-scene') {
+      saveRawScene(scene.desc, cam, out, path.basename(dir));
+      return;
+    }
+    const [w, h, frames, depth] = process.argv.slice(5, 9).map(Number);
+    const r = new PikeRenderer(scene, [w, h], { numBounces: depth });
+    await r.initDevice();
+    r.camera = { origin: r.tracePass.normalize(cam.origin), dir: cam.dir, fov: cam.fov };
+    for (let f = 0; f < frames; f++) r.tick();
+    await r.device.queue.onSubmittedWorkDone();
+    const img = r.readImage();
+    fs.writeFileSync(out, png.encode(w, h, r.postProcessPass.tonemap(img, w, h)));
+    if (process.argv[9]) write(process.argv[9], img);
+    r.destroy();
+    return;
+  }
   const scene = loadRawScene(dir);
   if (cmd === 'bvh') {
     const bvh = new BVH(native.load(), scene.desc);

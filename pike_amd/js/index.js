@@ -9,11 +9,16 @@ const { PostProcessPass } = require('./passes/postprocess_pass');
 const { BVH } = require('./bvh');
 const { EnvironmentGenerator } = require('./env_sampler');
 const { PikeRenderer } = require('./pike');
-const { loadRawScene } = require('./raw_scene');
+const { loadRawScene, saveRawScene } = require('./raw_scene');
 const { sceneDesc } = require('./scene_desc');
+const { Scene } = require('./scene');
+const { GLTFLoader } = require('./gltf_loader');
+const { TexturePacker } = require('./texture_packer');
+const png = require('./png');
 const native = require('./native');
 
 module.exports = {
   PikeDevice, RenderState, CameraPass, TracePass, ShadeHitPass, TraceShadowPass, ShadeMissPass, AccumulatePass,
-  PostProcessPass, BVH, EnvironmentGenerator, PikeRenderer, loadRawScene, sceneDesc, native,
+  PostProcessPass, BVH, EnvironmentGenerator, PikeRenderer, loadRawScene, saveRawScene, sceneDesc, Scene,
+  GLTFLoader, TexturePacker, png, native,
 };

@@ -30,4 +30,33 @@ function loadRawScene(dir) {
   return { desc, camera: m.camera, name: m.name };
 }
 
-module.exports = { loadRawScene };
+const DTYPES = [[Float64Array, 'f64'], [Float32Array, 'f32'], [Int32Array, 'i32'], [Uint32Array, 'u32'],
+  [Uint8Array, 'u8']];
+
+function putArray(dir, name, arr) {
+  fs.writeFileSync(path.join(dir, name), Buffer.from(arr.buffer, arr.byteOffset, arr.byteLength));
+  const d = DTYPES.find(([T]) => arr instanceof T);
+  return { file: name, dtype: d[1] };
+}
+
+// Writes a pk_scene_desc-shaped scene (and camera) in the same format.
+function saveRawScene(desc, camera, dir, name) {
+  fs.mkdirSync(dir, { recursive: true });
+  const m = {
+    name: name || 'scene',
+    positions: putArray(dir, 'positions.bin', desc.positions),
+    attributes: putArray(dir, 'attributes.bin', desc.attributes),
+    triangles: putArray(dir, 'triangles.bin', desc.triangles),
+    materials: putArray(dir, 'materials.bin', desc.materials),
+    atlas: { width: desc.atlas.width, height: desc.atlas.height, layers: desc.atlas.layers,
+      format: desc.atlas.format, data: putArray(dir, 'atlas.bin', desc.atlas.data) },
+    camera,
+  };
+  if (desc.env) {
+    m.env = { width: desc.env.width, height: desc.env.height, format: desc.env.format, sampler: desc.env.sampler,
+      data: putArray(dir, 'env.bin', desc.env.data) };
+  }
+  fs.writeFileSync(path.join(dir, 'scene.json'), JSON.stringify(m));
+}
+
+module.exports = { loadRawScene, saveRawScene };

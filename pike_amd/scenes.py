@@ -226,6 +226,30 @@ class SceneData:
     def num_triangles(self):
         return int(self.triangles.shape[0])
 
+    @classmethod
+    def load(cls, directory):
+        """Reads the raw format `save` (and pike_amd/js saveRawScene) writes."""
+        import json
+        import os
+
+        m = json.load(open(os.path.join(directory, "scene.json")))
+        dt = {"f64": np.float64, "f32": np.float32, "i32": np.int32, "u32": np.uint32, "u8": np.uint8}
+
+        def get(entry):
+            return np.fromfile(os.path.join(directory, entry["file"]), dt[entry["dtype"]])
+
+        a = m["atlas"]
+        tex = get(a["data"]).reshape(a["layers"], a["height"], a["width"], 4)
+        env, sampler = None, _ffi.PK_ENV_SAMPLER_BINS
+        if m.get("env"):
+            e = m["env"]
+            env = get(e["data"]).reshape(e["height"], e["width"], -1)
+            sampler = int(e.get("sampler", sampler))
+        cam = {k: (np.array(v) if isinstance(v, list) else v) for k, v in (m.get("camera") or {}).items()}
+        return cls(positions=get(m["positions"]).reshape(-1, 3), attributes=get(m["attributes"]).reshape(-1, 4),
+                   triangles=get(m["triangles"]).reshape(-1, 4), materials=get(m["materials"]).reshape(-1, 4),
+                   tex=tex, env=env, env_sampler=sampler, camera=cam, name=m.get("name", ""))
+
     def save(self, directory):
         """Raw arrays + scene.json manifest (read by pike_amd/js/raw_scene.js)."""
         import json
