@@ -383,34 +383,49 @@ def run_cpu_baseline(args, data, cam):
 
 
 def run_traversal(args, pike_amd, scenes):
+    """The 10M-triangle soup traced with 16M incoherent rays, closest hit and
+    any hit; per BVH kind (spatial-split SBVH, and the plain SAH BVH when
+    asked for) with its build time and traversal counters."""
     import numpy as np
 
-    t0 = time.time()
     data = scenes.config3(seed=3, num_tris=args.soup_tris)
-    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1))
-    log(f"config3 soup: {data.num_triangles} triangles, BVH {bvh.info()} ({time.time() - t0:.1f}s)")
+    threads = min(16, os.cpu_count() or 1)
+    kinds = {"sbvh": [True], "sah": [False], "both": [True, False]}[args.soup_bvh]
     ctx = pike_amd.Context(0)
-    scene = pike_amd.Scene(ctx, data, bvh)
-    rays = scenes.incoherent_rays(args.soup_rays, bvh.normalizer(), seed=33)
-    n = rays.shape[0]
-    d_rays = pike_amd.DeviceBuffer.from_array(ctx, rays)
-    del rays
-    ids = pike_amd.DeviceBuffer(ctx, 4 * n)
-    t = pike_amd.DeviceBuffer(ctx, 4 * n)
-    b = pike_amd.DeviceBuffer(ctx, 4 * n)
     out = {}
-    for name, anyh in (("closest", False), ("any", True)):
-        if not args.no_autotune:
-            before, after = ctx.autotune_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr)
-            log(f"autotune {name}: {before:.2f} -> {after:.2f} ms, {ctx.tuning()['rays_any' if anyh else 'rays_closest']}")
-        ctx.trace_rays_device(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr, True)
-        nodes, tris = ctx.trace_stats()
-        ms = ctx.time_trace_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr, args.trace_iters)
-        byts = traversal_bytes(n, nodes, tris)
-        hit_rate = float((ids.read(np.int32, n) >= 0).mean())
-        out[name] = dict(mrays_s=n / (ms * 1e-3) / 1e6, ms=ms, nodes_per_ray=nodes / n, tris_per_ray=tris / n,
-                         gbs=byts / (ms * 1e-3) / 1e9, hit_rate=hit_rate)
-        log(f"traversal {name}: {out[name]}")
+    for spatial in kinds:
+        kind = "sbvh" if spatial else "sah"
+        t0 = time.time()
+        bvh = pike_amd.BVH(data.positions, data.triangles, threads=threads, spatial_splits=spatial)
+        build_s = time.time() - t0
+        info = bvh.info()
+        log(f"config3 soup {kind}: {data.num_triangles} triangles, BVH {info} ({build_s:.1f}s, {threads} threads)")
+        scene = pike_amd.Scene(ctx, data, bvh)
+        rays = scenes.incoherent_rays(args.soup_rays, bvh.normalizer(), seed=33)
+        n = rays.shape[0]
+        d_rays = pike_amd.DeviceBuffer.from_array(ctx, rays)
+        del rays
+        ids = pike_amd.DeviceBuffer(ctx, 4 * n)
+        t = pike_amd.DeviceBuffer(ctx, 4 * n)
+        b = pike_amd.DeviceBuffer(ctx, 4 * n)
+        res = dict(build_s=build_s, build_threads=threads, leaf_references=info["leaf_triangles"],
+                   wide_nodes=info["wide_nodes"])
+        for name, anyh in (("closest", False), ("any", True)):
+            if not args.no_autotune:
+                before, after = ctx.autotune_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr)
+                log(f"autotune {kind} {name}: {before:.2f} -> {after:.2f} ms, "
+                    f"{ctx.tuning()['rays_any' if anyh else 'rays_closest']}")
+            ctx.trace_rays_device(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr, True)
+            nodes, tris = ctx.trace_stats()
+            ms = ctx.time_trace_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr, args.trace_iters)
+            byts = traversal_bytes(n, nodes, tris)
+            hit_rate = float((ids.read(np.int32, n) >= 0).mean())
+            res[name] = dict(mrays_s=n / (ms * 1e-3) / 1e6, ms=ms, nodes_per_ray=nodes / n, tris_per_ray=tris / n,
+                             gbs=byts / (ms * 1e-3) / 1e9, hit_rate=hit_rate)
+            log(f"traversal {kind} {name}: {res[name]}")
+        out[kind] = res
+        for obj in (scene, bvh, d_rays, ids, t, b):
+            obj.close()
     return out
 
 
@@ -434,6 +449,8 @@ def main():
                                                   "autotuning (e.g. the 'extra.autotune.shapes' of a previous run)")
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
+    ap.add_argument("--soup-bvh", choices=["sbvh", "sah", "both"], default="both",
+                    help="BVH kind(s) the soup is traced under: spatial splits (the workload's), plain SAH, or both")
     ap.add_argument("--trace-iters", type=int, default=5)
     ap.add_argument("--spp-per-pass", type=int, default=1,
                     help="samples per pixel traced per wavefront pass of the 1080p render (without autotuning)")

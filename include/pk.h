@@ -167,6 +167,15 @@ int pk_buffer_copy(pk_ctx* ctx, void* device_dst, const void* device_src, size_t
  * layout the kernels traverse. */
 int pk_bvh_build(const double* positions, uint32_t num_vertices, const int32_t* triangles,
                  uint32_t num_triangles, int num_threads, pk_bvh** out);
+/* The same with options.  PK_BVH_SPATIAL_SPLITS makes bvh.js's SBVH path
+ * live (Node :119-143, getSpatialSplit :231-274, the check the reference
+ * leaves disabled): spatial splits over 64 slabs when the object split's
+ * children overlap by more than 1e-5 of the root area.  Leaves then hold
+ * clipped fragments, so a triangle can appear in several leaves; the
+ * closest/any hit answer is unchanged. */
+#define PK_BVH_SPATIAL_SPLITS 1u
+int pk_bvh_build_ex(const double* positions, uint32_t num_vertices, const int32_t* triangles,
+                    uint32_t num_triangles, int num_threads, uint32_t flags, pk_bvh** out);
 int pk_bvh_destroy(pk_bvh* bvh);
 /* sizes: [0] binary nodes, [1] wide nodes, [2] leaf triangles, [3] binary depth,
  * [4] wide depth, [5] largest leaf */

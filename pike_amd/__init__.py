@@ -173,12 +173,13 @@ class DeviceBuffer:
 class BVH:
     """bvh.js SAH binary BVH + its 128-byte wide collapse (host side)."""
 
-    def __init__(self, positions, triangles, threads=0):
+    def __init__(self, positions, triangles, threads=0, spatial_splits=False):
         pos = np.ascontiguousarray(positions, dtype=np.float64)
         tri = np.ascontiguousarray(triangles, dtype=np.int32)
         h = ctypes.c_void_p()
-        _check(lib().pk_bvh_build(_ptr(pos), pos.shape[0], _ptr(tri), tri.shape[0], threads, ctypes.byref(h)),
-               "pk_bvh_build")
+        flags = _ffi.PK_BVH_SPATIAL_SPLITS if spatial_splits else 0
+        _check(lib().pk_bvh_build_ex(_ptr(pos), pos.shape[0], _ptr(tri), tri.shape[0], threads, flags,
+                                     ctypes.byref(h)), "pk_bvh_build_ex")
         self.h = h
         self.num_vertices = pos.shape[0]
         self.num_triangles = tri.shape[0]
@@ -193,7 +194,7 @@ class BVH:
         """The reference's flattened buffers (trace_pass.js:220-263)."""
         info = self.info()
         nodes = np.zeros((info["binary_nodes"], 5), np.int32)
-        tris = np.zeros((self.num_triangles, 4), np.int32)
+        tris = np.zeros((max(info["leaf_triangles"], self.num_triangles), 4), np.int32)
         verts = np.zeros((self.num_vertices, 4), np.float32)
         norm = np.zeros(4, np.float64)
         _check(lib().pk_bvh_export_reference(self.h, _ptr(nodes), _ptr(tris), _ptr(verts), _ptr(norm)),

@@ -27,6 +27,7 @@ PK_ENV_RGB32F = 2
 PK_ENV_SAMPLER_BINS = 0
 PK_ENV_SAMPLER_CDF = 1
 PK_PARAM_ATLAS = 0
+PK_BVH_SPATIAL_SPLITS = 1
 PK_PARAM_ENV = 1
 
 c_void_p = ctypes.c_void_p
@@ -135,6 +136,7 @@ SIGNATURES = {
     "pk_buffer_read": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "pk_buffer_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "pk_bvh_build":(c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_int, ctypes.POINTER(c_void_p)]),
+    "pk_bvh_build_ex": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_int, c_uint32, ctypes.POINTER(c_void_p)]),
     "pk_bvh_destroy": (c_int, [c_void_p]),
     "pk_bvh_info": (c_int, [c_void_p, c_void_p]),
     "pk_bvh_export_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

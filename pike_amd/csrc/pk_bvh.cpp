@@ -14,11 +14,16 @@
 // place, O(N) scratch per level) instead of bvh.js's per-node copies, and
 // builds independent subtrees on threads; the tree is the same.
 #include <algorithm>
+#include <iterator>
+#include <cfloat>
+#include <cstdio>
+#include <cstdlib>
 #include <atomic>
 #include <cmath>
 #include <cstring>
 #include <deque>
 #include <future>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -50,32 +55,47 @@ inline double box_area(const Box& b) {
   return (xl * yl + xl * zl + yl * zl) * 2;
 }
 
-// Chunked node store with lock-free allocation for the threaded build.
-class NodePool {
+// Chunked store with lock-free allocation and stable addresses for the
+// threaded builds (nodes; SBVH references).
+template <class T> class Pool {
  public:
-  explicit NodePool(size_t max_nodes) : pages_((max_nodes >> kShift) + 1) {}
+  explicit Pool(size_t max_items) : pages_((max_items >> kShift) + 1) {}
   uint32_t alloc() {
     uint32_t i = next_.fetch_add(1);
-    size_t p = i >> kShift;
-    if (!pages_[p].load(std::memory_order_acquire)) {
-      std::lock_guard<std::mutex> g(mu_);
-      if (!pages_[p].load(std::memory_order_relaxed)) pages_[p].store(new BNode[kPage], std::memory_order_release);
-    }
+    ensure_page(i >> kShift);
     return i;
   }
-  BNode& at(uint32_t i) { return pages_[i >> kShift].load(std::memory_order_relaxed)[i & (kPage - 1)]; }
+  // n consecutive items, or UINT32_MAX (nothing taken) if they do not fit
+  uint32_t try_alloc(uint32_t n) {
+    uint32_t i = next_.load();
+    do {
+      if (n == 0 || (((size_t)i + n - 1) >> kShift) >= pages_.size()) return UINT32_MAX;
+    } while (!next_.compare_exchange_weak(i, i + n));
+    for (size_t p = i >> kShift; p <= (((size_t)i + n - 1) >> kShift); ++p) ensure_page(p);
+    return i;
+  }
+  void ensure_page(size_t p) {
+    if (!pages_[p].load(std::memory_order_acquire)) {
+      std::lock_guard<std::mutex> g(mu_);
+      if (!pages_[p].load(std::memory_order_relaxed)) pages_[p].store(new T[kPage], std::memory_order_release);
+    }
+  }
+  size_t capacity() const { return pages_.size() << kShift; }
+  T& at(uint32_t i) { return pages_[i >> kShift].load(std::memory_order_relaxed)[i & (kPage - 1)]; }
+  const T& at(uint32_t i) const { return pages_[i >> kShift].load(std::memory_order_relaxed)[i & (kPage - 1)]; }
   uint32_t size() const { return next_.load(); }
-  ~NodePool() {
+  ~Pool() {
     for (auto& p : pages_) delete[] p.load();
   }
 
  private:
   static constexpr int kShift = 16;
   static constexpr size_t kPage = (size_t)1 << kShift;
-  std::vector<std::atomic<BNode*>> pages_;
+  std::vector<std::atomic<T*>> pages_;
   std::atomic<uint32_t> next_{0};
   std::mutex mu_;
 };
+using NodePool = Pool<BNode>;
 
 struct Build {
   const std::vector<Box>* ref_box;
@@ -197,6 +217,381 @@ struct Build {
   }
 };
 
+// ---------------------------------------------------------------------------
+// SBVH: bvh.js's spatial-split path (Node constructor :119-143,
+// getSpatialSplit :231-274, BoundingBox.axisIntersectTriangle,
+// primitives.js:52-80), which the reference ships disabled
+// (checkSpatialSplit = false), made live.  Decisions restated:
+//   * the object split as in the plain build; a spatial split is tried when
+//     the area of (front box at the best index) ∩ (back box there) exceeds
+//     kSplitAlpha of the root box area;
+//   * the node is cut into kSpatialBins equal slabs along the object split's
+//     axis; every reference gives one fragment per slab its box spans: the
+//     bounds of the triangle's part inside the slab (axisIntersectTriangle:
+//     edge/plane crossings and vertices within the slab widened by 2 ulp of
+//     1), here also intersected with the reference's own box;
+//   * a spatial split wins if cheaper than the object split; leaf iff the
+//     chosen cost > N; a leaf keeps the node's own references.
+// One deliberate change: bvh.js sweeps the fragments as if each were a
+// separate reference, so a triangle crossing m slabs counts m times and
+// spatial splits essentially never win.  Planes between slabs are evaluated
+// with the binned SBVH counts instead (Stich, Friedrich, Dietrich 2009): a
+// reference counts once on each side it reaches, with its box clipped to
+// that side; cost 0.5 + A(L)/A(P) N_L + A(R)/A(P) N_R as bvh.js's.
+// Children of a spatial split are re-sorted by centroid on every axis.  A
+// triangle can sit in several leaves.
+// ---------------------------------------------------------------------------
+constexpr double kSplitAlpha = 1.0e-5;  // bvh.js:4
+constexpr int kSpatialBins = 64;        // Node.NUM_BINS
+
+struct SRef {
+  uint32_t tri;
+  Box box;
+};
+
+struct SBuild {
+  const double* pos;
+  const int32_t* tris;
+  NodePool* pool;
+  Pool<SRef>* refs;
+  std::vector<uint8_t> mark;  // per reference id: in the left child
+  double root_area = 0;
+  std::mutex leaf_mu;
+  std::vector<uint32_t> leaf_out;
+  std::atomic<int> degenerate{0};
+  std::atomic<int> max_depth{0};
+  std::atomic<int64_t> spatial{0};
+
+  static double area_of(const Box& b) { return box_area(b); }
+
+  template <class BoxOf>
+  static void sweep_boxes(uint32_t n, BoxOf box_of, double parent_area, double* best_cost, uint32_t* best_index) {
+    std::vector<double> suf(n);
+    Box bb;
+    box_empty(bb);
+    for (uint32_t i = n; i-- > 0;) {
+      box_add(bb, box_of(i));
+      suf[i] = box_area(bb);
+    }
+    Box bf;
+    box_empty(bf);
+    double bc = INFINITY;
+    uint32_t bi = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      box_add(bf, box_of(i));
+      double cost = 0.5 + (box_area(bf) / parent_area) * (double)(i + 1) + (suf[i] / parent_area) * (double)(n - 1 - i);
+      if (cost < bc) {
+        bc = cost;
+        bi = i;
+      }
+    }
+    *best_cost = bc;
+    *best_index = bi;
+  }
+  void sweep(const std::vector<uint32_t>& l, double parent_area, double* best_cost, uint32_t* best_index) {
+    sweep_boxes((uint32_t)l.size(), [&](uint32_t i) -> const Box& { return refs->at(l[i]).box; }, parent_area,
+                best_cost, best_index);
+  }
+
+  // axisIntersectTriangle: bounds of the triangle's part inside slab [p0, p1]
+  // on axis a; returns false if it has no point there.
+  bool clip(uint32_t t, int a, double p0, double p1, Box* out) const {
+    const double lo = p0 - DBL_EPSILON * 2, hi = p1 + DBL_EPSILON * 2;
+    const double* v[3];
+    for (int k = 0; k < 3; ++k) v[k] = pos + (size_t)tris[(size_t)t * 4 + k] * 3;
+    Box b;
+    box_empty(b);
+    auto add = [&](const double* p) {
+      for (int k = 0; k < 3; ++k) {
+        b.lo[k] = std::min(p[k], b.lo[k]);
+        b.hi[k] = std::max(p[k], b.hi[k]);
+      }
+    };
+    auto add_lerp = [&](const double* x, const double* y, double s) {
+      double q[3];
+      for (int k = 0; k < 3; ++k) q[k] = x[k] + (y[k] - x[k]) * s;
+      add(q);
+    };
+    bool any = false;
+    for (int i = 0; i < 3; ++i) {
+      const double* x = v[i];
+      const double* y = v[(i + 1) % 3];
+      const double q0 = x[a], q1 = y[a];
+      const double tmin = (lo - q0) / (q1 - q0);
+      if (tmin >= 0 && tmin <= 1) {
+        add_lerp(x, y, tmin);
+        any = true;
+      }
+      const double tmax = (hi - q0) / (q1 - q0);
+      if (tmax >= 0 && tmax <= 1) {
+        add_lerp(x, y, tmax);
+        any = true;
+      }
+      if (q0 >= lo && q0 <= hi) {
+        add(x);
+        any = true;
+      }
+    }
+    *out = b;
+    return any;
+  }
+
+  void node(uint32_t ni, std::vector<uint32_t> ax[3], int depth, int par) {
+    int md = max_depth.load();
+    while (depth > md && !max_depth.compare_exchange_weak(md, depth)) {
+    }
+    const uint32_t n = (uint32_t)ax[0].size();
+    Box bounds;
+    box_empty(bounds);
+    for (uint32_t r : ax[0]) box_add(bounds, refs->at(r).box);
+    {
+      BNode& nd = pool->at(ni);
+      for (int k = 0; k < 3; ++k) {
+        nd.lo[k] = bounds.lo[k];
+        nd.hi[k] = bounds.hi[k];
+      }
+    }
+    const double parent_area = box_area(bounds);
+    double best_cost = INFINITY;
+    int best_axis = -1;
+    uint32_t best_index = 0;
+    for (int a = 0; a < 3; ++a) {
+      double c;
+      uint32_t i;
+      sweep(ax[a], parent_area, &c, &i);
+      if (c < best_cost) {
+        best_cost = c;
+        best_axis = a;
+        best_index = i;
+      }
+    }
+    // a spatial split's children: whole references kept, and new halves
+    std::vector<uint32_t> sp_left, sp_right, new_left, new_right;
+    bool spatial_split = false;
+    double split_cost = best_cost;
+    uint32_t split_index = best_index;
+    if (best_axis >= 0) {
+      // overlap of the object split's two child boxes (back box includes ref i)
+      Box f, b;
+      box_empty(f);
+      box_empty(b);
+      const std::vector<uint32_t>& l = ax[best_axis];
+      for (uint32_t i = 0; i <= best_index; ++i) box_add(f, refs->at(l[i]).box);
+      for (uint32_t i = best_index; i < n; ++i) box_add(b, refs->at(l[i]).box);
+      for (int k = 0; k < 3; ++k) {
+        f.lo[k] = std::max(f.lo[k], b.lo[k]);
+        f.hi[k] = std::min(f.hi[k], b.hi[k]);
+      }
+      if (root_area > 0 && box_area(f) / root_area > kSplitAlpha) {
+        const int a = best_axis;
+        const double lo = bounds.lo[a], span = bounds.hi[a] - lo;
+        if (span != 0) {
+          // per reference: fragment boxes of the slabs it reaches (clipped
+          // triangle, within the reference's own box); per slab: bounds,
+          // references entering and leaving there
+          struct Cut {
+            uint32_t ref;
+            int first, last;
+            size_t part;  // parts[part + i - first]: fragment in slab i
+          };
+          std::vector<Cut> cuts;
+          std::vector<Box> parts;
+          cuts.reserve(n);
+          parts.reserve(n + n / 2);
+          Box tmp[kSpatialBins];
+          Box slab_box[kSpatialBins];
+          uint32_t enter[kSpatialBins] = {0}, leave[kSpatialBins] = {0};
+          for (auto& sb : slab_box) box_empty(sb);
+          for (uint32_t r : l) {
+            const SRef& ref = refs->at(r);
+            const double rmin = ref.box.lo[a] - lo, rmax = ref.box.hi[a] - lo;
+            // a box starting on the node's upper face maps to slab 64: clamp
+            const int start = std::min(std::max(0, (int)std::floor((rmin / span) * kSpatialBins)), kSpatialBins - 1);
+            const int stop = std::max(std::min((int)std::floor((rmax / span) * kSpatialBins), kSpatialBins - 1), start);
+            Cut c{r, -1, -1, parts.size()};
+            if (start == stop) {
+              // within one slab: its fragment there is the reference itself
+              // (the slab's clip of the triangle contains the reference box)
+              c.first = c.last = start;
+              parts.push_back(ref.box);
+              box_add(slab_box[start], ref.box);
+              ++enter[start];
+              ++leave[start];
+              cuts.push_back(c);
+              continue;
+            }
+            for (int i = start; i <= stop; ++i) {
+              const double p0 = lo + (double)i * span / kSpatialBins;
+              const double p1 = lo + (double)(i + 1) * span / kSpatialBins;
+              Box fb;
+              if (!clip(ref.tri, a, p0, p1, &fb)) fb = Box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+              for (int k = 0; k < 3; ++k) {
+                fb.lo[k] = std::max(fb.lo[k], ref.box.lo[k]);
+                fb.hi[k] = std::min(fb.hi[k], ref.box.hi[k]);
+              }
+              const bool empty = !(fb.lo[0] <= fb.hi[0] && fb.lo[1] <= fb.hi[1] && fb.lo[2] <= fb.hi[2]);
+              if (!empty) {
+                if (c.first < 0) c.first = i;
+                c.last = i;
+                box_add(slab_box[i], fb);
+              }
+              tmp[i - start] = fb;
+            }
+            if (c.first < 0) {  // degenerate clip: keep the reference whole in its first slab
+              c.first = c.last = start;
+              parts.push_back(ref.box);
+              box_add(slab_box[start], ref.box);
+            } else {
+              parts.insert(parts.end(), tmp + (c.first - start), tmp + (c.last - start + 1));
+            }
+            ++enter[c.first];
+            ++leave[c.last];
+            cuts.push_back(std::move(c));
+          }
+          // sweep the 63 inner planes: cost 0.5 + A(L)/A(P) N_L + A(R)/A(P) N_R
+          double suf_area[kSpatialBins + 1];
+          uint32_t suf_n[kSpatialBins + 1];
+          {
+            Box bb;
+            box_empty(bb);
+            uint32_t cnt = 0;
+            suf_area[kSpatialBins] = 0;
+            suf_n[kSpatialBins] = 0;
+            for (int i = kSpatialBins; i-- > 0;) {
+              box_add(bb, slab_box[i]);
+              cnt += leave[i];
+              suf_area[i] = box_area(bb);
+              suf_n[i] = cnt;
+            }
+          }
+          Box bf;
+          box_empty(bf);
+          uint32_t nl = 0;
+          double sc = INFINITY;
+          int plane = -1;
+          for (int k = 1; k < kSpatialBins; ++k) {
+            box_add(bf, slab_box[k - 1]);
+            nl += enter[k - 1];
+            if (nl == 0 || suf_n[k] == 0) continue;
+            const double cost = 0.5 + (box_area(bf) / parent_area) * (double)nl +
+                                (suf_area[k] / parent_area) * (double)suf_n[k];
+            if (cost < sc) {
+              sc = cost;
+              plane = k;
+            }
+          }
+          if (plane > 0 && sc < best_cost) {
+            // references wholly on one side keep their id and box; a
+            // straddling one becomes two, each clipped to its side
+            std::vector<SRef> halves;
+            for (const Cut& c : cuts) {
+              if (c.last < plane) {
+                sp_left.push_back(c.ref);
+              } else if (c.first >= plane) {
+                sp_right.push_back(c.ref);
+              } else {
+                Box lb, rb;
+                box_empty(lb);
+                box_empty(rb);
+                for (int i = c.first; i <= c.last; ++i) box_add(i < plane ? lb : rb, parts[c.part + i - c.first]);
+                const uint32_t t = refs->at(c.ref).tri;
+                halves.push_back(SRef{t, lb});
+                halves.push_back(SRef{t, rb});
+              }
+            }
+            const uint32_t first = halves.empty() ? 0u : refs->try_alloc((uint32_t)halves.size());
+            if (first != UINT32_MAX) {
+              for (size_t i = 0; i < halves.size(); ++i) {
+                refs->at(first + (uint32_t)i) = halves[i];
+                (i % 2 ? new_right : new_left).push_back(first + (uint32_t)i);
+              }
+              spatial_split = true;
+              split_cost = sc;
+            } else {
+              sp_left.clear();
+              sp_right.clear();
+              new_left.clear();
+              new_right.clear();
+            }
+          }
+        }
+      }
+    }
+    if (best_axis < 0 || split_cost > (double)n) {
+      if (best_axis < 0) {
+        degenerate.store(1);
+        best_axis = 0;
+      }
+      std::lock_guard<std::mutex> g(leaf_mu);
+      BNode& nd = pool->at(ni);
+      nd.leaf = 1;
+      nd.split_axis = (uint8_t)best_axis;
+      nd.begin = (uint32_t)leaf_out.size();
+      nd.count = n;
+      for (uint32_t r : ax[best_axis]) leaf_out.push_back(refs->at(r).tri);
+      return;
+    }
+    pool->at(ni).split_axis = (uint8_t)best_axis;
+    std::vector<uint32_t> lax[3], rax[3];
+    if (spatial_split) {
+      // whole references keep their centroid order in every axis list (a
+      // subsequence of a sorted list is sorted); the new halves are sorted
+      // by centroid and merged in (stable: whole references first on ties)
+      spatial.fetch_add(1);
+      for (uint32_t r : sp_left) mark_at(r) = 1;
+      for (uint32_t r : sp_right) mark_at(r) = 2;
+      for (int k = 0; k < 3; ++k) {
+        auto by_centroid = [&](uint32_t x, uint32_t y) {
+          const Box& bx = refs->at(x).box;
+          const Box& by = refs->at(y).box;
+          return (bx.lo[k] + bx.hi[k]) * 0.5 < (by.lo[k] + by.hi[k]) * 0.5;
+        };
+        std::vector<uint32_t> kl, kr;
+        kl.reserve(sp_left.size());
+        kr.reserve(sp_right.size());
+        for (uint32_t r : ax[k]) {
+          const uint8_t m = mark_at(r);
+          if (m == 1) kl.push_back(r);
+          else if (m == 2) kr.push_back(r);
+        }
+        std::vector<uint32_t>().swap(ax[k]);
+        std::stable_sort(new_left.begin(), new_left.end(), by_centroid);
+        std::stable_sort(new_right.begin(), new_right.end(), by_centroid);
+        lax[k].reserve(kl.size() + new_left.size());
+        rax[k].reserve(kr.size() + new_right.size());
+        std::merge(kl.begin(), kl.end(), new_left.begin(), new_left.end(), std::back_inserter(lax[k]), by_centroid);
+        std::merge(kr.begin(), kr.end(), new_right.begin(), new_right.end(), std::back_inserter(rax[k]), by_centroid);
+      }
+      for (uint32_t r : sp_left) mark_at(r) = 0;
+      for (uint32_t r : sp_right) mark_at(r) = 0;
+    } else {
+      const std::vector<uint32_t>& split = ax[best_axis];
+      for (uint32_t i = 0; i <= split_index; ++i) mark_at(split[i]) = 1;
+      for (int k = 0; k < 3; ++k) {
+        for (uint32_t r : ax[k]) (mark_at(r) ? lax[k] : rax[k]).push_back(r);
+        std::vector<uint32_t>().swap(ax[k]);
+      }
+      for (uint32_t r : lax[0]) mark_at(r) = 0;
+    }
+    uint32_t l = pool->alloc(), r = pool->alloc();
+    {
+      BNode& p = pool->at(ni);
+      p.left = (int32_t)l;
+      p.right = (int32_t)r;
+    }
+    if (par > 0 && n > 32768) {
+      auto fut = std::async(std::launch::async, [&]() { node(l, lax, depth + 1, par - 1); });
+      node(r, rax, depth + 1, par - 1);
+      fut.get();
+    } else {
+      node(l, lax, depth + 1, 0);
+      node(r, rax, depth + 1, 0);
+    }
+  }
+
+  uint8_t& mark_at(uint32_t id) { return mark[id]; }
+};
+
 // JS ToInt16 (ECMAScript ToInt16: truncate, wrap modulo 2^16)
 inline int16_t js_to_int16(double x) {
   if (!std::isfinite(x)) return 0;
@@ -275,6 +670,11 @@ extern "C" {
 
 int pk_bvh_build(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
                  pk_bvh** out) {
+  return pk_bvh_build_ex(positions, nv, triangles, nt, num_threads, 0, out);
+}
+
+int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
+                    uint32_t flags, pk_bvh** out) {
   if (!positions || !triangles || !out || nt == 0 || nv == 0) {
     pk_set_error("pk_bvh_build: empty or null input");
     return PK_E_INVALID;
@@ -323,24 +723,53 @@ int pk_bvh_build(const double* positions, uint32_t nv, const int32_t* triangles,
     for (auto& t : ts) t.join();
   }
   std::vector<double>().swap(cen);
-  bvh->leaf_tris.resize(nt);
-  NodePool pool((size_t)nt * 2 + 2);
-  Build b;
-  b.ref_box = &ref_box;
-  for (int a = 0; a < 3; ++a) b.axis[a] = ax[a];
-  b.leaf_out = bvh->leaf_tris.data();
-  b.in_left.assign(nt, 0);
-  b.pool = &pool;
   int par = 0;
   for (int t = num_threads; t > 1; t >>= 1) ++par;
-  uint32_t root = pool.alloc();
-  b.node(root, 0, nt, 1, par + 1);
-  bvh->degenerate = b.degenerate.load();
-  bvh->binary_depth = b.max_depth.load();
+  bvh->spatial_splits = (flags & PK_BVH_SPATIAL_SPLITS) != 0;
+  // SBVH references: the triangles, then fragments (each spatial split
+  // adds at most 64 per reference it cuts; the pool bounds the total)
+  const size_t ref_cap = std::min((size_t)nt * 8 + (1u << 20), (size_t)UINT32_MAX - (1u << 17));
+  std::unique_ptr<NodePool> pool(new NodePool(bvh->spatial_splits ? ref_cap * 2 + 2 : (size_t)nt * 2 + 2));
+  uint32_t root = pool->alloc();
+  if (!bvh->spatial_splits) {
+    bvh->leaf_tris.resize(nt);
+    Build b;
+    b.ref_box = &ref_box;
+    for (int a = 0; a < 3; ++a) b.axis[a] = ax[a];
+    b.leaf_out = bvh->leaf_tris.data();
+    b.in_left.assign(nt, 0);
+    b.pool = pool.get();
+    b.node(root, 0, nt, 1, par + 1);
+    bvh->degenerate = b.degenerate.load();
+    bvh->binary_depth = b.max_depth.load();
+  } else {
+    Pool<SRef> refs(ref_cap);
+    for (uint32_t t = 0; t < nt; ++t) refs.at(refs.alloc()) = SRef{t, ref_box[t]};
+    SBuild b;
+    b.pos = positions;
+    b.tris = triangles;
+    b.pool = pool.get();
+    b.refs = &refs;
+    b.mark.assign(refs.capacity(), 0);
+    Box rb;
+    box_empty(rb);
+    for (uint32_t t = 0; t < nt; ++t) box_add(rb, ref_box[t]);
+    b.root_area = box_area(rb);
+    std::vector<uint32_t> lists[3];
+    for (int a = 0; a < 3; ++a) lists[a].assign(ax[a], ax[a] + nt);
+    std::vector<uint32_t>().swap(arr);
+    b.node(root, lists, 1, par + 1);
+    bvh->degenerate = b.degenerate.load();
+    bvh->binary_depth = b.max_depth.load();
+    bvh->leaf_tris.swap(b.leaf_out);
+    bvh->num_spatial_splits = b.spatial.load();
+  }
+  NodePool& pool_ref = *pool;
   // compact: renumber nodes in allocation order (root is 0)
-  const uint32_t nn = pool.size();
+  const uint32_t nn = pool_ref.size();
   bvh->nodes.resize(nn);
-  for (uint32_t i = 0; i < nn; ++i) bvh->nodes[i] = pool.at(i);
+  for (uint32_t i = 0; i < nn; ++i) bvh->nodes[i] = pool_ref.at(i);
+  pool.reset();
   int largest = 0;
   for (auto& n : bvh->nodes)
     if (n.leaf) largest = std::max(largest, (int)n.count);
@@ -515,6 +944,19 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
       for (int k = 0; k < 3; ++k) {
         b.lo[k] = std::min(l.lo[k], r.lo[k]);
         b.hi[k] = std::max(l.hi[k], r.hi[k]);
+      }
+    } else if (bvh->spatial_splits) {
+      // a leaf's references are clipped fragments: its double box, mapped
+      // to the normalized space and rounded outward to f32
+      const BNode& n = nodes[i];
+      for (int k = 0; k < 3; ++k) {
+        const double lo = (n.lo[k] - bvh->norm_center[k]) * bvh->norm_scale;
+        const double hi = (n.hi[k] - bvh->norm_center[k]) * bvh->norm_scale;
+        float fl = (float)lo, fh = (float)hi;
+        if ((double)fl > lo) fl = std::nextafter(fl, -INFINITY);
+        if ((double)fh < hi) fh = std::nextafter(fh, INFINITY);
+        b.lo[k] = fl;
+        b.hi[k] = fh;
       }
     } else {
       const BNode& n = nodes[i];

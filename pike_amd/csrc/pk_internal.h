@@ -28,6 +28,8 @@ struct pk_bvh {
   int degenerate = 0;
   int binary_depth = 0, wide_depth = 0, largest_leaf = 0;
   int64_t num_binary = 0;         // nodes of the bvh.js tree (before virtual splits)
+  bool spatial_splits = false;    // built with SBVH spatial splits (leaves hold fragments)
+  int64_t num_spatial_splits = 0;
   double norm_center[3] = {0, 0, 0};
   double norm_scale = 1;
   std::vector<float> verts;       // normalized f32 [nv][3]

@@ -11,6 +11,7 @@
 #include <node_api.h>
 
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -215,9 +216,12 @@ napi_value BvhBuild(napi_env env, napi_callback_info info) {
   if (!typed(env, a.argv[1], napi_int32_array, &tri, &nt, "triangles")) return nullptr;
   int32_t threads = 0;
   if (a.argc > 2) napi_get_value_int32(env, a.argv[2], &threads);
+  bool spatial = false;  // bvhBuild(positions, triangles, threads, spatialSplits)
+  if (a.argc > 3) napi_get_value_bool(env, a.argv[3], &spatial);
   pk_bvh* b = nullptr;
-  PK_CALL(pk_bvh_build((const double*)pos, (uint32_t)(np / 3), (const int32_t*)tri, (uint32_t)(nt / 4), threads, &b),
-          "pk_bvh_build");
+  PK_CALL(pk_bvh_build_ex((const double*)pos, (uint32_t)(np / 3), (const int32_t*)tri, (uint32_t)(nt / 4), threads,
+                          spatial ? PK_BVH_SPATIAL_SPLITS : 0u, &b),
+          "pk_bvh_build_ex");
   return wrap(env, kBvh, b);
 }
 
@@ -247,7 +251,8 @@ napi_value BvhExportReference(napi_env env, napi_callback_info info) {
   PK_CALL(pk_bvh_info(b, v), "pk_bvh_info");
   void *nodes, *tris, *verts, *norm;
   napi_value tn = make_typed(env, napi_int32_array, (size_t)v[0] * 5, 4, &nodes);
-  napi_value tt = make_typed(env, napi_int32_array, (size_t)nt * 4, 4, &tris);
+  // leaf triangle references (more than the triangles with spatial splits)
+  napi_value tt = make_typed(env, napi_int32_array, (size_t)std::max<int64_t>(v[2], nt) * 4, 4, &tris);
   napi_value tv = make_typed(env, napi_float32_array, (size_t)nv * 4, 4, &verts);
   napi_value tm = make_typed(env, napi_float64_array, 4, 8, &norm);
   if (!tn || !tt || !tv || !tm) return nullptr;

@@ -165,3 +165,87 @@ def test_bvh_build_rejects_bad_input():
     assert e.value.code == -1 and "out of range" in str(e.value)
     with pytest.raises(pike_amd.PikeError):
         pike_amd.BVH(pos, np.zeros((0, 4), np.int32))
+
+
+def _slivers(n, seed):
+    """Long thin triangles (one ~0.3-0.6 edge, ~0.002 wide) at random
+    orientations in the unit cube: the case spatial splits are for."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.2, 0.8, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = np.cross(d, rng.normal(size=(n, 3)))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    b = a + d * rng.uniform(0.3, 0.6, (n, 1))
+    c = a + w * 0.002
+    pos = np.stack([a, b, c], 1).reshape(-1, 3)
+    tri = np.concatenate([np.arange(3 * n).reshape(-1, 3), np.zeros((n, 1), np.int64)], 1).astype(np.int32)
+    return pos, tri
+
+
+def _reached_ids(nodes, tris, o, d, tmax):
+    """Triangle ids in the leaves whose quantized boxes the ray segment
+    [0, tmax] touches (a float64 model of the wide traversal without the
+    near-to-far order: closest-hit traversal can only find a hit through
+    such leaves)."""
+    inv = np.where(np.abs(d) > 1e-30, 1.0 / np.where(d == 0, 1, d), np.inf)
+    ids, stack = set(), [0]
+    while stack:
+        n = nodes[stack.pop()]
+        lo, hi = _decode_child_boxes(n)
+        for c in range(n["nchild"]):
+            with np.errstate(invalid="ignore"):
+                t0, t1 = (lo[c] - o) * inv, (hi[c] - o) * inv
+            tn = np.nanmax(np.minimum(t0, t1))
+            tf = np.nanmin(np.maximum(t0, t1))
+            if tn <= tf and tf >= 0 and tn <= tmax * (1 + 1e-9) + 1e-9:
+                if n["meta"][c]:
+                    ids.update(tris["id"][n["child"][c]:n["child"][c] + n["meta"][c]].tolist())
+                else:
+                    stack.append(n["child"][c])
+    return ids
+
+
+def test_spatial_splits_keep_every_closest_hit_reachable():
+    pos, tri = _slivers(1500, 3)
+    data = _data(pos, tri)
+    b = pike_amd.BVH(data.positions, data.triangles, spatial_splits=True)
+    info = b.info()
+    assert info["leaf_triangles"] > data.num_triangles  # fragments in several leaves
+    nodes, wt, tidx = b.export_wide()
+    assert set(wt["id"].tolist()) == set(range(data.num_triangles))
+    o = OracleScene(data, threads=8)
+    o.build_bvh()
+    nm = b.normalizer()
+    rays = scenes.incoherent_rays(3000, nm, seed=4)
+    ids, t, _, _ = o.trace(rays, "brute")
+    hits = np.flatnonzero(ids >= 0)
+    assert hits.size > 300
+    for r in hits:
+        reach = _reached_ids(nodes, wt, rays[r, :3].astype(np.float64), rays[r, 4:7].astype(np.float64), float(t[r]))
+        assert int(ids[r]) in reach, r
+    # the reference export holds every leaf reference; its triangles are the scene's
+    _, rtris, _, _ = b.export_reference()
+    assert rtris.shape[0] == info["leaf_triangles"]
+    assert {tuple(x) for x in rtris.tolist()} == {tuple(x) for x in data.triangles.tolist()}
+
+
+def test_spatial_splits_reduce_sah_cost_on_slivers():
+    """bvh.js's cost model on the same triangles: with spatial splits the
+    tree traverses fewer boxes per ray (measured with the model above)."""
+    pos, tri = _slivers(800, 5)
+    data = _data(pos, tri)
+    plain = pike_amd.BVH(data.positions, data.triangles)
+    sbvh = pike_amd.BVH(data.positions, data.triangles, spatial_splits=True)
+    rays = scenes.incoherent_rays(300, plain.normalizer(), seed=6)
+    o = OracleScene(data, threads=8)
+    o.build_bvh()
+    ids, t, _, _ = o.trace(rays, "brute")
+    tmax = np.where(ids >= 0, t, 10.0)
+
+    def tested(b):
+        nodes, wt, _ = b.export_wide()
+        return sum(len(_reached_ids(nodes, wt, rays[r, :3].astype(np.float64), rays[r, 4:7].astype(np.float64),
+                                    float(tmax[r]))) for r in range(len(rays)))
+
+    assert tested(sbvh) < tested(plain)
