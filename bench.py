@@ -61,8 +61,9 @@ def run_render(args, pike_amd, scenes):
     data = scenes.config2(seed=2, cells=args.cells)
     log(f"config2 scene: {data.num_triangles} triangles ({time.time() - t0:.1f}s)")
     t0 = time.time()
-    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1))
-    log(f"BVH: {bvh.info()} ({time.time() - t0:.1f}s)")
+    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1),
+                       spatial_splits=args.bvh == "sbvh")
+    log(f"BVH ({args.bvh}): {bvh.info()} ({time.time() - t0:.1f}s)")
     ctx = pike_amd.Context(0)
     scene = pike_amd.Scene(ctx, data, bvh)
     cam = scenes.normalized_camera(data.camera, bvh.normalizer())
@@ -150,7 +151,7 @@ def run_batching(args, pike_amd, scenes, ctx):
     t0 = time.time()
     sc, views = scenes.config4(seed=4)
     threads = min(16, os.cpu_count() or 1)
-    bvhs = [pike_amd.BVH(s.positions, s.triangles, threads=threads) for s in sc]
+    bvhs = [pike_amd.BVH(s.positions, s.triangles, threads=threads, spatial_splits=args.bvh == "sbvh") for s in sc]
     per_scene = [pike_amd.Scene(ctx, s, b) for s, b in zip(sc, bvhs)]
     merged = pike_amd.BatchScene(ctx, sc, bvhs)
     cams = [scenes.normalized_camera(c, bvhs[k].normalizer()) for k, _, _, _, c in views]
@@ -234,7 +235,8 @@ def run_diff(args, pike_amd, scenes):
     ATLAS, ENV = _ffi.PK_PARAM_ATLAS, _ffi.PK_PARAM_ENV
     t0 = time.time()
     data = scenes.config5(seed=5)
-    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1))
+    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1),
+                       spatial_splits=args.bvh == "sbvh")
     cam = scenes.normalized_camera(data.camera, bvh.normalizer())
     ctx = pike_amd.Context(0)
     scene = pike_amd.Scene(ctx, data, bvh)
@@ -449,6 +451,8 @@ def main():
                                                   "autotuning (e.g. the 'extra.autotune.shapes' of a previous run)")
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
+    ap.add_argument("--bvh", choices=["sah", "sbvh"], default="sah",
+                    help="BVH kind of the render, batching and differentiation scenes")
     ap.add_argument("--soup-bvh", choices=["sbvh", "sah", "both"], default="both",
                     help="BVH kind(s) the soup is traced under: spatial splits (the workload's), plain SAH, or both")
     ap.add_argument("--trace-iters", type=int, default=5)
@@ -523,9 +527,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural terrain + icospheres, log-normal HDR env with sun; seeded PCG)",
-        "config": {"workload": "plain wavefront path tracing, 3,121,152 triangles, 2048x1024 env (2D CDF "
-                               "importance sampling), 1920x1080, depth 8, 1 spp per step", "width": args.width, "height": args.height,
-                   "depth": args.depth, "triangles": res["triangles"]},
+        "config": {"workload": f"plain wavefront path tracing, {res['triangles']:,} triangles "
+                               f"({args.bvh.upper()} wide BVH), 2048x1024 env (2D CDF importance sampling), "
+                               f"{args.width}x{args.height}, depth {args.depth}, 1 spp per step",
+                   "width": args.width, "height": args.height, "depth": args.depth, "triangles": res["triangles"],
+                   "bvh": args.bvh},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -5,12 +5,15 @@
 // (trace_pass.js:220-263: BFS nodes of 5 words, leaf-ordered triangles,
 // normalized float32 vertices).
 
+// options: { spatialSplits: build with SBVH spatial splits (the branch
+// bvh.js:128-135 leaves disabled), threads: builder threads (0 = all) }
 class BVH {
-  constructor(native, desc) {
+  constructor(native, desc, options) {
+    const opts = options || {};
     this.native = native;
     this.numVertices = desc.positions.length / 3;
     this.numTriangles = desc.triangles.length / 4;
-    this.handle = native.bvhBuild(desc.positions, desc.triangles, 0);
+    this.handle = native.bvhBuild(desc.positions, desc.triangles, opts.threads || 0, !!opts.spatialSplits);
     const info = native.bvhInfo(this.handle);
     this.info = info;
     this.depth = info.binaryDepth;

@@ -12,6 +12,8 @@ const { PikeRenderer } = require('./pike');
 const { loadRawScene, saveRawScene } = require('./raw_scene');
 const { sceneDesc } = require('./scene_desc');
 const { Scene } = require('./scene');
+const { CameraController } = require('./camera_controller');
+const { Raycaster } = require('./raycaster');
 const { GLTFLoader } = require('./gltf_loader');
 const { TexturePacker } = require('./texture_packer');
 const png = require('./png');
@@ -20,5 +22,5 @@ const native = require('./native');
 module.exports = {
   PikeDevice, RenderState, CameraPass, TracePass, ShadeHitPass, TraceShadowPass, ShadeMissPass, AccumulatePass,
   PostProcessPass, BVH, EnvironmentGenerator, PikeRenderer, loadRawScene, saveRawScene, sceneDesc, Scene,
-  GLTFLoader, TexturePacker, png, native,
+  GLTFLoader, TexturePacker, png, native, CameraController, Raycaster,
 };

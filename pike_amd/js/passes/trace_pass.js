@@ -13,8 +13,10 @@ class TracePass {
     this.cameraPass = cameraPass;
     this.scene = scene;
     this.desc = sceneDesc(scene);
-    this.bvh = new BVH(device.native, this.desc);
+    this.bvh = new BVH(device.native, this.desc, scene.bvhOptions);
     this.sceneHandle = device.native.sceneUpload(device.ctx, this.desc, this.bvh.handle);
+    // what Raycaster needs to cast rays against the uploaded scene
+    this.bvh.tracer = { native: device.native, ctx: device.ctx, scene: this.sceneHandle };
   }
 
   static async create(device, cameraPass, scene) {

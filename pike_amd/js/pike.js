@@ -10,6 +10,7 @@ const { TracePass } = require('./passes/trace_pass');
 const { ShadeHitPass, TraceShadowPass, ShadeMissPass, AccumulatePass } = require('./passes/shade_passes');
 const { PostProcessPass } = require('./passes/postprocess_pass');
 const { EnvironmentGenerator } = require('./env_sampler');
+const { Raycaster } = require('./raycaster');
 
 class PikeRenderer {
   constructor(scene, resolution, options) {
@@ -37,6 +38,34 @@ class PikeRenderer {
     this.shadeMissPass = await ShadeMissPass.create(this.device, this.cameraPass, this.tracePass, envGenerator);
     this.accumulatePass = await AccumulatePass.create(this.device, this.resolution, this.renderState);
     this.postProcessPass = await PostProcessPass.create(this.device, 'rgba8unorm', null, this.accumulatePass);
+    this.raycaster = new Raycaster(this.tracePass.getBVH());   // pike.js:117
+  }
+
+  // autofocus on what the camera ray hits (pike.js:88-90)
+  focusCamera() {
+    this.focalDepth = 1 - 1 / this.raycaster.cast(this.camera);
+    return this.focalDepth;
+  }
+
+  // The reference builds its CameraController over the canvas with an
+  // onChange that refocuses and restarts accumulation (pike.js:65-76); a
+  // headless host hands over the controller it drives.  The controller's ray
+  // and field of view become the camera of the following ticks.
+  useController(controller) {
+    this.controller = controller;
+    const chained = controller.onChange;
+    const sync = () => {
+      const r = controller.getCameraRay();
+      this.camera = { origin: r.origin.slice(), dir: r.dir.slice(), fov: controller.getFov() };
+    };
+    sync();
+    controller.onChange = (state) => {
+      sync();
+      if (this.raycaster) this.focusCamera();
+      if (this.renderState) this.renderState.resetSamples();
+      chained(state);
+    };
+    return controller;
   }
 
   tick() {
