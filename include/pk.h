@@ -190,6 +190,40 @@ int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, 
  * CPU traversal cost model): wide_nodes * 128 bytes, leaf_triangles * 48
  * bytes of triangle records, and leaf_triangles * 4 words {i0,i1,i2,mat}. */
 int pk_bvh_export_wide(const pk_bvh* bvh, void* nodes128, void* tris48, int32_t* tri_index4);
+/* The normalizer (center xyz, scale) of any BVH, instanced ones included. */
+int pk_bvh_normalizer(const pk_bvh* bvh, double* normalizer4);
+
+/* ---- instancing: a two-level BVH (the reference bakes every prop into one
+ * triangle list, scene.js:159-170, and lists instancing as a TODO,
+ * README.md:20).  A mesh is geometry in its own object space; an instance
+ * places a mesh in the scene with an affine transform and may override its
+ * material.  One BVH is built per mesh and one over the instances' boxes;
+ * the scene's own triangles (pk_scene_desc positions/triangles, possibly
+ * none) form an implicit first instance with the identity transform.
+ * Triangle ids are those of the equivalent baked scene: the scene's own
+ * triangles first, then each instance's mesh triangles in instance order.
+ * Limits: fewer than 2^31 baked triangles and 2^31 instances. ---- */
+typedef struct pk_mesh_desc {
+  uint32_t num_vertices;
+  const double* positions;    /* object space [num_vertices][3] */
+  const uint32_t* attributes; /* [num_vertices][4] object-space tangent frame + uv, or NULL */
+  uint32_t num_triangles;
+  const int32_t* triangles;   /* [num_triangles][4] {i0, i1, i2, mat} */
+} pk_mesh_desc;
+
+typedef struct pk_instance_desc {
+  uint32_t mesh;              /* index into the meshes */
+  int32_t material;           /* material of all its triangles; -1: the mesh's own */
+  double transform[12];       /* object -> scene space, row-major 3x4 (invertible) */
+} pk_instance_desc;
+
+/* Builds the two-level BVH; pk_scene_upload then takes it with the scene
+ * desc (its materials, atlas, environment and the attributes of its own
+ * vertices).  flags as for pk_bvh_build_ex (applied to every mesh).  The
+ * reference layout export is not available for instanced BVHs. */
+int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
+                           const pk_instance_desc* instances, uint32_t num_instances, int num_threads,
+                           uint32_t flags, pk_bvh** out);
 
 /* ---- scene (TracePass / ShadeHitPass / ShadeMissPass resources) ---- */
 int pk_scene_upload(pk_ctx* ctx, const pk_scene_desc* desc, const pk_bvh* bvh, pk_scene** out);

@@ -291,6 +291,19 @@ struct or_scene {
   std::vector<double> erad64;
   // marginal (rows) / conditional (columns) CDF tables of the CDF sampler
   std::vector<float> row_cdf, col_cdf;
+  // instancing: per-mesh sub-scenes (the scene's own geometry first, when it
+  // has triangles) and the placements, each with its scene -> mesh map
+  struct Inst {
+    uint32_t mesh = 0;
+    int32_t material = -1;
+    double t[12] = {0};
+    double w2o64[12] = {0}, o2w64[9] = {0};
+    float w2o32[12] = {0}, o2w32[9] = {0};
+    int32_t id_base = 0;
+  };
+  bool instanced = false, built = false;
+  std::vector<std::unique_ptr<or_scene>> meshes;
+  std::vector<Inst> insts;
 };
 
 namespace {
@@ -474,9 +487,10 @@ template <> inline V3<double> vert<double>(const or_scene* s, uint32_t vi) {
 }
 
 template <class T> struct HitT {
-  int32_t id = -1;  // original triangle id
+  int32_t id = -1;  // original triangle id (baked numbering with instancing)
   T t;
   uint32_t bary = 0;
+  int32_t inst = -1, local = -1;  // instancing: placement and mesh triangle
 };
 
 // Literal trace.wgsl intersectScene (:148-189) over the 20-byte nodes.
@@ -655,6 +669,75 @@ template <class T> HitT<T> brute_trace(const or_scene* s, V3<T> o, V3<T> d, bool
       best.t = r;
       best.id = (int32_t)id;
       best.bary = bm;
+      if (any_hit) return best;
+    }
+  }
+  return best;
+}
+
+template <class T> const std::vector<CNode>& nodes_of(const or_scene* s);
+template <> const std::vector<CNode>& nodes_of<float>(const or_scene* s) { return s->cnodes32; }
+template <> const std::vector<CNode>& nodes_of<double>(const or_scene* s) { return s->cnodes64; }
+
+// Scene -> mesh map of a placement, in T: each output is ((a x + b y) + c z)
+// (+ d for points), the order the device evaluates.
+template <class T> const T* w2o_of(const or_scene::Inst& I);
+template <> const float* w2o_of<float>(const or_scene::Inst& I) { return I.w2o32; }
+template <> const double* w2o_of<double>(const or_scene::Inst& I) { return I.w2o64; }
+template <class T> const T* o2w_of(const or_scene::Inst& I);
+template <> const float* o2w_of<float>(const or_scene::Inst& I) { return I.o2w32; }
+template <> const double* o2w_of<double>(const or_scene::Inst& I) { return I.o2w64; }
+
+template <class T> V3<T> map_point(const T* m, V3<T> p) {
+  return v3(((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3], ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7],
+            ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11]);
+}
+template <class T> V3<T> map_dir(const T* m, V3<T> p) {
+  return v3((m[0] * p.x + m[1] * p.y) + m[2] * p.z, (m[4] * p.x + m[5] * p.y) + m[6] * p.z,
+            (m[8] * p.x + m[9] * p.y) + m[10] * p.z);
+}
+
+// Closest (or any) hit over the whole scene.  With instancing: argmin over
+// (t, baked id) of every placement's canonical hit, each found with the ray
+// mapped into that mesh's space (t is the same parameter in both spaces).
+// No placement is culled by its box, so the minimum is exact.
+template <class T> HitT<T> scene_trace(const or_scene* s, V3<T> o, V3<T> d, bool any_hit) {
+  if (!s->instanced) return canonical_trace<T>(s, nodes_of<T>(s), o, d, any_hit);
+  HitT<T> best;
+  best.t = T(100000.0);
+  for (size_t i = 0; i < s->insts.size(); ++i) {
+    const or_scene::Inst& I = s->insts[i];
+    const or_scene* m = s->meshes[I.mesh].get();
+    const T* w = w2o_of<T>(I);
+    HitT<T> h = canonical_trace<T>(m, nodes_of<T>(m), map_point<T>(w, o), map_dir<T>(w, d), any_hit);
+    if (h.id < 0) continue;
+    const int32_t gid = h.id + I.id_base;
+    if (h.t < best.t || (h.t == best.t && gid < best.id)) {
+      best = h;
+      best.id = gid;
+      best.inst = (int32_t)i;
+      best.local = h.id;
+      if (any_hit) return best;
+    }
+  }
+  return best;
+}
+
+template <class T> HitT<T> scene_brute(const or_scene* s, V3<T> o, V3<T> d, bool any_hit) {
+  if (!s->instanced) return brute_trace<T>(s, o, d, any_hit);
+  HitT<T> best;
+  best.t = T(100000.0);
+  for (size_t i = 0; i < s->insts.size(); ++i) {
+    const or_scene::Inst& I = s->insts[i];
+    const T* w = w2o_of<T>(I);
+    HitT<T> h = brute_trace<T>(s->meshes[I.mesh].get(), map_point<T>(w, o), map_dir<T>(w, d), any_hit);
+    if (h.id < 0) continue;
+    const int32_t gid = h.id + I.id_base;
+    if (h.t < best.t || (h.t == best.t && gid < best.id)) {
+      best = h;
+      best.id = gid;
+      best.inst = (int32_t)i;
+      best.local = h.id;
       if (any_hit) return best;
     }
   }
@@ -1167,17 +1250,42 @@ template <> double attr_uv<double>(const or_scene* s, uint32_t vi, int comp) {
 // getSurfaceInteraction (:179-200) with interpolateVertexAttribute (:363-394)
 template <class T, class U> SI<T> surface(const Ctx<T>& c, V3<T> ro, V3<T> rd, const HitT<T>& h, EV<U>* ev) {
   const or_scene* s = c.s;
-  const int32_t* tr = &s->tri[(size_t)h.id * 4];
+  // geometry of the hit: the scene's, or the placement's mesh
+  const or_scene* g = s;
+  const or_scene::Inst* inst = nullptr;
+  int32_t tid = h.id;
+  if (s->instanced) {
+    inst = &s->insts[(size_t)h.inst];
+    g = s->meshes[inst->mesh].get();
+    tid = h.local;
+  }
+  const int32_t* tr = &g->tri[(size_t)tid * 4];
   T bu = (T)(float)(h.bary & 0xffff) / T(65535), bv = (T)(float)(h.bary >> 16) / T(65535);
   V3<T> bary = v3(T(1) - bu - bv, bu, bv);
   auto interp = [&](int word) {
-    M3<T> m{attr_vec<T>(s, tr[0], word), attr_vec<T>(s, tr[1], word), attr_vec<T>(s, tr[2], word)};
+    M3<T> m{attr_vec<T>(g, tr[0], word), attr_vec<T>(g, tr[1], word), attr_vec<T>(g, tr[2], word)};
     return normalize(mat_vec(m, bary));
   };
   V3<T> tangent = interp(0), bitangent = interp(1), normal = interp(2);
-  T uvx = attr_uv<T>(s, tr[0], 0) * bary.x + attr_uv<T>(s, tr[1], 0) * bary.y + attr_uv<T>(s, tr[2], 0) * bary.z;
-  T uvy = attr_uv<T>(s, tr[0], 1) * bary.x + attr_uv<T>(s, tr[1], 1) * bary.y + attr_uv<T>(s, tr[2], 1) * bary.z;
-  const int32_t* mi = &s->mat[(size_t)tr[3] * 4];
+  if (inst) {
+    // mesh -> scene: tangents by the linear part, the normal by the
+    // transpose of the scene -> mesh linear part
+    const T* L = o2w_of<T>(*inst);
+    const T* A = w2o_of<T>(*inst);
+    auto lin = [&](V3<T> v) {
+      return v3((L[0] * v.x + L[1] * v.y) + L[2] * v.z, (L[3] * v.x + L[4] * v.y) + L[5] * v.z,
+                (L[6] * v.x + L[7] * v.y) + L[8] * v.z);
+    };
+    tangent = normalize(lin(tangent));
+    bitangent = normalize(lin(bitangent));
+    normal = normalize(v3((A[0] * normal.x + A[4] * normal.y) + A[8] * normal.z,
+                          (A[1] * normal.x + A[5] * normal.y) + A[9] * normal.z,
+                          (A[2] * normal.x + A[6] * normal.y) + A[10] * normal.z));
+  }
+  T uvx = attr_uv<T>(g, tr[0], 0) * bary.x + attr_uv<T>(g, tr[1], 0) * bary.y + attr_uv<T>(g, tr[2], 0) * bary.z;
+  T uvy = attr_uv<T>(g, tr[0], 1) * bary.x + attr_uv<T>(g, tr[1], 1) * bary.y + attr_uv<T>(g, tr[2], 1) * bary.z;
+  const int32_t mat_id = (inst && inst->material >= 0) ? inst->material : tr[3];
+  const int32_t* mi = &s->mat[(size_t)mat_id * 4];
   T nm[4], dc[4], mr[4], em[4];
   sample_tex<T>(s, mi[2], uvx, uvy, nm);
   V3<T> map_normal = v3((nm[0] - T(0.5)) * T(2), (nm[1] - T(0.5)) * T(2), (nm[2] - T(0)) * T(1));
@@ -1240,7 +1348,7 @@ template <class U> struct PixelOut {
 };
 
 template <class T>
-HitT<T> closest(const Ctx<T>& c, V3<T> o, V3<T> d) { return canonical_trace<T>(c.s, *c.nodes, o, d, false); }
+HitT<T> closest(const Ctx<T>& c, V3<T> o, V3<T> d) { return scene_trace<T>(c.s, o, d, false); }
 
 template <class T, class U>
 void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
@@ -1319,7 +1427,7 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
         V3<T> sdir = mat_vec(si.basis, es.wi);
         V3<U> sthr = scale * thr;
         out.shadow++;
-        HitT<T> oh = canonical_trace<T>(c.s, *c.nodes, si.origin, sdir, true);
+        HitT<T> oh = scene_trace<T>(c.s, si.origin, sdir, true);
         if (oh.id < 0) add(env_color<T, U>(c, sdir) * sthr);
       }
     }
@@ -1477,7 +1585,109 @@ or_scene* or_scene_create(const pk_scene_desc* d) {
 
 void or_scene_destroy(or_scene* s) { delete s; }
 
+int or_scene_set_instances(or_scene* s, const pk_mesh_desc* meshes, uint32_t num_meshes,
+                           const pk_instance_desc* instances, uint32_t num_instances) {
+  s->instanced = true;
+  s->meshes.clear();
+  s->insts.clear();
+  auto sub = [](uint32_t nv, const double* pos, const uint32_t* attr, uint32_t nt, const int32_t* tri) {
+    pk_scene_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.num_vertices = nv;
+    d.positions = pos;
+    d.attributes = attr;
+    d.num_triangles = nt;
+    d.triangles = tri;
+    d.env_format = PK_ENV_NONE;
+    return std::unique_ptr<or_scene>(or_scene_create(&d));
+  };
+  uint32_t first = 0;
+  if (s->nt > 0) {  // the scene's own triangles: the first placement, identity transform
+    s->meshes.push_back(sub(s->nv, s->pos.data(), s->attr.data(), s->nt, s->tri.data()));
+    or_scene::Inst I;
+    I.mesh = 0;
+    const double id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    std::memcpy(I.t, id, sizeof(id));
+    s->insts.push_back(I);
+    first = 1;
+  }
+  for (uint32_t m = 0; m < num_meshes; ++m)
+    s->meshes.push_back(sub(meshes[m].num_vertices, meshes[m].positions, meshes[m].attributes,
+                            meshes[m].num_triangles, meshes[m].triangles));
+  for (uint32_t i = 0; i < num_instances; ++i) {
+    if (instances[i].mesh >= num_meshes) return -1;
+    or_scene::Inst I;
+    I.mesh = first + instances[i].mesh;
+    I.material = instances[i].material;
+    std::memcpy(I.t, instances[i].transform, sizeof(I.t));
+    s->insts.push_back(I);
+  }
+  return s->insts.empty() ? -1 : 0;
+}
+
+int or_bvh_build(or_scene* s, int num_threads);
+
+// Instanced build: each mesh's BVH and normalizer as for a scene of its own;
+// the scene normalizer over the placements' boxes (their meshes' triangle
+// vertices mapped by x = ((T0 x + T1 y) + T2 z) + T3); per placement the
+// scene -> mesh map A = (s_m / s) L^-1 with L^-1 = adj(L) / det(L), offset
+// b = s_m ((L^-1 (c - t)) - c_m), and mesh -> scene directions (s / s_m) L,
+// composed in double and rounded to f32 once.
+static int build_instanced(or_scene* s, int num_threads) {
+  for (auto& m : s->meshes)
+    if (m->nt > 0 && or_bvh_build(m.get(), num_threads) < -1) return -2;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (const auto& I : s->insts) {
+    const or_scene& m = *s->meshes[I.mesh];
+    if (m.nt == 0) return -1;
+    for (size_t v = 0; v < (size_t)m.nt * 3; ++v) {
+      const double* p = &m.pos[(size_t)m.tri[(v / 3) * 4 + v % 3] * 3];
+      for (int k = 0; k < 3; ++k) {
+        const double x = ((I.t[k * 4] * p[0] + I.t[k * 4 + 1] * p[1]) + I.t[k * 4 + 2] * p[2]) + I.t[k * 4 + 3];
+        lo[k] = std::min(lo[k], x);
+        hi[k] = std::max(hi[k], x);
+      }
+    }
+  }
+  double span[3];
+  for (int k = 0; k < 3; ++k) span[k] = hi[k] - lo[k];
+  const double longest = std::max(std::max(span[0], span[1]), span[2]);
+  for (int k = 0; k < 3; ++k) s->norm_center[k] = lo[k] + span[k] * 0.5;
+  s->norm_scale = 2 / longest;
+  const double sc = s->norm_scale;
+  int32_t id_base = 0;
+  for (auto& I : s->insts) {
+    const or_scene& m = *s->meshes[I.mesh];
+    const double* t = I.t;
+    const double a = t[0], b = t[1], c = t[2], d = t[4], e = t[5], f = t[6], g = t[8], h = t[9], i = t[10];
+    const double A = e * i - f * h, B = f * g - d * i, C = d * h - e * g;
+    const double det = (a * A + b * B) + c * C;
+    if (!(std::fabs(det) > 0.0)) return -1;
+    const double r = 1.0 / det;
+    const double Li[9] = {A * r, (c * h - b * i) * r, (b * f - c * e) * r,
+                          B * r, (a * i - c * g) * r, (c * d - a * f) * r,
+                          C * r, (b * g - a * h) * r, (a * e - b * d) * r};
+    const double sm = m.norm_scale, k1 = sm / sc, k2 = sc / sm;
+    const double u[3] = {s->norm_center[0] - t[3], s->norm_center[1] - t[7], s->norm_center[2] - t[11]};
+    for (int row = 0; row < 3; ++row) {
+      for (int col = 0; col < 3; ++col) {
+        I.w2o64[row * 4 + col] = k1 * Li[row * 3 + col];
+        I.o2w64[row * 3 + col] = k2 * t[row * 4 + col];
+      }
+      const double v = (Li[row * 3] * u[0] + Li[row * 3 + 1] * u[1]) + Li[row * 3 + 2] * u[2];
+      I.w2o64[row * 4 + 3] = sm * (v - m.norm_center[row]);
+    }
+    for (int k = 0; k < 12; ++k) I.w2o32[k] = (float)I.w2o64[k];
+    for (int k = 0; k < 9; ++k) I.o2w32[k] = (float)I.o2w64[k];
+    I.id_base = id_base;
+    id_base += (int32_t)m.nt;
+  }
+  s->built = true;
+  return 0;
+}
+
 int or_bvh_build(or_scene* s, int num_threads) {
+  if (s->instanced) return build_instanced(s, num_threads);
   const uint32_t nt = s->nt;
   std::vector<Box> ref_box(nt);
   std::vector<std::array<double, 3>> centroid(nt);
@@ -1569,7 +1779,7 @@ int or_bvh_serialize(const or_scene* s, int32_t* nodes5, int32_t* tris4, float* 
 
 int or_trace(const or_scene* s, const float* rays, int64_t n, int mode, int any_hit, int num_threads, int32_t* ids,
              float* t, uint32_t* bary, uint32_t* flags) {
-  if (!s->root) return -1;
+  if (s->instanced ? (!s->built || mode == 0) : !s->root) return -1;
   parallel_for(n, num_threads, [&](int64_t i) {
     const float* r = &rays[i * 8];
     V3<float> o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
@@ -1581,8 +1791,8 @@ int or_trace(const or_scene* s, const float* rays, int64_t n, int mode, int any_
       bary[i] = lr.bary;
       fl = lr.overflow ? 1u : 0u;
     } else {
-      HitT<float> h = mode == 1 ? brute_trace<float>(s, o, d, any_hit != 0)
-                                : canonical_trace<float>(s, s->cnodes32, o, d, any_hit != 0);
+      HitT<float> h = mode == 1 ? scene_brute<float>(s, o, d, any_hit != 0)
+                                : scene_trace<float>(s, o, d, any_hit != 0);
       ids[i] = h.id;
       t[i] = h.t;
       bary[i] = h.bary;
@@ -1615,7 +1825,7 @@ int or_env_tables(const or_scene* s, int32_t* bins2, int32_t* coords2, float* pd
 
 int or_render(const or_scene* s, const pk_camera* cam, const or_render_params* p, float* accum, float* color_out,
               uint64_t* counters) {
-  if (!s->root) return -1;
+  if (s->instanced ? !s->built : !s->root) return -1;
   if (p->precision == 64) return render_t<double>(s, cam, p, accum, color_out, counters);
   return render_t<float>(s, cam, p, accum, color_out, counters);
 }
@@ -1650,7 +1860,7 @@ int or_scene_write_params(or_scene* s, int which, int64_t offset, const float* s
 
 int or_gradient(const or_scene* s, const pk_camera* cam, const or_render_params* p, const float* d_image,
                 const int32_t* which, const int64_t* index, int64_t n, double* out, double* value) {
-  if (s->cnodes64.empty() || p->num_frames == 0) return -1;
+  if ((s->instanced ? !s->built : s->cnodes64.empty()) || p->num_frames == 0) return -1;
   if (p->precision == 32) return gradient_t<float>(s, cam, p, d_image, which, index, n, out, value);
   return gradient_t<double>(s, cam, p, d_image, which, index, n, out, value);
 }

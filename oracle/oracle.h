@@ -31,6 +31,10 @@ extern "C" {
 typedef struct or_scene or_scene;
 
 or_scene* or_scene_create(const pk_scene_desc* desc);
+/* Instancing (pk_bvh_build_instanced semantics), before or_bvh_build: the
+ * scene's own triangles become the first placement with the identity. */
+int or_scene_set_instances(or_scene* s, const pk_mesh_desc* meshes, uint32_t num_meshes,
+                           const pk_instance_desc* instances, uint32_t num_instances);
 void or_scene_destroy(or_scene* s);
 /* Literal bvh.js build (float64 SAH); returns 0 or -1 on a degenerate node. */
 int or_bvh_build(or_scene* s, int num_threads);

@@ -30,6 +30,8 @@ def load():
     lib.or_scene_create.restype = vp
     lib.or_scene_create.argtypes = [vp]
     lib.or_scene_destroy.argtypes = [vp]
+    lib.or_scene_set_instances.restype = i32
+    lib.or_scene_set_instances.argtypes = [vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32]
     lib.or_bvh_build.restype = i32
     lib.or_bvh_build.argtypes = [vp, i32]
     lib.or_bvh_info.argtypes = [vp, vp]
@@ -85,6 +87,11 @@ class OracleScene:
         self.h = self.lib.or_scene_create(ctypes.byref(self.desc))
         self.threads = threads or min(16, os.cpu_count() or 1)
         self.built = False
+        if data.instances:
+            meshes, insts, keep = data.instance_descs()
+            assert self.lib.or_scene_set_instances(self.h, ctypes.byref(meshes), len(data.meshes), ctypes.byref(insts),
+                                                   len(data.instances)) == 0
+            self._keep = (meshes, insts, keep)
 
     def build_bvh(self):
         rc = self.lib.or_bvh_build(self.h, self.threads)

@@ -219,8 +219,27 @@ class BVH:
 
     def normalizer(self):
         norm = np.zeros(4, np.float64)
-        _check(lib().pk_bvh_export_reference(self.h, None, None, None, _ptr(norm)), "pk_bvh_export_reference")
+        _check(lib().pk_bvh_normalizer(self.h, _ptr(norm)), "pk_bvh_normalizer")
         return norm
+
+    @classmethod
+    def for_scene(cls, data, threads=0, spatial_splits=False):
+        """The BVH a SceneData needs: single-level, or two-level when it has
+        instances (pk_bvh_build_instanced)."""
+        if not data.instances:
+            return cls(data.positions, data.triangles, threads, spatial_splits)
+        desc = data.desc()
+        meshes, insts, keep = data.instance_descs()  # noqa: F841 (arrays stay alive through the call)
+        h = ctypes.c_void_p()
+        flags = _ffi.PK_BVH_SPATIAL_SPLITS if spatial_splits else 0
+        _check(lib().pk_bvh_build_instanced(ctypes.byref(desc), meshes, len(data.meshes), insts, len(data.instances),
+                                            threads, flags, ctypes.byref(h)), "pk_bvh_build_instanced")
+        b = cls.__new__(cls)
+        b.h = h
+        b.num_vertices = int(np.asarray(data.positions).shape[0])
+        b.num_triangles = data.num_triangles
+        b.instanced = True
+        return b
 
     def close(self):
         if self.h:

@@ -38,6 +38,24 @@ c_float = ctypes.c_float
 c_size_t = ctypes.c_size_t
 
 
+class MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_vertices", c_uint32),
+        ("positions", c_void_p),
+        ("attributes", c_void_p),
+        ("num_triangles", c_uint32),
+        ("triangles", c_void_p),
+    ]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [
+        ("mesh", c_uint32),
+        ("material", ctypes.c_int32),
+        ("transform", ctypes.c_double * 12),
+    ]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [
         ("num_vertices", c_uint32),
@@ -141,6 +159,9 @@ SIGNATURES = {
     "pk_bvh_info": (c_int, [c_void_p, c_void_p]),
     "pk_bvh_export_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pk_bvh_export_wide": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pk_bvh_normalizer": (c_int, [c_void_p, c_void_p]),
+    "pk_bvh_build_instanced": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_int, c_uint32,
+                                       ctypes.POINTER(c_void_p)]),
     "pk_scene_upload": (c_int, [c_void_p, ctypes.POINTER(SceneDesc), c_void_p, ctypes.POINTER(c_void_p)]),
     "pk_scene_destroy": (c_int, [c_void_p]),
     "pk_scene_env_info": (c_int, [c_void_p, c_void_p]),

@@ -607,10 +607,6 @@ inline uint32_t quantize_pair(double a, double b) {
   return (uint32_t)(uint16_t)s0 | ((uint32_t)(uint16_t)s1 << 16);
 }
 
-struct FBox {
-  float lo[3], hi[3];
-};
-
 // Conservative 8-bit quantization of child boxes relative to the node box.
 void quantize_node(pk::WNode& w, const FBox* cb, int nc) {
   const double m = pk::kBoxMargin;
@@ -663,6 +659,8 @@ void quantize_node(pk::WNode& w, const FBox* cb, int nc) {
 }
 
 }  // namespace
+
+void pk_quantize_node(pk::WNode& w, const FBox* cb, int nc) { quantize_node(w, cb, nc); }
 
 void pk_bvh_make_wide(pk_bvh* bvh);
 
@@ -809,9 +807,23 @@ int pk_bvh_info(const pk_bvh* bvh, int64_t* info) {
   return PK_OK;
 }
 
+int pk_bvh_normalizer(const pk_bvh* bvh, double* norm4) {
+  if (!bvh || !norm4) {
+    pk_set_error("pk_bvh_normalizer: null argument");
+    return PK_E_INVALID;
+  }
+  for (int k = 0; k < 3; ++k) norm4[k] = bvh->norm_center[k];
+  norm4[3] = bvh->norm_scale;
+  return PK_OK;
+}
+
 int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, float* verts4, double* norm4) {
   if (!bvh) {
     pk_set_error("pk_bvh_export_reference: null bvh");
+    return PK_E_INVALID;
+  }
+  if (bvh->instanced) {
+    pk_set_error("pk_bvh_export_reference: an instanced BVH has no reference layout");
     return PK_E_INVALID;
   }
   // BFS (bvh.js:46-74): siblings adjacent, left child of the k-th interior

@@ -39,6 +39,10 @@ struct DevScene {
   const uint32_t* col_guide;
   uint32_t ew, eh, nbins;
   int env_mode;              // 0 none, 1 bins, 2 cdf
+  // instancing: nodes[0] is the top level; hits carry virtual leaf indices
+  const InstRecord* inst;
+  const uint32_t* inst_leaf_base;  // ascending, one per instance
+  uint32_t num_inst;               // 0: a single-level BVH
 };
 
 struct Hit {
@@ -99,7 +103,11 @@ struct Stack {
   int lane;
 };
 
-// State of one ray's traversal, kept in registers between steps.
+// State of one ray's traversal, kept in registers between steps.  With
+// instancing, o/d are the ray in the space of the BVH being walked and
+// wo/wd the scene-space ray; id_delta/leaf_delta turn the current mesh's
+// triangle ids and record indices into baked ids and virtual leaves.  The
+// single-level kernels never touch those fields, so they cost no registers.
 struct RayState {
   v3 o, d;
   float invx, invy, invz;
@@ -109,15 +117,33 @@ struct RayState {
   int sp;
   bool have;       // `node` is to be visited
   bool overflow;
+  v3 wo, wd;
+  uint32_t id_delta, leaf_delta;
 };
+
+__device__ inline void ray_set_inv(RayState& r) {
+  const v3 d = r.d;
+  r.invx = 1.0f / (pkm::abs_(d.x) < 1e-30f ? pkm::copysign_(1e-30f, d.x) : d.x);
+  r.invy = 1.0f / (pkm::abs_(d.y) < 1e-30f ? pkm::copysign_(1e-30f, d.y) : d.y);
+  r.invz = 1.0f / (pkm::abs_(d.z) < 1e-30f ? pkm::copysign_(1e-30f, d.z) : d.z);
+}
+
+// x' = A x + b with A, b the rows of a 3x4 (f32, no contraction); the
+// oracle evaluates the same sums in the same order
+__device__ inline v3 xform_point(const float* m, v3 p) {
+  return pkm::mk(((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3], ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7],
+                 ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11]);
+}
+__device__ inline v3 xform_dir(const float* m, v3 v) {
+  return pkm::mk((m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[4] * v.x + m[5] * v.y) + m[6] * v.z,
+                 (m[8] * v.x + m[9] * v.y) + m[10] * v.z);
+}
 
 template <bool kAnyHit>
 __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
   r.o = o;
   r.d = d;
-  r.invx = 1.0f / (pkm::abs_(d.x) < 1e-30f ? pkm::copysign_(1e-30f, d.x) : d.x);
-  r.invy = 1.0f / (pkm::abs_(d.y) < 1e-30f ? pkm::copysign_(1e-30f, d.y) : d.y);
-  r.invz = 1.0f / (pkm::abs_(d.z) < 1e-30f ? pkm::copysign_(1e-30f, d.z) : d.z);
+  ray_set_inv(r);
   r.tbest = kMaxT;
   r.cull = kAnyHit ? kMaxT * 2.0f : cull_bound(kMaxT);
   r.best_id = 0xffffffffu;
@@ -127,6 +153,22 @@ __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
   r.sp = 0;
   r.have = true;
   r.overflow = false;
+  r.id_delta = 0;
+  r.leaf_delta = 0;
+}
+
+__device__ inline void stack_push(const Stack& stk, RayState& r, uint32_t node, float t) {
+  const int lane = stk.lane;
+  if (r.sp < kStackLds) {
+    stk.lds_node[r.sp * 64 + lane] = node;
+    stk.lds_t[r.sp * 64 + lane] = t;
+    ++r.sp;
+  } else if (r.sp < kStackMax) {
+    stk.spill[(r.sp - kStackLds) * 64 + lane] = make_uint2(node, __float_as_uint(t));
+    ++r.sp;
+  } else {
+    r.overflow = true;
+  }
 }
 
 __device__ inline void cswap(uint32_t& a, uint32_t& b) {
@@ -138,8 +180,13 @@ __device__ inline void cswap(uint32_t& a, uint32_t& b) {
 // One traversal step: visit the current node (box-test its 8 children,
 // intersect the hit leaf children, continue with the nearest hit internal
 // child and push the others far-to-near), or pop the next live stack entry.
+// With kInst, visiting an instance child maps the ray into its mesh space,
+// pushes a return marker (entry distance 0, so it is never culled) and goes
+// on at the mesh's root; popping the marker maps the ray back.  Distances
+// are the same parameter in both spaces (the map is affine and the direction
+// is not renormalized), so tbest and the culling bound carry over.
 // Returns true when the ray is finished.
-template <bool kAnyHit, bool kCount>
+template <bool kAnyHit, bool kCount, bool kInst = false>
 __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
   const int lane = stk.lane;
   if (!r.have) {
@@ -162,6 +209,37 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
       }
     }
     if (!r.have) return true;
+  }
+  if (kInst) {
+    if (r.node == kReturnMarker) {
+      r.o = r.wo;
+      r.d = r.wd;
+      ray_set_inv(r);
+      r.id_delta = 0;
+      r.leaf_delta = 0;
+      r.have = false;
+      return r.sp == 0;
+    }
+    if (r.node & kInstChild) {
+      if (r.sp >= kStackMax) {  // no room for the way back: skip it, flagged
+        r.overflow = true;
+        r.have = false;
+        return r.sp == 0;
+      }
+      stack_push(stk, r, kReturnMarker, 0.0f);
+      const float4* ip = reinterpret_cast<const float4*>(S.inst + (r.node & ~kInstChild));
+      const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
+      const uint4 ref = reinterpret_cast<const uint4*>(ip)[3];
+      const float m[12] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+      r.wo = r.o;
+      r.wd = r.d;
+      r.o = xform_point(m, r.o);
+      r.d = xform_dir(m, r.d);
+      ray_set_inv(r);
+      r.node = ref.x;
+      r.id_delta = ref.y;
+      r.leaf_delta = ref.z;
+    }
   }
   r.have = false;
   if (kCount) st->nodes++;
@@ -226,10 +304,11 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
       if (kCount) st->tris++;
       uint32_t id, bary = 0;
       const float t = intersect_tri(S.tris, k, r.o, r.d, eps, &id, &bary);
+      if (kInst) id += r.id_delta;
       if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
         r.tbest = t;
         r.best_id = id;
-        r.best_leaf = k;
+        r.best_leaf = kInst ? k + r.leaf_delta : k;
         r.best_bary = bary;
         if (kAnyHit) return true;
         r.cull = cull_bound(t);
@@ -271,16 +350,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
       for (int j = 0; j < 8; ++j)
         if ((uint32_t)j == s) nx = child[j];
       const float t = __uint_as_float(k[i] & ~7u);  // rounded down: conservative
-      if (r.sp < kStackLds) {
-        stk.lds_node[r.sp * 64 + lane] = nx;
-        stk.lds_t[r.sp * 64 + lane] = t;
-        ++r.sp;
-      } else if (r.sp < kStackMax) {
-        stk.spill[(r.sp - kStackLds) * 64 + lane] = make_uint2(nx, __float_as_uint(t));
-        ++r.sp;
-      } else {
-        r.overflow = true;
-      }
+      stack_push(stk, r, nx, t);
     }
   }
   return false;

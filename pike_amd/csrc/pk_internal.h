@@ -37,7 +37,20 @@ struct pk_bvh {
   std::vector<pk::WNode> wnodes;
   std::vector<pk::TriRecord> wtris;
   std::vector<int32_t> wtri_index;  // leaf order -> {i0,i1,i2,mat} [n][4]
+  // instancing (pk_bvh_build_instanced): wnodes/wtris/wtri_index hold the
+  // top-level nodes followed by every mesh's; nt/nv are the scene's own
+  bool instanced = false;
+  std::vector<pk::InstRecord> inst;   // the scene's own geometry first, if any
+  std::vector<uint32_t> inst_attrs;   // the meshes' vertex attributes [n][4], after the scene's own
+  int64_t baked_triangles = 0;        // triangles of the equivalent baked scene
 };
+
+// Child boxes of a wide node in normalized space and their conservative
+// 8-bit quantization (pk_bvh.cpp).
+struct FBox {
+  float lo[3], hi[3];
+};
+void pk_quantize_node(pk::WNode& w, const FBox* cb, int nc);
 
 // Environment importance-sampling tables (render/env_sampler.js).
 struct EnvTables {

@@ -154,7 +154,7 @@ __device__ inline void add_color(const FrameDev& F, uint32_t pid, v3 L) {
 // one atomic.  This keeps lanes busy on incoherent rays without paying an
 // atomic per ray.  Every wave leaves the loop once the queue is exhausted
 // and its own lanes are done.
-template <bool kAnyHit, bool kCount, class Load, class Finish, class Flush>
+template <bool kAnyHit, bool kCount, bool kInst, class Load, class Finish, class Flush>
 __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* cursor, uint32_t refill_min,
                                   const Stack& stk, TraverseStats* st, uint32_t* overflow, Load load,
                                   Finish finish, Flush flush) {
@@ -164,7 +164,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   bool active = false, done = false, exhausted = false;
   uint32_t slot = 0;
   while (true) {
-    if (active && ray_step<kAnyHit, kCount>(S, r, stk, st)) {
+    if (active && ray_step<kAnyHit, kCount, kInst>(S, r, stk, st)) {
       active = false;
       done = true;
       if (r.overflow) *overflow = 1;
@@ -196,7 +196,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   }
 }
 
-template <bool kCount>
+template <bool kCount, bool kInst>
 __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevScene S, FrameDev F, int bounce, uint32_t refill_min) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
   uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
   const float4* qo = F.qo[in];
   const float4* qd = F.qd[in];
-  trace_loop<false, kCount>(
+  trace_loop<false, kCount, kInst>(
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = qo[slot], d4 = qd[slot];
@@ -463,7 +463,22 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
                                  uint32_t bary_mask, uint32_t seed, v3 thr, ShadeOut& o, ShadeGrad* g) {
   o.emit = o.scatter = o.bounce = o.shadow = false;
   // getSurfaceInteraction (:179-200), interpolateVertexAttribute (:363-394)
-  const int4 tri = S.tri_index[leaf];
+  // With instancing `leaf` is a virtual leaf index: the instance holding it
+  // is the last one whose leaf_base is <= leaf.
+  const InstRecord* inst = nullptr;
+  uint32_t rec = leaf;
+  if (S.num_inst) {
+    uint32_t lo = 0, hi = S.num_inst;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (S.inst_leaf_base[mid] <= leaf) lo = mid;
+      else hi = mid;
+    }
+    inst = S.inst + lo;
+    rec = leaf - inst->leaf_delta;
+  }
+  int4 tri = S.tri_index[rec];
+  if (inst && inst->material >= 0) tri.w = inst->material;
   const float bu = (float)(bary_mask & 0xffffu) / 65535.0f, bv = (float)(bary_mask >> 16) / 65535.0f;
   const v3 bary = pkm::mk(1.0f - bu - bv, bu, bv);
   v3 tng, btg, nrm;
@@ -481,6 +496,21 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
     m.c1 = attr_vec(S, tri.y, 2);
     m.c2 = attr_vec(S, tri.z, 2);
     nrm = pkm::normalize(pkm::mat_mul(m, bary));
+  }
+  if (inst) {
+    // mesh -> scene: tangents by the linear part, the normal by the inverse
+    // transpose (the transpose of the scene -> mesh linear part)
+    const float* L = inst->o2w;
+    const float* A = inst->w2o;
+    auto lin = [&](v3 v) {
+      return pkm::mk((L[0] * v.x + L[1] * v.y) + L[2] * v.z, (L[3] * v.x + L[4] * v.y) + L[5] * v.z,
+                     (L[6] * v.x + L[7] * v.y) + L[8] * v.z);
+    };
+    tng = pkm::normalize(lin(tng));
+    btg = pkm::normalize(lin(btg));
+    nrm = pkm::normalize(pkm::mk((A[0] * nrm.x + A[4] * nrm.y) + A[8] * nrm.z,
+                                 (A[1] * nrm.x + A[5] * nrm.y) + A[9] * nrm.z,
+                                 (A[2] * nrm.x + A[6] * nrm.y) + A[10] * nrm.z));
   }
   const uint32_t w0 = S.attrs[tri.x].w, w1 = S.attrs[tri.y].w, w2 = S.attrs[tri.z].w;
   const float uvx = pkm::unpack_snorm16(w0 & 0xffffu) * bary.x + pkm::unpack_snorm16(w1 & 0xffffu) * bary.y +
@@ -653,7 +683,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
 // ---------------------------------------------------------------------------
 // trace_shadow.wgsl main (:192-207) + shade_miss for unoccluded shadow rays
 // ---------------------------------------------------------------------------
-template <bool kCount>
+template <bool kCount, bool kInst>
 __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScene S, FrameDev F, uint32_t refill_min) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
@@ -667,7 +697,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
   const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, overflow = 0, pid = 0;
-  trace_loop<true, kCount>(
+  trace_loop<true, kCount, kInst>(
       S, n, &F.cnt[kCntWorkShadow], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = F.so[slot], d4 = F.sd[slot];
@@ -816,6 +846,7 @@ __device__ inline void wave_scatter_add(float* dst, uint32_t key, bool valid, co
 //     and d alpha / d roughness = 2 * roughness, scattered bilinearly to the
 //     4 texels of the lookup.
 // Nothing per path is stored between the passes.
+template <bool kInst>
 __global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
@@ -870,7 +901,7 @@ __global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
         r.best_bary = 0;
         if (alive) {
           ray_init<false>(r, o, d);
-          while (!ray_step<false, false>(S, r, stk, &st)) {
+          while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
           }
           if (r.overflow) overflow = 1;
         }
@@ -904,7 +935,7 @@ __global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
         if (hit && so.shadow) {
           RayState q;
           ray_init<true>(q, so.s_o, so.s_d);
-          while (!ray_step<true, false>(S, q, stk, &st)) {
+          while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
           }
           if (q.overflow) overflow = 1;
           vis = q.best_id == 0xffffffffu;
@@ -985,7 +1016,7 @@ __global__ void __launch_bounds__(256) k_loss_l2(const float4* img, const float4
 // ---------------------------------------------------------------------------
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
 // ---------------------------------------------------------------------------
-template <bool kAnyHit, bool kCount>
+template <bool kAnyHit, bool kCount, bool kInst>
 __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int32_t* ids,
                                                        float* tout, uint32_t* bary, uint32_t* cursor,
                                                        uint2* spill_all, unsigned long long* stats,
@@ -996,7 +1027,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_rays(DevScene 
   const Stack stk{lds_node, lds_t, spill_all + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
-  trace_loop<kAnyHit, kCount>(
+  trace_loop<kAnyHit, kCount, kInst>(
       S, n, cursor, refill_min, stk, &st, &overflow,
       [&](uint32_t i, RayState& r) {
         const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
@@ -1043,8 +1074,11 @@ void launch_raygen(const FrameDev& F, hipStream_t s) {
 void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
   FrameDev F = F0;
   F.spill = L.spill;
-  if (L.count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
-  else hipLaunchKernelGGL(k_trace_closest<false>, dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  const bool inst = S.num_inst != 0;
+  if (L.count && inst) hipLaunchKernelGGL((k_trace_closest<true, true>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else if (L.count) hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else if (inst) hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
   hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, S, F, bounce);
@@ -1052,8 +1086,11 @@ void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t 
 void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
   FrameDev F = F0;
   F.spill = L.spill;
-  if (L.count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
-  else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  const bool inst = S.num_inst != 0;
+  if (L.count && inst) hipLaunchKernelGGL((k_trace_shadow<true, true>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else hipLaunchKernelGGL((k_trace_shadow<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
 }
 void launch_accumulate(const FrameDev& F, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
@@ -1068,7 +1105,8 @@ void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy_f4, dim3(min(cdiv(n, 256), 8192u)), dim3(256), 0, s, dst, src, n);
 }
 void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(64), 0, s, S, A);
+  if (S.num_inst) hipLaunchKernelGGL(k_adjoint<true>, dim3(grid), dim3(64), 0, s, S, A);
+  else hipLaunchKernelGGL(k_adjoint<false>, dim3(grid), dim3(64), 0, s, S, A);
 }
 void launch_loss_l2(const float4* img, const float4* target, float4* grad, double* loss, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_loss_l2, dim3(cdiv(n, 256)), dim3(256), 0, s, img, target, grad, loss, n);
@@ -1076,15 +1114,19 @@ void launch_loss_l2(const float4* img, const float4* target, float4* grad, doubl
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,
                        uint32_t* overflow, const TraceLaunch& L, hipStream_t s) {
-#define PK_LAUNCH_RAYS(A, C)                                                                                    \
-  hipLaunchKernelGGL((k_trace_rays<A, C>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor, spill, \
+#define PK_LAUNCH_RAYS(A, C, I)                                                                                    \
+  hipLaunchKernelGGL((k_trace_rays<A, C, I>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor, spill, \
                      stats, overflow, L.refill)
-  if (any_hit) {
-    if (L.count) PK_LAUNCH_RAYS(true, true);
-    else PK_LAUNCH_RAYS(true, false);
-  } else {
-    if (L.count) PK_LAUNCH_RAYS(false, true);
-    else PK_LAUNCH_RAYS(false, false);
+  const int variant = (any_hit ? 4 : 0) | (L.count ? 2 : 0) | (S.num_inst ? 1 : 0);
+  switch (variant) {
+    case 0: PK_LAUNCH_RAYS(false, false, false); break;
+    case 1: PK_LAUNCH_RAYS(false, false, true); break;
+    case 2: PK_LAUNCH_RAYS(false, true, false); break;
+    case 3: PK_LAUNCH_RAYS(false, true, true); break;
+    case 4: PK_LAUNCH_RAYS(true, false, false); break;
+    case 5: PK_LAUNCH_RAYS(true, false, true); break;
+    case 6: PK_LAUNCH_RAYS(true, true, false); break;
+    default: PK_LAUNCH_RAYS(true, true, true); break;
   }
 #undef PK_LAUNCH_RAYS
 }

@@ -53,4 +53,24 @@ struct TriRecord {        // 48 bytes
 };
 static_assert(sizeof(TriRecord) == 48, "triangle record must be 48 bytes");
 
+// Instancing.  A top-level wide node's child with kInstChild set (and meta
+// 0) is instance child & ~kInstChild; entering it maps the ray into the
+// instance's mesh space (where that mesh's wide BVH and triangle records
+// live) and pushes kReturnMarker, whose pop maps the ray back.
+constexpr uint32_t kInstChild = 0x80000000u;
+constexpr uint32_t kReturnMarker = 0xffffffffu;
+
+// 128 bytes: the traversal reads the first 64, shading the rest.
+struct alignas(64) InstRecord {
+  float w2o[12];        // scene (normalized) -> mesh space, row-major 3x4
+  uint32_t root;        // the mesh's root wide node
+  uint32_t id_delta;    // baked triangle id = mesh triangle id + id_delta
+  uint32_t leaf_delta;  // virtual leaf index = triangle record index + leaf_delta
+  int32_t material;     // -1: the triangles' own
+  float o2w[9];         // mesh -> scene linear part, row-major 3x3
+  uint32_t leaf_base;   // first virtual leaf index of this instance
+  uint32_t pad[6];
+};
+static_assert(sizeof(InstRecord) == 128, "instance record must be 128 bytes");
+
 }  // namespace pk

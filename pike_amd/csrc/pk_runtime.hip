@@ -338,6 +338,19 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
       return PK_E_INVALID;
     }
   }
+  if (bvh->instanced) {
+    // mesh triangles and instance overrides must name existing materials
+    for (size_t i = 3; i < bvh->wtri_index.size(); i += 4)
+      if (bvh->wtri_index[i] < 0 || (uint32_t)bvh->wtri_index[i] >= nm) {
+        pk_set_error("pk_scene_upload: a mesh triangle has material id out of range");
+        return PK_E_INVALID;
+      }
+    for (size_t i = 0; i < bvh->inst.size(); ++i)
+      if (bvh->inst[i].material >= (int32_t)nm) {
+        pk_set_error("pk_scene_upload: instance material " + std::to_string(bvh->inst[i].material) + " out of range");
+        return PK_E_INVALID;
+      }
+  }
   if (d->materials)
     for (uint32_t i = 0; i < nm * 4; ++i)
       if (d->materials[i] < 0 || (uint32_t)d->materials[i] >= layers) {
@@ -371,15 +384,36 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
   D.nodes = nodes;
   D.tris = tris;
   D.tri_index = tidx;
-  std::vector<uint32_t> zero_attr;
+  std::vector<uint32_t> all_attr;
   const void* attr_src = d->attributes;
-  if (!attr_src) {
-    zero_attr.assign((size_t)d->num_vertices * 4, 0u);
-    attr_src = zero_attr.data();
+  size_t attr_count = d->num_vertices;
+  if (!attr_src || bvh->instanced) {
+    // the scene's own vertices (zero if absent), then the meshes'
+    if (d->attributes) all_attr.assign(d->attributes, d->attributes + (size_t)d->num_vertices * 4);
+    else all_attr.assign((size_t)d->num_vertices * 4, 0u);
+    all_attr.insert(all_attr.end(), bvh->inst_attrs.begin(), bvh->inst_attrs.end());
+    if (all_attr.empty()) all_attr.assign(4, 0u);
+    attr_src = all_attr.data();
+    attr_count = all_attr.size() / 4;
   }
   uint4* attrs;
-  if ((rc = upload(s->allocs, &attrs, attr_src, d->num_vertices, st))) return fail(rc);
+  if ((rc = upload(s->allocs, &attrs, attr_src, attr_count, st))) return fail(rc);
   D.attrs = attrs;
+  D.inst = nullptr;
+  D.inst_leaf_base = nullptr;
+  D.num_inst = 0;
+  std::vector<uint32_t> bases;  // lives until the stream is synchronized below
+  if (bvh->instanced) {
+    bases.resize(bvh->inst.size());
+    for (size_t i = 0; i < bases.size(); ++i) bases[i] = bvh->inst[i].leaf_base;
+    pk::InstRecord* ir;
+    uint32_t* lb;
+    if ((rc = upload(s->allocs, &ir, bvh->inst.data(), bvh->inst.size(), st))) return fail(rc);
+    if ((rc = upload(s->allocs, &lb, bases.data(), bases.size(), st))) return fail(rc);
+    D.inst = ir;
+    D.inst_leaf_base = lb;
+    D.num_inst = (uint32_t)bvh->inst.size();
+  }
   const int32_t default_mat[4] = {0, 0, 0, 0};
   int4* mats;
   if ((rc = upload(s->allocs, &mats, d->materials ? (const void*)d->materials : (const void*)default_mat, nm, st)))
@@ -1034,6 +1068,11 @@ int pk_scene_upload_batch(pk_ctx* c, const pk_scene_desc* descs, const pk_bvh* c
   for (uint32_t k = 0; k < count; ++k) {
     const pk_scene_desc& d = descs[k];
     const pk_bvh* b = bvhs[k];
+    if (b && b->instanced) {
+      pk_set_error("pk_scene_upload_batch: scene " + std::to_string(k) + " has an instanced BVH; batches take "
+                   "single-level BVHs");
+      return PK_E_INVALID;
+    }
     if (!b || d.num_triangles != b->nt || d.num_vertices != b->nv || !d.materials || !d.tex_data) {
       pk_set_error("pk_scene_upload_batch: scene " + std::to_string(k) +
                    " lacks materials/atlas or its BVH was built for other geometry");

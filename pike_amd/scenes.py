@@ -209,6 +209,44 @@ def terrain(cells, seed, amplitude=0.2):
 # ---------------------------------------------------------------------------
 # scene container
 # ---------------------------------------------------------------------------
+class Mesh:
+    """Object-space geometry for instancing."""
+
+    def __init__(self, positions, attributes, triangles):
+        self.positions = np.asarray(positions, np.float64)
+        self.attributes = np.asarray(attributes, np.uint32)
+        self.triangles = np.asarray(triangles, np.int32)
+
+
+class Instance:
+    """A placement of mesh `mesh`: object -> scene affine `transform` (3x4),
+    `material` for all its triangles (-1: the mesh's own)."""
+
+    def __init__(self, mesh, transform, material=-1):
+        self.mesh = int(mesh)
+        self.transform = np.asarray(transform, np.float64).reshape(3, 4)
+        self.material = int(material)
+
+
+def _unpack_snorm8x3(words):
+    b = np.stack([(words >> s) & 0xFF for s in (0, 8, 16)], 1).astype(np.uint8).view(np.int8).astype(np.float64)
+    return np.maximum(b / 127.0, -1.0)
+
+
+def transform_attributes(attrs, L):
+    """Tangent/bitangent by L, normal by inv(L)^T, normalized and re-quantized
+    (uv unchanged)."""
+    a = np.asarray(attrs, np.uint32)
+    if a.size == 0:
+        return a.reshape(0, 4)
+    out = a.copy()
+    N = np.linalg.inv(L).T
+    for word, M in ((0, L), (1, L), (2, N)):
+        v = _unpack_snorm8x3(a[:, word]) @ M.T
+        out[:, word] = quantize_snorm8x3(_normalize(v))
+    return out
+
+
 @dataclass
 class SceneData:
     positions: np.ndarray          # (nv, 3) float64 world space
@@ -220,11 +258,68 @@ class SceneData:
     env_sampler: int = _ffi.PK_ENV_SAMPLER_BINS
     camera: dict = field(default_factory=dict)   # world-space camera
     name: str = ""
+    meshes: list = field(default_factory=list)     # [Mesh] instanced geometry (object space)
+    instances: list = field(default_factory=list)  # [Instance] placements of the meshes
     _keep: list = field(default_factory=list, repr=False)
 
     @property
     def num_triangles(self):
         return int(self.triangles.shape[0])
+
+    @property
+    def baked_triangles(self):
+        """Triangles of the equivalent baked scene (own + every instance's)."""
+        return self.num_triangles + sum(int(self.meshes[i.mesh].triangles.shape[0]) for i in self.instances)
+
+    def instance_descs(self):
+        """ctypes pk_mesh_desc / pk_instance_desc arrays (+ the arrays they point into)."""
+        keep = []
+        meshes = (_ffi.MeshDesc * max(1, len(self.meshes)))()
+        for k, m in enumerate(self.meshes):
+            pos = np.ascontiguousarray(m.positions, np.float64)
+            attr = np.ascontiguousarray(m.attributes, np.uint32)
+            tri = np.ascontiguousarray(m.triangles, np.int32)
+            keep += [pos, attr, tri]
+            meshes[k].num_vertices = pos.shape[0]
+            meshes[k].positions = pos.ctypes.data
+            meshes[k].attributes = attr.ctypes.data
+            meshes[k].num_triangles = tri.shape[0]
+            meshes[k].triangles = tri.ctypes.data
+        insts = (_ffi.InstanceDesc * max(1, len(self.instances)))()
+        for k, it in enumerate(self.instances):
+            insts[k].mesh = it.mesh
+            insts[k].material = it.material
+            t = np.asarray(it.transform, np.float64).reshape(12)
+            for j in range(12):
+                insts[k].transform[j] = float(t[j])
+        return meshes, insts, keep
+
+    def bake(self):
+        """The equivalent single-level scene (what scene.js:159-170 builds):
+        every instance's mesh transformed into the scene, in instance order,
+        after the scene's own triangles.  Tangents map by the linear part,
+        normals by its inverse transpose, re-quantized to snorm8."""
+        pos, attr, tri = [np.asarray(self.positions, np.float64)], [np.asarray(self.attributes, np.uint32)], \
+            [np.asarray(self.triangles, np.int32)]
+        nv = pos[0].shape[0]
+        for it in self.instances:
+            m = self.meshes[it.mesh]
+            T = np.asarray(it.transform, np.float64).reshape(3, 4)
+            p = np.asarray(m.positions, np.float64)
+            # ((T0 x + T1 y) + T2 z) + T3 per row, as the instanced build evaluates it
+            w = np.stack([((T[k, 0] * p[:, 0] + T[k, 1] * p[:, 1]) + T[k, 2] * p[:, 2]) + T[k, 3]
+                          for k in range(3)], 1)
+            pos.append(w)
+            attr.append(transform_attributes(m.attributes, T[:, :3]))
+            t = np.asarray(m.triangles, np.int32).copy()
+            t[:, :3] += nv
+            if it.material >= 0:
+                t[:, 3] = it.material
+            tri.append(t)
+            nv += p.shape[0]
+        return SceneData(positions=np.concatenate(pos), attributes=np.concatenate(attr), triangles=np.concatenate(tri),
+                         materials=self.materials, tex=self.tex, env=self.env, env_sampler=self.env_sampler,
+                         camera=dict(self.camera), name=self.name + "-baked")
 
     @classmethod
     def load(cls, directory):
@@ -448,12 +543,15 @@ def config1(seed=1, num_spheres=64, subdiv=2, env_size=(64, 32)):
 C2_CAMERA = dict(origin=(0.5, 0.42, 1.2), target=(0.5, 0.02, 0.4), fov=1.0)
 
 
-def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024), env_sampler=None):
+def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024), env_sampler=None, instanced=False):
     """Terrain of cells^2 quads (fBm, 6 octaves, amplitude 0.2) plus
     num_spheres icospheres (subdivision 4); log-normal (sigma 1.5) HDR
     environment with a 0.5-degree sun at 1e4 x mean radiance, importance
     sampled through the marginal/conditional CDF (pass
-    env_sampler=PK_ENV_SAMPLER_BINS for env_sampler.js's bins)."""
+    env_sampler=PK_ENV_SAMPLER_BINS for env_sampler.js's bins).  With
+    instanced=True the spheres are instances of one unit icosphere mesh
+    (uniform scale + translation, own material each) over the terrain; bake()
+    of that scene places the same vertices as the baked default."""
     rng = Rng(seed)
     tp, ta, tt = terrain(cells, seed, 0.2)
     packer = AtlasPacker()
@@ -468,16 +566,25 @@ def config2(seed=2, cells=1024, num_spheres=200, subdiv=4, env_size=(2048, 1024)
     albedo = rng.uniform(num_spheres * 3, 0.2, 0.8).reshape(num_spheres, 3)
     rough = rng.uniform(num_spheres, 0.05, 1.0)
     metal = rng.uniform(num_spheres) < 0.3
+    instances = []
     for i in range(num_spheres):
         mats.append(packer.material(albedo[i], rough[i], 1.0 if metal[i] else 0.0))
         c = np.array([cx[i], cy[i], cz[i]])
-        parts.append((c + r[i] * uv_, sa, f, i + 1))
+        if instanced:
+            instances.append(Instance(0, [[r[i], 0, 0, c[0]], [0, r[i], 0, c[1]], [0, 0, r[i], c[2]]], i + 1))
+        else:
+            parts.append((c + r[i] * uv_, sa, f, i + 1))
     pos, attr, tris = _merge(parts)
+    meshes = []
+    if instanced:
+        mp, ma, mt = _merge([(uv_, sa, f, 0)])
+        meshes.append(Mesh(mp, ma, mt))
     env = lognormal_sun_env(env_size[0], env_size[1], seed + 17)
     return SceneData(positions=pos, attributes=attr, triangles=tris, materials=np.array(mats, dtype=np.int32),
                      tex=packer.atlas(), env=env,
                      env_sampler=_ffi.PK_ENV_SAMPLER_CDF if env_sampler is None else env_sampler,
-                     camera=camera(C2_CAMERA["origin"], C2_CAMERA["target"], C2_CAMERA["fov"]), name="c2")
+                     camera=camera(C2_CAMERA["origin"], C2_CAMERA["target"], C2_CAMERA["fov"]),
+                     name="c2-instanced" if instanced else "c2", meshes=meshes, instances=instances)
 
 
 def config3(seed=3, num_tris=10_000_000):
@@ -594,6 +701,81 @@ def config4(seed=4, num_scenes=32, num_views=64, tri_range=(1_000, 500_000), spp
         o = (0.5 + 0.75 * math.cos(az[v]), 0.45, 0.5 + 0.75 * math.sin(az[v]))
         views.append((k, w, h, int(spp[v]), camera(o, (0.5, 0.05, 0.5), 1.0)))
     return sc, views
+
+
+def _rotation(axis, angle):
+    """Rodrigues rotation matrix about a unit axis."""
+    x, y, z = axis
+    c, s = np.cos(angle), np.sin(angle)
+    C = 1 - c
+    return np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
+                     [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
+                     [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+
+
+def box_mesh():
+    """Unit cube [-1, 1]^3 with per-face normals/tangents/uvs (24 vertices)."""
+    pos, nrm, tng, uv, tri = [], [], [], [], []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[axis] = sign
+            u = np.zeros(3)
+            u[(axis + 1) % 3] = 1.0
+            v = np.cross(n, u)
+            base = len(pos)
+            for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                pos.append(n + a * u + b * v)
+                nrm.append(n)
+                tng.append(u)
+                uv.append(((a + 1) / 2, (b + 1) / 2))
+            tri += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
+    p, a, t = _merge([(np.array(pos), vertex_attributes(np.array(nrm), np.array(tng), np.array(uv)),
+                       np.array(tri), 0)])
+    return Mesh(p, a, t)
+
+
+def instanced_scene(seed, num_instances=24, subdiv=2, ground=True, env=True):
+    """Parity scene for instancing: an icosphere and a box mesh placed with
+    random rotations, non-uniform scales (one mirrored, one sheared) and
+    translations in the unit cube, per-instance materials (some left to the
+    mesh's own), over the ground quad of config1."""
+    rng = np.random.default_rng(seed)
+    packer = AtlasPacker()
+    mats = [packer.material((0.6, 0.55, 0.5), 0.7, 0.0)]
+    uv_, f = icosphere(subdiv)
+    sp, sa, st = _merge([(uv_, _sphere_attrs(uv_), f, 0)])
+    meshes = [Mesh(sp, sa, st), box_mesh()]
+    instances = []
+    for i in range(num_instances):
+        ax = rng.normal(size=3)
+        R = _rotation(ax / np.linalg.norm(ax), rng.uniform(0, 2 * np.pi))
+        S = np.diag(rng.uniform(0.03, 0.12, 3))
+        if i == 1:
+            S[0, 0] = -S[0, 0]            # mirrored
+        M = R @ S
+        if i == 2:
+            M = M @ np.array([[1.0, 0.4, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])  # sheared
+        T = np.concatenate([M, rng.uniform(0.1, 0.9, (3, 1))], 1)
+        mat = -1
+        if rng.uniform() < 0.75:
+            mats.append(packer.material(rng.uniform(0.2, 0.8, 3), rng.uniform(0.05, 1.0),
+                                        1.0 if rng.uniform() < 0.3 else 0.0))
+            mat = len(mats) - 1
+        instances.append(Instance(i % 2, T, mat))
+    if ground:
+        g = C1_GROUND
+        lo, hi = g["extent"]
+        qp = np.array([[lo, g["y"], lo], [hi, g["y"], lo], [hi, g["y"], hi], [lo, g["y"], hi]], dtype=np.float64)
+        qa = vertex_attributes(np.tile([0.0, 1.0, 0.0], (4, 1)), np.tile([1.0, 0.0, 0.0], (4, 1)),
+                               np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float64))
+        pos, attr, tris = _merge([(qp, qa, np.array([[0, 2, 1], [0, 3, 2]]), 0)])
+    else:
+        pos, attr, tris = np.zeros((0, 3)), np.zeros((0, 4), np.uint32), np.zeros((0, 4), np.int32)
+    return SceneData(positions=pos, attributes=attr, triangles=tris, materials=np.array(mats, dtype=np.int32),
+                     tex=packer.atlas(), env=constant_env_rgbe(64, 32) if env else None,
+                     camera=camera(C1_CAMERA["origin"], C1_CAMERA["target"], C1_CAMERA["fov"]),
+                     name="instanced", meshes=meshes, instances=instances)
 
 
 def random_scene(seed, num_spheres=6, subdiv=1, env=True):
