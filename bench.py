@@ -58,12 +58,12 @@ def run_render(args, pike_amd, scenes):
     import numpy as np
 
     t0 = time.time()
-    data = scenes.config2(seed=2, cells=args.cells)
-    log(f"config2 scene: {data.num_triangles} triangles ({time.time() - t0:.1f}s)")
+    data = scenes.config2(seed=2, cells=args.cells, instanced=args.instanced)
+    log(f"config2 scene: {data.baked_triangles} triangles, {len(data.instances)} instances "
+        f"({time.time() - t0:.1f}s)")
     t0 = time.time()
-    bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1),
-                       spatial_splits=args.bvh == "sbvh")
-    log(f"BVH ({args.bvh}): {bvh.info()} ({time.time() - t0:.1f}s)")
+    bvh = pike_amd.BVH.for_scene(data, threads=min(16, os.cpu_count() or 1), spatial_splits=args.bvh == "sbvh")
+    log(f"BVH ({args.bvh}{', two-level' if args.instanced else ''}): {bvh.info()} ({time.time() - t0:.1f}s)")
     ctx = pike_amd.Context(0)
     scene = pike_amd.Scene(ctx, data, bvh)
     cam = scenes.normalized_camera(data.camera, bvh.normalizer())
@@ -118,7 +118,7 @@ def run_render(args, pike_amd, scenes):
                     kernel_ms=None, profile_frames=0, profile_counters=None,
                     roofline=dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
                                   traffic=None, bytes_per_launch=None, launch_ms=None),
-                    triangles=data.num_triangles, autotune=tuned), data, bvh, cam
+                    triangles=data.baked_triangles, autotune=tuned), data, bvh, cam
     frame.reset()
     frame.count_nodes(True)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=pf, use_graph=False)
@@ -138,7 +138,7 @@ def run_render(args, pike_amd, scenes):
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                       traffic=measured_traffic(), kernel="k_trace_closest", bytes_per_launch=bytes_launch,
                       launch_ms=t_launch * 1e3),
-        triangles=data.num_triangles, autotune=tuned)
+        triangles=data.baked_triangles, autotune=tuned)
     return result, data, bvh, cam
 
 
@@ -451,6 +451,8 @@ def main():
                                                   "autotuning (e.g. the 'extra.autotune.shapes' of a previous run)")
     ap.add_argument("--soup-tris", type=int, default=10_000_000)
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
+    ap.add_argument("--instanced", action="store_true",
+                    help="render scene with its 200 icospheres as instances of one mesh (two-level BVH)")
     ap.add_argument("--bvh", choices=["sah", "sbvh"], default="sah",
                     help="BVH kind of the render, batching and differentiation scenes")
     ap.add_argument("--soup-bvh", choices=["sbvh", "sah", "both"], default="both",
@@ -528,10 +530,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (procedural terrain + icospheres, log-normal HDR env with sun; seeded PCG)",
         "config": {"workload": f"plain wavefront path tracing, {res['triangles']:,} triangles "
-                               f"({args.bvh.upper()} wide BVH), 2048x1024 env (2D CDF importance sampling), "
+                               f"({args.bvh.upper()} wide BVH{', 200 instanced icospheres' if args.instanced else ''}), "
+                               f"2048x1024 env (2D CDF importance sampling), "
                                f"{args.width}x{args.height}, depth {args.depth}, 1 spp per step",
                    "width": args.width, "height": args.height, "depth": args.depth, "triangles": res["triangles"],
-                   "bvh": args.bvh},
+                   "bvh": args.bvh, "instanced": args.instanced},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,
