@@ -83,20 +83,23 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   const uint32_t W = F.W, H = F.H, B = W * H;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   const FrameParams P = *F.params;
-  const uint32_t n = B * pass_samples(F, P);
+  const uint32_t ns = pass_samples(F, P);
+  const uint32_t n = B * ns;
   if (idx == 0) {
     F.cnt[kCntRays0] = n;
     F.cnt[kCntHits] = 0;
     F.cnt[kCntWorkTrace] = 0;
   }
   if (idx >= n) return;
-  const uint32_t s = idx / B, pixel = idx - s * B;
+  // paths are pixel-major (the samples of a pixel are neighbours in the
+  // queues); the color buffer is sample-major (color_slot)
+  const uint32_t pixel = idx / ns, s = idx - pixel * ns;
   v3 o, d;
   camera_ray(pixel % W, pixel / W, W, H, P.sample + s, P.cam, &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
   F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  F.color[(size_t)s * B + pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Batch ray generation: one thread per global path; pixels of views whose
@@ -135,13 +138,24 @@ __global__ void k_batch_round_begin(FrameDev F) {
   F.cnt[kCntWorkTrace] = 0;
 }
 
-__device__ inline void add_color(const FrameDev& F, uint32_t pid, v3 L) {
-  float4 c = F.color[pid];
+// Color slot of path pid: single-view passes number paths pixel-major
+// (pid = pixel * ns + s) and keep colors sample-major (s * W * H + pixel),
+// the layout k_accumulate folds and pk_read_color returns; batch paths map
+// one to one.
+__device__ inline uint32_t color_slot(const FrameDev& F, uint32_t ns, uint32_t pid) {
+  if (F.views || ns <= 1) return pid;
+  const uint32_t pixel = pid / ns;
+  return (pid - pixel * ns) * (F.W * F.H) + pixel;
+}
+
+__device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v3 L) {
+  const uint32_t c_at = color_slot(F, ns, pid);
+  float4 c = F.color[c_at];
   c.x += L.x;
   c.y += L.y;
   c.z += L.z;
   c.w += 1.0f;
-  F.color[pid] = c;
+  F.color[c_at] = c;
 }
 
 // ---------------------------------------------------------------------------
@@ -213,6 +227,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
   uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
   const float4* qo = F.qo[in];
   const float4* qd = F.qd[in];
+  const uint32_t ns = F.views ? 1u : pass_samples(F, *F.params);
   trace_loop<false, kCount, kInst>(
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
@@ -225,7 +240,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
         traced++;
         if (r.best_id == 0xffffffffu) {  // miss: shade_miss.wgsl envColor x throughput
           const float4 t4 = F.thr[pid];
-          add_color(F, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
+          add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
       },
       [&](bool done, uint32_t slot, const RayState& r) {
@@ -646,6 +661,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
   const uint32_t nh = F.cnt[kCntHits];
   const FrameParams P = *F.params;
+  const uint32_t ns = F.views ? 1u : pass_samples(F, P);
   ShadeOut o;
   o.emit = o.scatter = o.bounce = o.shadow = false;
   uint32_t pid = 0, view_bits = 0;
@@ -656,15 +672,15 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
     pid = __float_as_uint(o4.w);
     view_bits = __float_as_uint(d4.w);
     // the pixel index the seeds use is local to the view; a single-view
-    // pass holds samples P.sample .. P.sample + S - 1, one per W*H paths
-    const uint32_t B = F.W * F.H;
-    const uint32_t s = F.views ? 0u : pid / B;
-    const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid - s * B;
+    // pass holds samples P.sample .. P.sample + ns - 1 of every pixel,
+    // numbered pixel-major
+    const uint32_t s = F.views ? 0u : pid % ns;
+    const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid / ns;
     const float4 t4 = F.thr[pid];
     shade_hit<false>(S, P.env_theta, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), __uint_as_float(hr.z),
                      hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample + s), pkm::mk(t4.x, t4.y, t4.z), o,
                      nullptr);
-    if (o.emit) add_color(F, pid, o.L);
+    if (o.emit) add_color(F, ns, pid, o.L);
   }
   const uint32_t bs = wave_append(&F.cnt[kCntRays0 + (in ^ 1)], o.bounce);
   if (o.bounce) {
@@ -693,6 +709,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
     F.cnt[kCntWorkTrace] = 0;
   }
   const uint32_t n = F.cnt[kCntShadow];
+  const uint32_t ns = F.views ? 1u : pass_samples(F, *F.params);
   const float theta = F.params->env_theta;
   const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
@@ -709,7 +726,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
         traced++;
         if (r.best_id == 0xffffffffu) {  // unoccluded: env radiance x NEE throughput
           const float4 t4 = F.sthr[slot];
-          add_color(F, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
+          add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
       },
       [&](bool, uint32_t, const RayState&) {});
