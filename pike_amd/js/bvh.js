@@ -5,6 +5,8 @@
 // (trace_pass.js:220-263: BFS nodes of 5 words, leaf-ordered triangles,
 // normalized float32 vertices).
 
+// A desc with meshes and instances gets the two-level build
+// (pk_bvh_build_instanced), which has no reference export.
 // options: { spatialSplits: build with SBVH spatial splits (the branch
 // bvh.js:128-135 leaves disabled), threads: builder threads (0 = all) }
 class BVH {
@@ -13,7 +15,10 @@ class BVH {
     this.native = native;
     this.numVertices = desc.positions.length / 3;
     this.numTriangles = desc.triangles.length / 4;
-    this.handle = native.bvhBuild(desc.positions, desc.triangles, opts.threads || 0, !!opts.spatialSplits);
+    this.instanced = !!(desc.instances && desc.instances.length);
+    this.handle = this.instanced
+      ? native.bvhBuildInstanced(desc, desc.meshes, desc.instances, opts.threads || 0, !!opts.spatialSplits)
+      : native.bvhBuild(desc.positions, desc.triangles, opts.threads || 0, !!opts.spatialSplits);
     const info = native.bvhInfo(this.handle);
     this.info = info;
     this.depth = info.binaryDepth;
@@ -30,7 +35,7 @@ class BVH {
   }
 
   normalizer() {
-    return this.exportReference().normalizer;
+    return this.native.bvhNormalizer(this.handle);
   }
 
   destroy() {

@@ -7,9 +7,10 @@
 //   node cli.js adjoint <scene_dir> <out_dir> <d_image.f32> <width> <height> <spp> <depth>
 //        path-replay adjoint of the frame's image (pk_adjoint): writes the
 //        atlas and environment gradients (float32, PK_PARAM_* layouts)
-//   node cli.js load-scene <scene.json> <out_dir>
+//   node cli.js load-scene <scene.json> <out_dir> [--instanced]
 //        Scene.load (scene JSON + glTF props + environment PNG) baked into
-//        the raw scene format above
+//        the raw scene format above (--instanced: props sharing a file become
+//        placements of one mesh)
 //   node cli.js render-scene <scene.json> <out.png> <width> <height> <frames> <depth> [out.f32]
 //        loads, renders (PikeRenderer), tonemaps (PostProcessPass) and writes
 //        a PNG (and optionally the linear running mean).  The camera is the
@@ -34,13 +35,18 @@ function write(file, arr) {
 // seen from +z
 function sceneCamera(scene) {
   const c = scene.json.camera;
-  const pos = scene.desc.positions;
   const lo = [Infinity, Infinity, Infinity], hi = [-Infinity, -Infinity, -Infinity];
-  for (let i = 0; i < pos.length; i += 3)
-    for (let k = 0; k < 3; k++) {
-      lo[k] = Math.min(lo[k], pos[i + k]);
-      hi[k] = Math.max(hi[k], pos[i + k]);
-    }
+  const add = (pos, T) => {
+    for (let i = 0; i < pos.length; i += 3)
+      for (let k = 0; k < 3; k++) {
+        const x = T ? T[4 * k] * pos[i] + T[4 * k + 1] * pos[i + 1] + T[4 * k + 2] * pos[i + 2] + T[4 * k + 3]
+          : pos[i + k];
+        lo[k] = Math.min(lo[k], x);
+        hi[k] = Math.max(hi[k], x);
+      }
+  };
+  add(scene.desc.positions, null);
+  for (const inst of scene.desc.instances || []) add(scene.desc.meshes[inst.mesh].positions, inst.transform);
   const center = [0, 1, 2].map((k) => (lo[k] + hi[k]) / 2);
   const extent = Math.max(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
   const origin = c && c.origin ? c.origin : [center[0], center[1], center[2] + 1.5 * extent];
@@ -52,9 +58,12 @@ function sceneCamera(scene) {
 }
 
 async function main() {
+  // --instanced: props sharing a glTF file become placements of one mesh
+  const instancing = process.argv.includes('--instanced');
+  process.argv = process.argv.filter((a) => a !== '--instanced');
   const [cmd, dir, out] = process.argv.slice(2);
   if (cmd === 'load-scene' || cmd === 'render-scene') {
-    const scene = await new Scene().load(dir);
+    const scene = await new Scene().load(dir, { instancing });
     sceneDesc(scene);
     const cam = sceneCamera(scene);
     if (cmd === 'load-scene') {

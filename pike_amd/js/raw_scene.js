@@ -27,6 +27,12 @@ function loadRawScene(dir) {
     atlas: Object.assign({}, m.atlas, { data: readArray(dir, m.atlas.data) }),
   };
   if (m.env) desc.env = Object.assign({}, m.env, { data: readArray(dir, m.env.data) });
+  if (m.instances) {
+    desc.meshes = m.meshes.map((x) => ({ positions: readArray(dir, x.positions),
+      attributes: readArray(dir, x.attributes), triangles: readArray(dir, x.triangles) }));
+    desc.instances = m.instances.map((i) => ({ mesh: i.mesh, material: i.material,
+      transform: Float64Array.from(i.transform) }));
+  }
   return { desc, camera: m.camera, name: m.name };
 }
 
@@ -55,6 +61,13 @@ function saveRawScene(desc, camera, dir, name) {
   if (desc.env) {
     m.env = { width: desc.env.width, height: desc.env.height, format: desc.env.format, sampler: desc.env.sampler,
       data: putArray(dir, 'env.bin', desc.env.data) };
+  }
+  if (desc.instances && desc.instances.length) {
+    m.meshes = desc.meshes.map((x, k) => ({ positions: putArray(dir, `mesh${k}_positions.bin`, x.positions),
+      attributes: putArray(dir, `mesh${k}_attributes.bin`, x.attributes),
+      triangles: putArray(dir, `mesh${k}_triangles.bin`, x.triangles) }));
+    m.instances = desc.instances.map((i) => ({ mesh: i.mesh, material: i.material,
+      transform: Array.from(i.transform) }));
   }
   fs.writeFileSync(path.join(dir, 'scene.json'), JSON.stringify(m));
 }

@@ -10,6 +10,12 @@
 // result has the reference's host representation (attributes[], indices[],
 // materials[], texturePacker, env), which scene_desc.js turns into
 // pk_scene_desc.  Paths in the JSON are relative to its directory.
+// With load(uri, {instancing: true}) props that share a glTF file (and skip
+// lists) become placements of one mesh baked in the file's own space, each
+// with its prop transform as a 3x4 matrix: scene.meshes [{attributes,
+// indices}] and scene.instances [{mesh, material: -1, transform}], the
+// scene's own triangle lists empty (pk_bvh_build_instanced builds the two
+// levels; the reference bakes every prop, scene.js:159-170).
 // Differences from the reference, all fixes: props without transforms use
 // the identity; materials are deduplicated by (file, material index) where
 // the reference keys them by a per-primitive random id; the bitangent is
@@ -55,6 +61,19 @@ function quantizeVec3(v) {
   return new Uint32Array(bytes.buffer);
 }
 
+// The prop transform (rotations in order, then uniform scale, then
+// translation, as _transform applies them) as a row-major 3x4 matrix.
+function instanceTransform(t) {
+  const cols = [[1, 0, 0], [0, 1, 0], [0, 0, 1]].map((e) => {
+    let r = e;
+    for (const rot of t.rotate) r = rotateArbitrary(r, rot.axis, rot.angle);
+    return [r[0] * t.scale, r[1] * t.scale, r[2] * t.scale];
+  });
+  return [cols[0][0], cols[1][0], cols[2][0], t.translate[0],
+    cols[0][1], cols[1][1], cols[2][1], t.translate[1],
+    cols[0][2], cols[1][2], cols[2][2], t.translate[2]];
+}
+
 function quantizeVec2(v) {
   const shorts = new Int16Array(2);
   shorts[0] = v[0] * (v[0] > 0 ? 32767 : 32768);
@@ -71,7 +90,8 @@ class Scene {
     this.texturePacker = null;
   }
 
-  async load(uri) {
+  async load(uri, options) {
+    const opts = options || {};
     const file = path.resolve(uri);
     const dir = path.dirname(file);
     this.json = JSON.parse(fs.readFileSync(file, 'utf8'));
@@ -87,7 +107,8 @@ class Scene {
     this.attributes = [];
     this.indices = [];
     this.lights = [];
-    this._createBuffers();
+    if (opts.instancing) this._createInstances(props);
+    else this._createBuffers();
     this.env = this.json.environment ? png.decode(fs.readFileSync(path.join(dir, this.json.environment))) : null;
     return this;
   }
@@ -114,8 +135,32 @@ class Scene {
 
   _createBuffers() {
     let offset = 0;
-    for (const group of this.groups) {
-      const t = group.transforms;
+    for (const group of this.groups) offset = this._bakeGroup(group, group.transforms, this.attributes, this.indices, offset);
+  }
+
+  _createInstances(props) {
+    const identity = { rotate: [], scale: 1, translate: [0, 0, 0] };
+    const meshOf = {};
+    this.meshes = [];
+    this.instances = [];
+    this.groups.forEach((group, gi) => {
+      const key = JSON.stringify([props[gi].uri, [...group.skipMaterials].sort(), [...group.skipMeshes].sort()]);
+      if (meshOf[key] === undefined) {
+        const m = { attributes: [], indices: [] };
+        this._bakeGroup(group, identity, m.attributes, m.indices, 0);
+        meshOf[key] = m.indices.length ? this.meshes.length : -1;  // nothing left after the skip lists
+        if (m.indices.length) this.meshes.push(m);
+      }
+      if (meshOf[key] >= 0) {
+        this.instances.push({ mesh: meshOf[key], material: -1, transform: instanceTransform(group.transforms) });
+      }
+    });
+  }
+
+  // bakes one prop's primitives with transform t into attributes/indices,
+  // vertex numbering from `offset`; returns the next offset
+  _bakeGroup(group, t, attributes, indices, offset) {
+    {
       for (const mesh of group.meshes) {
         if (mesh.desc.name !== undefined && group.skipMeshes.has(mesh.desc.name)) continue;
         for (const primitive of mesh.primitives) {
@@ -129,7 +174,7 @@ class Scene {
           }
           const n = primitive.indexCount();
           for (let i = 0; i + 2 < n; i += 3) {
-            this.indices.push({ i0: offset + primitive.indexAt(i)[0], i1: offset + primitive.indexAt(i + 1)[0],
+            indices.push({ i0: offset + primitive.indexAt(i)[0], i1: offset + primitive.indexAt(i + 1)[0],
               i2: offset + primitive.indexAt(i + 2)[0], matId });
           }
           const count = primitive.attributeCount();
@@ -142,7 +187,7 @@ class Scene {
             const tangent = normalize(this._transform(st.slice(0, 3), t, true));
             const c = cross(normal, tangent);
             const bitangent = normalize([c[0] * (st[3] || 1), c[1] * (st[3] || 1), c[2] * (st[3] || 1)]);
-            this.attributes.push({ pos, tangent: quantizeVec3(tangent), bitangent: quantizeVec3(bitangent),
+            attributes.push({ pos, tangent: quantizeVec3(tangent), bitangent: quantizeVec3(bitangent),
               normal: quantizeVec3(normal), uv: quantizeVec2(primitive.uvAt(i)) });
             if (primitive.material.hasEmissive()) {
               lo = lo ? lo.map((x, k) => Math.min(x, pos[k])) : pos.slice();
@@ -153,7 +198,8 @@ class Scene {
         }
       }
     }
+    return offset;
   }
 }
 
-module.exports = { Scene, quantizeVec3, quantizeVec2, rotateArbitrary };
+module.exports = { Scene, quantizeVec3, quantizeVec2, rotateArbitrary, instanceTransform };

@@ -15,13 +15,13 @@ function word(v) {
   return v[0] >>> 0;
 }
 
-function sceneDesc(scene) {
-  if (scene.desc) return scene.desc;
-  const nv = scene.attributes.length;
+// attributes[] / indices[] of the host representation -> typed arrays
+function flatten(attrs, indices) {
+  const nv = attrs.length;
   const positions = new Float64Array(nv * 3);
   const attributes = new Uint32Array(nv * 4);
   for (let i = 0; i < nv; i++) {
-    const a = scene.attributes[i];
+    const a = attrs[i];
     positions[3 * i] = a.pos[0];
     positions[3 * i + 1] = a.pos[1];
     positions[3 * i + 2] = a.pos[2];
@@ -30,20 +30,32 @@ function sceneDesc(scene) {
     attributes[4 * i + 2] = word(a.normal);
     attributes[4 * i + 3] = word(a.uv);
   }
-  const nt = scene.indices.length;
+  const nt = indices.length;
   const triangles = new Int32Array(nt * 4);
   for (let i = 0; i < nt; i++) {
-    const t = scene.indices[i];
+    const t = indices[i];
     triangles[4 * i] = t.i0;
     triangles[4 * i + 1] = t.i1;
     triangles[4 * i + 2] = t.i2;
     triangles[4 * i + 3] = t.matId;
   }
+  return { positions, attributes, triangles };
+}
+
+function sceneDesc(scene) {
+  if (scene.desc) return scene.desc;
+  const { positions, attributes, triangles } = flatten(scene.attributes, scene.indices);
   const materials = new Int32Array(scene.materials.length * 4);
   scene.materials.forEach((m, i) => {
     materials.set([m.diffMap, m.metRoughMap, m.normMap, m.emitMap], 4 * i);
   });
   const desc = { positions, attributes, triangles, materials };
+  if (scene.instances && scene.instances.length) {
+    // instanced scenes (Scene.load(uri, {instancing: true}))
+    desc.meshes = scene.meshes.map((m) => flatten(m.attributes, m.indices));
+    desc.instances = scene.instances.map((i) => ({ mesh: i.mesh, material: i.material,
+      transform: Float64Array.from(i.transform) }));
+  }
   if (scene.texturePacker) {
     const res = scene.texturePacker.getResolution();
     desc.atlas = { width: res[0], height: res[1], layers: res[2], format: PK_TEX_RGBA8,

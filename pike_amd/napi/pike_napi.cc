@@ -225,6 +225,114 @@ napi_value BvhBuild(napi_env env, napi_callback_info info) {
   return wrap(env, kBvh, b);
 }
 
+bool scene_desc(napi_env env, napi_value o, pk_scene_desc* d);
+
+// bvhBuildInstanced(desc, meshes, instances, threads, spatialSplits):
+// meshes [{positions: Float64Array, triangles: Int32Array, attributes?:
+// Uint32Array}], instances [{mesh, material (-1 default), transform: 12
+// numbers, row-major 3x4}]
+napi_value BvhBuildInstanced(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 3)) return nullptr;
+  pk_scene_desc d;
+  if (!scene_desc(env, a.argv[0], &d)) return nullptr;
+  uint32_t nm = 0, ni = 0;
+  if (napi_get_array_length(env, a.argv[1], &nm) != napi_ok ||
+      napi_get_array_length(env, a.argv[2], &ni) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "bvhBuildInstanced: meshes and instances must be arrays");
+    return nullptr;
+  }
+  std::vector<pk_mesh_desc> meshes(nm);
+  for (uint32_t m = 0; m < nm; ++m) {
+    napi_value o, v;
+    void* p;
+    size_t n;
+    napi_get_element(env, a.argv[1], m, &o);
+    pk_mesh_desc& md = meshes[m];
+    std::memset(&md, 0, sizeof(md));
+    if (napi_get_named_property(env, o, "positions", &v) != napi_ok ||
+        !typed(env, v, napi_float64_array, &p, &n, "mesh.positions"))
+      return nullptr;
+    md.positions = (const double*)p;
+    md.num_vertices = (uint32_t)(n / 3);
+    if (napi_get_named_property(env, o, "triangles", &v) != napi_ok ||
+        !typed(env, v, napi_int32_array, &p, &n, "mesh.triangles"))
+      return nullptr;
+    md.triangles = (const int32_t*)p;
+    md.num_triangles = (uint32_t)(n / 4);
+    bool has = false;
+    napi_has_named_property(env, o, "attributes", &has);
+    if (has) {
+      napi_get_named_property(env, o, "attributes", &v);
+      if (!typed(env, v, napi_uint32_array, &p, &n, "mesh.attributes")) return nullptr;
+      if (n != (size_t)md.num_vertices * 4) {
+        napi_throw_range_error(env, nullptr, "mesh.attributes must hold 4 words per vertex");
+        return nullptr;
+      }
+      md.attributes = (const uint32_t*)p;
+    }
+  }
+  std::vector<pk_instance_desc> insts(ni);
+  for (uint32_t i = 0; i < ni; ++i) {
+    napi_value o, v;
+    napi_get_element(env, a.argv[2], i, &o);
+    pk_instance_desc& id = insts[i];
+    std::memset(&id, 0, sizeof(id));
+    get_u32(env, o, "mesh", &id.mesh, 0);
+    double mat = -1;
+    bool has = false;
+    napi_has_named_property(env, o, "material", &has);
+    if (has) {
+      napi_get_named_property(env, o, "material", &v);
+      napi_get_value_double(env, v, &mat);
+    }
+    id.material = (int32_t)mat;
+    uint32_t len = 0;
+    if (napi_get_named_property(env, o, "transform", &v) != napi_ok) return nullptr;
+    bool is_typed = false;
+    napi_is_typedarray(env, v, &is_typed);
+    if (is_typed) {
+      void* p;
+      size_t n;
+      if (!typed(env, v, napi_float64_array, &p, &n, "instance.transform")) return nullptr;
+      len = (uint32_t)n;
+      if (len == 12) std::memcpy(id.transform, p, sizeof(id.transform));
+    } else if (napi_get_array_length(env, v, &len) == napi_ok && len == 12) {
+      for (uint32_t k = 0; k < 12; ++k) {
+        napi_value e;
+        napi_get_element(env, v, k, &e);
+        napi_get_value_double(env, e, &id.transform[k]);
+      }
+    }
+    if (len != 12) {
+      napi_throw_range_error(env, nullptr, "instance.transform must hold 12 numbers (row-major 3x4)");
+      return nullptr;
+    }
+  }
+  int32_t threads = 0;
+  if (a.argc > 3) napi_get_value_int32(env, a.argv[3], &threads);
+  bool spatial = false;
+  if (a.argc > 4) napi_get_value_bool(env, a.argv[4], &spatial);
+  pk_bvh* b = nullptr;
+  PK_CALL(pk_bvh_build_instanced(&d, meshes.data(), nm, insts.data(), ni, threads,
+                                 spatial ? PK_BVH_SPATIAL_SPLITS : 0u, &b),
+          "pk_bvh_build_instanced");
+  return wrap(env, kBvh, b);
+}
+
+// bvhNormalizer(bvh) -> Float64Array [cx, cy, cz, scale]
+napi_value BvhNormalizer(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 1)) return nullptr;
+  auto* b = (pk_bvh*)unwrap(env, a.argv[0], kBvh, "bvh");
+  if (!b) return nullptr;
+  void* p;
+  napi_value out = make_typed(env, napi_float64_array, 4, 8, &p);
+  if (!out) return nullptr;
+  PK_CALL(pk_bvh_normalizer(b, (double*)p), "pk_bvh_normalizer");
+  return out;
+}
+
 napi_value BvhInfo(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, &a, 1)) return nullptr;
@@ -737,6 +845,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"frameSetSamplesPerPass", FrameSetSamplesPerPass}, {"autotuneSamples", AutotuneSamples},
       {"sceneWriteParams", SceneWriteParams}, {"sceneReadParams", SceneReadParams},
       {"adjoint", Adjoint},
+      {"bvhBuildInstanced", BvhBuildInstanced}, {"bvhNormalizer", BvhNormalizer},
   };
   for (const Fn& f : fns) {
     napi_value v;

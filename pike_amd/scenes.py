@@ -341,9 +341,14 @@ class SceneData:
             env = get(e["data"]).reshape(e["height"], e["width"], -1)
             sampler = int(e.get("sampler", sampler))
         cam = {k: (np.array(v) if isinstance(v, list) else v) for k, v in (m.get("camera") or {}).items()}
+        meshes = [Mesh(get(x["positions"]).reshape(-1, 3), get(x["attributes"]).reshape(-1, 4),
+                       get(x["triangles"]).reshape(-1, 4)) for x in m.get("meshes", [])]
+        instances = [Instance(i["mesh"], np.array(i["transform"], np.float64), i.get("material", -1))
+                     for i in m.get("instances", [])]
         return cls(positions=get(m["positions"]).reshape(-1, 3), attributes=get(m["attributes"]).reshape(-1, 4),
                    triangles=get(m["triangles"]).reshape(-1, 4), materials=get(m["materials"]).reshape(-1, 4),
-                   tex=tex, env=env, env_sampler=sampler, camera=cam, name=m.get("name", ""))
+                   tex=tex, env=env, env_sampler=sampler, camera=cam, name=m.get("name", ""), meshes=meshes,
+                   instances=instances)
 
     def save(self, directory):
         """Raw arrays + scene.json manifest (read by pike_amd/js/raw_scene.js)."""
@@ -375,6 +380,13 @@ class SceneData:
             m["env"] = {"width": int(env.shape[1]), "height": int(env.shape[0]),
                         "format": _ffi.PK_ENV_RGBE8 if env.dtype == np.uint8 else _ffi.PK_ENV_RGB32F,
                         "sampler": int(self.env_sampler), "data": put("env.bin", env, env.dtype)}
+        if self.instances:
+            m["meshes"] = [{"positions": put(f"mesh{k}_positions.bin", x.positions, np.float64),
+                            "attributes": put(f"mesh{k}_attributes.bin", x.attributes, np.uint32),
+                            "triangles": put(f"mesh{k}_triangles.bin", x.triangles, np.int32)}
+                           for k, x in enumerate(self.meshes)]
+            m["instances"] = [{"mesh": i.mesh, "material": i.material, "transform": i.transform.ravel().tolist()}
+                              for i in self.instances]
         with open(os.path.join(directory, "scene.json"), "w") as f:
             json.dump(m, f)
 
