@@ -8,18 +8,23 @@
 
 namespace pk {
 
-// device counters (uint32)
+// Device counters (uint32), each in a 4 KB page of its own: atomics and
+// loads on one cache line serialize at the L2, so a queue length that a
+// kernel's waves read must not share a line with the counters another queue's
+// appends are hammering.
+constexpr int kCounterStride = 1024;  // words
 enum : int {
-  kCntRays0 = 0,      // ray queue 0 length
-  kCntRays1 = 1,      // ray queue 1 length
-  kCntHits = 2,       // hit queue length
-  kCntShadow = 3,     // shadow queue length
-  kCntWorkTrace = 4,  // trace_closest work cursor
-  kCntWorkShadow = 5, // trace_shadow work cursor
-  kCntOverflow = 6,   // stack overflow flag
-  kCntRaysCursor = 7, // pk_trace_rays work cursor
-  kNumCounters = 16
+  kCntRays0 = 0 * kCounterStride,      // ray queue 0 length
+  kCntRays1 = 1 * kCounterStride,      // ray queue 1 length
+  kCntHits = 2 * kCounterStride,       // hit queue length
+  kCntShadow = 3 * kCounterStride,     // shadow queue length
+  kCntWorkTrace = 4 * kCounterStride,  // trace_closest work cursor
+  kCntWorkShadow = 5 * kCounterStride, // trace_shadow work cursor
+  kCntOverflow = 6 * kCounterStride,   // stack overflow flag
+  kNumCounters = 7 * kCounterStride
 };
+// counter of ray queue q (0 or 1)
+__host__ __device__ inline int ray_counter(int q) { return q ? kCntRays1 : kCntRays0; }
 // device statistics (uint64)
 enum : int {
   kStatRays = 0,
@@ -67,7 +72,7 @@ struct FrameDev {
   const ViewDev* views;  // batch views sorted by offset (nullptr: single view)
   uint32_t nviews;
   // Samples per pass: a single-view pass traces samples sample .. sample+S-1
-  // of every pixel as one wavefront, path id = s * W * H + pixel (batches use 1).
+  // of every pixel as one wavefront, path id = pixel * S + s (batches use 1).
   uint32_t S;
 };
 

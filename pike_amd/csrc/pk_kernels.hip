@@ -42,6 +42,40 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + (uint32_t)__popcll(below);
 }
 
+// Two appends aggregated over a block of kWaves waves: per-wave counts meet
+// in LDS and each queue takes one global atomic per block.  Appends to one
+// counter serialize at the L2, so per-wave atomics from every wave of a
+// 16M-thread launch made k_shade atomic-bound; a block pays a quarter of
+// them.  Every thread of the block must call it (it synchronizes the block).
+template <int kWaves>
+__device__ inline void block_append2(uint32_t* c0, bool p0, uint32_t* c1, bool p1, uint32_t* slot0,
+                                     uint32_t* slot1) {
+  __shared__ uint32_t wcnt[2][kWaves];
+  __shared__ uint32_t base[2];
+  const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+  const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+  if (lane == 0) {
+    wcnt[0][wave] = (uint32_t)__popcll(m0);
+    wcnt[1][wave] = (uint32_t)__popcll(m1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 || threadIdx.x == 64) {  // the two queues' atomics from two waves
+    const int q = threadIdx.x == 0 ? 0 : 1;
+    uint32_t t = 0;
+    for (int w = 0; w < kWaves; ++w) t += wcnt[q][w];
+    base[q] = t ? atomicAdd(q == 0 ? c0 : c1, t) : 0u;
+  }
+  __syncthreads();
+  uint32_t o0 = base[0], o1 = base[1];
+  for (int w = 0; w < wave; ++w) {
+    o0 += wcnt[0][w];
+    o1 += wcnt[1][w];
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  *slot0 = o0 + (uint32_t)__popcll(m0 & below);
+  *slot1 = o1 + (uint32_t)__popcll(m1 & below);
+}
+
 // ---------------------------------------------------------------------------
 // camera.wgsl main (:75-92) / createPrimaryRay (:58-73)
 // ---------------------------------------------------------------------------
@@ -164,10 +198,14 @@ __device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v
 // Persistent traversal loop shared by the three trace kernels.  Each lane
 // owns one ray at a time; a lane that finishes stays idle until at least
 // `refill_min` lanes of the wave are idle, then the wave compacts the
-// finished results (flush) and pulls that many new rays from the queue with
-// one atomic.  This keeps lanes busy on incoherent rays without paying an
-// atomic per ray.  Every wave leaves the loop once the queue is exhausted
-// and its own lanes are done.
+// finished results (flush) and refills the idle lanes from the slots it has
+// claimed, claiming kClaimChunk more with one atomic when those run out.
+// This keeps lanes busy on incoherent rays while the queue cursor, whose
+// atomics serialize at the L2, sees one atomic per 256 rays; each wave also
+// works through contiguous (neighbouring-pixel) slots.  Every wave leaves the
+// loop once the queue is exhausted and its own lanes are done.
+constexpr uint32_t kClaimChunk = 256;  // slots per cursor atomic (>= 64)
+
 template <bool kAnyHit, bool kCount, bool kInst, class Load, class Finish, class Flush>
 __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* cursor, uint32_t refill_min,
                                   const Stack& stk, TraverseStats* st, uint32_t* overflow, Load load,
@@ -177,6 +215,11 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   r.best_id = 0xffffffffu;
   bool active = false, done = false, exhausted = false;
   uint32_t slot = 0;
+  // slots this wave has claimed and not yet started (wave-uniform); claims
+  // take kClaimChunk slots per atomic, since atomics on one counter
+  // serialize at the L2
+  uint32_t pool_lo = 0, pool_hi = 0;
+  bool claimed_all = false;
   while (true) {
     if (active && ray_step<kAnyHit, kCount, kInst>(S, r, stk, st)) {
       active = false;
@@ -193,17 +236,37 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
       if (nidle == 64u) break;
       continue;
     }
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(cursor, nidle);
-    base = __shfl(base, 0);
+    const uint32_t have = pool_hi - pool_lo;
+    uint32_t fresh = 0;
+    bool got = false;
+    if (have < nidle && !claimed_all) {
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(cursor, kClaimChunk);
+      fresh = __shfl(b, 0);
+      got = true;
+      claimed_all = fresh >= n || n - fresh <= kClaimChunk;
+    }
     if (!active) {
-      slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-      if (slot < n) {
+      // idle lanes take the pool's slots first, then the fresh chunk's
+      const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+      const uint32_t s2 = rank < have ? pool_lo + rank : (got ? fresh + (rank - have) : 0xffffffffu);
+      if (s2 < n) {
+        slot = s2;
         load(slot, r);
         active = true;
       }
     }
-    if (base + nidle >= n) {
+    if (have >= nidle) {
+      pool_lo += nidle;
+    } else if (got) {
+      pool_lo = fresh + (nidle - have);
+      pool_hi = fresh + kClaimChunk;
+    } else {
+      pool_lo = pool_hi;
+    }
+    pool_hi = min(pool_hi, n);
+    pool_lo = min(pool_lo, pool_hi);
+    if (claimed_all && pool_lo >= pool_hi) {
       exhausted = true;
       if (__ballot(active) == 0ull) break;
     }
@@ -217,10 +280,10 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
   const int lane = (int)threadIdx.x;
   const int in = bounce & 1;
   if (blockIdx.x == 0 && lane == 0) {
-    F.cnt[kCntRays0 + (in ^ 1)] = 0;
+    F.cnt[ray_counter(in ^ 1)] = 0;
     F.cnt[kCntShadow] = 0;
   }
-  const uint32_t n = F.cnt[kCntRays0 + in];
+  const uint32_t n = F.cnt[ray_counter(in)];
   const float theta = F.params->env_theta;
   const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
@@ -244,6 +307,8 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
         }
       },
       [&](bool done, uint32_t slot, const RayState& r) {
+        // a wave's flush holds rays of the chunk it claimed, so the hit
+        // queue keeps the ray queue's (pixel) order chunk by chunk
         const bool is_hit = done && r.best_id != 0xffffffffu;
         const uint32_t hs = wave_append(&F.cnt[kCntHits], is_hit);
         if (is_hit) {
@@ -655,7 +720,9 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
   }
 }
 
-__global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounce) {
+constexpr int kShadeWaves = 4;  // k_shade blocks: 256 threads
+
+__global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev F, int bounce) {
   const int in = bounce & 1;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
@@ -682,13 +749,14 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, FrameDev F, int bounc
                      nullptr);
     if (o.emit) add_color(F, ns, pid, o.L);
   }
-  const uint32_t bs = wave_append(&F.cnt[kCntRays0 + (in ^ 1)], o.bounce);
+  // (every thread of the block reaches this point: block-wide append)
+  uint32_t bs, ss;
+  block_append2<kShadeWaves>(&F.cnt[ray_counter(in ^ 1)], o.bounce, &F.cnt[kCntShadow], o.shadow, &bs, &ss);
   if (o.bounce) {
     F.qo[in ^ 1][bs] = make_float4(o.b_o.x, o.b_o.y, o.b_o.z, __uint_as_float(pid));
     F.qd[in ^ 1][bs] = make_float4(o.b_d.x, o.b_d.y, o.b_d.z, __uint_as_float(view_bits));
     F.thr[pid] = make_float4(o.b_thr.x, o.b_thr.y, o.b_thr.z, 0.0f);
   }
-  const uint32_t ss = wave_append(&F.cnt[kCntShadow], o.shadow);
   if (o.shadow) {
     F.so[ss] = make_float4(o.s_o.x, o.s_o.y, o.s_o.z, __uint_as_float(pid));
     F.sd[ss] = make_float4(o.s_d.x, o.s_d.y, o.s_d.z, __uint_as_float(view_bits));
@@ -1098,7 +1166,8 @@ void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, con
   else hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, S, F, bounce);
+  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H * F.S, kShadeWaves * 64)), dim3(kShadeWaves * 64), 0, s, S, F,
+                     bounce);
 }
 void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
   FrameDev F = F0;

@@ -836,9 +836,9 @@ int pk_read_counters(pk_frame* f, pk_counters* out) {
   }
   PK_HIP(hipStreamSynchronize(f->ctx->stream));
   unsigned long long st[pk::kNumStats];
-  uint32_t cnt[pk::kNumCounters];
+  uint32_t overflow = 0;
   PK_HIP(hipMemcpy(st, f->dev.stats, sizeof(st), hipMemcpyDeviceToHost));
-  PK_HIP(hipMemcpy(cnt, f->dev.cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  PK_HIP(hipMemcpy(&overflow, f->dev.cnt + pk::kCntOverflow, sizeof(overflow), hipMemcpyDeviceToHost));
   out->rays_traced = st[pk::kStatRays];
   out->shadow_rays = st[pk::kStatShadow];
   out->hits = st[pk::kStatHits];
@@ -847,7 +847,7 @@ int pk_read_counters(pk_frame* f, pk_counters* out) {
   out->shadow_nodes = st[pk::kStatShadowNodes];
   out->shadow_tris = st[pk::kStatShadowTris];
   out->frames = (uint32_t)st[pk::kStatFrames];
-  out->stack_overflow = cnt[pk::kCntOverflow];
+  out->stack_overflow = overflow;
   return PK_OK;
 }
 
