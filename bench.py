@@ -5,9 +5,9 @@ subdivision-4 icospheres (3,121,152 triangles), 2048x1024 log-normal HDR
 environment with a 0.5-degree sun, importance-sampled via the 2D
 (marginal/conditional) CDF; 1920x1080, depth 8, FP32.  One step = one sample per pixel of the frame through
 raygen -> {closest hit -> shade -> shadow any-hit} x 8 -> accumulate,
-replayed from one captured hipGraph; the autotuner may trace several samples
-per pixel in one wavefront pass (the image equals one-sample frames bit for
-bit).  value = samples per pixel per second (frames/s at 1080p); scene, BVH
+replayed from one captured hipGraph; the autotuner chooses how many samples
+per pixel one wavefront pass traces (up to 64; the image equals one-sample
+frames bit for bit).  value = samples per pixel per second (frames/s at 1080p); scene, BVH
 and queues are resident in HBM before timing.
 
 Also reported: traversal microbenchmark (10M-triangle soup, 16M incoherent
