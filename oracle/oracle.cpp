@@ -302,8 +302,12 @@ struct or_scene {
     int32_t id_base = 0;
   };
   bool instanced = false, built = false;
-  std::vector<std::unique_ptr<or_scene>> meshes;
-  std::vector<Inst> insts;
+  std::vector<std::unique_ptr<or_scene>> meshes;  // [0]: the scene's own triangles, when it has any
+  std::vector<Inst> insts;                        // the placements (of the other meshes)
+  int32_t own = -1;                               // index of the scene's own triangles in meshes
+  // a normalizer imposed from outside (the scene's, for its own triangles)
+  bool force_norm = false;
+  double force_center[3] = {0, 0, 0}, force_scale = 1;
 };
 
 namespace {
@@ -316,6 +320,10 @@ void make_normalizer(or_scene* s) {
   longest = std::max(std::max(span[0], span[1]), span[2]);
   for (int k = 0; k < 3; ++k) s->norm_center[k] = b.lo[k] + span[k] * 0.5;
   s->norm_scale = 2 / longest;
+  if (s->force_norm) {
+    for (int k = 0; k < 3; ++k) s->norm_center[k] = s->force_center[k];
+    s->norm_scale = s->force_scale;
+  }
   s->vf.resize((size_t)s->nv * 3);
   s->vd.resize((size_t)s->nv * 3);
   for (size_t i = 0; i < (size_t)s->nv * 3; ++i) {
@@ -705,6 +713,12 @@ template <class T> HitT<T> scene_trace(const or_scene* s, V3<T> o, V3<T> d, bool
   if (!s->instanced) return canonical_trace<T>(s, nodes_of<T>(s), o, d, any_hit);
   HitT<T> best;
   best.t = T(100000.0);
+  if (s->own >= 0) {  // the scene's own triangles, in the scene's space
+    const or_scene* w = s->meshes[(size_t)s->own].get();
+    best = canonical_trace<T>(w, nodes_of<T>(w), o, d, any_hit);
+    best.local = best.id;
+    if (any_hit && best.id >= 0) return best;
+  }
   for (size_t i = 0; i < s->insts.size(); ++i) {
     const or_scene::Inst& I = s->insts[i];
     const or_scene* m = s->meshes[I.mesh].get();
@@ -727,6 +741,11 @@ template <class T> HitT<T> scene_brute(const or_scene* s, V3<T> o, V3<T> d, bool
   if (!s->instanced) return brute_trace<T>(s, o, d, any_hit);
   HitT<T> best;
   best.t = T(100000.0);
+  if (s->own >= 0) {
+    best = brute_trace<T>(s->meshes[(size_t)s->own].get(), o, d, any_hit);
+    best.local = best.id;
+    if (any_hit && best.id >= 0) return best;
+  }
   for (size_t i = 0; i < s->insts.size(); ++i) {
     const or_scene::Inst& I = s->insts[i];
     const T* w = w2o_of<T>(I);
@@ -1254,7 +1273,7 @@ template <class T, class U> SI<T> surface(const Ctx<T>& c, V3<T> ro, V3<T> rd, c
   const or_scene* g = s;
   const or_scene::Inst* inst = nullptr;
   int32_t tid = h.id;
-  if (s->instanced) {
+  if (s->instanced && h.inst >= 0) {
     inst = &s->insts[(size_t)h.inst];
     g = s->meshes[inst->mesh].get();
     tid = h.local;
@@ -1602,13 +1621,10 @@ int or_scene_set_instances(or_scene* s, const pk_mesh_desc* meshes, uint32_t num
     return std::unique_ptr<or_scene>(or_scene_create(&d));
   };
   uint32_t first = 0;
-  if (s->nt > 0) {  // the scene's own triangles: the first placement, identity transform
+  s->own = -1;
+  if (s->nt > 0) {  // the scene's own triangles (traced in the scene's space, not placed)
     s->meshes.push_back(sub(s->nv, s->pos.data(), s->attr.data(), s->nt, s->tri.data()));
-    or_scene::Inst I;
-    I.mesh = 0;
-    const double id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-    std::memcpy(I.t, id, sizeof(id));
-    s->insts.push_back(I);
+    s->own = 0;
     first = 1;
   }
   for (uint32_t m = 0; m < num_meshes; ++m)
@@ -1622,21 +1638,28 @@ int or_scene_set_instances(or_scene* s, const pk_mesh_desc* meshes, uint32_t num
     std::memcpy(I.t, instances[i].transform, sizeof(I.t));
     s->insts.push_back(I);
   }
-  return s->insts.empty() ? -1 : 0;
+  return (s->insts.empty() && s->own < 0) ? -1 : 0;
 }
 
 int or_bvh_build(or_scene* s, int num_threads);
 
 // Instanced build: each mesh's BVH and normalizer as for a scene of its own;
-// the scene normalizer over the placements' boxes (their meshes' triangle
-// vertices mapped by x = ((T0 x + T1 y) + T2 z) + T3); per placement the
-// scene -> mesh map A = (s_m / s) L^-1 with L^-1 = adj(L) / det(L), offset
+// the scene's own triangles in the scene's normalized space (the normalizer
+// over them and the placements' boxes); per placement the scene -> mesh map
+// A = (s_m / s) L^-1 with L^-1 = adj(L) / det(L), offset
 // b = s_m ((L^-1 (c - t)) - c_m), and mesh -> scene directions (s / s_m) L,
 // composed in double and rounded to f32 once.
 static int build_instanced(or_scene* s, int num_threads) {
-  for (auto& m : s->meshes)
-    if (m->nt > 0 && or_bvh_build(m.get(), num_threads) < -1) return -2;
+  // the scene's box: its own triangles' vertices and the placements' meshes'
+  // triangle vertices mapped by x = ((T0 x + T1 y) + T2 z) + T3
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t v = 0; v < (size_t)s->nt * 3; ++v) {
+    const double* p = &s->pos[(size_t)s->tri[(v / 3) * 4 + v % 3] * 3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], p[k]);
+      hi[k] = std::max(hi[k], p[k]);
+    }
+  }
   for (const auto& I : s->insts) {
     const or_scene& m = *s->meshes[I.mesh];
     if (m.nt == 0) return -1;
@@ -1655,7 +1678,18 @@ static int build_instanced(or_scene* s, int num_threads) {
   for (int k = 0; k < 3; ++k) s->norm_center[k] = lo[k] + span[k] * 0.5;
   s->norm_scale = 2 / longest;
   const double sc = s->norm_scale;
-  int32_t id_base = 0;
+  // the scene's own triangles keep the scene's normalizer (the baked scene's
+  // f32 vertices); every mesh gets its own
+  for (size_t mi = 0; mi < s->meshes.size(); ++mi) {
+    or_scene* m = s->meshes[mi].get();
+    if ((int32_t)mi == s->own) {
+      m->force_norm = true;
+      for (int k = 0; k < 3; ++k) m->force_center[k] = s->norm_center[k];
+      m->force_scale = sc;
+    }
+    if (m->nt > 0 && or_bvh_build(m, num_threads) < -1) return -2;
+  }
+  int32_t id_base = (int32_t)s->nt;
   for (auto& I : s->insts) {
     const or_scene& m = *s->meshes[I.mesh];
     const double* t = I.t;

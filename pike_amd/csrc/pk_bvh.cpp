@@ -957,6 +957,8 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
         b.lo[k] = std::min(l.lo[k], r.lo[k]);
         b.hi[k] = std::max(l.hi[k], r.hi[k]);
       }
+    } else if (nodes[i].inst >= 0) {
+      b = bvh->inst_boxes[(size_t)nodes[i].inst];  // a placement's leaf (instanced top level)
     } else if (bvh->spatial_splits) {
       // a leaf's references are clipped fragments: its double box, mapped
       // to the normalized space and rounded outward to f32
@@ -1067,6 +1069,9 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
         bvh->wnodes.emplace_back();
         q.push_back({kids[i], wi, it.depth + 1});
         w.child[i] = wi;
+        w.meta[i] = 0;
+      } else if (nodes[kids[i]].inst >= 0) {
+        w.child[i] = pk::kInstChild | (uint32_t)nodes[kids[i]].inst;
         w.meta[i] = 0;
       } else {
         uint32_t first;

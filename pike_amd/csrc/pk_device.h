@@ -43,6 +43,7 @@ struct DevScene {
   const InstRecord* inst;
   const uint32_t* inst_leaf_base;  // ascending, one per instance
   uint32_t num_inst;               // 0: a single-level BVH
+  uint32_t world_leaves;           // leaves below this are the scene's own triangles
 };
 
 struct Hit {

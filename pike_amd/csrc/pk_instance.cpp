@@ -2,11 +2,11 @@
 //
 // The reference bakes every prop into one triangle list (scene.js:159-170)
 // and lists instancing as a TODO (README.md:20).  Here each mesh gets its own
-// wide BVH, built by pk_bvh_build_ex in the mesh's own normalized space, and
-// one top-level wide BVH (SAH over the instances' boxes, one instance per
-// leaf) places them in the scene's normalized space.  The scene's own
-// triangles are an implicit first instance with the identity transform, so
-// every closest hit goes through exactly one instance.
+// wide BVH, built by pk_bvh_build_ex in the mesh's own normalized space.  The
+// top level is the scene's own triangles' BVH, in the scene's normalized
+// space exactly as a baked scene would have it, with the placements inserted
+// as leaves of their own (one placement per leaf); a scene with no triangles
+// of its own gets an SAH over the placements' boxes instead.
 //
 // Transforms, all composed in double and rounded to f32 once:
 //   scene normalizer  x_n = (x - c) * s   (trace_pass.js:202-211 over the
@@ -23,6 +23,7 @@
 #include <deque>
 #include <memory>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -226,7 +227,139 @@ struct MeshIn {
   const int32_t* tris;
 };
 
+// Inserts placement leaves into a binary BVH in place: each goes beside the
+// node where it grows the tree's total surface area least (the insertion
+// search of Bittner et al. 2013: best-first over the area growth inherited
+// from the ancestors, bounded below by the placement's own area), then the
+// ancestors are refit.  The root stays at index 0.
+void insert_placements(std::vector<BNode>& nodes, const std::vector<DBox>& boxes) {
+  auto area = [](const double* lo, const double* hi) {
+    const double x = std::max(hi[0] - lo[0], 0.0), y = std::max(hi[1] - lo[1], 0.0), z = std::max(hi[2] - lo[2], 0.0);
+    return x * y + x * z + y * z;
+  };
+  auto merged = [](const BNode& n, const DBox& b, double* lo, double* hi) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(n.lo[k], b.lo[k]);
+      hi[k] = std::max(n.hi[k], b.hi[k]);
+    }
+  };
+  std::vector<int32_t> parent(nodes.size(), -1);
+  for (size_t i = 0; i < nodes.size(); ++i)
+    if (!nodes[i].leaf) {
+      parent[(size_t)nodes[i].left] = (int32_t)i;
+      parent[(size_t)nodes[i].right] = (int32_t)i;
+    }
+  for (size_t p = 0; p < boxes.size(); ++p) {
+    const DBox& b = boxes[p];
+    const double ab = area(b.lo, b.hi);
+    using Item = std::pair<double, int32_t>;  // inherited growth, node
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+    pq.push({0.0, 0});
+    double best_cost = INFINITY;
+    int32_t best = 0;
+    while (!pq.empty()) {
+      const Item it = pq.top();
+      pq.pop();
+      if (it.first + ab >= best_cost) break;
+      const BNode& x = nodes[(size_t)it.second];
+      double lo[3], hi[3];
+      merged(x, b, lo, hi);
+      const double u = area(lo, hi);
+      if (it.first + u < best_cost) {
+        best_cost = it.first + u;
+        best = it.second;
+      }
+      if (!x.leaf) {
+        const double inherited = it.first + u - area(x.lo, x.hi);
+        if (inherited + ab < best_cost) {
+          pq.push({inherited, x.left});
+          pq.push({inherited, x.right});
+        }
+      }
+    }
+    BNode leaf;
+    for (int k = 0; k < 3; ++k) {
+      leaf.lo[k] = b.lo[k];
+      leaf.hi[k] = b.hi[k];
+    }
+    leaf.leaf = 1;
+    leaf.count = 0;
+    leaf.inst = (int32_t)p;
+    const int32_t li = (int32_t)nodes.size();
+    nodes.push_back(leaf);
+    parent.push_back(-1);
+    int32_t sib = best;
+    if (sib == 0) {  // beside the root: the old root moves out of index 0
+      const int32_t moved = (int32_t)nodes.size();
+      nodes.push_back(nodes[0]);
+      parent.push_back(0);
+      if (!nodes[(size_t)moved].leaf) {
+        parent[(size_t)nodes[(size_t)moved].left] = moved;
+        parent[(size_t)nodes[(size_t)moved].right] = moved;
+      }
+      sib = moved;
+      BNode root;
+      merged(nodes[(size_t)moved], b, root.lo, root.hi);
+      root.leaf = 0;
+      root.left = moved;
+      root.right = li;
+      nodes[0] = root;
+      parent[(size_t)li] = 0;
+      continue;
+    }
+    const int32_t gp = parent[(size_t)sib];
+    const int32_t pi = (int32_t)nodes.size();
+    BNode pn;
+    merged(nodes[(size_t)sib], b, pn.lo, pn.hi);
+    pn.leaf = 0;
+    pn.left = sib;
+    pn.right = li;
+    nodes.push_back(pn);
+    parent.push_back(gp);
+    if (nodes[(size_t)gp].left == sib) nodes[(size_t)gp].left = pi;
+    else nodes[(size_t)gp].right = pi;
+    parent[(size_t)sib] = pi;
+    parent[(size_t)li] = pi;
+    for (int32_t a = gp; a >= 0; a = parent[(size_t)a]) {
+      BNode& n = nodes[(size_t)a];
+      const BNode& l = nodes[(size_t)n.left];
+      const BNode& r = nodes[(size_t)n.right];
+      for (int k = 0; k < 3; ++k) {
+        n.lo[k] = std::min(l.lo[k], r.lo[k]);
+        n.hi[k] = std::max(l.hi[k], r.hi[k]);
+      }
+    }
+  }
+}
+
+// Every child reference of the merged wide layout points inside it: internal
+// children at nodes, leaf ranges inside the triangle records, placement
+// children at placements (top level only) -- the kernels follow them
+// unchecked.
+bool wide_layout_ok(const pk_bvh& b, uint32_t top_nodes) {
+  const size_t nn = b.wnodes.size(), nt = b.wtris.size();
+  for (size_t i = 0; i < nn; ++i) {
+    const pk::WNode& w = b.wnodes[i];
+    if (w.nchild == 0 || w.nchild > pk::kWide) return false;
+    for (int c = 0; c < w.nchild; ++c) {
+      const uint32_t ch = w.child[c];
+      if (w.meta[c]) {
+        if ((size_t)ch + w.meta[c] > nt) return false;
+      } else if (ch & pk::kInstChild) {
+        if (i >= top_nodes || (ch & ~pk::kInstChild) >= b.inst.size()) return false;
+      } else if (ch >= nn) {
+        return false;
+      }
+    }
+  }
+  for (const pk::InstRecord& r : b.inst)
+    if (r.root < top_nodes || r.root >= nn) return false;
+  return true;
+}
+
 }  // namespace
+
+void pk_bvh_make_wide(pk_bvh* bvh);
 
 extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
                                       const pk_instance_desc* instances, uint32_t num_instances, int num_threads,
@@ -249,58 +382,60 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
     pk_set_error("pk_bvh_build_instanced: more than 2^31 instances");
     return PK_E_LIMIT;
   }
-  // mesh 0 of the build is the scene's own geometry (if any)
-  std::vector<MeshIn> mesh;
-  if (has_world)
-    mesh.push_back({scene->num_vertices, scene->num_triangles, scene->positions, scene->attributes, scene->triangles});
-  const uint32_t first_user = (uint32_t)mesh.size();
+  std::vector<MeshIn> mesh(num_meshes);
   for (uint32_t m = 0; m < num_meshes; ++m) {
     const pk_mesh_desc& d = meshes[m];
-    mesh.push_back({d.num_vertices, d.num_triangles, d.positions, d.attributes, d.triangles});
+    mesh[m] = {d.num_vertices, d.num_triangles, d.positions, d.attributes, d.triangles};
   }
   struct Inst {
     uint32_t mesh;
     int32_t material;
     double t[12];
   };
-  std::vector<Inst> inst;
-  if (has_world) inst.push_back({0, -1, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0}});
-  std::vector<char> used(mesh.size(), 0);
-  if (has_world) used[0] = 1;
+  std::vector<Inst> inst(num_instances);
+  std::vector<char> used(num_meshes, 0);
   for (uint32_t i = 0; i < num_instances; ++i) {
     const pk_instance_desc& d = instances[i];
     if (d.mesh >= num_meshes) {
       pk_set_error("pk_bvh_build_instanced: instance " + std::to_string(i) + " references a missing mesh");
       return PK_E_INVALID;
     }
-    Inst x;
-    x.mesh = first_user + d.mesh;
-    x.material = d.material;
-    std::memcpy(x.t, d.transform, sizeof(x.t));
-    for (double v : x.t)
+    inst[i].mesh = d.mesh;
+    inst[i].material = d.material;
+    std::memcpy(inst[i].t, d.transform, sizeof(inst[i].t));
+    for (double v : inst[i].t)
       if (!std::isfinite(v)) {
         pk_set_error("pk_bvh_build_instanced: instance " + std::to_string(i) + " has a non-finite transform");
         return PK_E_INVALID;
       }
-    inst.push_back(x);
-    used[x.mesh] = 1;
+    used[d.mesh] = 1;
   }
-  // per-mesh BVHs (mesh-space normalized), only for meshes in use
-  std::vector<std::unique_ptr<pk_bvh>> mb(mesh.size());
-  for (size_t m = 0; m < mesh.size(); ++m) {
+  // the scene's own triangles' BVH and the meshes' (each normalized on its own
+  // for now; the scene's own is renormalized below)
+  std::unique_ptr<pk_bvh> world;
+  if (has_world) {
+    pk_bvh* b = nullptr;
+    const int rc = pk_bvh_build_ex(scene->positions, scene->num_vertices, scene->triangles, scene->num_triangles,
+                                   num_threads, flags, &b);
+    if (rc) return rc;
+    world.reset(b);
+  }
+  std::vector<std::unique_ptr<pk_bvh>> mb(num_meshes);
+  for (uint32_t m = 0; m < num_meshes; ++m) {
     if (!used[m]) continue;
     pk_bvh* b = nullptr;
     const int rc = pk_bvh_build_ex(mesh[m].pos, mesh[m].nv, mesh[m].tris, mesh[m].nt, num_threads, flags, &b);
     if (rc) return rc;  // the error names the problem (empty mesh, bad index, ...)
     mb[m].reset(b);
   }
-  // scene-space boxes of the instances over their meshes' triangle vertices
+  // scene-space boxes of the placements over their meshes' triangle vertices;
+  // the scene's box also holds its own triangles (the root of their BVH)
   const size_t ni = inst.size();
   std::vector<DBox> wbox(ni);
   DBox all;
   for (int k = 0; k < 3; ++k) {
-    all.lo[k] = INFINITY;
-    all.hi[k] = -INFINITY;
+    all.lo[k] = has_world ? world->nodes[0].lo[k] : INFINITY;
+    all.hi[k] = has_world ? world->nodes[0].hi[k] : -INFINITY;
   }
   for (size_t i = 0; i < ni; ++i) {
     const MeshIn& m = mesh[inst[i].mesh];
@@ -342,10 +477,12 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
   }
   const double s = bvh->norm_scale;
   const double* c = bvh->norm_center;
-  // instance records: transforms, baked id and virtual leaf numbering
+  const uint32_t world_leaves = has_world ? (uint32_t)world->wtris.size() : 0u;
+  // placement records: transforms, baked id and virtual leaf numbering (the
+  // scene's own triangles and their records come first)
   std::vector<FBox> nbox(ni);
   bvh->inst.resize(ni);
-  uint64_t id_base = 0, leaf_base = 0;
+  uint64_t id_base = scene->num_triangles, leaf_base = world_leaves;
   for (size_t i = 0; i < ni; ++i) {
     const pk_bvh& m = *mb[inst[i].mesh];
     Inv3 L;
@@ -387,40 +524,63 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
   }
   bvh->baked_triangles = (int64_t)id_base;
   // top level
-  std::vector<TNode> top;
-  int top_depth = 0;
-  {
+  int top_depth = 0, top_wide_depth = 0;
+  if (has_world) {
+    // the scene's own triangles in the scene's normalized space (the same f32
+    // vertices a baked scene has), placements inserted beside them
+    pk_bvh& w = *world;
+    for (int k = 0; k < 3; ++k) w.norm_center[k] = c[k];
+    w.norm_scale = s;
+    for (size_t i = 0; i < (size_t)w.nv * 3; ++i) w.verts[i] = (float)((scene->positions[i] - c[i % 3]) * s);
+    w.inst_boxes = nbox;
+    insert_placements(w.nodes, wbox);
+    pk_bvh_make_wide(&w);
+    bvh->wnodes.swap(w.wnodes);
+    bvh->wtris.swap(w.wtris);
+    bvh->wtri_index.swap(w.wtri_index);
+    top_depth = w.binary_depth + 1;
+    top_wide_depth = w.wide_depth;
+    bvh->num_binary += w.num_binary + 2 * (int64_t)ni;
+    bvh->largest_leaf = w.largest_leaf;
+    bvh->num_spatial_splits += w.num_spatial_splits;
+    bvh->degenerate |= w.degenerate;
+  } else {
+    std::vector<TNode> top;
     std::vector<uint32_t> items(ni);
     std::iota(items.begin(), items.end(), 0u);
     top_depth = build_top(top, nbox, std::move(items));
+    top_wide_depth = collapse_top(top, bvh->wnodes);
+    bvh->num_binary += (int64_t)top.size();
   }
-  const int top_wide_depth = collapse_top(top, bvh->wnodes);
+  bvh->world_leaves = (uint32_t)bvh->wtris.size();
+  if (bvh->world_leaves != world_leaves) {
+    pk_set_error("pk_bvh_build_instanced: internal error (top-level triangle records changed)");
+    return PK_E_INVALID;
+  }
+  const uint32_t top_nodes = (uint32_t)bvh->wnodes.size();
   // append the meshes' wide BVHs; mesh vertices follow the scene's own
-  std::vector<uint32_t> root(mesh.size(), 0), toff(mesh.size(), 0), voff(mesh.size(), 0);
-  uint32_t vnext = has_world ? scene->num_vertices : 0;
+  std::vector<uint32_t> root(num_meshes, 0), toff(num_meshes, 0);
+  uint32_t vnext = scene->num_vertices;
   int blas_depth = 0, blas_wide_depth = 0;
-  for (size_t m = 0; m < mesh.size(); ++m) {
+  for (uint32_t m = 0; m < num_meshes; ++m) {
     if (!used[m]) continue;
     const pk_bvh& b = *mb[m];
     const uint32_t noff = (uint32_t)bvh->wnodes.size(), to = (uint32_t)bvh->wtris.size();
     root[m] = noff;
     toff[m] = to;
-    voff[m] = (has_world && m == 0) ? 0u : vnext;
     for (pk::WNode w : b.wnodes) {
       for (int i = 0; i < w.nchild; ++i) w.child[i] += w.meta[i] ? to : noff;
       bvh->wnodes.push_back(w);
     }
     bvh->wtris.insert(bvh->wtris.end(), b.wtris.begin(), b.wtris.end());
     for (size_t i = 0; i < b.wtri_index.size(); i += 4) {
-      for (int v = 0; v < 3; ++v) bvh->wtri_index.push_back(b.wtri_index[i + v] + (int32_t)voff[m]);
+      for (int v = 0; v < 3; ++v) bvh->wtri_index.push_back(b.wtri_index[i + v] + (int32_t)vnext);
       bvh->wtri_index.push_back(b.wtri_index[i + 3]);
     }
-    if (!(has_world && m == 0)) {
-      const MeshIn& mi = mesh[m];
-      if (mi.attrs) bvh->inst_attrs.insert(bvh->inst_attrs.end(), mi.attrs, mi.attrs + (size_t)mi.nv * 4);
-      else bvh->inst_attrs.insert(bvh->inst_attrs.end(), (size_t)mi.nv * 4, 0u);
-      vnext += mi.nv;
-    }
+    const MeshIn& mi = mesh[m];
+    if (mi.attrs) bvh->inst_attrs.insert(bvh->inst_attrs.end(), mi.attrs, mi.attrs + (size_t)mi.nv * 4);
+    else bvh->inst_attrs.insert(bvh->inst_attrs.end(), (size_t)mi.nv * 4, 0u);
+    vnext += mi.nv;
     bvh->num_binary += b.num_binary;
     blas_depth = std::max(blas_depth, b.binary_depth);
     blas_wide_depth = std::max(blas_wide_depth, b.wide_depth);
@@ -437,7 +597,10 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
     r.root = root[inst[i].mesh];
     r.leaf_delta = r.leaf_base - toff[inst[i].mesh];
   }
-  bvh->num_binary += (int64_t)top.size();
+  if (!wide_layout_ok(*bvh, top_nodes)) {
+    pk_set_error("pk_bvh_build_instanced: internal error (a child reference leaves the layout)");
+    return PK_E_INVALID;
+  }
   bvh->binary_depth = top_depth + blas_depth;
   bvh->wide_depth = top_wide_depth + blas_wide_depth;
   *out = guard.release();

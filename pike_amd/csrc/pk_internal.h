@@ -8,6 +8,11 @@
 #include "../../include/pk.h"
 #include "pk_layout.h"
 
+// Child box of a wide node in normalized space (f32, rounded outward).
+struct FBox {
+  float lo[3], hi[3];
+};
+
 // Binary BVH node (render/bvh.js Node): float64 world-space bounds.
 struct BNode {
   double lo[3], hi[3];
@@ -18,6 +23,7 @@ struct BNode {
   // Oversized leaves (count > kLeafMax) get virtual halves that only the wide
   // collapse follows; the reference export still sees the original leaf.
   int32_t vleft = -1, vright = -1;
+  int32_t inst = -1;  // a placement's leaf (instanced top level), else -1
 };
 
 struct pk_bvh {
@@ -40,16 +46,14 @@ struct pk_bvh {
   // instancing (pk_bvh_build_instanced): wnodes/wtris/wtri_index hold the
   // top-level nodes followed by every mesh's; nt/nv are the scene's own
   bool instanced = false;
-  std::vector<pk::InstRecord> inst;   // the scene's own geometry first, if any
+  std::vector<pk::InstRecord> inst;   // the placements
+  uint32_t world_leaves = 0;          // top-level triangle records (the scene's own), first in wtris
+  std::vector<FBox> inst_boxes;       // normalized boxes of placement leaves (by placement), for the collapse
   std::vector<uint32_t> inst_attrs;   // the meshes' vertex attributes [n][4], after the scene's own
   int64_t baked_triangles = 0;        // triangles of the equivalent baked scene
 };
 
-// Child boxes of a wide node in normalized space and their conservative
-// 8-bit quantization (pk_bvh.cpp).
-struct FBox {
-  float lo[3], hi[3];
-};
+// Conservative 8-bit quantization of wide-node child boxes (pk_bvh.cpp).
 void pk_quantize_node(pk::WNode& w, const FBox* cb, int nc);
 
 // Environment importance-sampling tables (render/env_sampler.js).

@@ -543,11 +543,12 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
                                  uint32_t bary_mask, uint32_t seed, v3 thr, ShadeOut& o, ShadeGrad* g) {
   o.emit = o.scatter = o.bounce = o.shadow = false;
   // getSurfaceInteraction (:179-200), interpolateVertexAttribute (:363-394)
-  // With instancing `leaf` is a virtual leaf index: the instance holding it
-  // is the last one whose leaf_base is <= leaf.
+  // With instancing `leaf` is a virtual leaf index: below world_leaves it is
+  // one of the scene's own triangle records, else the placement holding it is
+  // the last one whose leaf_base is <= leaf.
   const InstRecord* inst = nullptr;
   uint32_t rec = leaf;
-  if (S.num_inst) {
+  if (S.num_inst && leaf >= S.world_leaves) {
     uint32_t lo = 0, hi = S.num_inst;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;

@@ -402,6 +402,7 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
   D.inst = nullptr;
   D.inst_leaf_base = nullptr;
   D.num_inst = 0;
+  D.world_leaves = bvh->instanced ? bvh->world_leaves : 0u;
   std::vector<uint32_t> bases;  // lives until the stream is synchronized below
   if (bvh->instanced) {
     bases.resize(bvh->inst.size());

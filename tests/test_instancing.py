@@ -69,6 +69,12 @@ def test_oracle_instanced_hits_match_the_baked_scene(seed, ground):
     i2, t2, _, _ = ob.trace(rays, "canonical")
     assert (i1 >= 0).mean() > 0.2
     assert (i1 == i2).mean() > 0.9995
+    # the scene's own triangles are traced in the scene's space, as baked:
+    # their hits are bit-identical
+    own = (i2 >= 0) & (i2 < d.num_triangles) & (i1 == i2)
+    if d.num_triangles:
+        assert own.sum() > 100
+        np.testing.assert_array_equal(t1[own].view(np.uint32), t2[own].view(np.uint32))
     both = (i1 >= 0) & (i1 == i2)
     rel = np.abs(t1[both].astype(np.float64) - t2[both]) / t2[both]
     assert np.median(rel) < 1e-6 and np.quantile(rel, 0.99) < 1e-4, (np.median(rel), np.quantile(rel, 0.99))
