@@ -142,6 +142,43 @@ def run_render(args, pike_amd, scenes):
     return result, data, bvh, cam
 
 
+def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
+    """The 1080p workload with its 200 icospheres as placements of one mesh
+    (two-level BVH) next to the baked layout's sizes: build time, layout
+    bytes and spp/s with the render leg's launch shapes and pass size."""
+    import numpy as np
+
+    data = scenes.config2(seed=2, cells=args.cells, instanced=True)
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    bvh = pike_amd.BVH.for_scene(data, threads=threads)
+    build_s = time.time() - t0
+    info = bvh.info()
+    scene = pike_amd.Scene(ctx, data, bvh)
+    frame = pike_amd.Frame(ctx, args.width, args.height)
+    frame.set_camera(scenes.normalized_camera(data.camera, bvh.normalizer()))
+    frame.set_samples_per_pass(spp_per_pass)
+    frame.render(scene, max_depth=args.depth, first_sample=0, num_frames=spp_per_pass, use_graph=True)
+    ctx.sync()
+    frame.reset()
+    ctx.sync()
+    steps = max(args.steps, spp_per_pass)
+    t0 = time.perf_counter()
+    frame.render(scene, max_depth=args.depth, first_sample=0, num_frames=steps, use_graph=True)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    assert np.isfinite(frame.read_image()).all(), "non-finite pixels"
+    out = dict(spp_s=steps / dt, steps=steps, build_s=build_s, placements=len(data.instances),
+               baked_triangles=data.baked_triangles, wide_nodes=info["wide_nodes"],
+               triangle_records=info["leaf_triangles"],
+               layout_mb=(info["wide_nodes"] * 128 + info["leaf_triangles"] * 48) / 1e6,
+               overflow=frame.counters()["stack_overflow"])
+    log(f"instanced c2: {out}")
+    for obj in (frame, scene, bvh):
+        obj.close()
+    return out
+
+
 def run_batching(args, pike_amd, scenes, ctx):
     """Config 4: 32 scenes x 64 views (aggregate samples/s), plain per-view
     wavefront renders vs one batched graph launch; the images are identical
@@ -434,7 +471,7 @@ def run_traversal(args, pike_amd, scenes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -446,6 +483,7 @@ def main():
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the instanced (two-level BVH) render leg")
     ap.add_argument("--batch-depth", type=int, default=4)
     ap.add_argument("--tuning", default="", help="JSON {class: {waves_per_cu, refill_min}} to apply instead of "
                                                   "autotuning (e.g. the 'extra.autotune.shapes' of a previous run)")
@@ -492,6 +530,7 @@ def main():
         print(json.dumps({"mode": "diff-only", "differentiation_c5": run_diff(args, pike_amd, scenes)}))
         return
     if args.mode == "render-only":
+        args.no_instanced = True
         args.no_cpu = True
         args.no_traversal = True
         args.no_batch = True
@@ -507,6 +546,14 @@ def main():
     cpu = None if args.no_cpu else run_cpu_baseline(args, data, cam)
     if not args.no_traversal:
         extra["traversal_10M_soup"] = run_traversal(args, pike_amd, scenes)
+    if not args.no_instanced:
+        ictx = pike_amd.Context(0)
+        if res["autotune"]:
+            for cls, shape in res["autotune"]["shapes"].items():
+                ictx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+        extra["instanced_c2"] = run_instanced(args, pike_amd, scenes, ictx, max(1, args.spp_per_pass))
+        extra["instanced_c2"]["baked_layout_mb"] = (bvh.info()["wide_nodes"] * 128 +
+                                                    bvh.info()["leaf_triangles"] * 48) / 1e6
     if not args.no_batch:
         bctx = pike_amd.Context(0)
         if res["autotune"]:
