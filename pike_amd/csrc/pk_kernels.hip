@@ -242,7 +242,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
     if (have < nidle && !claimed_all) {
       uint32_t b = 0;
       if (lane == 0) b = atomicAdd(cursor, kClaimChunk);
-      fresh = __shfl(b, 0);
+      fresh = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(b, 0));  // wave-uniform: a scalar
       got = true;
       claimed_all = fresh >= n || n - fresh <= kClaimChunk;
     }
