@@ -62,7 +62,8 @@ def run_render(args, pike_amd, scenes):
     log(f"config2 scene: {data.baked_triangles} triangles, {len(data.instances)} instances "
         f"({time.time() - t0:.1f}s)")
     t0 = time.time()
-    bvh = pike_amd.BVH.for_scene(data, threads=min(16, os.cpu_count() or 1), spatial_splits=args.bvh == "sbvh")
+    bvh = pike_amd.BVH.for_scene(data, threads=min(16, os.cpu_count() or 1), spatial_splits=args.bvh == "sbvh",
+                                 sah_collapse=args.collapse == "sah")
     log(f"BVH ({args.bvh}{', two-level' if args.instanced else ''}): {bvh.info()} ({time.time() - t0:.1f}s)")
     ctx = pike_amd.Context(0)
     scene = pike_amd.Scene(ctx, data, bvh)
@@ -435,7 +436,8 @@ def run_traversal(args, pike_amd, scenes):
     for spatial in kinds:
         kind = "sbvh" if spatial else "sah"
         t0 = time.time()
-        bvh = pike_amd.BVH(data.positions, data.triangles, threads=threads, spatial_splits=spatial)
+        bvh = pike_amd.BVH(data.positions, data.triangles, threads=threads, spatial_splits=spatial,
+                           sah_collapse=args.soup_collapse == "sah")
         build_s = time.time() - t0
         info = bvh.info()
         log(f"config3 soup {kind}: {data.num_triangles} triangles, BVH {info} ({build_s:.1f}s, {threads} threads)")
@@ -448,7 +450,7 @@ def run_traversal(args, pike_amd, scenes):
         t = pike_amd.DeviceBuffer(ctx, 4 * n)
         b = pike_amd.DeviceBuffer(ctx, 4 * n)
         res = dict(build_s=build_s, build_threads=threads, leaf_references=info["leaf_triangles"],
-                   wide_nodes=info["wide_nodes"])
+                   wide_nodes=info["wide_nodes"], collapse=args.soup_collapse)
         for name, anyh in (("closest", False), ("any", True)):
             if not args.no_autotune:
                 before, after = ctx.autotune_rays(scene, d_rays.ptr, n, anyh, ids.ptr, t.ptr, b.ptr)
@@ -491,6 +493,11 @@ def main():
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
     ap.add_argument("--instanced", action="store_true",
                     help="render scene with its 200 icospheres as instances of one mesh (two-level BVH)")
+    ap.add_argument("--collapse", choices=["greedy", "sah"], default="greedy",
+                    help="binary -> 8-wide collapse of the render scenes: open the largest child, or the "
+                         "SAH-optimal dynamic program")
+    ap.add_argument("--soup-collapse", choices=["greedy", "sah"], default="sah",
+                    help="the same for the traversal soup (incoherent rays gain from the SAH-optimal collapse)")
     ap.add_argument("--bvh", choices=["sah", "sbvh"], default="sah",
                     help="BVH kind of the render, batching and differentiation scenes")
     ap.add_argument("--soup-bvh", choices=["sbvh", "sah", "both"], default="both",

@@ -174,6 +174,11 @@ int pk_bvh_build(const double* positions, uint32_t num_vertices, const int32_t* 
  * clipped fragments, so a triangle can appear in several leaves; the
  * closest/any hit answer is unchanged. */
 #define PK_BVH_SPATIAL_SPLITS 1u
+/* PK_BVH_SAH_COLLAPSE collapses the binary tree into the 8-wide layout by the
+ * SAH-optimal dynamic program (a subtree becomes a wide node, a leaf of up to
+ * 8 triangles, or is spread over its parent's child slots) instead of
+ * greedily opening the largest child.  Hits are unchanged. */
+#define PK_BVH_SAH_COLLAPSE 2u
 int pk_bvh_build_ex(const double* positions, uint32_t num_vertices, const int32_t* triangles,
                     uint32_t num_triangles, int num_threads, uint32_t flags, pk_bvh** out);
 int pk_bvh_destroy(pk_bvh* bvh);

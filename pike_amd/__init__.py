@@ -173,11 +173,11 @@ class DeviceBuffer:
 class BVH:
     """bvh.js SAH binary BVH + its 128-byte wide collapse (host side)."""
 
-    def __init__(self, positions, triangles, threads=0, spatial_splits=False):
+    def __init__(self, positions, triangles, threads=0, spatial_splits=False, sah_collapse=False):
         pos = np.ascontiguousarray(positions, dtype=np.float64)
         tri = np.ascontiguousarray(triangles, dtype=np.int32)
         h = ctypes.c_void_p()
-        flags = _ffi.PK_BVH_SPATIAL_SPLITS if spatial_splits else 0
+        flags = (_ffi.PK_BVH_SPATIAL_SPLITS if spatial_splits else 0) | (_ffi.PK_BVH_SAH_COLLAPSE if sah_collapse else 0)
         _check(lib().pk_bvh_build_ex(_ptr(pos), pos.shape[0], _ptr(tri), tri.shape[0], threads, flags,
                                      ctypes.byref(h)), "pk_bvh_build_ex")
         self.h = h
@@ -223,15 +223,15 @@ class BVH:
         return norm
 
     @classmethod
-    def for_scene(cls, data, threads=0, spatial_splits=False):
+    def for_scene(cls, data, threads=0, spatial_splits=False, sah_collapse=False):
         """The BVH a SceneData needs: single-level, or two-level when it has
         instances (pk_bvh_build_instanced)."""
         if not data.instances:
-            return cls(data.positions, data.triangles, threads, spatial_splits)
+            return cls(data.positions, data.triangles, threads, spatial_splits, sah_collapse)
         desc = data.desc()
         meshes, insts, keep = data.instance_descs()  # noqa: F841 (arrays stay alive through the call)
         h = ctypes.c_void_p()
-        flags = _ffi.PK_BVH_SPATIAL_SPLITS if spatial_splits else 0
+        flags = (_ffi.PK_BVH_SPATIAL_SPLITS if spatial_splits else 0) | (_ffi.PK_BVH_SAH_COLLAPSE if sah_collapse else 0)
         _check(lib().pk_bvh_build_instanced(ctypes.byref(desc), meshes, len(data.meshes), insts, len(data.instances),
                                             threads, flags, ctypes.byref(h)), "pk_bvh_build_instanced")
         b = cls.__new__(cls)

@@ -28,6 +28,7 @@ PK_ENV_SAMPLER_BINS = 0
 PK_ENV_SAMPLER_CDF = 1
 PK_PARAM_ATLAS = 0
 PK_BVH_SPATIAL_SPLITS = 1
+PK_BVH_SAH_COLLAPSE = 2
 PK_PARAM_ENV = 1
 
 c_void_p = ctypes.c_void_p

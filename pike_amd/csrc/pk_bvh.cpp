@@ -724,6 +724,7 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
   int par = 0;
   for (int t = num_threads; t > 1; t >>= 1) ++par;
   bvh->spatial_splits = (flags & PK_BVH_SPATIAL_SPLITS) != 0;
+  bvh->sah_collapse = (flags & PK_BVH_SAH_COLLAPSE) != 0;
   // SBVH references: the triangles, then fragments (each spatial split
   // adds at most 64 per reference it cuts; the pool bounds the total)
   const size_t ref_cap = std::min((size_t)nt * 8 + (1u << 20), (size_t)UINT32_MAX - (1u << 17));
@@ -1031,10 +1032,151 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
       for (int c = 0; c < 4; ++c) bvh->wtri_index.push_back(tr[c]);
     }
   };
+  // SAH-optimal collapse (Ylitie et al. 2017): for every binary node x and
+  // slot budget k, the least SAH cost of representing x's subtree with at most
+  // k children of the wide node above it -- as one child (a wide node of its
+  // own, or a leaf holding its whole subtree when that has at most
+  // kMergeLeaf triangles) or dissolved into its two children's budgets.
+  // Costs: kCostNode per node visit, kCostTri per triangle test, weighted by
+  // surface area.
+  constexpr int kMergeLeaf = 4;
+  constexpr float kCostNode = 1.0f, kCostTri = 1.2f;
+  enum : uint8_t { kAsLeaf = 0, kAsNode = 1, kSplit = 2, kAsPlacement = 3 };
+  const size_t nb = nodes.size();
+  std::vector<float> slot_cost, dist_cost;
+  std::vector<uint8_t> slot_how, split_at;
+  std::vector<uint32_t> tri_count;
+  if (bvh->sah_collapse) {
+    slot_cost.assign(nb * pk::kWide, 0.0f);  // [x][k-1]
+    dist_cost.assign(nb * pk::kWide, 0.0f);
+    slot_how.assign(nb * pk::kWide, kAsNode);
+    split_at.assign(nb * pk::kWide, 1);
+    tri_count.assign(nb, 0);
+    for (size_t oi = order.size(); oi-- > 0;) {  // children before parents
+      const int32_t x = order[oi];
+      const float ax = (float)area(x);
+      float* sc = &slot_cost[(size_t)x * pk::kWide];
+      uint8_t* sh = &slot_how[(size_t)x * pk::kWide];
+      if (!is_internal(x)) {
+        const bool placement = nodes[x].inst >= 0;
+        tri_count[x] = placement ? 0xffffffffu : nodes[x].count;
+        const float c = placement ? ax * kCostNode : ax * kCostTri * (float)nodes[x].count;
+        for (int k = 0; k < pk::kWide; ++k) {
+          sc[k] = c;
+          sh[k] = placement ? kAsPlacement : kAsLeaf;
+        }
+        continue;
+      }
+      const int32_t l = kid_l(x), r = kid_r(x);
+      const uint32_t tl = tri_count[l], tr = tri_count[r];
+      tri_count[x] = (tl == 0xffffffffu || tr == 0xffffffffu) ? 0xffffffffu : tl + tr;
+      const float* sl = &slot_cost[(size_t)l * pk::kWide];
+      const float* sr = &slot_cost[(size_t)r * pk::kWide];
+      float* dc = &dist_cost[(size_t)x * pk::kWide];
+      uint8_t* sa = &split_at[(size_t)x * pk::kWide];
+      for (int k = 2; k <= pk::kWide; ++k) {
+        float best = INFINITY;
+        int bj = 1;
+        for (int j = 1; j < k; ++j) {
+          const float c = sl[j - 1] + sr[k - j - 1];
+          if (c < best) {
+            best = c;
+            bj = j;
+          }
+        }
+        dc[k - 1] = best;
+        sa[k - 1] = (uint8_t)bj;
+      }
+      const float as_node = ax * kCostNode + dc[pk::kWide - 1];
+      const bool mergeable = tri_count[x] <= (uint32_t)kMergeLeaf;
+      const float as_leaf = mergeable ? ax * kCostTri * (float)tri_count[x] : INFINITY;
+      const float one = std::min(as_node, as_leaf);
+      sc[0] = one;
+      sh[0] = as_leaf < as_node ? kAsLeaf : kAsNode;
+      for (int k = 2; k <= pk::kWide; ++k) {
+        if (dc[k - 1] < one) {
+          sc[k - 1] = dc[k - 1];
+          sh[k - 1] = kSplit;
+        } else {
+          sc[k - 1] = one;
+          sh[k - 1] = sh[0];
+        }
+      }
+    }
+  }
+  // the children of the wide node for internal binary node b under the
+  // collapse's decisions: (binary node, how)
+  std::vector<std::pair<int32_t, uint8_t>> pending;
+  auto expand = [&](int32_t b, std::vector<std::pair<int32_t, uint8_t>>& out) {
+    pending.clear();
+    const int j = split_at[(size_t)b * pk::kWide + pk::kWide - 1];
+    std::vector<std::pair<int32_t, int>> work{{kid_r(b), pk::kWide - j}, {kid_l(b), j}};
+    while (!work.empty()) {
+      const auto [x, k] = work.back();
+      work.pop_back();
+      const uint8_t how = slot_how[(size_t)x * pk::kWide + k - 1];
+      if (how == kSplit) {
+        const int jj = split_at[(size_t)x * pk::kWide + k - 1];
+        work.push_back({kid_r(x), k - jj});
+        work.push_back({kid_l(x), jj});
+      } else {
+        out.push_back({x, how});
+      }
+    }
+  };
+  // a leaf child holding every triangle of binary subtree x, in order
+  auto emit_subtree = [&](int32_t x, uint32_t* first, uint32_t* count) {
+    *first = (uint32_t)bvh->wtris.size();
+    std::vector<int32_t> st{x};
+    while (!st.empty()) {
+      const int32_t y = st.back();
+      st.pop_back();
+      if (is_internal(y)) {
+        st.push_back(kid_r(y));
+        st.push_back(kid_l(y));
+      } else {
+        uint32_t f;
+        emit_leaf(y, &f);
+      }
+    }
+    *count = (uint32_t)bvh->wtris.size() - *first;
+  };
+  std::vector<std::pair<int32_t, uint8_t>> chosen;
   while (!q.empty()) {
     QItem it = q.front();
     q.pop_front();
     wdepth = std::max(wdepth, it.depth);
+    if (bvh->sah_collapse && is_internal(it.bnode)) {
+      chosen.clear();
+      expand(it.bnode, chosen);
+      FBox cb[pk::kWide];
+      pk::WNode w;
+      std::memset(&w, 0, sizeof(w));
+      const int nk = (int)chosen.size();
+      w.nchild = (uint8_t)nk;
+      for (int i = 0; i < nk; ++i) {
+        const int32_t x = chosen[i].first;
+        cb[i] = fb[x];
+        if (chosen[i].second == kAsNode) {
+          const uint32_t wi = (uint32_t)bvh->wnodes.size();
+          bvh->wnodes.emplace_back();
+          q.push_back({x, wi, it.depth + 1});
+          w.child[i] = wi;
+          w.meta[i] = 0;
+        } else if (chosen[i].second == kAsPlacement) {
+          w.child[i] = pk::kInstChild | (uint32_t)nodes[x].inst;
+          w.meta[i] = 0;
+        } else {
+          uint32_t first, count;
+          emit_subtree(x, &first, &count);
+          w.child[i] = first;
+          w.meta[i] = (uint8_t)count;
+        }
+      }
+      quantize_node(w, cb, nk);
+      bvh->wnodes[it.wnode] = w;
+      continue;
+    }
     int32_t kids[pk::kWide];
     int nk = 0;
     if (is_internal(it.bnode)) {

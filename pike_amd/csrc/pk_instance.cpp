@@ -462,6 +462,7 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
   std::unique_ptr<pk_bvh> guard(bvh);
   bvh->instanced = true;
   bvh->spatial_splits = (flags & PK_BVH_SPATIAL_SPLITS) != 0;
+  bvh->sah_collapse = (flags & PK_BVH_SAH_COLLAPSE) != 0;
   bvh->nt = scene->num_triangles;
   bvh->nv = scene->num_vertices;
   {

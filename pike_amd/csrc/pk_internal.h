@@ -35,6 +35,7 @@ struct pk_bvh {
   int binary_depth = 0, wide_depth = 0, largest_leaf = 0;
   int64_t num_binary = 0;         // nodes of the bvh.js tree (before virtual splits)
   bool spatial_splits = false;    // built with SBVH spatial splits (leaves hold fragments)
+  bool sah_collapse = false;      // wide collapse by the SAH-optimal dynamic program
   int64_t num_spatial_splits = 0;
   double norm_center[3] = {0, 0, 0};
   double norm_scale = 1;

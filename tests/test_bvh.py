@@ -249,3 +249,35 @@ def test_spatial_splits_reduce_sah_cost_on_slivers():
                                     float(tmax[r]))) for r in range(len(rays)))
 
     assert tested(sbvh) < tested(plain)
+
+
+@pytest.mark.parametrize("spatial", [False, True])
+def test_sah_collapse_layout_and_reachability(spatial):
+    """PK_BVH_SAH_COLLAPSE: every triangle in exactly one leaf record (or its
+    fragments, with spatial splits), every child reference in range, merged
+    leaves of at most 4 triangles, and every closest hit reachable through
+    the quantized boxes its ray passes."""
+    pos, tri = _slivers(800, 11) if spatial else (None, None)
+    data = _data(pos, tri) if spatial else scenes.config1(seed=12, num_spheres=20)
+    b = pike_amd.BVH(data.positions, data.triangles, spatial_splits=spatial, sah_collapse=True)
+    g = pike_amd.BVH(data.positions, data.triangles, spatial_splits=spatial)
+    nodes, wt, _ = b.export_wide()
+    assert b.info()["leaf_triangles"] == g.info()["leaf_triangles"]
+    assert nodes["meta"].max() <= max(4, g.info()["largest_leaf"])
+    if not spatial:
+        np.testing.assert_array_equal(np.sort(wt["id"]), np.arange(data.num_triangles))
+    else:
+        assert set(wt["id"].tolist()) == set(range(data.num_triangles))
+    for n in nodes:
+        for c in range(int(n["nchild"])):
+            ch, m = int(n["child"][c]), int(n["meta"][c])
+            assert (ch + m <= len(wt)) if m else (ch < len(nodes))
+    o = OracleScene(data, threads=8)
+    o.build_bvh()
+    rays = scenes.incoherent_rays(2000, b.normalizer(), seed=13)
+    ids, t, _, _ = o.trace(rays, "brute")
+    hits = np.flatnonzero(ids >= 0)
+    assert hits.size > 100
+    for r in hits:
+        reach = _reached_ids(nodes, wt, rays[r, :3].astype(np.float64), rays[r, 4:7].astype(np.float64), float(t[r]))
+        assert int(ids[r]) in reach, r

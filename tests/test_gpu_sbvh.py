@@ -65,3 +65,21 @@ def test_render_with_spatial_splits_equals_plain_bvh(ctx):
         imgs.append(fr.read_image())
         assert fr.counters()["stack_overflow"] == 0
     np.testing.assert_array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("spatial", [False, True])
+def test_sah_collapse_matches_canonical(ctx, spatial):
+    """The SAH-optimal wide collapse (PK_BVH_SAH_COLLAPSE) changes the tree,
+    not the answer: hits equal the oracle's canonical ones."""
+    data = scenes.config3(seed=31, num_tris=150_000)
+    bvh = pike_amd.BVH(data.positions, data.triangles, threads=8, spatial_splits=spatial, sah_collapse=True)
+    scene = pike_amd.Scene(ctx, data, bvh)
+    orc = OracleScene(data, threads=8)
+    orc.build_bvh()
+    rays = scenes.incoherent_rays(80_000, bvh.normalizer(), seed=32)
+    ids, t, bary = ctx.trace_rays(scene, rays)
+    ids_o, t_o, b_o, _ = orc.trace(rays, "canonical")
+    _assert_same(ids, t, bary, ids_o, t_o, b_o)
+    occ, _, _ = ctx.trace_rays(scene, rays, any_hit=True)
+    occ_o, _, _, _ = orc.trace(rays, "canonical", any_hit=True)
+    np.testing.assert_array_equal(occ >= 0, occ_o >= 0)
