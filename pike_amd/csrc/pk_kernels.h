@@ -112,6 +112,8 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s);
 // Path-replay adjoint of a frame's image (k_adjoint).
+// words a path vertex's gradient terms occupy in AdjointArgs::vtx
+constexpr int kAdjVtxWords = 20;
 struct AdjointArgs {
   const float4* d_image;  // dLoss/dImage per pixel (rgb)
   float* g_atlas;         // gradient in the atlas layout [layer][y][x][4]
@@ -121,6 +123,7 @@ struct AdjointArgs {
   pk_camera cam;
   uint32_t* cursor;       // zeroed before the launch
   uint2* spill;           // grid * kSpillPerLane * 64 entries
+  float* vtx;             // grid * depth * kAdjVtxWords * 64 floats
   uint32_t* overflow;
 };
 void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, hipStream_t s);

@@ -918,26 +918,29 @@ __device__ inline void wave_scatter_add(float* dst, uint32_t key, bool valid, co
 }
 
 // One thread per path (pixel, sample); persistent one-wave blocks pull
-// 64-path batches from A.cursor.  Each path is replayed twice from its
-// seeds with the same arithmetic as the wavefront frame:
-//   pass 0 sums its radiance L (equal to the frame's color for that sample);
-//   pass 1 walks it again with rem = L, subtracting each contribution as it
-//   is met, so that at a scattering vertex rem is the radiance that flows
-//   through that vertex's continuation weight.  With delta = dLoss/dI / N:
-//     environment texel e of a miss or an unoccluded NEE ray:
-//       dLoss/dE_e += delta * (throughput of that contribution);
-//     base color / alpha at a vertex with throughput thr:
-//       NEE:   delta * E * thr * d(nee weight)/d(theta)
-//       bounce: delta * rem * d(bsdf)/d(theta) / bsdf
-//     and d alpha / d roughness = 2 * roughness, scattered bilinearly to the
-//     4 texels of the lookup.
-// Nothing per path is stored between the passes.
+// 64-path batches from A.cursor.  Each path is replayed once from its seeds
+// with the same arithmetic as the wavefront frame, summing its radiance L
+// (equal to the frame's color for that sample).  With delta = dLoss/dI / N:
+//   environment texel e of a miss or an unoccluded NEE ray:
+//     dLoss/dE_e += delta * (throughput of that contribution),
+//   scattered as the contribution is met;
+//   base color / alpha at a vertex with throughput thr:
+//     NEE:   delta * E * thr * d(nee weight)/d(theta)
+//     bounce: delta * rem * d(bsdf)/d(theta) / bsdf
+//   and d alpha / d roughness = 2 * roughness, scattered bilinearly to the
+//   4 texels of the lookup.  rem, the radiance that flows through a vertex's
+//   continuation weight, is L minus the contributions met up to and
+//   including that vertex, so the vertex terms wait in A.vtx (kAdjVtxWords
+//   words per vertex, lane-interleaved per wave) until the path has ended,
+//   and are then scattered with rem = L - c_0 - ... - c_b, subtracted in
+//   path order.
 template <bool kInst>
 __global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
   const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  float* const vtx = A.vtx + (size_t)blockIdx.x * A.depth * kAdjVtxWords * 64 + lane;
   const uint32_t B = A.W * A.H;
   const uint32_t total = B * A.num_frames;
   const uint32_t plane = S.tw * S.th;
@@ -960,119 +963,166 @@ __global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
       delta = pkm::mk(g4.x * A.inv_frames, g4.y * A.inv_frames, g4.z * A.inv_frames);
       live = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f;
     }
-    v3 sum = zero, rem = zero;
-    for (int pass = 0; pass < 2; ++pass) {
-      // contributions in the frame's order: pass 0 adds, pass 1 subtracts
-      auto contribute = [&](v3 c) {
-        if (pass == 0) {
-          sum.x += c.x;
-          sum.y += c.y;
-          sum.z += c.z;
-        } else {
-          rem.x -= c.x;
-          rem.y -= c.y;
-          rem.z -= c.z;
+    v3 sum = zero;
+    uint32_t nv = 0;  // vertices this lane recorded
+    bool alive = live;
+    v3 o = zero, d = zero, thr = pkm::mk(1.0f, 1.0f, 1.0f);
+    if (alive) camera_ray(pixel % A.W, pixel / A.W, A.W, A.H, sample, A.cam, &o, &d);
+    for (uint32_t b = 0; b < A.depth; ++b) {
+      if (__ballot(alive) == 0ull) break;
+      float* const rec = vtx + (size_t)b * kAdjVtxWords * 64;
+      const bool at_vertex = alive;
+      v3 c = zero;  // this vertex's contribution (at most one of the three)
+      // closest hit (trace.wgsl) and miss shading (shade_miss.wgsl)
+      RayState r;
+      r.best_id = 0xffffffffu;
+      r.tbest = kMaxT;
+      r.best_leaf = 0;
+      r.best_bary = 0;
+      if (alive) {
+        ray_init<false>(r, o, d);
+        while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
         }
-      };
-      bool alive = live;
-      v3 o = zero, d = zero, thr = pkm::mk(1.0f, 1.0f, 1.0f);
-      if (alive) camera_ray(pixel % A.W, pixel / A.W, A.W, A.H, sample, A.cam, &o, &d);
-      for (uint32_t b = 0; b < A.depth; ++b) {
-        if (__ballot(alive) == 0ull) break;
-        // closest hit (trace.wgsl) and miss shading (shade_miss.wgsl)
-        RayState r;
-        r.best_id = 0xffffffffu;
-        r.tbest = kMaxT;
-        r.best_leaf = 0;
-        r.best_bary = 0;
-        if (alive) {
-          ray_init<false>(r, o, d);
-          while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
-          }
-          if (r.overflow) overflow = 1;
-        }
-        const bool miss = alive && r.best_id == 0xffffffffu;
-        if (miss) {
-          contribute(pkm::mul(env_color(S, A.theta, d), thr));
+        if (r.overflow) overflow = 1;
+      }
+      const bool miss = alive && r.best_id == 0xffffffffu;
+      if (miss) {
+        c = pkm::mul(env_color(S, A.theta, d), thr);
+        alive = false;
+      }
+      {
+        const bool valid = miss && S.env_mode != 0;
+        const float ev[3] = {delta.x * thr.x, delta.y * thr.y, delta.z * thr.z};
+        wave_scatter_add<3>(A.g_env, valid ? env_texel(S, A.theta, d) : 0u, valid, ev, 0);
+      }
+      // shading (shade_hit.wgsl)
+      ShadeOut so;
+      so.emit = so.scatter = so.bounce = so.shadow = false;
+      ShadeGrad sg;
+      sg.u = sg.v = sg.rough = 0.0f;
+      sg.layer_base = sg.layer_rough = 0;
+      const bool hit = alive;
+      if (hit) {
+        shade_hit<true>(S, A.theta, o, d, r.tbest, r.best_leaf, r.best_bary, shade_seed(b, pixel, sample), thr, so,
+                        &sg);
+        if (so.emit) {
+          c = so.L;
           alive = false;
         }
-        if (pass == 1) {
-          const bool valid = miss && S.env_mode != 0;
-          const float ev[3] = {delta.x * thr.x, delta.y * thr.y, delta.z * thr.z};
-          wave_scatter_add<3>(A.g_env, valid ? env_texel(S, A.theta, d) : 0u, valid, ev, 0);
+      }
+      // NEE shadow ray (trace_shadow.wgsl) and its environment term
+      bool vis = false;
+      if (hit && so.shadow) {
+        RayState q;
+        ray_init<true>(q, so.s_o, so.s_d);
+        while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
         }
-        // shading (shade_hit.wgsl)
-        ShadeOut so;
-        so.emit = so.scatter = so.bounce = so.shadow = false;
-        ShadeGrad sg;
-        sg.u = sg.v = sg.rough = 0.0f;
-        sg.layer_base = sg.layer_rough = 0;
-        const bool hit = alive;
-        if (hit) {
-          shade_hit<true>(S, A.theta, o, d, r.tbest, r.best_leaf, r.best_bary, shade_seed(b, pixel, sample), thr,
-                          so, &sg);
-          if (so.emit) {
-            contribute(so.L);
-            alive = false;
-          }
-        }
-        // NEE shadow ray (trace_shadow.wgsl) and its environment term
-        bool vis = false;
-        if (hit && so.shadow) {
-          RayState q;
-          ray_init<true>(q, so.s_o, so.s_d);
-          while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
-          }
-          if (q.overflow) overflow = 1;
-          vis = q.best_id == 0xffffffffu;
-        }
-        v3 E = zero;
-        if (vis) {
-          E = env_color(S, A.theta, so.s_d);
-          contribute(pkm::mul(E, so.s_thr));
-        }
-        if (pass == 1) {
-          const float ev[3] = {delta.x * so.s_thr.x, delta.y * so.s_thr.y, delta.z * so.s_thr.z};
-          wave_scatter_add<3>(A.g_env, vis ? env_texel(S, A.theta, so.s_d) : 0u, vis, ev, 0);
-          const bool scat = hit && so.scatter;
+        if (q.overflow) overflow = 1;
+        vis = q.best_id == 0xffffffffu;
+      }
+      v3 E = zero;
+      if (vis) {
+        E = env_color(S, A.theta, so.s_d);
+        c = pkm::mul(E, so.s_thr);
+      }
+      {
+        const float ev[3] = {delta.x * so.s_thr.x, delta.y * so.s_thr.y, delta.z * so.s_thr.z};
+        wave_scatter_add<3>(A.g_env, vis ? env_texel(S, A.theta, so.s_d) : 0u, vis, ev, 0);
+      }
+      if (at_vertex) {
+        sum.x += c.x;
+        sum.y += c.y;
+        sum.z += c.z;
+        const bool scat = hit && so.scatter;
+        // the last vertex's continuation is never traced
+        const bool cont = scat && so.bounce && b + 1 < A.depth;
+        rec[0 * 64] = c.x;
+        rec[1 * 64] = c.y;
+        rec[2 * 64] = c.z;
+        rec[3 * 64] = __uint_as_float((scat ? 1u : 0u) | (cont ? 2u : 0u));
+        if (scat) {
           v3 ga = zero;
           float gr = 0.0f;
-          if (scat) {
-            if (vis) {
-              const v3 w = pkm::mul(pkm::mul(delta, E), thr);
-              ga = pkm::scale(w, sg.dn_base);
-              gr = pkm::dot(w, sg.dn_alpha);
-            }
-            // the last vertex's continuation is never traced
-            if (so.bounce && b + 1 < A.depth) {
-              const float q0 = sg.bsdf.x != 0.0f ? delta.x * rem.x / sg.bsdf.x : 0.0f;
-              const float q1 = sg.bsdf.y != 0.0f ? delta.y * rem.y / sg.bsdf.y : 0.0f;
-              const float q2 = sg.bsdf.z != 0.0f ? delta.z * rem.z / sg.bsdf.z : 0.0f;
-              ga = pkm::add(ga, pkm::mk(q0 * sg.db_base, q1 * sg.db_base, q2 * sg.db_base));
-              gr += q0 * sg.db_alpha.x + q1 * sg.db_alpha.y + q2 * sg.db_alpha.z;
-            }
-            gr *= 2.0f * sg.rough;
+          if (vis) {
+            const v3 w = pkm::mul(pkm::mul(delta, E), thr);
+            ga = pkm::scale(w, sg.dn_base);
+            gr = pkm::dot(w, sg.dn_alpha);
           }
-          const Footprint fp = tex_footprint(S, sg.u, sg.v);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const float va[3] = {ga.x * fp.w[j], ga.y * fp.w[j], ga.z * fp.w[j]};
-            wave_scatter_add<3>(A.g_atlas, (uint32_t)sg.layer_base * plane + fp.idx[j], scat, va, 0);
-            const float vr[1] = {gr * fp.w[j]};
-            wave_scatter_add<1>(A.g_atlas, (uint32_t)sg.layer_rough * plane + fp.idx[j], scat, vr, 1);
+          rec[4 * 64] = ga.x;
+          rec[5 * 64] = ga.y;
+          rec[6 * 64] = ga.z;
+          rec[7 * 64] = gr;
+          rec[8 * 64] = sg.rough;
+          rec[9 * 64] = sg.u;
+          rec[10 * 64] = sg.v;
+          rec[11 * 64] = __int_as_float(sg.layer_base);
+          rec[12 * 64] = __int_as_float(sg.layer_rough);
+          if (cont) {
+            rec[13 * 64] = sg.bsdf.x;
+            rec[14 * 64] = sg.bsdf.y;
+            rec[15 * 64] = sg.bsdf.z;
+            rec[16 * 64] = sg.db_base;
+            rec[17 * 64] = sg.db_alpha.x;
+            rec[18 * 64] = sg.db_alpha.y;
+            rec[19 * 64] = sg.db_alpha.z;
           }
         }
-        if (hit && !so.emit) {
-          if (so.bounce) {
-            o = so.b_o;
-            d = so.b_d;
-            thr = so.b_thr;
-          } else {
-            alive = false;
-          }
+        nv = b + 1;
+      }
+      if (hit && !so.emit) {
+        if (so.bounce) {
+          o = so.b_o;
+          d = so.b_d;
+          thr = so.b_thr;
+        } else {
+          alive = false;
         }
       }
-      if (pass == 0) rem = sum;
+    }
+    // the path has ended: scatter the vertex terms, subtracting each
+    // contribution from rem in path order
+    v3 rem = sum;
+    for (uint32_t b = 0; __ballot(b < nv) != 0ull; ++b) {
+      const float* const rec = vtx + (size_t)b * kAdjVtxWords * 64;
+      bool scat = false;
+      v3 ga = zero;
+      float gr = 0.0f, u = 0.0f, v = 0.0f;
+      int layer_base = 0, layer_rough = 0;
+      if (b < nv) {
+        rem.x -= rec[0 * 64];
+        rem.y -= rec[1 * 64];
+        rem.z -= rec[2 * 64];
+        const uint32_t flags = __float_as_uint(rec[3 * 64]);
+        scat = (flags & 1u) != 0u;
+        if (scat) {
+          ga = pkm::mk(rec[4 * 64], rec[5 * 64], rec[6 * 64]);
+          gr = rec[7 * 64];
+          const float rough = rec[8 * 64];
+          u = rec[9 * 64];
+          v = rec[10 * 64];
+          layer_base = __float_as_int(rec[11 * 64]);
+          layer_rough = __float_as_int(rec[12 * 64]);
+          if (flags & 2u) {
+            const v3 bsdf = pkm::mk(rec[13 * 64], rec[14 * 64], rec[15 * 64]);
+            const float db_base = rec[16 * 64];
+            const v3 db_alpha = pkm::mk(rec[17 * 64], rec[18 * 64], rec[19 * 64]);
+            const float q0 = bsdf.x != 0.0f ? delta.x * rem.x / bsdf.x : 0.0f;
+            const float q1 = bsdf.y != 0.0f ? delta.y * rem.y / bsdf.y : 0.0f;
+            const float q2 = bsdf.z != 0.0f ? delta.z * rem.z / bsdf.z : 0.0f;
+            ga = pkm::add(ga, pkm::mk(q0 * db_base, q1 * db_base, q2 * db_base));
+            gr += q0 * db_alpha.x + q1 * db_alpha.y + q2 * db_alpha.z;
+          }
+          gr *= 2.0f * rough;
+        }
+      }
+      const Footprint fp = tex_footprint(S, u, v);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float va[3] = {ga.x * fp.w[j], ga.y * fp.w[j], ga.z * fp.w[j]};
+        wave_scatter_add<3>(A.g_atlas, (uint32_t)layer_base * plane + fp.idx[j], scat, va, 0);
+        const float vr[1] = {gr * fp.w[j]};
+        wave_scatter_add<1>(A.g_atlas, (uint32_t)layer_rough * plane + fp.idx[j], scat, vr, 1);
+      }
     }
   }
   overflow = wave_sum(overflow);

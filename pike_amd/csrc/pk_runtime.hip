@@ -81,6 +81,8 @@ struct pk_frame {
   float* adj_env = nullptr;
   size_t adj_env_n = 0;
   uint32_t* adj_cursor = nullptr;
+  float* adj_vtx = nullptr;  // per-wave path vertex records
+  size_t adj_vtx_n = 0;
   double* loss = nullptr;
 };
 
@@ -1440,6 +1442,12 @@ int pk_adjoint(pk_frame* f, pk_scene* s, const pk_render_params* p, const float*
   // 2 waves per SIMD fit the adjoint's registers; the spill area holds
   // kMaxWavesPerCu waves per CU
   const uint32_t grid = (uint32_t)f->ctx->num_cus * kAdjointWavesPerCu;
+  const size_t nvtx = (size_t)grid * p->max_depth * pk::kAdjVtxWords * 64;
+  if (f->adj_vtx_n < nvtx) {
+    if ((rc = dev_alloc(f->allocs, &f->adj_vtx, nvtx))) return rc;
+    f->adj_vtx_n = nvtx;
+  }
+  A.vtx = f->adj_vtx;
   pk::launch_adjoint(s->dev, A, grid, st);
   PK_HIP(hipGetLastError());
   if (!g->on_device) {
