@@ -82,6 +82,8 @@ def run_render(args, pike_amd, scenes):
         for cls, shape in cfg.get("shapes", cfg).items():
             if isinstance(shape, dict):
                 ctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+        if args.scheduler == "auto" and "scheduler" in cfg:
+            ctx.set_scheduler(cfg["scheduler"])
         tuned = dict(shapes=ctx.tuning(), samples_per_pass=frame.samples_per_pass, source="--tuning")
     elif not args.no_autotune:
         t0 = time.time()
@@ -92,6 +94,13 @@ def run_render(args, pike_amd, scenes):
         tuned = dict(ms_before=before, ms_after=after, shapes=ctx.tuning(), samples_per_pass=args.spp_per_pass,
                      ms_per_sample_s1=s_before, ms_per_sample_chosen=s_after, seconds=time.time() - t0)
         log(f"autotune: traversal {before:.2f} -> {after:.2f} ms/pass, {ctx.tuning()} ({tuned['seconds']:.1f}s)")
+        if args.scheduler == "auto":
+            sc0, sc1 = frame.autotune_scheduler(scene, max_depth=depth, samples=2 * max(1, args.spp_per_pass))
+            log(f"autotune scheduler: {ctx.scheduler()} ({sc0:.3f} -> {sc1:.3f} ms/sample)")
+    if args.scheduler != "auto":
+        ctx.set_scheduler(args.scheduler)
+    if tuned is not None:
+        tuned["scheduler"] = ctx.scheduler()
     # warmup (also instantiates the graph)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=max(1, args.warmup), use_graph=True)
     ctx.sync()
@@ -119,7 +128,7 @@ def run_render(args, pike_amd, scenes):
                     kernel_ms=None, profile_frames=0, profile_counters=None,
                     roofline=dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
                                   traffic=None, bytes_per_launch=None, launch_ms=None),
-                    triangles=data.baked_triangles, autotune=tuned), data, bvh, cam
+                    triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler()), data, bvh, cam
     frame.reset()
     frame.count_nodes(True)
     frame.render(scene, max_depth=depth, first_sample=0, num_frames=pf, use_graph=False)
@@ -139,7 +148,7 @@ def run_render(args, pike_amd, scenes):
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                       traffic=measured_traffic(), kernel="k_trace_closest", bytes_per_launch=bytes_launch,
                       launch_ms=t_launch * 1e3),
-        triangles=data.baked_triangles, autotune=tuned)
+        triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler())
     return result, data, bvh, cam
 
 
@@ -159,6 +168,10 @@ def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
     frame = pike_amd.Frame(ctx, args.width, args.height)
     frame.set_camera(scenes.normalized_camera(data.camera, bvh.normalizer()))
     frame.set_samples_per_pass(spp_per_pass)
+    if args.scheduler == "auto":
+        frame.autotune_scheduler(scene, max_depth=args.depth, samples=2 * spp_per_pass)
+    else:
+        ctx.set_scheduler(args.scheduler)
     frame.render(scene, max_depth=args.depth, first_sample=0, num_frames=spp_per_pass, use_graph=True)
     ctx.sync()
     frame.reset()
@@ -171,7 +184,7 @@ def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
     assert np.isfinite(frame.read_image()).all(), "non-finite pixels"
     out = dict(spp_s=steps / dt, steps=steps, build_s=build_s, placements=len(data.instances),
                baked_triangles=data.baked_triangles, wide_nodes=info["wide_nodes"],
-               triangle_records=info["leaf_triangles"],
+               triangle_records=info["leaf_triangles"], scheduler=ctx.scheduler(),
                layout_mb=(info["wide_nodes"] * 128 + info["leaf_triangles"] * 48) / 1e6,
                overflow=frame.counters()["stack_overflow"])
     log(f"instanced c2: {out}")
@@ -282,7 +295,11 @@ def run_diff(args, pike_amd, scenes):
     spp, depth = args.diff_spp, args.diff_depth
     frame = pike_amd.Frame(ctx, W, H)
     frame.set_camera(cam)
-    frame.set_samples_per_pass(spp)  # the 16 samples of an iteration are one wavefront
+    frame.set_samples_per_pass(spp)  # the 16 samples of an iteration are one pass
+    if args.scheduler == "auto":
+        frame.autotune_scheduler(scene, max_depth=depth, samples=4 * spp)
+    else:
+        ctx.set_scheduler(args.scheduler)
     frame.render(scene, max_depth=depth, num_frames=args.diff_target_spp)
     target = pike_amd.DeviceBuffer(ctx, W * H * 16)
     frame.read_image_device(target)
@@ -331,7 +348,7 @@ def run_diff(args, pike_amd, scenes):
     t_adj = (time.perf_counter() - t0) / args.diff_iters
     out = dict(iters_per_s=args.diff_iters / dt, ms_per_iter=dt / args.diff_iters * 1e3, forward_ms=t_fwd * 1e3,
                adjoint_ms=t_adj * 1e3, adjoint_mpaths_s=W * H * spp / t_adj / 1e6, resolution=[W, H], spp=spp,
-               depth=depth, triangles=data.num_triangles,
+               depth=depth, triangles=data.num_triangles, scheduler=ctx.scheduler(),
                params=int(info["tex_width"] * info["tex_height"] * 4 + eh * ew * 3))
     log(f"differentiation: {out['iters_per_s']:.1f} iters/s (forward {out['forward_ms']:.2f} ms, adjoint "
         f"{out['adjoint_ms']:.2f} ms)")
@@ -502,6 +519,9 @@ def main():
                     help="BVH kind of the render, batching and differentiation scenes")
     ap.add_argument("--soup-bvh", choices=["sbvh", "sah", "both"], default="both",
                     help="BVH kind(s) the soup is traced under: spatial splits (the workload's), plain SAH, or both")
+    ap.add_argument("--scheduler", choices=["auto", "wavefront", "paths"], default="auto",
+                    help="pass scheduler of the single-view render legs: per-bounce wavefront kernels, the path "
+                         "megakernel, or the faster of the two by timing (auto)")
     ap.add_argument("--trace-iters", type=int, default=5)
     ap.add_argument("--spp-per-pass", type=int, default=1,
                     help="samples per pixel traced per wavefront pass of the 1080p render (without autotuning)")
@@ -588,7 +608,7 @@ def main():
                                f"2048x1024 env (2D CDF importance sampling), "
                                f"{args.width}x{args.height}, depth {args.depth}, 1 spp per step",
                    "width": args.width, "height": args.height, "depth": args.depth, "triangles": res["triangles"],
-                   "bvh": args.bvh, "instanced": args.instanced},
+                   "bvh": args.bvh, "instanced": args.instanced, "scheduler": res["scheduler"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,

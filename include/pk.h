@@ -131,10 +131,24 @@ int pk_sync(pk_ctx* ctx);
 #define PK_TUNE_SHADOW 2       /* any hit, NEE shadow rays */
 #define PK_TUNE_RAYS_CLOSEST 3 /* pk_trace_rays*, closest hit */
 #define PK_TUNE_RAYS_ANY 4     /* pk_trace_rays*, any hit */
-#define PK_TUNE_CLASSES 5
+#define PK_TUNE_PATHS 5        /* path megakernel (PK_SCHED_PATHS); waves_per_cu in [1, 16], refill_min unused */
+#define PK_TUNE_CLASSES 6
 int pk_ctx_set_tuning(pk_ctx* ctx, int kernel_class, uint32_t waves_per_cu, uint32_t refill_min);
 int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_cu, uint32_t* refill_min,
                       uint32_t* num_cus);
+/* Pass scheduler of single-view frames (no reference counterpart: the
+ * reference always runs its passes as wavefront compute dispatches,
+ * pathtracer.js:201-266).  PK_SCHED_WAVEFRONT: raygen, then per bounce
+ * trace_closest / shade / trace_shadow over queues in device memory.
+ * PK_SCHED_PATHS: one persistent megakernel in which each lane carries a
+ * path through every bounce in registers.  Both give bit-identical color
+ * buffers; batches, node counting (pk_frame_count_nodes) and
+ * pk_render_profile always use the wavefront kernels.  Captured graphs are
+ * re-recorded after a change. */
+#define PK_SCHED_WAVEFRONT 0
+#define PK_SCHED_PATHS 1
+int pk_ctx_set_scheduler(pk_ctx* ctx, int scheduler);
+int pk_ctx_get_scheduler(const pk_ctx* ctx, int* scheduler);
 /* Autotuner: coordinate search over (waves_per_cu, refill_min) for each
  * render kernel class, timing that class's kernels with HIP events over
  * `frames` profiled frames of this scene, camera and resolution; keeps the
@@ -149,6 +163,12 @@ int pk_autotune_render(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uin
  * chosen one. */
 int pk_autotune_samples_per_pass(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t max_samples,
                                  float* ms2);
+/* Pass scheduler: times `samples` samples of graph-replayed passes of this
+ * scene, camera and resolution under PK_SCHED_WAVEFRONT and PK_SCHED_PATHS
+ * (with the context's launch shapes and the frame's samples per pass) and
+ * leaves the faster set in the context and the frame reset.  ms2 =
+ * milliseconds per sample under the scheduler set before and the chosen one. */
+int pk_autotune_scheduler(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t samples, float* ms2);
 /* The same for the standalone ray queries on device-resident rays. */
 int pk_autotune_rays(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids,
                      float* d_t, uint32_t* d_bary, float* ms2);

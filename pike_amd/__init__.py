@@ -64,7 +64,19 @@ class Context:
     def sync(self):
         _check(lib().pk_sync(self.h), "pk_sync")
 
-    TUNE_CLASSES = {"primary": 0, "secondary": 1, "shadow": 2, "rays_closest": 3, "rays_any": 4}
+    TUNE_CLASSES = {"primary": 0, "secondary": 1, "shadow": 2, "rays_closest": 3, "rays_any": 4, "paths": 5}
+    SCHEDULERS = {"wavefront": 0, "paths": 1}
+
+    def set_scheduler(self, name):
+        """How single-view frames run a pass: "wavefront" (per-bounce kernels
+        over device queues) or "paths" (one path megakernel); the images are
+        bit-identical."""
+        _check(lib().pk_ctx_set_scheduler(self.h, self.SCHEDULERS[name]), "pk_ctx_set_scheduler")
+
+    def scheduler(self):
+        v = ctypes.c_int()
+        _check(lib().pk_ctx_get_scheduler(self.h, ctypes.byref(v)), "pk_ctx_get_scheduler")
+        return {c: n for n, c in self.SCHEDULERS.items()}[v.value]
 
     def set_tuning(self, kernel_class, waves_per_cu, refill_min):
         cls = self.TUNE_CLASSES[kernel_class] if isinstance(kernel_class, str) else kernel_class
@@ -448,6 +460,14 @@ class Frame:
         ms = (ctypes.c_float * 2)()
         _check(lib().pk_autotune_samples_per_pass(self.h, scene.h, max_depth, max_samples, ms),
                "pk_autotune_samples_per_pass")
+        return float(ms[0]), float(ms[1])
+
+    def autotune_scheduler(self, scene, max_depth=4, samples=8):
+        """Choose the context's pass scheduler ("wavefront" or "paths") by
+        timing `samples` samples under each (resets the accumulation);
+        returns (ms/sample before, after)."""
+        ms = (ctypes.c_float * 2)()
+        _check(lib().pk_autotune_scheduler(self.h, scene.h, max_depth, samples, ms), "pk_autotune_scheduler")
         return float(ms[0]), float(ms[1])
 
     def profile(self, scene, max_depth=4, first_sample=0, num_frames=1, env_theta=0.0):

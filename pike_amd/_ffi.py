@@ -144,6 +144,8 @@ SIGNATURES = {
     "pk_ctx_destroy": (c_int, [c_void_p]),
     "pk_sync": (c_int, [c_void_p]),
     "pk_ctx_set_tuning": (c_int, [c_void_p, c_int, c_uint32, c_uint32]),
+    "pk_ctx_set_scheduler": (c_int, [c_void_p, c_int]),
+    "pk_ctx_get_scheduler": (c_int, [c_void_p, ctypes.POINTER(c_int)]),
     "pk_ctx_get_tuning": (c_int, [c_void_p, c_int, ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
                                   ctypes.POINTER(c_uint32)]),
     "pk_autotune_render": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_void_p]),
@@ -201,6 +203,7 @@ SIGNATURES = {
     "pk_frame_get_samples_per_pass": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "pk_autotune_samples_per_pass": (c_int, [c_void_p, c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                              ctypes.POINTER(c_float)]),
+    "pk_autotune_scheduler": (c_int, [c_void_p, c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(c_float)]),
     "pk_scene_param_info": (c_int, [c_void_p, c_void_p]),
     "pk_scene_write_params": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, ctypes.c_size_t, c_int, c_int]),
     "pk_scene_read_params": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, ctypes.c_size_t]),

@@ -108,6 +108,12 @@ void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, cons
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s);
 void launch_trace_shadow(const DevScene& S, const FrameDev& F, const TraceLaunch& L, hipStream_t s);
 void launch_accumulate(const FrameDev& F, hipStream_t s);
+// Single-view pass as one persistent path megakernel (k_render_paths): the
+// same color buffer as raygen + depth x (trace_closest, shade, trace_shadow).
+// Waves per SIMD its registers are compiled for; L.grid must not exceed
+// 4 * kMegaWavesPerSimd one-wave blocks per CU (the spill area's size).
+constexpr int kMegaWavesPerSimd = 4;
+void launch_render_paths(const DevScene& S, const FrameDev& F, uint32_t depth, const TraceLaunch& L, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s);

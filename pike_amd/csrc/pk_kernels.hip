@@ -816,6 +816,161 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
 }
 
 // ---------------------------------------------------------------------------
+// Path megakernel: a single-view pass in one persistent launch.  Each lane
+// owns one path at a time and takes it through every bounce (closest hit,
+// miss / emission / shading, NEE shadow ray) in registers, summing its
+// contributions in the order the wavefront kernels add them to its color
+// slot; when the path ends it writes that slot (rgb sum, contribution
+// count) once and takes the next path of the pass.  The per-bounce queues
+// (rays, hits, shadow rays, throughput) never reach memory, and a lane whose
+// path ended refills between bounces, so waves stay full while path lengths
+// differ.  Path numbering, seeds and arithmetic are the wavefront's, so the
+// color buffer is bit-identical to the one raygen / trace_closest / shade /
+// trace_shadow produce.  The work cursor (kCntWorkTrace) is zeroed before
+// the launch.
+// ---------------------------------------------------------------------------
+template <bool kInst>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWavesPerSimd)))
+    k_render_paths(DevScene S, FrameDev F, uint32_t depth) {
+  __shared__ uint32_t lds_node[kStackLds * 64];
+  __shared__ float lds_t[kStackLds * 64];
+  const int lane = (int)threadIdx.x;
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const FrameParams P = *F.params;
+  const uint32_t ns = pass_samples(F, P);
+  const uint32_t B = F.W * F.H;
+  const uint32_t n = B * ns;
+  uint32_t* const cursor = &F.cnt[kCntWorkTrace];
+  TraverseStats st = {0, 0};
+  uint32_t traced = 0, hits = 0, shadows = 0, overflow = 0;
+  const v3 zero = pkm::mk(0.0f, 0.0f, 0.0f);
+  bool has = false;
+  uint32_t pixel = 0, s = 0, b = 0;
+  v3 o = zero, d = zero, thr = zero, L = zero;
+  float cnt = 0.0f;
+  // paths claimed by this wave and not yet started (wave-uniform)
+  uint32_t pool_lo = 0, pool_hi = 0;
+  bool claimed_all = n == 0;
+  while (true) {
+    // lanes without a path take the next ones of the pool
+    const unsigned long long idle = __ballot(!has);
+    if (idle && !(claimed_all && pool_lo >= pool_hi)) {
+      const uint32_t nidle = (uint32_t)__popcll(idle);
+      const uint32_t have = pool_hi - pool_lo;
+      uint32_t fresh = 0;
+      bool got = false;
+      if (have < nidle && !claimed_all) {
+        uint32_t c0 = 0;
+        if (lane == 0) c0 = atomicAdd(cursor, kClaimChunk);
+        fresh = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(c0, 0));
+        got = true;
+        claimed_all = fresh >= n || n - fresh <= kClaimChunk;
+      }
+      if (!has) {
+        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        const uint32_t p = rank < have ? pool_lo + rank : (got ? fresh + (rank - have) : 0xffffffffu);
+        if (p < n) {
+          has = true;
+          pixel = p / ns;
+          s = p - pixel * ns;
+          b = 0;
+          thr = pkm::mk(1.0f, 1.0f, 1.0f);
+          L = zero;
+          cnt = 0.0f;
+          camera_ray(pixel % F.W, pixel / F.W, F.W, F.H, P.sample + s, P.cam, &o, &d);
+        }
+      }
+      if (have >= nidle) {
+        pool_lo += nidle;
+      } else if (got) {
+        pool_lo = fresh + (nidle - have);
+        pool_hi = fresh + kClaimChunk;
+      } else {
+        pool_lo = pool_hi;
+      }
+      pool_hi = min(pool_hi, n);
+      pool_lo = min(pool_lo, pool_hi);
+    }
+    if (__ballot(has) == 0ull) break;
+    // closest hit (trace.wgsl) and miss shading (shade_miss.wgsl)
+    RayState r;
+    r.best_id = 0xffffffffu;
+    r.d = d;
+    if (has) {
+      ray_init<false>(r, o, d);
+      while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
+      }
+      if (r.overflow) overflow = 1;
+      traced++;
+    }
+    bool end = false;
+    ShadeOut so;
+    so.emit = so.scatter = so.bounce = so.shadow = false;
+    if (has) {
+      if (r.best_id == 0xffffffffu) {
+        const v3 c = pkm::mul(env_color(S, P.env_theta, r.d), thr);
+        L.x += c.x;
+        L.y += c.y;
+        L.z += c.z;
+        cnt += 1.0f;
+        end = true;
+      } else {
+        hits++;
+        shade_hit<false>(S, P.env_theta, o, d, r.tbest, r.best_leaf, r.best_bary, shade_seed(b, pixel, P.sample + s),
+                         thr, so, nullptr);
+        if (so.emit) {
+          L.x += so.L.x;
+          L.y += so.L.y;
+          L.z += so.L.z;
+          cnt += 1.0f;
+        }
+      }
+    }
+    // NEE shadow ray (trace_shadow.wgsl), unoccluded: env radiance x NEE throughput
+    if (so.shadow) {
+      RayState q;
+      ray_init<true>(q, so.s_o, so.s_d);
+      while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
+      }
+      if (q.overflow) overflow = 1;
+      shadows++;
+      if (q.best_id == 0xffffffffu) {
+        const v3 c = pkm::mul(env_color(S, P.env_theta, q.d), so.s_thr);
+        L.x += c.x;
+        L.y += c.y;
+        L.z += c.z;
+        cnt += 1.0f;
+      }
+    }
+    if (has && !end) {
+      // the continuation of the last bounce is never traced
+      if (so.bounce && b + 1u < depth) {
+        o = so.b_o;
+        d = so.b_d;
+        thr = so.b_thr;
+        b++;
+      } else {
+        end = true;
+      }
+    }
+    if (end) {
+      F.color[(size_t)s * B + pixel] = make_float4(L.x, L.y, L.z, cnt);
+      has = false;
+    }
+  }
+  traced = wave_sum(traced);
+  hits = wave_sum(hits);
+  shadows = wave_sum(shadows);
+  overflow = wave_sum(overflow);
+  if (lane == 0) {
+    atomicAdd(&F.stats[kStatRays], (unsigned long long)traced);
+    atomicAdd(&F.stats[kStatHits], (unsigned long long)hits);
+    atomicAdd(&F.stats[kStatShadow], (unsigned long long)shadows);
+    if (overflow) atomicOr(&F.cnt[kCntOverflow], 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // accumulate.wgsl main (:20-34): running mean over frames.  A pass of S
 // samples folds them in sample order with the same per-sample update, so the
 // mean equals S one-sample frames bit for bit.
@@ -935,7 +1090,7 @@ __device__ inline void wave_scatter_add(float* dst, uint32_t key, bool valid, co
 //   and are then scattered with rem = L - c_0 - ... - c_b, subtracted in
 //   path order.
 template <bool kInst>
-__global__ void __launch_bounds__(64) k_adjoint(DevScene S, AdjointArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_adjoint(DevScene S, AdjointArgs A) {
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
@@ -1228,6 +1383,13 @@ void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunc
   else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
   else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
   else hipLaunchKernelGGL((k_trace_shadow<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+}
+void launch_render_paths(const DevScene& S, const FrameDev& F0, uint32_t depth, const TraceLaunch& L, hipStream_t s) {
+  FrameDev F = F0;
+  F.spill = L.spill;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, F.cnt + kCntWorkTrace, 0u, 1u);
+  if (S.num_inst) hipLaunchKernelGGL(k_render_paths<true>, dim3(L.grid), dim3(64), 0, s, S, F, depth);
+  else hipLaunchKernelGGL(k_render_paths<false>, dim3(L.grid), dim3(64), 0, s, S, F, depth);
 }
 void launch_accumulate(const FrameDev& F, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);

@@ -35,6 +35,7 @@ struct pk_ctx {
     uint32_t refill;  // idle lanes that trigger a wave-level ray refill
   } shape[PK_TUNE_CLASSES];
   uint32_t tuning_gen = 0;  // bumped by pk_ctx_set_tuning; captured graphs keep the old shape
+  int sched = PK_SCHED_WAVEFRONT;
   uint32_t* cursors = nullptr;  // per-launch work cursors for ray queries
   uint32_t ncursors = 0;
   uint2* spill = nullptr;
@@ -92,7 +93,7 @@ namespace {
 // of the 4 SIMDs (the register and LDS budget of the traversal kernels).
 constexpr uint32_t kMaxWavesPerCu = 4 * pk::kTraceWavesPerSimd;
 // One-wave adjoint blocks per CU (its registers allow 2 waves per SIMD).
-constexpr uint32_t kAdjointWavesPerCu = 8;
+constexpr uint32_t kAdjointWavesPerCu = 16;
 
 pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
   return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill};
@@ -173,6 +174,7 @@ int pk_ctx_create(int device, pk_ctx** out) {
   c->shape[PK_TUNE_SHADOW] = {8, 64};
   c->shape[PK_TUNE_RAYS_CLOSEST] = {kMaxWavesPerCu, 16};
   c->shape[PK_TUNE_RAYS_ANY] = {kMaxWavesPerCu, 24};
+  c->shape[PK_TUNE_PATHS] = {4 * pk::kMegaWavesPerSimd, 64};
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess)
     e = hipMalloc((void**)&c->spill, (size_t)c->num_cus * kMaxWavesPerCu * pk::kSpillPerLane * 64 * sizeof(uint2));
@@ -223,13 +225,33 @@ int pk_ctx_set_tuning(pk_ctx* c, int cls, uint32_t waves_per_cu, uint32_t refill
     pk_set_error("pk_ctx_set_tuning: null context or unknown kernel class");
     return PK_E_INVALID;
   }
-  if (waves_per_cu < 1 || waves_per_cu > kMaxWavesPerCu || refill_min < 1 || refill_min > 64) {
-    pk_set_error("pk_ctx_set_tuning: waves_per_cu must be in [1, " + std::to_string(kMaxWavesPerCu) +
+  const uint32_t wmax = cls == PK_TUNE_PATHS ? 4u * pk::kMegaWavesPerSimd : kMaxWavesPerCu;
+  if (waves_per_cu < 1 || waves_per_cu > wmax || refill_min < 1 || refill_min > 64) {
+    pk_set_error("pk_ctx_set_tuning: waves_per_cu must be in [1, " + std::to_string(wmax) +
                  "] and refill_min in [1, 64]");
     return PK_E_LIMIT;
   }
   c->shape[cls] = {waves_per_cu, refill_min};
   c->tuning_gen++;
+  return PK_OK;
+}
+
+int pk_ctx_set_scheduler(pk_ctx* c, int sched) {
+  if (!c || (sched != PK_SCHED_WAVEFRONT && sched != PK_SCHED_PATHS)) {
+    pk_set_error("pk_ctx_set_scheduler: null context or unknown scheduler");
+    return PK_E_INVALID;
+  }
+  if (sched != c->sched) c->tuning_gen++;
+  c->sched = sched;
+  return PK_OK;
+}
+
+int pk_ctx_get_scheduler(const pk_ctx* c, int* sched) {
+  if (!c || !sched) {
+    pk_set_error("pk_ctx_get_scheduler: null argument");
+    return PK_E_INVALID;
+  }
+  *sched = c->sched;
   return PK_OK;
 }
 
@@ -740,6 +762,12 @@ int pk_accumulate(pk_frame* f) {
 }
 
 static void enqueue_frame(pk_frame* f, pk_scene* s, uint32_t depth, hipStream_t st) {
+  if (f->ctx->sched == PK_SCHED_PATHS && !f->dev.views && !f->count_nodes && depth > 0) {
+    pk::launch_render_paths(s->dev, f->dev, depth, trace_launch(f, PK_TUNE_PATHS), st);
+    pk::launch_accumulate(f->dev, st);
+    pk::launch_advance(f->dev, st);
+    return;
+  }
   pk::launch_raygen(f->dev, st);
   for (uint32_t b = 0; b < depth; ++b) {
     pk::launch_trace_closest(s->dev, f->dev, (int)b, trace_launch(f, closest_class(b)), st);
@@ -1602,6 +1630,51 @@ extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t d
   if (ms2) {
     ms2[0] = at_start;
     ms2[1] = best;
+  }
+  return pk_frame_reset(f);
+}
+
+// Pass scheduler: times `samples` samples of graph-replayed passes under each
+// scheduler (after one warm pass, which also records the graph) and keeps
+// the faster.  Which wins depends on the scene: the wavefront kernels refill
+// lanes inside traversal and run more waves per SIMD, the megakernel saves
+// the queues' memory traffic and the per-bounce launches.
+extern "C" int pk_autotune_scheduler(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t samples, float* ms2) {
+  if (!f || !s || depth == 0 || depth > 64 || samples == 0 || f->batch_scene) {
+    pk_set_error("pk_autotune_scheduler: null argument, depth outside [1, 64], zero samples or a batch");
+    return PK_E_INVALID;
+  }
+  if (!f->has_camera) {
+    pk_set_error("pk_autotune_scheduler: no camera set");
+    return PK_E_STATE;
+  }
+  pk_ctx* c = f->ctx;
+  hipStream_t st = c->stream;
+  const int start = c->sched;
+  float ms[2] = {0, 0};
+  pk_render_params p{f->W, f->H, depth, 0, samples, f->theta, 1};
+  int rc = PK_OK;
+  for (int sched : {PK_SCHED_WAVEFRONT, PK_SCHED_PATHS}) {
+    if ((rc = pk_ctx_set_scheduler(c, sched))) break;
+    pk_render_params warm = p;
+    warm.num_frames = f->dev.S;
+    if ((rc = pk_render(f, s, &warm))) break;
+    PK_HIP(hipEventRecord(c->ev0, st));
+    if ((rc = pk_render(f, s, &p))) break;
+    PK_HIP(hipEventRecord(c->ev1, st));
+    PK_HIP(hipEventSynchronize(c->ev1));
+    PK_HIP(hipEventElapsedTime(&ms[sched], c->ev0, c->ev1));
+    ms[sched] /= (float)samples;
+  }
+  if (rc) {
+    (void)pk_ctx_set_scheduler(c, start);
+    return rc;
+  }
+  const int best = ms[PK_SCHED_PATHS] < ms[PK_SCHED_WAVEFRONT] ? PK_SCHED_PATHS : PK_SCHED_WAVEFRONT;
+  if ((rc = pk_ctx_set_scheduler(c, best))) return rc;
+  if (ms2) {
+    ms2[0] = ms[start];
+    ms2[1] = ms[best];
   }
   return pk_frame_reset(f);
 }

@@ -751,6 +751,55 @@ napi_value AutotuneSamples(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// autotuneScheduler(frame, scene, depth, samples): keeps the faster pass
+// scheduler in the frame's context
+napi_value AutotuneScheduler(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 3)) return nullptr;
+  FRAME_ARG(0);
+  SCENE_ARG(1);
+  uint32_t depth = 4, samples = 8;
+  napi_get_value_uint32(env, a.argv[2], &depth);
+  if (a.argc > 3) napi_get_value_uint32(env, a.argv[3], &samples);
+  float ms[2];
+  PK_CALL(pk_autotune_scheduler(f, s, depth, samples, ms), "pk_autotune_scheduler");
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "msPerSampleBefore", ms[0]);
+  set_num(env, o, "msPerSampleAfter", ms[1]);
+  return o;
+}
+
+// ctxSetScheduler(ctx, "wavefront" | "paths"); ctxScheduler(ctx)
+napi_value CtxSetScheduler(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 2)) return nullptr;
+  auto* c = (pk_ctx*)unwrap(env, a.argv[0], kCtx, "context");
+  if (!c) return nullptr;
+  char name[16] = {0};
+  size_t len = 0;
+  napi_get_value_string_utf8(env, a.argv[1], name, sizeof(name), &len);
+  const std::string n(name, len);
+  if (n != "wavefront" && n != "paths") {
+    napi_throw_error(env, nullptr, "ctxSetScheduler: scheduler must be \"wavefront\" or \"paths\"");
+    return nullptr;
+  }
+  PK_CALL(pk_ctx_set_scheduler(c, n == "paths" ? PK_SCHED_PATHS : PK_SCHED_WAVEFRONT), "pk_ctx_set_scheduler");
+  return undefined(env);
+}
+
+napi_value CtxScheduler(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, &a, 1)) return nullptr;
+  auto* c = (pk_ctx*)unwrap(env, a.argv[0], kCtx, "context");
+  if (!c) return nullptr;
+  int sched = 0;
+  PK_CALL(pk_ctx_get_scheduler(c, &sched), "pk_ctx_get_scheduler");
+  napi_value v;
+  napi_create_string_utf8(env, sched == PK_SCHED_PATHS ? "paths" : "wavefront", NAPI_AUTO_LENGTH, &v);
+  return v;
+}
+
 // sceneWriteParams(scene, which, offset, Float32Array, alsoSampling)
 napi_value SceneWriteParams(napi_env env, napi_callback_info info) {
   Args a;
@@ -843,6 +892,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"counters", Counters},         {"autotune", Autotune},
       {"traceRays", TraceRays},
       {"frameSetSamplesPerPass", FrameSetSamplesPerPass}, {"autotuneSamples", AutotuneSamples},
+      {"autotuneScheduler", AutotuneScheduler}, {"ctxSetScheduler", CtxSetScheduler},
+      {"ctxScheduler", CtxScheduler},
       {"sceneWriteParams", SceneWriteParams}, {"sceneReadParams", SceneReadParams},
       {"adjoint", Adjoint},
       {"bvhBuildInstanced", BvhBuildInstanced}, {"bvhNormalizer", BvhNormalizer},
