@@ -195,8 +195,10 @@ def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
 
 def run_batching(args, pike_amd, scenes, ctx):
     """Config 4: 32 scenes x 64 views (aggregate samples/s), plain per-view
-    wavefront renders vs one batched graph launch; the images are identical
-    (checked on every view), so the comparison is at equal error."""
+    renders (one sample per pass, and multi-sample passes) vs one batched
+    graph launch, each under both pass schedulers (per-bounce wavefront
+    kernels, path megakernel; the faster is reported); the images are
+    identical (checked on every view), so the comparison is at equal error."""
     import numpy as np
 
     t0 = time.time()
@@ -244,25 +246,43 @@ def run_batching(args, pike_amd, scenes, ctx):
         batch.render(max_depth=depth, use_graph=True)
         ctx.sync()
 
-    plain()
-    plain_ms()
-    batched()  # warm-up: instantiates every graph
-    t0 = time.perf_counter()
-    plain()
-    t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    plain_ms()
-    t_plain_ms = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    batched()
-    t_batch = time.perf_counter() - t0
-    same = all(np.array_equal(batch.read_view(i).view(np.uint32), frames[i].read_image().view(np.uint32)) and
-               np.array_equal(frames_ms[i].read_image().view(np.uint32), frames[i].read_image().view(np.uint32))
-               for i in range(len(views)))
+    # each method under each pass scheduler; the images must be identical
+    # across all of them
+    ref = None
+    per = {}
+    same = True
+    for sched in ("wavefront", "paths"):
+        ctx.set_scheduler(sched)
+        plain()
+        plain_ms()
+        batched()  # warm-up: instantiates every graph
+        t0 = time.perf_counter()
+        plain()
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        plain_ms()
+        t_plain_ms = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        batched()
+        t_batch = time.perf_counter() - t0
+        if ref is None:
+            ref = [frames[i].read_image() for i in range(len(views))]
+        same = same and all(
+            np.array_equal(batch.read_view(i).view(np.uint32), ref[i].view(np.uint32)) and
+            np.array_equal(frames_ms[i].read_image().view(np.uint32), ref[i].view(np.uint32)) and
+            np.array_equal(frames[i].read_image().view(np.uint32), ref[i].view(np.uint32))
+            for i in range(len(views)))
+        per[sched] = dict(plain_s=t_plain, plain_multisample_s=t_plain_ms, batched_s=t_batch)
+        log(f"batching ({sched}): plain {samples / t_plain / 1e6:.0f}, plain multi-sample "
+            f"{samples / t_plain_ms / 1e6:.0f}, batched {samples / t_batch / 1e6:.0f} Msamples/s")
+    ctx.set_scheduler("wavefront")
+    best = {k: min(v[k] for v in per.values()) for k in ("plain_s", "plain_multisample_s", "batched_s")}
+    t_plain, t_plain_ms, t_batch = best["plain_s"], best["plain_multisample_s"], best["batched_s"]
     out = dict(samples=samples, plain_s=t_plain, plain_multisample_s=t_plain_ms, batched_s=t_batch,
                plain_msamples_s=samples / t_plain / 1e6, plain_multisample_msamples_s=samples / t_plain_ms / 1e6,
                batched_msamples_s=samples / t_batch / 1e6, speedup=t_plain / t_batch,
                speedup_vs_plain_multisample=t_plain_ms / t_batch, images_identical=same,
+               batched_scheduler=min(per, key=lambda k: per[k]["batched_s"]), per_scheduler=per,
                depth=depth, scenes=len(sc), views=len(views))
     log(f"batching: plain {out['plain_msamples_s']:.0f} Msamples/s, plain multi-sample "
         f"{out['plain_multisample_msamples_s']:.0f}, batched {out['batched_msamples_s']:.0f} Msamples/s "

@@ -136,15 +136,15 @@ int pk_sync(pk_ctx* ctx);
 int pk_ctx_set_tuning(pk_ctx* ctx, int kernel_class, uint32_t waves_per_cu, uint32_t refill_min);
 int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_cu, uint32_t* refill_min,
                       uint32_t* num_cus);
-/* Pass scheduler of single-view frames (no reference counterpart: the
+/* Pass scheduler of frames (no reference counterpart: the
  * reference always runs its passes as wavefront compute dispatches,
  * pathtracer.js:201-266).  PK_SCHED_WAVEFRONT: raygen, then per bounce
  * trace_closest / shade / trace_shadow over queues in device memory.
  * PK_SCHED_PATHS: one persistent megakernel in which each lane carries a
  * path through every bounce in registers.  Both give bit-identical color
- * buffers; batches, node counting (pk_frame_count_nodes) and
- * pk_render_profile always use the wavefront kernels.  Captured graphs are
- * re-recorded after a change. */
+ * buffers, for single-view frames and batches (pk_batch_render) alike; node
+ * counting (pk_frame_count_nodes) and pk_render_profile always use the
+ * wavefront kernels.  Captured graphs are re-recorded after a change. */
 #define PK_SCHED_WAVEFRONT 0
 #define PK_SCHED_PATHS 1
 int pk_ctx_set_scheduler(pk_ctx* ctx, int scheduler);

@@ -816,20 +816,21 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
 }
 
 // ---------------------------------------------------------------------------
-// Path megakernel: a single-view pass in one persistent launch.  Each lane
-// owns one path at a time and takes it through every bounce (closest hit,
-// miss / emission / shading, NEE shadow ray) in registers, summing its
-// contributions in the order the wavefront kernels add them to its color
-// slot; when the path ends it writes that slot (rgb sum, contribution
-// count) once and takes the next path of the pass.  The per-bounce queues
-// (rays, hits, shadow rays, throughput) never reach memory, and a lane whose
-// path ended refills between bounces, so waves stay full while path lengths
-// differ.  Path numbering, seeds and arithmetic are the wavefront's, so the
-// color buffer is bit-identical to the one raygen / trace_closest / shade /
-// trace_shadow produce.  The work cursor (kCntWorkTrace) is zeroed before
-// the launch.
+// Path megakernel: a pass in one persistent launch.  Each lane owns one path
+// at a time and takes it through every bounce (closest hit, miss / emission /
+// shading, NEE shadow ray) in registers, summing its contributions in the
+// order the wavefront kernels add them to its color slot; when the path ends
+// it writes that slot (rgb sum, contribution count) once and takes the next
+// path of the pass.  The per-bounce queues (rays, hits, shadow rays,
+// throughput) never reach memory, and lanes whose path ended refill between
+// bounces, so waves stay full while path lengths differ.  Path numbering,
+// seeds and arithmetic are the wavefront's (single view: pixel-major
+// samples; batch: one path per global pixel, with the view's camera and
+// root, and no path for views whose samples are spent), so the color buffer
+// is bit-identical to the one raygen / trace_closest / shade / trace_shadow
+// produce.  The work cursor (kCntWorkTrace) is zeroed before the launch.
 // ---------------------------------------------------------------------------
-template <bool kInst>
+template <bool kInst, bool kBatch>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWavesPerSimd)))
     k_render_paths(DevScene S, FrameDev F, uint32_t depth) {
   __shared__ uint32_t lds_node[kStackLds * 64];
@@ -837,7 +838,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   const int lane = (int)threadIdx.x;
   const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   const FrameParams P = *F.params;
-  const uint32_t ns = pass_samples(F, P);
+  const uint32_t ns = kBatch ? 1u : pass_samples(F, P);
   const uint32_t B = F.W * F.H;
   const uint32_t n = B * ns;
   uint32_t* const cursor = &F.cnt[kCntWorkTrace];
@@ -845,16 +846,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   uint32_t traced = 0, hits = 0, shadows = 0, overflow = 0;
   const v3 zero = pkm::mk(0.0f, 0.0f, 0.0f);
   bool has = false;
-  uint32_t pixel = 0, s = 0, b = 0;
+  uint32_t pixel = 0, sample = 0, slot = 0, root = 0, b = 0;
   v3 o = zero, d = zero, thr = zero, L = zero;
   float cnt = 0.0f;
   // paths claimed by this wave and not yet started (wave-uniform)
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = n == 0;
   while (true) {
-    // lanes without a path take the next ones of the pool
-    const unsigned long long idle = __ballot(!has);
-    if (idle && !(claimed_all && pool_lo >= pool_hi)) {
+    // lanes without a path take the next ones of the pool (again while a
+    // batch path of a finished view left a lane idle)
+    for (unsigned long long idle = __ballot(!has); idle && !(claimed_all && pool_lo >= pool_hi);
+         idle = __ballot(!has)) {
       const uint32_t nidle = (uint32_t)__popcll(idle);
       const uint32_t have = pool_hi - pool_lo;
       uint32_t fresh = 0;
@@ -870,14 +872,28 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
         const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         const uint32_t p = rank < have ? pool_lo + rank : (got ? fresh + (rank - have) : 0xffffffffu);
         if (p < n) {
-          has = true;
-          pixel = p / ns;
-          s = p - pixel * ns;
+          if (kBatch) {
+            const ViewDev& V = F.views[view_of(F, p)];
+            if (P.sample < V.spp) {
+              has = true;
+              pixel = p - V.offset;
+              sample = P.sample;
+              slot = p;
+              root = V.root;
+              camera_ray(pixel % V.W, pixel / V.W, V.W, V.H, sample, V.cam, &o, &d);
+            }
+          } else {
+            has = true;
+            pixel = p / ns;
+            const uint32_t s = p - pixel * ns;
+            sample = P.sample + s;
+            slot = s * B + pixel;
+            camera_ray(pixel % F.W, pixel / F.W, F.W, F.H, sample, P.cam, &o, &d);
+          }
           b = 0;
           thr = pkm::mk(1.0f, 1.0f, 1.0f);
           L = zero;
           cnt = 0.0f;
-          camera_ray(pixel % F.W, pixel / F.W, F.W, F.H, P.sample + s, P.cam, &o, &d);
         }
       }
       if (have >= nidle) {
@@ -898,6 +914,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
     r.d = d;
     if (has) {
       ray_init<false>(r, o, d);
+      if (kBatch) r.node = root;
       while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
       }
       if (r.overflow) overflow = 1;
@@ -916,8 +933,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
         end = true;
       } else {
         hits++;
-        shade_hit<false>(S, P.env_theta, o, d, r.tbest, r.best_leaf, r.best_bary, shade_seed(b, pixel, P.sample + s),
-                         thr, so, nullptr);
+        shade_hit<false>(S, P.env_theta, o, d, r.tbest, r.best_leaf, r.best_bary, shade_seed(b, pixel, sample), thr,
+                         so, nullptr);
         if (so.emit) {
           L.x += so.L.x;
           L.y += so.L.y;
@@ -930,6 +947,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
     if (so.shadow) {
       RayState q;
       ray_init<true>(q, so.s_o, so.s_d);
+      if (kBatch) q.node = root;
       while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
       }
       if (q.overflow) overflow = 1;
@@ -954,7 +972,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
       }
     }
     if (end) {
-      F.color[(size_t)s * B + pixel] = make_float4(L.x, L.y, L.z, cnt);
+      F.color[slot] = make_float4(L.x, L.y, L.z, cnt);
       has = false;
     }
   }
@@ -1388,8 +1406,10 @@ void launch_render_paths(const DevScene& S, const FrameDev& F0, uint32_t depth, 
   FrameDev F = F0;
   F.spill = L.spill;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, F.cnt + kCntWorkTrace, 0u, 1u);
-  if (S.num_inst) hipLaunchKernelGGL(k_render_paths<true>, dim3(L.grid), dim3(64), 0, s, S, F, depth);
-  else hipLaunchKernelGGL(k_render_paths<false>, dim3(L.grid), dim3(64), 0, s, S, F, depth);
+  // batches hold no instances (pk_scene_upload_batch rejects them)
+  if (F.views) hipLaunchKernelGGL((k_render_paths<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, depth);
+  else if (S.num_inst) hipLaunchKernelGGL((k_render_paths<true, false>), dim3(L.grid), dim3(64), 0, s, S, F, depth);
+  else hipLaunchKernelGGL((k_render_paths<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, depth);
 }
 void launch_accumulate(const FrameDev& F, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);

@@ -92,8 +92,13 @@ namespace {
 // One-wave traversal blocks per CU at most: kTraceWavesPerSimd waves on each
 // of the 4 SIMDs (the register and LDS budget of the traversal kernels).
 constexpr uint32_t kMaxWavesPerCu = 4 * pk::kTraceWavesPerSimd;
-// One-wave adjoint blocks per CU (its registers allow 2 waves per SIMD).
+// One-wave adjoint blocks per CU: k_adjoint is compiled for 4 waves per SIMD
+// (128 VGPRs, the rest spilled to scratch), which measured 1.35x faster on
+// the 512x512 differentiation workload than the 2 waves per SIMD its
+// registers need unspilled.
 constexpr uint32_t kAdjointWavesPerCu = 16;
+static_assert(kAdjointWavesPerCu <= kMaxWavesPerCu, "adjoint grid exceeds the spill area");
+static_assert(4 * pk::kMegaWavesPerSimd <= kMaxWavesPerCu, "path megakernel grid exceeds the spill area");
 
 pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
   return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill};
@@ -762,7 +767,7 @@ int pk_accumulate(pk_frame* f) {
 }
 
 static void enqueue_frame(pk_frame* f, pk_scene* s, uint32_t depth, hipStream_t st) {
-  if (f->ctx->sched == PK_SCHED_PATHS && !f->dev.views && !f->count_nodes && depth > 0) {
+  if (f->ctx->sched == PK_SCHED_PATHS && !f->count_nodes && depth > 0) {
     pk::launch_render_paths(s->dev, f->dev, depth, trace_launch(f, PK_TUNE_PATHS), st);
     pk::launch_accumulate(f->dev, st);
     pk::launch_advance(f->dev, st);
@@ -1217,8 +1222,14 @@ int pk_batch_create(pk_ctx* c, pk_scene* s, const pk_view* views, uint32_t nview
 static void enqueue_batch(pk_frame* f, pk_scene* s, uint32_t depth, float theta, hipStream_t st) {
   uint32_t rounds = 0;
   for (const auto& v : f->views) rounds = std::max(rounds, v.spp);
+  const bool paths = f->ctx->sched == PK_SCHED_PATHS && !f->count_nodes;
   for (uint32_t r = 0; r < rounds; ++r) {
     pk::launch_begin(f->dev, r, theta, f->cam, st);
+    if (paths) {
+      pk::launch_render_paths(s->dev, f->dev, depth, trace_launch(f, PK_TUNE_PATHS), st);
+      pk::launch_accumulate(f->dev, st);
+      continue;
+    }
     pk::launch_raygen(f->dev, st);
     for (uint32_t b = 0; b < depth; ++b) {
       pk::launch_trace_closest(s->dev, f->dev, (int)b, trace_launch(f, closest_class(b)), st);
@@ -1467,8 +1478,7 @@ int pk_adjoint(pk_frame* f, pk_scene* s, const pk_render_params* p, const float*
   A.cursor = f->adj_cursor;
   A.spill = f->ctx->spill;
   A.overflow = f->dev.cnt + pk::kCntOverflow;
-  // 2 waves per SIMD fit the adjoint's registers; the spill area holds
-  // kMaxWavesPerCu waves per CU
+  // the spill area holds kMaxWavesPerCu >= kAdjointWavesPerCu waves per CU
   const uint32_t grid = (uint32_t)f->ctx->num_cus * kAdjointWavesPerCu;
   const size_t nvtx = (size_t)grid * p->max_depth * pk::kAdjVtxWords * 64;
   if (f->adj_vtx_n < nvtx) {

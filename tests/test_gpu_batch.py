@@ -18,8 +18,11 @@ def _small_batch():
     return sc, views
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_equals_per_view_render(ctx, use_graph):
+@pytest.mark.parametrize("use_graph,sched", [(True, "wavefront"), (False, "wavefront"), (True, "paths"),
+                                             (False, "paths")])
+def test_batch_equals_per_view_render(ctx, use_graph, sched):
+    """Under either pass scheduler for the batch; the single views always
+    render through the wavefront kernels."""
     sc, views = _small_batch()
     bvhs = [pike_amd.BVH(s.positions, s.triangles) for s in sc]
     merged = pike_amd.BatchScene(ctx, sc, bvhs)
@@ -29,7 +32,11 @@ def test_batch_equals_per_view_render(ctx, use_graph):
         cam = scenes.camera((0.5 + 0.8 * np.cos(az), 0.4, 0.5 + 0.8 * np.sin(az)), (0.5, 0.05, 0.5), 1.0)
         cams.append(scenes.normalized_camera(cam, bvhs[k].normalizer()))
     batch = pike_amd.Batch(ctx, merged, [(k, w, h, spp, cams[i]) for i, (k, w, h, spp) in enumerate(views)])
-    batch.render(max_depth=4, use_graph=use_graph)
+    ctx.set_scheduler(sched)
+    try:
+        batch.render(max_depth=4, use_graph=use_graph)
+    finally:
+        ctx.set_scheduler("wavefront")
     assert batch.counters()["stack_overflow"] == 0
     for i, (k, w, h, spp) in enumerate(views):
         scene = pike_amd.Scene(ctx, sc[k], bvhs[k])
