@@ -178,6 +178,16 @@ __device__ inline void cswap(uint32_t& a, uint32_t& b) {
   b = hi;
 }
 
+// c[s] for a per-lane slot s in [0, 8): a 3-level select tree on the slot's
+// bits (7 conditional moves) instead of a compare-and-move per slot
+__device__ inline uint32_t select8(const uint32_t* c, uint32_t s) {
+  const bool b0 = (s & 1u) != 0u, b1 = (s & 2u) != 0u, b2 = (s & 4u) != 0u;
+  const uint32_t a0 = b0 ? c[1] : c[0], a1 = b0 ? c[3] : c[2];
+  const uint32_t a2 = b0 ? c[5] : c[4], a3 = b0 ? c[7] : c[6];
+  const uint32_t d0 = b1 ? a1 : a0, d1 = b1 ? a3 : a2;
+  return b2 ? d1 : d0;
+}
+
 // One traversal step: visit the current node (box-test its 8 children,
 // intersect the hit leaf children, continue with the nearest hit internal
 // child and push the others far-to-near), or pop the next live stack entry.
@@ -297,10 +307,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     const int c = __builtin_ctz(lm);
     lm &= lm - 1;
     const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
-    uint32_t first = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j == c) first = child[j];
+    const uint32_t first = select8(child, (uint32_t)c);
     for (uint32_t k = first; k < first + m; ++k) {
       if (kCount) st->tris++;
       uint32_t id, bary = 0;
@@ -333,23 +340,12 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   cswap(k[2], k[4]); cswap(k[3], k[5]);
   cswap(k[1], k[4]); cswap(k[3], k[6]);
   cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
-  {
-    const uint32_t s = k[0] & 7u;
-    uint32_t nx = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if ((uint32_t)j == s) nx = child[j];
-    r.node = nx;
-    r.have = true;
-  }
+  r.node = select8(child, k[0] & 7u);
+  r.have = true;
 #pragma unroll
   for (int i = 7; i >= 1; --i) {
     if (i < m) {
-      const uint32_t s = k[i] & 7u;
-      uint32_t nx = 0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if ((uint32_t)j == s) nx = child[j];
+      const uint32_t nx = select8(child, k[i] & 7u);
       const float t = __uint_as_float(k[i] & ~7u);  // rounded down: conservative
       stack_push(stk, r, nx, t);
     }
