@@ -292,9 +292,10 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     const float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
     const float tfar = fminf(fminf(tfx, tfy), tfz);
     tn[c] = tnear;
-    const bool h = (c < nchild) && tnear <= tfar && tnear <= r.cull;
+    const bool h = tnear <= tfar && tnear <= r.cull;
     hitmask |= h ? (1u << c) : 0u;
   }
+  hitmask &= (1u << nchild) - 1u;  // the slots past the node's children
   uint32_t leafmask = 0;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
