@@ -650,6 +650,10 @@ void quantize_node(pk::WNode& w, const FBox* cb, int nc) {
       qhi[k][i] = 0;
     }
   }
+  uint8_t lm = 0;
+  for (int i = 0; i < nc; ++i)
+    if (w.meta[i]) lm |= (uint8_t)(1u << i);
+  w.leafmask = lm;
   w.ox = origin[0];
   w.oy = origin[1];
   w.oz = origin[2];

@@ -296,12 +296,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     hitmask |= h ? (1u << c) : 0u;
   }
   hitmask &= (1u << nchild) - 1u;  // the slots past the node's children
-  uint32_t leafmask = 0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
-    if (m) leafmask |= 1u << c;
-  }
+  const uint32_t leafmask = L6.z & 0xffu;
   // leaf children first: their hits tighten the bound for the internal ones
   uint32_t lm = hitmask & leafmask;
   while (lm) {

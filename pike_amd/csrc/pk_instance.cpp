@@ -334,13 +334,17 @@ void insert_placements(std::vector<BNode>& nodes, const std::vector<DBox>& boxes
 
 // Every child reference of the merged wide layout points inside it: internal
 // children at nodes, leaf ranges inside the triangle records, placement
-// children at placements (top level only) -- the kernels follow them
-// unchecked.
+// children at placements (top level only), and each node's leaf mask agrees
+// with its leaf counts -- the kernels follow them unchecked.
 bool wide_layout_ok(const pk_bvh& b, uint32_t top_nodes) {
   const size_t nn = b.wnodes.size(), nt = b.wtris.size();
   for (size_t i = 0; i < nn; ++i) {
     const pk::WNode& w = b.wnodes[i];
     if (w.nchild == 0 || w.nchild > pk::kWide) return false;
+    uint32_t lm = 0;
+    for (int c = 0; c < w.nchild; ++c)
+      if (w.meta[c]) lm |= 1u << c;
+    if (w.leafmask != lm) return false;
     for (int c = 0; c < w.nchild; ++c) {
       const uint32_t ch = w.child[c];
       if (w.meta[c]) {

@@ -26,7 +26,9 @@ constexpr double kBoxMargin = 1.0 / 131072.0;
 // 128-byte wide node.  Child i (i < nchild) has box
 //   lo = origin + qlo[i] * 2^(e-127),  hi = origin + qhi[i] * 2^(e-127)
 // per axis; meta[i] == 0 marks an internal child (child[i] = node index),
-// meta[i] = k > 0 a leaf of k triangles starting at triangle child[i].
+// meta[i] = k > 0 a leaf of k triangles starting at triangle child[i];
+// leafmask bit i = (meta[i] != 0), so traversal reads it with one byte
+// extract instead of testing eight bytes.
 struct alignas(128) WNode {
   float ox, oy, oz;       // 0
   uint8_t ex, ey, ez;     // 12: biased exponents
@@ -39,7 +41,9 @@ struct alignas(128) WNode {
   uint8_t qhiy[8];        // 80
   uint8_t qloz[8];        // 88
   uint8_t qhiz[8];        // 96
-  uint32_t pad[6];        // 104..127
+  uint8_t leafmask;       // 104
+  uint8_t pad8[3];        // 105
+  uint32_t pad[5];        // 108..127
 };
 static_assert(sizeof(WNode) == 128, "wide node must be one 128-byte line");
 

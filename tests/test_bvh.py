@@ -121,6 +121,10 @@ def test_wide_layout_is_conservative_and_complete():
     data = scenes.config1(seed=5)
     b = pike_amd.BVH(data.positions, data.triangles)
     nodes, tris, tidx = b.export_wide()
+    # leafmask bit c = (meta[c] != 0) over the node's children, zero past them
+    bits = (nodes["meta"] != 0) & (np.arange(8)[None, :] < nodes["nchild"][:, None])
+    np.testing.assert_array_equal(nodes["leafmask"], (bits * (1 << np.arange(8))).sum(1))
+    assert (nodes["leafmask"] != 0).any() and (nodes["leafmask"] != 255).any()
     _, _, verts, _ = b.export_reference()
     V = verts[:, :3].astype(np.float64)
     seen = np.zeros(len(tris), np.int64)
