@@ -16,13 +16,18 @@ constexpr int kCounterStride = 1024;  // words
 enum : int {
   kCntRays0 = 0 * kCounterStride,      // ray queue 0 length
   kCntRays1 = 1 * kCounterStride,      // ray queue 1 length
-  kCntHits = 2 * kCounterStride,       // hit queue length
+  kCntHits = 2 * kCounterStride,       // hit queue length (records and holes)
   kCntShadow = 3 * kCounterStride,     // shadow queue length
   kCntWorkTrace = 4 * kCounterStride,  // trace_closest work cursor
   kCntWorkShadow = 5 * kCounterStride, // trace_shadow work cursor
   kCntOverflow = 6 * kCounterStride,   // stack overflow flag
   kNumCounters = 7 * kCounterStride
 };
+// trace_closest reserves the hit queue in runs of kHitRun records per wave
+// (one atomic per run instead of one per refill); the unused end of a
+// wave's last run is filled with holes, so the queue holds at most
+// kHitRun - 1 holes per wave of the launch beyond the hits.
+constexpr uint32_t kHitRun = 256;
 // counter of ray queue q (0 or 1)
 __host__ __device__ inline int ray_counter(int q) { return q ? kCntRays1 : kCntRays0; }
 // device statistics (uint64)
@@ -58,7 +63,7 @@ struct FrameDev {
   float4* qo[2];   // ray queues: origin.xyz, pixel id bits
   float4* qd[2];   //             direction.xyz
   float4* thr;     // path throughput per pixel
-  uint4* hits;     // {ray slot, leaf triangle, t bits, bary mask}
+  uint4* hits;     // {ray slot, leaf triangle, t bits, bary mask}; x = ~0: a hole
   float4* so;      // shadow queue: origin.xyz, pixel id bits
   float4* sd;      //               direction.xyz
   float4* sthr;    //               NEE throughput
@@ -74,6 +79,7 @@ struct FrameDev {
   // Samples per pass: a single-view pass traces samples sample .. sample+S-1
   // of every pixel as one wavefront, path id = pixel * S + s (batches use 1).
   uint32_t S;
+  uint32_t hit_cap;  // records the hit queue holds (paths + holes)
 };
 
 // samples the current pass traces (fewer than S at the end of a run)

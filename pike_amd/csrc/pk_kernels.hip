@@ -291,6 +291,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
   const float4* qo = F.qo[in];
   const float4* qd = F.qd[in];
   const uint32_t ns = F.views ? 1u : pass_samples(F, *F.params);
+  uint32_t res_lo = 0, res_hi = 0;  // this wave's unused run of hit slots (wave-uniform)
   trace_loop<false, kCount, kInst>(
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
@@ -307,15 +308,32 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
         }
       },
       [&](bool done, uint32_t slot, const RayState& r) {
-        // a wave's flush holds rays of the chunk it claimed, so the hit
-        // queue keeps the ray queue's (pixel) order chunk by chunk
+        // hit records go to this wave's reserved run of the hit queue, in
+        // the order the wave finishes them (its claimed chunk's pixel order)
         const bool is_hit = done && r.best_id != 0xffffffffu;
-        const uint32_t hs = wave_append(&F.cnt[kCntHits], is_hit);
+        const unsigned long long m = __ballot(is_hit);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (cnt == 0) return;
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t rem = res_hi - res_lo;
+        uint32_t at = res_lo + rank;
+        if (rem < cnt) {  // the run ends: continue in a fresh one
+          uint32_t b = 0;
+          if (lane == 0) b = atomicAdd(&F.cnt[kCntHits], kHitRun);
+          const uint32_t fresh = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(b, 0));
+          if (rank >= rem) at = fresh + (rank - rem);
+          res_lo = fresh + (cnt - rem);
+          res_hi = fresh + kHitRun;
+        } else {
+          res_lo += cnt;
+        }
         if (is_hit) {
-          F.hits[hs] = make_uint4(slot, r.best_leaf, __float_as_uint(r.tbest), r.best_bary);
+          F.hits[at] = make_uint4(slot, r.best_leaf, __float_as_uint(r.tbest), r.best_bary);
           hits++;
         }
       });
+  // the unused end of the wave's run: holes k_shade skips
+  for (uint32_t i = res_lo + (uint32_t)lane; i < res_hi; i += 64u) F.hits[i] = make_uint4(0xffffffffu, 0u, 0u, 0u);
   traced = wave_sum(traced);
   hits = wave_sum(hits);
   if (kCount) {
@@ -727,14 +745,33 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
   const int in = bounce & 1;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
-  const uint32_t nh = F.cnt[kCntHits];
+  const uint32_t nh = F.cnt[kCntHits];  // hit records and holes
   const FrameParams P = *F.params;
   const uint32_t ns = F.views ? 1u : pass_samples(F, P);
   ShadeOut o;
   o.emit = o.scatter = o.bounce = o.shadow = false;
   uint32_t pid = 0, view_bits = 0;
-  if (idx < nh) {
-    const uint4 hr = F.hits[idx];
+  // compact the block's hit records in LDS (their order kept), dropping the
+  // holes at the ends of trace_closest's per-wave runs
+  __shared__ uint4 s_hit[kShadeWaves * 64];
+  __shared__ uint32_t s_wcnt[kShadeWaves];
+  {
+    const uint4 h = idx < nh ? F.hits[idx] : make_uint4(0xffffffffu, 0u, 0u, 0u);
+    const bool is_hit = h.x != 0xffffffffu;
+    const unsigned long long m = __ballot(is_hit);
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t at = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w) at += s_wcnt[w];
+    if (is_hit) s_hit[at] = h;
+    __syncthreads();
+  }
+  uint32_t nb = 0;
+#pragma unroll
+  for (int w = 0; w < kShadeWaves; ++w) nb += s_wcnt[w];
+  if (threadIdx.x < nb) {
+    const uint4 hr = s_hit[threadIdx.x];
     const uint32_t slot = hr.x;
     const float4 o4 = F.qo[in][slot], d4 = F.qd[in][slot];
     pid = __float_as_uint(o4.w);
@@ -1390,8 +1427,7 @@ void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, con
   else hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.W * F.H * F.S, kShadeWaves * 64)), dim3(kShadeWaves * 64), 0, s, S, F,
-                     bounce);
+  hipLaunchKernelGGL(k_shade, dim3(cdiv(F.hit_cap, kShadeWaves * 64)), dim3(kShadeWaves * 64), 0, s, S, F, bounce);
 }
 void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
   FrameDev F = F0;

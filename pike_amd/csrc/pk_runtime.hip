@@ -554,7 +554,15 @@ static int alloc_paths(pk_frame* f, size_t paths) {
     if ((rc = dev_alloc(f->path_allocs, &F.qd[q], paths))) return rc;
   }
   if ((rc = dev_alloc(f->path_allocs, &F.thr, paths))) return rc;
-  if ((rc = dev_alloc(f->path_allocs, &F.hits, paths))) return rc;
+  // the hit queue also holds the holes at the end of each trace_closest
+  // wave's last run (fewer than kHitRun per wave of the largest grid)
+  const size_t hit_cap = paths + (size_t)f->ctx->num_cus * kMaxWavesPerCu * pk::kHitRun;
+  if (hit_cap >= (1ull << 32)) {
+    pk_set_error("frame: hit queue beyond 2^32 records");
+    return PK_E_LIMIT;
+  }
+  if ((rc = dev_alloc(f->path_allocs, &F.hits, hit_cap))) return rc;
+  F.hit_cap = (uint32_t)hit_cap;
   if ((rc = dev_alloc(f->path_allocs, &F.so, paths))) return rc;
   if ((rc = dev_alloc(f->path_allocs, &F.sd, paths))) return rc;
   if ((rc = dev_alloc(f->path_allocs, &F.sthr, paths))) return rc;
