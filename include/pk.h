@@ -150,7 +150,9 @@ int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_c
 int pk_ctx_set_scheduler(pk_ctx* ctx, int scheduler);
 int pk_ctx_get_scheduler(const pk_ctx* ctx, int* scheduler);
 /* Autotuner: coordinate search over (waves_per_cu, refill_min) for each
- * render kernel class, timing that class's kernels with HIP events over
+ * render kernel class (refill_min in {8, 16, ..., 64} at 24 waves per CU,
+ * then waves_per_cu in {4, 8, ..., 24} at the best threshold), timing that
+ * class's kernels with HIP events over
  * `frames` profiled frames of this scene, camera and resolution; keeps the
  * fastest shape per class in the context.  ms2 receives the traversal
  * milliseconds per frame before and after. */

@@ -1516,7 +1516,7 @@ int pk_adjoint(pk_frame* f, pk_scene* s, const pk_render_params* p, const float*
 namespace {
 
 const uint32_t kTuneWaves[] = {4, 8, 12, 16, 20, 24};
-const uint32_t kTuneRefill[] = {16, 32, 48, 64};
+const uint32_t kTuneRefill[] = {8, 16, 24, 32, 40, 48, 56, 64};
 
 // Milliseconds per frame spent in each render traversal class.
 int time_render_classes(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t frames, double ms[3]) {
@@ -1576,19 +1576,26 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
   int rc = time_render_classes(f, s, depth, frames, base);
   if (rc) return rc;
   const double before = base[0] + base[1] + base[2];
+  // per class: the refill threshold at the widest launch, then the launch
+  // width at that threshold
   for (int cls = PK_TUNE_PRIMARY; cls <= PK_TUNE_SHADOW; ++cls) {
     pk_ctx::Shape best = c->shape[cls];
     double best_ms = base[cls];
-    for (uint32_t w : kTuneWaves)
-      for (uint32_t r : kTuneRefill) {
-        c->shape[cls] = {w, r};
-        double ms[3];
-        if ((rc = time_render_classes(f, s, depth, frames, ms))) return rc;
-        if (ms[cls] < best_ms) {
-          best_ms = ms[cls];
-          best = c->shape[cls];
-        }
+    auto trial = [&](uint32_t w, uint32_t r) -> int {
+      c->shape[cls] = {w, r};
+      double ms[3];
+      const int e = time_render_classes(f, s, depth, frames, ms);
+      if (!e && ms[cls] < best_ms) {
+        best_ms = ms[cls];
+        best = c->shape[cls];
       }
+      return e;
+    };
+    for (uint32_t r : kTuneRefill)
+      if ((rc = trial(kMaxWavesPerCu, r))) return rc;
+    const uint32_t r_best = best.refill;
+    for (uint32_t w : kTuneWaves)
+      if ((rc = trial(w, r_best))) return rc;
     c->shape[cls] = best;
   }
   double after[3];
