@@ -297,27 +297,33 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   }
   hitmask &= (1u << nchild) - 1u;  // the slots past the node's children
   const uint32_t leafmask = L6.z & 0xffu;
-  // leaf children first: their hits tighten the bound for the internal ones
+  // leaf children first: their hits tighten the bound for the internal ones.
+  // One loop over the lane's triangles of all its hit leaf children (the
+  // next child's range is fetched when one ends), so the wave iterates as
+  // often as its busiest lane has triangles, not once per child round.
   uint32_t lm = hitmask & leafmask;
-  while (lm) {
-    const int c = __builtin_ctz(lm);
-    lm &= lm - 1;
-    const uint32_t m = ((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu;
-    const uint32_t first = select8(child, (uint32_t)c);
-    for (uint32_t k = first; k < first + m; ++k) {
-      if (kCount) st->tris++;
-      uint32_t id, bary = 0;
-      const float t = intersect_tri(S.tris, k, r.o, r.d, eps, &id, &bary);
-      if (kInst) id += r.id_delta;
-      if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
-        r.tbest = t;
-        r.best_id = id;
-        r.best_leaf = kInst ? k + r.leaf_delta : k;
-        r.best_bary = bary;
-        if (kAnyHit) return true;
-        r.cull = cull_bound(t);
-      }
+  uint32_t tri = 0, tend = 0;
+  while (true) {
+    if (tri == tend) {
+      if (lm == 0) break;
+      const int c = __builtin_ctz(lm);
+      lm &= lm - 1;
+      tri = select8(child, (uint32_t)c);
+      tend = tri + (((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu);
     }
+    if (kCount) st->tris++;
+    uint32_t id, bary = 0;
+    const float t = intersect_tri(S.tris, tri, r.o, r.d, eps, &id, &bary);
+    if (kInst) id += r.id_delta;
+    if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
+      r.tbest = t;
+      r.best_id = id;
+      r.best_leaf = kInst ? tri + r.leaf_delta : tri;
+      r.best_bary = bary;
+      if (kAnyHit) return true;
+      r.cull = cull_bound(t);
+    }
+    ++tri;
   }
   // internal children, sorted near-to-far: key = entry distance bits (>= 0,
   // so order-preserving) with the slot in the low 3 bits
