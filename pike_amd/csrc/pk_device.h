@@ -325,6 +325,22 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     }
     ++tri;
   }
+  if (kAnyHit) {
+    // occlusion asks for any hit, not the nearest (the bound never
+    // tightens): the hit internal children in slot order, unsorted, the
+    // lowest visited next and the rest pushed so that lower slots pop first
+    uint32_t im = hitmask & ~leafmask;
+    if (im == 0) return r.sp == 0;
+    r.node = select8(child, (uint32_t)__builtin_ctz(im));
+    r.have = true;
+    im &= im - 1u;
+    while (im) {
+      const int c = 31 - __builtin_clz(im);
+      im &= ~(1u << c);
+      stack_push(stk, r, select8(child, (uint32_t)c), 0.0f);
+    }
+    return false;
+  }
   // internal children, sorted near-to-far: key = entry distance bits (>= 0,
   // so order-preserving) with the slot in the low 3 bits
   uint32_t k[8];
