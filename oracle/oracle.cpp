@@ -1917,6 +1917,11 @@ void or_math_eval(int fn, const float* x, const float* y, float* out, int64_t n)
         out[i] = pkm::u2f(pkm::pcg_next(&seed));
         break;
       }
+      // the kernels' integer unpacking (x = the packed integer's value)
+      case 11: out[i] = pkm::unpack_unorm16((uint32_t)x[i]); break;
+      case 12: out[i] = pkm::unpack_snorm16((uint32_t)(int32_t)x[i] & 0xffffu); break;
+      case 13: out[i] = pkm::unpack_snorm8((uint32_t)(int32_t)x[i] & 0xffu); break;
+      case 14: out[i] = pkm::unpack_unorm8((uint32_t)x[i]); break;
       default: out[i] = 0;
     }
   }

@@ -66,7 +66,7 @@ def _p(a):
 
 
 MATH_FNS = {"sin": 0, "cos": 1, "tan": 2, "atan2": 3, "acos": 4, "log2": 5, "exp2": 6, "pow": 7, "sqrt": 8,
-            "div": 9, "pcg": 10}
+            "div": 9, "pcg": 10, "unorm16": 11, "snorm16": 12, "snorm8": 13, "unorm8": 14}
 
 
 def math_eval(fn, x, y=None):

@@ -417,7 +417,7 @@ __device__ inline float env_pdf(const DevScene& S, float theta, v3 dir) {
 
 __device__ inline float texel(const DevScene& S, const void* tex, int layer, int x, int y, int c) {
   size_t i = (((size_t)layer * S.th + (size_t)y) * S.tw + (size_t)x) * 4 + c;
-  if (S.tfmt == 0) return (float)((const uint8_t*)tex)[i] / 255.0f;
+  if (S.tfmt == 0) return pkm::unpack_unorm8(((const uint8_t*)tex)[i]);
   return ((const float*)tex)[i];
 }
 

@@ -578,7 +578,7 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
   }
   int4 tri = S.tri_index[rec];
   if (inst && inst->material >= 0) tri.w = inst->material;
-  const float bu = (float)(bary_mask & 0xffffu) / 65535.0f, bv = (float)(bary_mask >> 16) / 65535.0f;
+  const float bu = pkm::unpack_unorm16(bary_mask & 0xffffu), bv = pkm::unpack_unorm16(bary_mask >> 16);
   const v3 bary = pkm::mk(1.0f - bu - bv, bu, bv);
   v3 tng, btg, nrm;
   {

@@ -125,3 +125,19 @@ def test_pcg_uniformity():
     expected = len(seeds) / 256
     chi2 = float(((counts - expected) ** 2 / expected).sum())
     assert chi2 < 340  # 255 dof, p ~ 0.0003
+
+
+@pytest.mark.parametrize("fn,lo,hi,den", [("unorm16", 0, 65535, 65535), ("snorm16", -32768, 32767, 32767),
+                                          ("snorm8", -128, 127, 127), ("unorm8", 0, 255, 255)])
+def test_integer_unpacking_is_the_correctly_rounded_quotient(fn, lo, hi, den):
+    """The kernels unpack quantized integers (bary masks, uv, tangent frames,
+    RGBA8 texels) with a reciprocal product and one fma correction instead of
+    a division; over every integer of each domain that equals the correctly
+    rounded q / den (numpy's float32 division), clamped at -1 for snorm as
+    WGSL's unpack does."""
+    q = np.arange(lo, hi + 1).astype(np.float32)
+    got = oracle.math_eval(fn, q)
+    want = q / np.float32(den)
+    if fn.startswith("snorm"):
+        want = np.maximum(want, np.float32(-1.0))
+    np.testing.assert_array_equal(got.view(np.uint32), want.astype(np.float32).view(np.uint32))
