@@ -120,6 +120,9 @@ struct RayState {
   bool overflow;
   v3 wo, wd;
   uint32_t id_delta, leaf_delta;
+  // leaf children found by a node visit and not yet intersected: the node
+  // and its hit leaf-child mask (0: none)
+  uint32_t pnode, pmask;
 };
 
 __device__ inline void ray_set_inv(RayState& r) {
@@ -156,6 +159,8 @@ __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
   r.overflow = false;
   r.id_delta = 0;
   r.leaf_delta = 0;
+  r.pnode = 0;
+  r.pmask = 0;
 }
 
 __device__ inline void stack_push(const Stack& stk, RayState& r, uint32_t node, float t) {
@@ -188,6 +193,40 @@ __device__ inline uint32_t select8(const uint32_t* c, uint32_t s) {
   return b2 ? d1 : d0;
 }
 
+// Intersect the triangles of the leaf children in `lm` (child words and
+// leaf counts of one node), one loop over all of them so the wave iterates
+// as often as its busiest lane has triangles.  Returns true for an any-hit
+// ray that found its hit.
+template <bool kAnyHit, bool kCount, bool kInst>
+__device__ inline bool test_leaves(const DevScene& S, RayState& r, const uint32_t* child, uint32_t meta0,
+                                   uint32_t meta1, uint32_t lm, TraverseStats* st) {
+  const float eps = kAnyHit ? kEpsShadow : kEpsTrace;
+  uint32_t tri = 0, tend = 0;
+  while (true) {
+    if (tri == tend) {
+      if (lm == 0) break;
+      const int c = __builtin_ctz(lm);
+      lm &= lm - 1;
+      tri = select8(child, (uint32_t)c);
+      tend = tri + (((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu);
+    }
+    if (kCount) st->tris++;
+    uint32_t id, bary = 0;
+    const float t = intersect_tri(S.tris, tri, r.o, r.d, eps, &id, &bary);
+    if (kInst) id += r.id_delta;
+    if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
+      r.tbest = t;
+      r.best_id = id;
+      r.best_leaf = kInst ? tri + r.leaf_delta : tri;
+      r.best_bary = bary;
+      if (kAnyHit) return true;
+      r.cull = cull_bound(t);
+    }
+    ++tri;
+  }
+  return false;
+}
+
 // One traversal step: visit the current node (box-test its 8 children,
 // intersect the hit leaf children, continue with the nearest hit internal
 // child and push the others far-to-near), or pop the next live stack entry.
@@ -200,6 +239,28 @@ __device__ inline uint32_t select8(const uint32_t* c, uint32_t s) {
 template <bool kAnyHit, bool kCount, bool kInst = false>
 __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
   const int lane = stk.lane;
+  // Leaf phase.  A node visit defers its hit leaf children; the lane then
+  // waits while the others visit nodes, until at least half the wave's
+  // lanes hold deferred leaves (or no lane has a node left to visit), and
+  // those lanes intersect them together: on incoherent rays the triangle
+  // loop would otherwise run with a few lanes of the wave active.
+  const bool pend = r.pmask != 0u;
+  const unsigned long long pm = __ballot(pend);
+  if (pm) {
+    const unsigned long long inner = __ballot(!pend && (r.have || r.sp > 0));
+    if (inner == 0ull || 2 * __popcll(pm) >= __popcll(__ballot(true))) {
+      if (pend) {
+        const uint4* lp = reinterpret_cast<const uint4*>(S.nodes + r.pnode);
+        const uint4 C1 = lp[1], C2 = lp[2], C3 = lp[3];
+        const uint32_t pchild[8] = {C1.x, C1.y, C1.z, C1.w, C2.x, C2.y, C2.z, C2.w};
+        const uint32_t lm = r.pmask;
+        r.pmask = 0;
+        if (test_leaves<kAnyHit, kCount, kInst>(S, r, pchild, C3.x, C3.y, lm, st)) return true;
+      }
+      return !r.have && r.sp == 0 && r.pmask == 0u;
+    }
+  }
+  if (pend) return false;  // waits for the leaf phase
   if (!r.have) {
     while (r.sp > 0) {
       --r.sp;
@@ -254,8 +315,8 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   }
   r.have = false;
   if (kCount) st->nodes++;
-  const float eps = kAnyHit ? kEpsShadow : kEpsTrace;
-  const uint4* np = reinterpret_cast<const uint4*>(S.nodes + r.node);
+  const uint32_t visited = r.node;
+  const uint4* np = reinterpret_cast<const uint4*>(S.nodes + visited);
   const uint4 L0 = np[0], L1 = np[1], L2 = np[2], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
   const float ox = __uint_as_float(L0.x), oy = __uint_as_float(L0.y), oz = __uint_as_float(L0.z);
   const uint32_t ew = L0.w;
@@ -274,7 +335,6 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   const uint32_t nz0 = negz ? L6.x : L5.z, nz1 = negz ? L6.y : L5.w;
   const uint32_t fz0 = negz ? L5.z : L6.x, fz1 = negz ? L5.w : L6.y;
   const uint32_t child[8] = {L1.x, L1.y, L1.z, L1.w, L2.x, L2.y, L2.z, L2.w};
-  const uint32_t meta0 = L3.x, meta1 = L3.y;
   float tn[8];
   uint32_t hitmask = 0;
 #pragma unroll
@@ -297,40 +357,18 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   }
   hitmask &= (1u << nchild) - 1u;  // the slots past the node's children
   const uint32_t leafmask = L6.z & 0xffu;
-  // leaf children first: their hits tighten the bound for the internal ones.
-  // One loop over the lane's triangles of all its hit leaf children (the
-  // next child's range is fetched when one ends), so the wave iterates as
-  // often as its busiest lane has triangles, not once per child round.
-  uint32_t lm = hitmask & leafmask;
-  uint32_t tri = 0, tend = 0;
-  while (true) {
-    if (tri == tend) {
-      if (lm == 0) break;
-      const int c = __builtin_ctz(lm);
-      lm &= lm - 1;
-      tri = select8(child, (uint32_t)c);
-      tend = tri + (((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu);
-    }
-    if (kCount) st->tris++;
-    uint32_t id, bary = 0;
-    const float t = intersect_tri(S.tris, tri, r.o, r.d, eps, &id, &bary);
-    if (kInst) id += r.id_delta;
-    if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
-      r.tbest = t;
-      r.best_id = id;
-      r.best_leaf = kInst ? tri + r.leaf_delta : tri;
-      r.best_bary = bary;
-      if (kAnyHit) return true;
-      r.cull = cull_bound(t);
-    }
-    ++tri;
+  // hit leaf children: intersected in the next leaf phase
+  const uint32_t lmask = hitmask & leafmask;
+  if (lmask) {
+    r.pnode = visited;
+    r.pmask = lmask;
   }
   if (kAnyHit) {
     // occlusion asks for any hit, not the nearest (the bound never
     // tightens): the hit internal children in slot order, unsorted, the
     // lowest visited next and the rest pushed so that lower slots pop first
     uint32_t im = hitmask & ~leafmask;
-    if (im == 0) return r.sp == 0;
+    if (im == 0) return r.sp == 0 && r.pmask == 0u;
     r.node = select8(child, (uint32_t)__builtin_ctz(im));
     r.have = true;
     im &= im - 1u;
@@ -351,7 +389,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     k[c] = live ? ((__float_as_uint(tn[c]) & ~7u) | (uint32_t)c) : 0xffffffffu;
     m += live ? 1 : 0;
   }
-  if (m == 0) return r.sp == 0;
+  if (m == 0) return r.sp == 0 && r.pmask == 0u;
   cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
   cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
   cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
