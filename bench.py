@@ -161,7 +161,7 @@ def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
     data = scenes.config2(seed=2, cells=args.cells, instanced=True)
     threads = min(16, os.cpu_count() or 1)
     t0 = time.time()
-    bvh = pike_amd.BVH.for_scene(data, threads=threads)
+    bvh = pike_amd.BVH.for_scene(data, threads=threads, sah_collapse=args.collapse == "sah")
     build_s = time.time() - t0
     info = bvh.info()
     scene = pike_amd.Scene(ctx, data, bvh)
@@ -204,7 +204,8 @@ def run_batching(args, pike_amd, scenes, ctx):
     t0 = time.time()
     sc, views = scenes.config4(seed=4)
     threads = min(16, os.cpu_count() or 1)
-    bvhs = [pike_amd.BVH(s.positions, s.triangles, threads=threads, spatial_splits=args.bvh == "sbvh") for s in sc]
+    bvhs = [pike_amd.BVH(s.positions, s.triangles, threads=threads, spatial_splits=args.bvh == "sbvh",
+                         sah_collapse=args.collapse == "sah") for s in sc]
     per_scene = [pike_amd.Scene(ctx, s, b) for s, b in zip(sc, bvhs)]
     merged = pike_amd.BatchScene(ctx, sc, bvhs)
     cams = [scenes.normalized_camera(c, bvhs[k].normalizer()) for k, _, _, _, c in views]
@@ -307,7 +308,7 @@ def run_diff(args, pike_amd, scenes):
     t0 = time.time()
     data = scenes.config5(seed=5)
     bvh = pike_amd.BVH(data.positions, data.triangles, threads=min(16, os.cpu_count() or 1),
-                       spatial_splits=args.bvh == "sbvh")
+                       spatial_splits=args.bvh == "sbvh", sah_collapse=args.collapse == "sah")
     cam = scenes.normalized_camera(data.camera, bvh.normalizer())
     ctx = pike_amd.Context(0)
     scene = pike_amd.Scene(ctx, data, bvh)
@@ -530,7 +531,7 @@ def main():
     ap.add_argument("--soup-rays", type=int, default=16_000_000)
     ap.add_argument("--instanced", action="store_true",
                     help="render scene with its 200 icospheres as instances of one mesh (two-level BVH)")
-    ap.add_argument("--collapse", choices=["greedy", "sah"], default="greedy",
+    ap.add_argument("--collapse", choices=["greedy", "sah"], default="sah",
                     help="binary -> 8-wide collapse of the render scenes: open the largest child, or the "
                          "SAH-optimal dynamic program")
     ap.add_argument("--soup-collapse", choices=["greedy", "sah"], default="sah",
@@ -628,7 +629,8 @@ def main():
                                f"2048x1024 env (2D CDF importance sampling), "
                                f"{args.width}x{args.height}, depth {args.depth}, 1 spp per step",
                    "width": args.width, "height": args.height, "depth": args.depth, "triangles": res["triangles"],
-                   "bvh": args.bvh, "instanced": args.instanced, "scheduler": res["scheduler"]},
+                   "bvh": args.bvh, "collapse": args.collapse, "instanced": args.instanced,
+                   "scheduler": res["scheduler"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": extra,
