@@ -3,7 +3,7 @@
 // role of render/util/gltf_loader.js (GLTFLoader.load :38-64, node
 // hierarchy :86-114, accessors :302-343, PBR metallic-roughness materials
 // :121-213), written for Node (files instead of fetch, PNG decoding through
-// png.js) and with the reference's loader bugs fixed (SURVEY A.13):
+// png.js) and with these loader bugs of the reference fixed:
 //   * missing roughnessFactor / metallicFactor default to 1 (glTF spec), not 0;
 //   * byteStride is read from the accessor's bufferView (the reference looks
 //     it up in the accessor table);
