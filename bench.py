@@ -241,6 +241,7 @@ def run_batching(args, pike_amd, scenes, ctx):
         ctx.sync()
 
     batch = pike_amd.Batch(ctx, merged, [(k, w, h, spp, cams[i]) for i, (k, w, h, spp, _) in enumerate(views)])
+    total_px = sum(w * h for _, w, h, _, _ in views)
 
     def batched():
         batch.reset()
@@ -263,9 +264,22 @@ def run_batching(args, pike_amd, scenes, ctx):
         t0 = time.perf_counter()
         plain_ms()
         t_plain_ms = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        batched()
-        t_batch = time.perf_counter() - t0
+        # batched rounds of 1, 2, 4, ... samples per view pixel (below 2^27
+        # paths per round); the fastest is the batched figure
+        t_round = {}
+        for spr in (1, 2, 4, 8):
+            if total_px * spr >= (1 << 27):
+                break
+            batch.set_samples_per_round(spr)
+            batched()  # warm-up: records this shape's graph
+            t0 = time.perf_counter()
+            batched()
+            t_round[spr] = time.perf_counter() - t0
+            if not all(np.array_equal(batch.read_view(i).view(np.uint32), frames[i].read_image().view(np.uint32))
+                       for i in range(len(views))):
+                same = False
+        best_spr = min(t_round, key=t_round.get)
+        t_batch = t_round[best_spr]
         if ref is None:
             ref = [frames[i].read_image() for i in range(len(views))]
         same = same and all(
@@ -273,9 +287,12 @@ def run_batching(args, pike_amd, scenes, ctx):
             np.array_equal(frames_ms[i].read_image().view(np.uint32), ref[i].view(np.uint32)) and
             np.array_equal(frames[i].read_image().view(np.uint32), ref[i].view(np.uint32))
             for i in range(len(views)))
-        per[sched] = dict(plain_s=t_plain, plain_multisample_s=t_plain_ms, batched_s=t_batch)
+        per[sched] = dict(plain_s=t_plain, plain_multisample_s=t_plain_ms, batched_s=t_batch,
+                          batched_samples_per_round=best_spr, batched_s_by_samples_per_round=t_round)
         log(f"batching ({sched}): plain {samples / t_plain / 1e6:.0f}, plain multi-sample "
-            f"{samples / t_plain_ms / 1e6:.0f}, batched {samples / t_batch / 1e6:.0f} Msamples/s")
+            f"{samples / t_plain_ms / 1e6:.0f}, batched {samples / t_batch / 1e6:.0f} Msamples/s "
+            f"({best_spr} samples per round; " +
+            ", ".join(f"{k}: {samples / v / 1e6:.0f}" for k, v in t_round.items()) + ")")
     ctx.set_scheduler("wavefront")
     best = {k: min(v[k] for v in per.values()) for k in ("plain_s", "plain_multisample_s", "batched_s")}
     t_plain, t_plain_ms, t_batch = best["plain_s"], best["plain_multisample_s"], best["batched_s"]

@@ -281,7 +281,9 @@ int pk_render(pk_frame* frame, pk_scene* scene, const pk_render_params* params);
  * pk_render and of the per-pass entry points traces `samples` consecutive
  * samples of every pixel as one wavefront (width*height*samples paths, below
  * 2^28), so small frames fill the GPU.  The accumulated image equals that of
- * one-sample frames bit for bit.  Reallocates the frame's queues. */
+ * one-sample frames bit for bit.  On a batch frame (pk_batch_create) it sets
+ * the samples per round of pk_batch_render (total view pixels * samples
+ * paths).  Reallocates the frame's queues. */
 int pk_frame_set_samples_per_pass(pk_frame* frame, uint32_t samples);
 int pk_frame_get_samples_per_pass(const pk_frame* frame, uint32_t* samples);
 /* Running-mean image, float RGBA, width*height*16 bytes. */

@@ -365,6 +365,17 @@ class Batch:
     def render(self, max_depth=4, env_theta=0.0, use_graph=True):
         _check(lib().pk_batch_render(self.h, max_depth, float(env_theta), int(use_graph)), "pk_batch_render")
 
+    def set_samples_per_round(self, samples):
+        """Samples per view pixel traced in one round of render() (default
+        1); the images are identical for any value."""
+        _check(lib().pk_frame_set_samples_per_pass(self.h, int(samples)), "pk_frame_set_samples_per_pass")
+
+    @property
+    def samples_per_round(self):
+        v = ctypes.c_uint32()
+        _check(lib().pk_frame_get_samples_per_pass(self.h, ctypes.byref(v)), "pk_frame_get_samples_per_pass")
+        return v.value
+
     def read_view(self, i):
         _, w, hh, _ = self.views[i]
         out = np.empty((hh, w, 4), np.float32)

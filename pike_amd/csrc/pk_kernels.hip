@@ -136,25 +136,26 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   F.color[(size_t)s * B + pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// Batch ray generation: one thread per global path; pixels of views whose
-// sample budget is spent this round emit nothing, the rest are compacted
-// into ray queue 0 (d.w carries the view).  The queue counter is zeroed by
-// the previous round's trace_shadow / by pk_batch_render before round 0.
+// Batch ray generation: one thread per (global pixel p, sample s) of the
+// round, path id p * S + s; samples at or beyond their view's spp emit
+// nothing, the rest are compacted into ray queue 0 (d.w carries the view).
+// The queue counter is zeroed by k_batch_round_begin before the round.
 __global__ void __launch_bounds__(256) k_raygen_batch(FrameDev F) {
-  const uint32_t B = F.W;
+  const uint32_t B = F.W, S = F.S;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   const FrameParams P = *F.params;
   bool emit = false;
   uint32_t view = 0;
   v3 o = pkm::mk(0.0f, 0.0f, 0.0f), d = o;
-  if (idx < B) {
-    view = view_of(F, idx);
+  if (idx < B * S) {
+    const uint32_t p = idx / S, sm = idx - p * S;
+    view = view_of(F, p);
     const ViewDev& V = F.views[view];
-    if (P.sample < V.spp) {
-      const uint32_t lp = idx - V.offset;
-      camera_ray(lp % V.W, lp / V.W, V.W, V.H, P.sample, V.cam, &o, &d);
+    if (P.sample + sm < V.spp) {
+      const uint32_t lp = p - V.offset;
+      camera_ray(lp % V.W, lp / V.W, V.W, V.H, P.sample + sm, V.cam, &o, &d);
       F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-      F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      F.color[(size_t)sm * B + p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       emit = true;
     }
   }
@@ -172,12 +173,12 @@ __global__ void k_batch_round_begin(FrameDev F) {
   F.cnt[kCntWorkTrace] = 0;
 }
 
-// Color slot of path pid: single-view passes number paths pixel-major
-// (pid = pixel * ns + s) and keep colors sample-major (s * W * H + pixel),
-// the layout k_accumulate folds and pk_read_color returns; batch paths map
-// one to one.
+// Color slot of path pid: passes number paths pixel-major (pid = pixel * ns
+// + s; batch pixels are global, ns = the round's S) and keep colors
+// sample-major (s * W * H + pixel), the layout k_accumulate folds and
+// pk_read_color returns.
 __device__ inline uint32_t color_slot(const FrameDev& F, uint32_t ns, uint32_t pid) {
-  if (F.views || ns <= 1) return pid;
+  if (ns <= 1) return pid;
   const uint32_t pixel = pid / ns;
   return (pid - pixel * ns) * (F.W * F.H) + pixel;
 }
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
   uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
   const float4* qo = F.qo[in];
   const float4* qd = F.qd[in];
-  const uint32_t ns = F.views ? 1u : pass_samples(F, *F.params);
+  const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   uint32_t res_lo = 0, res_hi = 0;  // this wave's unused run of hit slots (wave-uniform)
   trace_loop<false, kCount, kInst>(
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
@@ -747,7 +748,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
   const uint32_t nh = F.cnt[kCntHits];  // hit records and holes
   const FrameParams P = *F.params;
-  const uint32_t ns = F.views ? 1u : pass_samples(F, P);
+  const uint32_t ns = F.views ? F.S : pass_samples(F, P);
   ShadeOut o;
   o.emit = o.scatter = o.bounce = o.shadow = false;
   uint32_t pid = 0, view_bits = 0;
@@ -779,8 +780,8 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
     // the pixel index the seeds use is local to the view; a single-view
     // pass holds samples P.sample .. P.sample + ns - 1 of every pixel,
     // numbered pixel-major
-    const uint32_t s = F.views ? 0u : pid % ns;
-    const uint32_t pixel = F.views ? pid - F.views[view_bits].offset : pid / ns;
+    const uint32_t s = pid % ns;
+    const uint32_t pixel = F.views ? pid / ns - F.views[view_bits].offset : pid / ns;
     const float4 t4 = F.thr[pid];
     shade_hit<false>(S, P.env_theta, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), __uint_as_float(hr.z),
                      hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample + s), pkm::mk(t4.x, t4.y, t4.z), o,
@@ -815,7 +816,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
     F.cnt[kCntWorkTrace] = 0;
   }
   const uint32_t n = F.cnt[kCntShadow];
-  const uint32_t ns = F.views ? 1u : pass_samples(F, *F.params);
+  const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   const float theta = F.params->env_theta;
   const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   TraverseStats st = {0, 0};
@@ -861,9 +862,9 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
 // path of the pass.  The per-bounce queues (rays, hits, shadow rays,
 // throughput) never reach memory, and lanes whose path ended refill between
 // bounces, so waves stay full while path lengths differ.  Path numbering,
-// seeds and arithmetic are the wavefront's (single view: pixel-major
-// samples; batch: one path per global pixel, with the view's camera and
-// root, and no path for views whose samples are spent), so the color buffer
+// seeds and arithmetic are the wavefront's (pixel-major samples; a batch
+// numbers global pixels, with the view's camera and root, and has no path
+// for samples beyond the view's spp), so the color buffer
 // is bit-identical to the one raygen / trace_closest / shade / trace_shadow
 // produce.  The work cursor (kCntWorkTrace) is zeroed before the launch.
 // ---------------------------------------------------------------------------
@@ -875,7 +876,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   const int lane = (int)threadIdx.x;
   const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
   const FrameParams P = *F.params;
-  const uint32_t ns = kBatch ? 1u : pass_samples(F, P);
+  const uint32_t ns = kBatch ? F.S : pass_samples(F, P);
   const uint32_t B = F.W * F.H;
   const uint32_t n = B * ns;
   uint32_t* const cursor = &F.cnt[kCntWorkTrace];
@@ -910,12 +911,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
         const uint32_t p = rank < have ? pool_lo + rank : (got ? fresh + (rank - have) : 0xffffffffu);
         if (p < n) {
           if (kBatch) {
-            const ViewDev& V = F.views[view_of(F, p)];
-            if (P.sample < V.spp) {
+            const uint32_t gp = p / ns, sm = p - gp * ns;
+            const ViewDev& V = F.views[view_of(F, gp)];
+            if (P.sample + sm < V.spp) {
               has = true;
-              pixel = p - V.offset;
-              sample = P.sample;
-              slot = p;
+              pixel = gp - V.offset;
+              sample = P.sample + sm;
+              slot = sm * B + gp;
               root = V.root;
               camera_ray(pixel % V.W, pixel / V.W, V.W, V.H, sample, V.cam, &o, &d);
             }
@@ -1035,9 +1037,14 @@ __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= B) return;
   const FrameParams P = *F.params;
-  // batch: views whose sample budget is spent this round keep their mean
-  if (F.views && P.sample >= F.views[view_of(F, idx)].spp) return;
-  const uint32_t ns = pass_samples(F, P);
+  // batch: a view's pixels fold the round's samples below its spp (views
+  // whose budget is spent keep their mean)
+  uint32_t ns = pass_samples(F, P);
+  if (F.views) {
+    const uint32_t spp = F.views[view_of(F, idx)].spp;
+    ns = P.sample < spp ? min(F.S, spp - P.sample) : 0u;
+    if (ns == 0) return;
+  }
   float4 a = F.accum[idx];
   for (uint32_t k = 0; k < ns; ++k) {
     const float s = (float)(P.sample + k);
@@ -1412,7 +1419,7 @@ void launch_advance(const FrameDev& F, hipStream_t s) { hipLaunchKernelGGL(k_adv
 void launch_raygen(const FrameDev& F, hipStream_t s) {
   if (F.views) {
     hipLaunchKernelGGL(k_batch_round_begin, dim3(1), dim3(1), 0, s, F);
-    hipLaunchKernelGGL(k_raygen_batch, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+    hipLaunchKernelGGL(k_raygen_batch, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, F);
   } else {
     hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, F);
   }

@@ -645,8 +645,8 @@ int pk_frame_get_samples_per_pass(const pk_frame* f, uint32_t* samples) {
 }
 
 int pk_frame_set_samples_per_pass(pk_frame* f, uint32_t samples) {
-  if (!f || samples == 0 || f->batch_scene) {
-    pk_set_error("pk_frame_set_samples_per_pass: null frame, zero samples or a batch frame");
+  if (!f || samples == 0) {
+    pk_set_error("pk_frame_set_samples_per_pass: null frame or zero samples");
     return PK_E_INVALID;
   }
   const uint64_t paths = (uint64_t)f->W * f->H * samples;
@@ -677,7 +677,7 @@ int pk_frame_reset(pk_frame* f) {
   }
   const uint32_t B = f->W * f->H;
   pk::launch_fill_f4(f->dev.accum, make_float4(0, 0, 0, 0), B, f->ctx->stream);
-  pk::launch_fill_f4(f->dev.color, make_float4(0, 0, 0, 0), B * (f->batch_scene ? 1u : f->dev.S), f->ctx->stream);
+  pk::launch_fill_f4(f->dev.color, make_float4(0, 0, 0, 0), B * f->dev.S, f->ctx->stream);
   pk::launch_fill_u32((uint32_t*)f->dev.stats, 0u, pk::kNumStats * 2, f->ctx->stream);
   pk::launch_fill_u32(f->dev.cnt, 0u, pk::kNumCounters, f->ctx->stream);
   PK_HIP(hipGetLastError());
@@ -1227,12 +1227,16 @@ int pk_batch_create(pk_ctx* c, pk_scene* s, const pk_view* views, uint32_t nview
   return PK_OK;
 }
 
+// A batch render runs rounds of S samples (the frame's samples per pass):
+// round r traces samples r*S .. r*S+S-1 of every view pixel whose view has
+// that many.
 static void enqueue_batch(pk_frame* f, pk_scene* s, uint32_t depth, float theta, hipStream_t st) {
-  uint32_t rounds = 0;
-  for (const auto& v : f->views) rounds = std::max(rounds, v.spp);
+  uint32_t max_spp = 0;
+  for (const auto& v : f->views) max_spp = std::max(max_spp, v.spp);
+  const uint32_t S = f->dev.S, rounds = (max_spp + S - 1) / S;
   const bool paths = f->ctx->sched == PK_SCHED_PATHS && !f->count_nodes;
   for (uint32_t r = 0; r < rounds; ++r) {
-    pk::launch_begin(f->dev, r, theta, f->cam, st);
+    pk::launch_begin(f->dev, r * S, theta, f->cam, st);
     if (paths) {
       pk::launch_render_paths(s->dev, f->dev, depth, trace_launch(f, PK_TUNE_PATHS), st);
       pk::launch_accumulate(f->dev, st);

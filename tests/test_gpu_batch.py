@@ -18,11 +18,14 @@ def _small_batch():
     return sc, views
 
 
-@pytest.mark.parametrize("use_graph,sched", [(True, "wavefront"), (False, "wavefront"), (True, "paths"),
-                                             (False, "paths")])
-def test_batch_equals_per_view_render(ctx, use_graph, sched):
-    """Under either pass scheduler for the batch; the single views always
-    render through the wavefront kernels."""
+@pytest.mark.parametrize("use_graph,sched,per_round", [(True, "wavefront", 1), (False, "wavefront", 1),
+                                                       (True, "paths", 1), (False, "paths", 1),
+                                                       (True, "wavefront", 2), (True, "wavefront", 3),
+                                                       (False, "paths", 2), (True, "paths", 4)])
+def test_batch_equals_per_view_render(ctx, use_graph, sched, per_round):
+    """Under either pass scheduler for the batch and with rounds of several
+    samples (views whose spp is not a multiple end inside a round); the
+    single views always render through the wavefront kernels."""
     sc, views = _small_batch()
     bvhs = [pike_amd.BVH(s.positions, s.triangles) for s in sc]
     merged = pike_amd.BatchScene(ctx, sc, bvhs)
@@ -32,6 +35,8 @@ def test_batch_equals_per_view_render(ctx, use_graph, sched):
         cam = scenes.camera((0.5 + 0.8 * np.cos(az), 0.4, 0.5 + 0.8 * np.sin(az)), (0.5, 0.05, 0.5), 1.0)
         cams.append(scenes.normalized_camera(cam, bvhs[k].normalizer()))
     batch = pike_amd.Batch(ctx, merged, [(k, w, h, spp, cams[i]) for i, (k, w, h, spp) in enumerate(views)])
+    batch.set_samples_per_round(per_round)
+    assert batch.samples_per_round == per_round
     ctx.set_scheduler(sched)
     try:
         batch.render(max_depth=4, use_graph=use_graph)
