@@ -227,9 +227,14 @@ __device__ inline bool test_leaves(const DevScene& S, RayState& r, const uint32_
   return false;
 }
 
-// One traversal step: visit the current node (box-test its 8 children,
-// intersect the hit leaf children, continue with the nearest hit internal
-// child and push the others far-to-near), or pop the next live stack entry.
+// One traversal step, wave-cooperative (every lane of the wave that still
+// traces calls it together): either a leaf phase, in which the lanes holding
+// deferred leaf children intersect their triangles, or a node step, in
+// which the other lanes visit their current node (box-test its 8 children,
+// defer the hit leaf children, continue with the nearest hit internal child
+// and push the others far-to-near) or pop the next live stack entry.  The
+// canonical closest hit does not depend on the visiting order, so deferring
+// leaves changes only how much work the wave does, not the result.
 // With kInst, visiting an instance child maps the ray into its mesh space,
 // pushes a return marker (entry distance 0, so it is never culled) and goes
 // on at the mesh's root; popping the marker maps the ray back.  Distances
