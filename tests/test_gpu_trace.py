@@ -125,3 +125,35 @@ def test_node_counters_match_between_runs(ctx):
     ms = ctx.time_trace_rays(scene, rays.ptr, n, False, ids.ptr, t.ptr, b.ptr, 2)
     assert ms > 0
     _assert_same(ids.read(np.int32, n), t.read(np.float32, n), b.read(np.uint32, n), ids_o, t_o, b_o)
+
+
+@pytest.mark.parametrize("refill", [1, 64])
+def test_overlapping_boxes_deep_stacks(ctx, refill):
+    """Large random triangles spanning the cube: every node's children
+    overlap, so rays push far more entries than the LDS part of the stack
+    holds (the global spill area is used) and most lanes carry deferred leaf
+    children at once; under the extreme refill thresholds (a lane refilled
+    as soon as it is idle, or only once the whole wave is) the hits stay the
+    canonical ones."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    pos = rng.uniform(0.0, 1.0, (3 * n, 3))
+    tri = np.concatenate([np.arange(3 * n).reshape(-1, 3), np.zeros((n, 1), np.int64)], 1).astype(np.int32)
+    from .test_bvh import _data
+    data = _data(pos, tri)
+    bvh, scene, orc = _setup(ctx, data)
+    rays = scenes.incoherent_rays(20_000, bvh.normalizer(), seed=12)
+    old = ctx.tuning()
+    try:
+        for cls in ("rays_closest", "rays_any"):
+            ctx.set_tuning(cls, old[cls]["waves_per_cu"], refill)
+        ids, t, bary = ctx.trace_rays(scene, rays)
+        occ, _, _ = ctx.trace_rays(scene, rays, any_hit=True)
+    finally:
+        for cls in ("rays_closest", "rays_any"):
+            ctx.set_tuning(cls, old[cls]["waves_per_cu"], old[cls]["refill_min"])
+    ids_o, t_o, b_o, _ = orc.trace(rays, "canonical")
+    assert (ids_o >= 0).mean() > 0.5
+    _assert_same(ids, t, bary, ids_o, t_o, b_o)
+    occ_o, _, _, _ = orc.trace(rays, "canonical", any_hit=True)
+    np.testing.assert_array_equal(occ >= 0, occ_o >= 0)
