@@ -158,7 +158,7 @@ int pk_ctx_get_scheduler(const pk_ctx* ctx, int* scheduler);
  * milliseconds per frame before and after. */
 int pk_autotune_render(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t frames, float* ms2);
 /* Queue capacity: chooses the frame's samples per pass (1, 2, 4, ... up to
- * max_samples, below 2^27 paths) by timing max_samples samples of
+ * max_samples, below 2^28 paths) by timing max_samples samples of
  * graph-replayed passes of this scene, camera and resolution; leaves the
  * fastest set and the frame reset.  ms2 = milliseconds per sample at the
  * frame's previous setting (0 if it was not among the candidates) and at the

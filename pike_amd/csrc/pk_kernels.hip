@@ -133,7 +133,10 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
   F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  F.color[(size_t)s * B + pixel] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  // the pass's color slots s * B + pixel are exactly [0, B * ns): each
+  // thread clears one of them in address order (coalesced) rather than its
+  // own path's slot, which lies B * 16 bytes from its neighbours'
+  F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Batch ray generation: one thread per (global pixel p, sample s) of the
@@ -151,11 +154,13 @@ __global__ void __launch_bounds__(256) k_raygen_batch(FrameDev F) {
     const uint32_t p = idx / S, sm = idx - p * S;
     view = view_of(F, p);
     const ViewDev& V = F.views[view];
+    // color slots sm * B + p cover [0, B * S): clear them in address order
+    // (slots of samples beyond a view's spp are never folded)
+    F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (P.sample + sm < V.spp) {
       const uint32_t lp = p - V.offset;
       camera_ray(lp % V.W, lp / V.W, V.W, V.H, P.sample + sm, V.cam, &o, &d);
       F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-      F.color[(size_t)sm * B + p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       emit = true;
     }
   }

@@ -1616,7 +1616,7 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
 // Queue capacity: samples per pass in 1, 2, 4, ... up to max_samples (and
 // below kTunePathCap paths), each timed over max_samples samples of
 // graph-replayed passes; keeps the fastest per sample.
-constexpr uint64_t kTunePathCap = 1ull << 27;
+constexpr uint64_t kTunePathCap = 1ull << 28;  // the frame's own limit
 
 extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t max_samples,
                                             float* ms2) {
