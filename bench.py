@@ -264,11 +264,11 @@ def run_batching(args, pike_amd, scenes, ctx):
         t0 = time.perf_counter()
         plain_ms()
         t_plain_ms = time.perf_counter() - t0
-        # batched rounds of 1, 2, 4, ... samples per view pixel (below 2^27
-        # paths per round); the fastest is the batched figure
+        # batched rounds of 1, 2, 4, ... samples per view pixel (below 2^28
+        # paths per round, the frame's limit); the fastest is the batched figure
         t_round = {}
         for spr in (1, 2, 4, 8):
-            if total_px * spr >= (1 << 27):
+            if total_px * spr >= (1 << 28):
                 break
             batch.set_samples_per_round(spr)
             batched()  # warm-up: records this shape's graph
