@@ -290,7 +290,8 @@ int pk_frame_get_samples_per_pass(const pk_frame* frame, uint32_t* samples);
 int pk_read_image(pk_frame* frame, float* rgba, size_t bytes);
 /* The same image copied to device memory (asynchronous, context stream). */
 int pk_read_image_device(pk_frame* frame, float* d_rgba, size_t bytes);
-/* The current frame's color buffer (RenderState.colorBuffer) before accumulation. */
+/* The current frame's color buffer (RenderState.colorBuffer) before
+ * accumulation: with several samples per pass, the first sample's. */
 int pk_read_color(pk_frame* frame, float* rgba, size_t bytes);
 int pk_read_counters(pk_frame* frame, pk_counters* out);
 /* Count BVH nodes visited and triangles tested in this frame's trace kernels

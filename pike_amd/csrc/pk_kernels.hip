@@ -110,9 +110,9 @@ __device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t
   *o_out = pkm::add(dof, org);
 }
 
-// One thread per path of the pass: path idx = s * W * H + pixel traces
-// sample P.sample + s of that pixel; the paths of samples at or beyond the
-// run's limit are the tail of the index range and are not emitted.
+// One thread per path of the pass: path idx = pixel * ns + s traces sample
+// P.sample + s of that pixel (ns = the pass's samples, fewer than S at the
+// end of a run).
 __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   const uint32_t W = F.W, H = F.H, B = W * H;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -126,17 +126,14 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   }
   if (idx >= n) return;
   // paths are pixel-major (the samples of a pixel are neighbours in the
-  // queues); the color buffer is sample-major (color_slot)
+  // queues and in the color buffer)
   const uint32_t pixel = idx / ns, s = idx - pixel * ns;
   v3 o, d;
   camera_ray(pixel % W, pixel / W, W, H, P.sample + s, P.cam, &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
   F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  // the pass's color slots s * B + pixel are exactly [0, B * ns): each
-  // thread clears one of them in address order (coalesced) rather than its
-  // own path's slot, which lies B * 16 bytes from its neighbours'
-  F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  F.color[(size_t)pixel * F.S + s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Batch ray generation: one thread per (global pixel p, sample s) of the
@@ -154,8 +151,8 @@ __global__ void __launch_bounds__(256) k_raygen_batch(FrameDev F) {
     const uint32_t p = idx / S, sm = idx - p * S;
     view = view_of(F, p);
     const ViewDev& V = F.views[view];
-    // color slots sm * B + p cover [0, B * S): clear them in address order
-    // (slots of samples beyond a view's spp are never folded)
+    // this path's color slot (p * S + sm; slots of samples beyond a view's
+    // spp are never folded)
     F.color[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (P.sample + sm < V.spp) {
       const uint32_t lp = p - V.offset;
@@ -179,13 +176,15 @@ __global__ void k_batch_round_begin(FrameDev F) {
 }
 
 // Color slot of path pid: passes number paths pixel-major (pid = pixel * ns
-// + s; batch pixels are global, ns = the round's S) and keep colors
-// sample-major (s * W * H + pixel), the layout k_accumulate folds and
-// pk_read_color returns.
+// + s; batch pixels are global, ns = the round's S) and keep each pixel's S
+// color slots together (pixel * S + s), the layout k_accumulate folds: the
+// paths a wave finishes together are neighbours, so their read-modify-write
+// of the color slots touch a few cache lines instead of one line (and page)
+// each.
 __device__ inline uint32_t color_slot(const FrameDev& F, uint32_t ns, uint32_t pid) {
-  if (ns <= 1) return pid;
+  if (ns == F.S) return pid;
   const uint32_t pixel = pid / ns;
-  return (pid - pixel * ns) * (F.W * F.H) + pixel;
+  return pixel * F.S + (pid - pixel * ns);
 }
 
 __device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v3 L) {
@@ -922,7 +921,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
               has = true;
               pixel = gp - V.offset;
               sample = P.sample + sm;
-              slot = sm * B + gp;
+              slot = p;  // gp * S + sm
               root = V.root;
               camera_ray(pixel % V.W, pixel / V.W, V.W, V.H, sample, V.cam, &o, &d);
             }
@@ -931,7 +930,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
             pixel = p / ns;
             const uint32_t s = p - pixel * ns;
             sample = P.sample + s;
-            slot = s * B + pixel;
+            slot = color_slot(F, ns, p);
             camera_ray(pixel % F.W, pixel / F.W, F.W, F.H, sample, P.cam, &o, &d);
           }
           b = 0;
@@ -1037,30 +1036,55 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
 // samples folds them in sample order with the same per-sample update, so the
 // mean equals S one-sample frames bit for bit.
 // ---------------------------------------------------------------------------
+// A pixel's S color slots are adjacent, so one thread per pixel reading its
+// own would stride S * 16 bytes across the wave: a block stages kAccPix
+// pixels' slots in LDS, kAccChunk samples at a time, with coalesced loads,
+// and one wave folds them in sample order.
+constexpr uint32_t kAccPix = 64, kAccChunk = 32;
 __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
-  const uint32_t B = F.W * F.H;
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= B) return;
+  __shared__ float4 tile[kAccPix * kAccChunk];
+  const uint32_t B = F.W * F.H, S = F.S;
+  const uint32_t pix0 = blockIdx.x * kAccPix;
+  const uint32_t t = threadIdx.x;
   const FrameParams P = *F.params;
-  // batch: a view's pixels fold the round's samples below its spp (views
-  // whose budget is spent keep their mean)
-  uint32_t ns = pass_samples(F, P);
-  if (F.views) {
-    const uint32_t spp = F.views[view_of(F, idx)].spp;
-    ns = P.sample < spp ? min(F.S, spp - P.sample) : 0u;
-    if (ns == 0) return;
+  // the folding wave's pixel and its sample count this pass (batch: a view's
+  // pixels fold the round's samples below its spp; views whose budget is
+  // spent keep their mean)
+  const uint32_t idx = pix0 + t;
+  uint32_t ns = 0;
+  float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (t < kAccPix && idx < B) {
+    ns = pass_samples(F, P);
+    if (F.views) {
+      const uint32_t spp = F.views[view_of(F, idx)].spp;
+      ns = P.sample < spp ? min(S, spp - P.sample) : 0u;
+    }
+    a = F.accum[idx];
   }
-  float4 a = F.accum[idx];
-  for (uint32_t k = 0; k < ns; ++k) {
-    const float s = (float)(P.sample + k);
-    const float s1 = (float)(P.sample + k + 1u);
-    const float4 c = F.color[(size_t)k * B + idx];
-    a.x = (pkm::max_(c.x, 0.0f) + a.x * s) / s1;
-    a.y = (pkm::max_(c.y, 0.0f) + a.y * s) / s1;
-    a.z = (pkm::max_(c.z, 0.0f) + a.z * s) / s1;
+  const uint32_t nmax = F.views ? S : pass_samples(F, P);
+  for (uint32_t c0 = 0; c0 < nmax; c0 += kAccChunk) {
+    for (uint32_t e = t; e < kAccPix * kAccChunk; e += blockDim.x) {
+      const uint32_t pl = e / kAccChunk, k = e - pl * kAccChunk;
+      if (pix0 + pl < B && c0 + k < nmax) tile[e] = F.color[(size_t)(pix0 + pl) * S + c0 + k];
+    }
+    __syncthreads();
+    if (t < kAccPix) {
+      const uint32_t kend = ns > c0 ? min(kAccChunk, ns - c0) : 0u;
+      for (uint32_t k = 0; k < kend; ++k) {
+        const float sm = (float)(P.sample + c0 + k);
+        const float s1 = (float)(P.sample + c0 + k + 1u);
+        const float4 c = tile[t * kAccChunk + k];
+        a.x = (pkm::max_(c.x, 0.0f) + a.x * sm) / s1;
+        a.y = (pkm::max_(c.y, 0.0f) + a.y * sm) / s1;
+        a.z = (pkm::max_(c.z, 0.0f) + a.z * sm) / s1;
+      }
+    }
+    __syncthreads();
   }
-  a.w = 1.0f;
-  F.accum[idx] = a;
+  if (t < kAccPix && idx < B && ns > 0) {
+    a.w = 1.0f;
+    F.accum[idx] = a;
+  }
 }
 
 // Frame bookkeeping kernels (no memset/memcpy nodes, so frames can be
@@ -1460,7 +1484,7 @@ void launch_render_paths(const DevScene& S, const FrameDev& F0, uint32_t depth, 
   else hipLaunchKernelGGL((k_render_paths<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, depth);
 }
 void launch_accumulate(const FrameDev& F, hipStream_t s) {
-  hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, 256)), dim3(256), 0, s, F);
+  hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, kAccPix)), dim3(256), 0, s, F);
 }
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);

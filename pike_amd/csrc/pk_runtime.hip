@@ -869,7 +869,10 @@ int pk_read_color(pk_frame* f, float* rgba, size_t bytes) {
     return PK_E_INVALID;
   }
   PK_HIP(hipStreamSynchronize(f->ctx->stream));
-  PK_HIP(hipMemcpy(rgba, f->dev.color, (size_t)f->W * f->H * 16, hipMemcpyDeviceToHost));
+  // sample 0 of each pixel: a pixel's S color slots are adjacent
+  const size_t S = f->dev.S;
+  if (S == 1) PK_HIP(hipMemcpy(rgba, f->dev.color, (size_t)f->W * f->H * 16, hipMemcpyDeviceToHost));
+  else PK_HIP(hipMemcpy2D(rgba, 16, f->dev.color, 16 * S, 16, (size_t)f->W * f->H, hipMemcpyDeviceToHost));
   return PK_OK;
 }
 
