@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   camera_ray(pixel % W, pixel / W, W, H, P.sample + s, P.cam, &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
-  F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  // (camera rays carry throughput 1: bounce 0 does not read F.thr)
   F.color[(size_t)pixel * F.S + s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -157,7 +157,6 @@ __global__ void __launch_bounds__(256) k_raygen_batch(FrameDev F) {
     if (P.sample + sm < V.spp) {
       const uint32_t lp = p - V.offset;
       camera_ray(lp % V.W, lp / V.W, V.W, V.H, P.sample + sm, V.cam, &o, &d);
-      F.thr[idx] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
       emit = true;
     }
   }
@@ -308,7 +307,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
       [&](uint32_t, const RayState& r) {
         traced++;
         if (r.best_id == 0xffffffffu) {  // miss: shade_miss.wgsl envColor x throughput
-          const float4 t4 = F.thr[pid];
+          const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.thr[pid];
           add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
       },
@@ -786,7 +785,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
     // numbered pixel-major
     const uint32_t s = pid % ns;
     const uint32_t pixel = F.views ? pid / ns - F.views[view_bits].offset : pid / ns;
-    const float4 t4 = F.thr[pid];
+    const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.thr[pid];  // camera rays: 1
     shade_hit<false>(S, P.env_theta, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), __uint_as_float(hr.z),
                      hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample + s), pkm::mk(t4.x, t4.y, t4.z), o,
                      nullptr);
