@@ -3,12 +3,13 @@
 Workload (the plain method at scale): 2,097,152-triangle fBm terrain plus 200
 subdivision-4 icospheres (3,121,152 triangles), 2048x1024 log-normal HDR
 environment with a 0.5-degree sun, importance-sampled via the 2D
-(marginal/conditional) CDF; 1920x1080, depth 8, FP32.  One step = one sample per pixel of the frame through
-raygen -> {closest hit -> shade -> shadow any-hit} x 8 -> accumulate,
-replayed from one captured hipGraph; the autotuner chooses how many samples
-per pixel one wavefront pass traces (up to 64; the image equals one-sample
-frames bit for bit).  value = samples per pixel per second (frames/s at 1080p); scene, BVH
-and queues are resident in HBM before timing.
+(marginal/conditional) CDF; 1920x1080, depth 8, FP32.  One step = one
+wavefront pass: S samples of every pixel of the frame (S chosen by the
+autotuner, up to 128; the image equals S one-sample frames bit for bit)
+through raygen -> {closest hit -> shade -> shadow any-hit} x 8 ->
+accumulate, replayed from one captured hipGraph.  value = samples per pixel
+per second (frames/s at 1080p); scene, BVH and queues are resident in HBM
+before timing.
 
 Also reported: traversal microbenchmark (10M-triangle soup, 16M incoherent
 rays, closest and any hit, Mrays/s), the roofline of the dominant kernel
@@ -39,19 +40,49 @@ def traversal_bytes(rays, nodes, tris):
     return rays * RAY_BYTES + nodes * NODE_BYTES + tris * TRI_BYTES
 
 
+TRAFFIC_SUMMARY = "render_c2_1080p_hbm_traffic.md"
+COUNTER_SUMMARY = "render_c2_1080p_counters.md"
+
+
+def summary_row(name, kernel):
+    """The row of `kernel` in the first markdown table of profiles/`name`
+    whose header starts with "Kernel", as {column header: float or text};
+    {} when the summary or the row is absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    header = None
+    try:
+        for line in open(path):
+            if not line.startswith("|"):
+                header = None
+                continue
+            cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+            if cells and cells[0] == "Kernel":
+                header = cells
+            elif header and cells[0] == kernel:
+                row = {}
+                for h, c in zip(header[1:], cells[1:]):
+                    try:
+                        row[h] = float(c)
+                    except ValueError:
+                        row[h] = c
+                return row
+    except OSError:
+        return {}
+    return {}
+
+
 def measured_traffic(kernel="k_trace_closest"):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known
     size); None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "render_c2_1080p_hbm_traffic.md")
-    try:
-        for line in open(path):
-            cells = [c.strip().strip("`") for c in line.split("|")]
-            if len(cells) > 7 and cells[1] == kernel:
-                return float(cells[7]) * 1e6
-    except OSError:
-        return None
-    return None
+    v = summary_row(TRAFFIC_SUMMARY, kernel).get("HBM MB/dispatch")
+    return v * 1e6 if isinstance(v, float) else None
+
+
+def measured_counters(kernel):
+    """Derived PMC figures of `kernel` (lanes per VALU instruction, L2 hit
+    rate, ...) from the committed counter summary."""
+    return {k: v for k, v in summary_row(COUNTER_SUMMARY, kernel).items() if isinstance(v, float)}
 
 
 def run_render(args, pike_amd, scenes):
@@ -101,22 +132,26 @@ def run_render(args, pike_amd, scenes):
         ctx.set_scheduler(args.scheduler)
     if tuned is not None:
         tuned["scheduler"] = ctx.scheduler()
+    # One step = one whole wavefront pass: S samples of every pixel (S = the
+    # frame's samples per pass), so every timed pass is a full one.
+    S = max(1, frame.samples_per_pass)
     # warmup (also instantiates the graph)
-    frame.render(scene, max_depth=depth, first_sample=0, num_frames=max(1, args.warmup), use_graph=True)
+    frame.render(scene, max_depth=depth, first_sample=0, num_frames=max(1, args.warmup) * S, use_graph=True)
     ctx.sync()
     frame.reset()
     ctx.sync()
+    samples = args.steps * S
     t0 = time.perf_counter()
-    frame.render(scene, max_depth=depth, first_sample=0, num_frames=args.steps, use_graph=True)
+    frame.render(scene, max_depth=depth, first_sample=0, num_frames=samples, use_graph=True)
     ctx.sync()
     dt = time.perf_counter() - t0
     cnt = frame.counters()
     img = frame.read_image()
     assert np.isfinite(img).all(), "non-finite pixels"
-    spp_s = args.steps / dt
+    spp_s = samples / dt
     rays_total = cnt["rays_traced"] + cnt["shadow_rays"]
-    log(f"render: {args.steps} frames in {dt:.3f}s = {spp_s:.2f} spp/s, {rays_total / dt / 1e6:.0f} Mrays/s, "
-        f"overflow={cnt['stack_overflow']}")
+    log(f"render: {args.steps} passes of {S} samples per pixel ({samples} spp) in {dt:.3f}s = {spp_s:.2f} spp/s, "
+        f"{rays_total / dt / 1e6:.0f} Mrays/s, overflow={cnt['stack_overflow']}")
     # kernel-level profile: counted run for algorithmic bytes, uncounted for
     # time, over whole passes
     pf = args.profile_frames
@@ -128,6 +163,7 @@ def run_render(args, pike_amd, scenes):
                     kernel_ms=None, profile_frames=0, profile_counters=None,
                     roofline=dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
                                   traffic=None, bytes_per_launch=None, launch_ms=None),
+                    samples_per_step=S,
                     triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler()), data, bvh, cam
     frame.reset()
     frame.count_nodes(True)
@@ -142,12 +178,27 @@ def run_render(args, pike_amd, scenes):
     achieved = bytes_launch / t_launch / 1e9
     log(f"profile ({pf} frames): {ms}; per trace_closest launch {bytes_launch / 1e6:.1f} MB in "
         f"{t_launch * 1e3:.3f} ms -> {achieved:.0f} GB/s")
+    traffic = measured_traffic()
+    pmc = measured_counters("k_trace_closest")
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                traffic=traffic, kernel="k_trace_closest", bytes_per_launch=bytes_launch, launch_ms=t_launch * 1e3,
+                # `achieved` prices the kernel's algorithmic node/triangle/ray
+                # bytes, most of which L2 and MALL serve; the DRAM figures
+                # divide the PMC-measured HBM bytes per launch by this run's
+                # launch time.  The kernel's limiter is VALU issue with idle
+                # lanes (divergent traversal), read from the PMC summary.
+                achieved_basis="algorithmic bytes per launch (L2/MALL hits included)",
+                traffic_source=f"profiles/{TRAFFIC_SUMMARY} (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
+                dram_gbs=traffic / t_launch / 1e9 if traffic else None,
+                dram_frac=traffic / t_launch / 1e9 / HBM_PEAK_GBS if traffic else None,
+                limiter="valu_issue (divergent lanes), not HBM",
+                valu_lanes_active=pmc.get("lanes active per VALU instruction"),
+                l2_hit_rate=pmc.get("L2 hit rate"),
+                lds_bank_conflict_frac=pmc.get("LDS bank-conflict cycles / LDS instructions"),
+                counters_source=f"profiles/{COUNTER_SUMMARY}" if pmc else None)
     result = dict(
         spp_s=spp_s, ms_per_step=dt / args.steps * 1e3, rays_per_s=rays_total / dt, counters=cnt,
-        kernel_ms=ms, profile_frames=pf, profile_counters=cc,
-        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=measured_traffic(), kernel="k_trace_closest", bytes_per_launch=bytes_launch,
-                      launch_ms=t_launch * 1e3),
+        kernel_ms=ms, profile_frames=pf, profile_counters=cc, roofline=roof, samples_per_step=S,
         triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler())
     return result, data, bvh, cam
 
@@ -176,7 +227,7 @@ def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
     ctx.sync()
     frame.reset()
     ctx.sync()
-    steps = max(args.steps, spp_per_pass)
+    steps = args.steps * spp_per_pass  # whole passes, as the headline leg
     t0 = time.perf_counter()
     frame.render(scene, max_depth=args.depth, first_sample=0, num_frames=steps, use_graph=True)
     ctx.sync()
@@ -456,7 +507,10 @@ def run_diff(args, pike_amd, scenes):
 def run_cpu_baseline(args, data, cam):
     from oracle.oracle import OracleScene
 
-    threads = min(16, os.cpu_count() or 1)
+    # Every core this process may run on, up to the 16 host cores the GPU
+    # host allots to one GPU (the count the machine reports is recorded too).
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, affinity))
     t0 = time.time()
     orc = OracleScene(data, threads=threads)
     orc.build_bvh()
@@ -465,14 +519,15 @@ def run_cpu_baseline(args, data, cam):
     t0 = time.perf_counter()
     orc.render(cam, args.width, args.height, args.depth, 0, 1, precision=32)
     t1 = time.perf_counter() - t0
-    frames = int(max(1, min(args.steps, round(args.cpu_seconds / max(t1, 1e-3)))))
+    frames = int(max(1, round(args.cpu_seconds / max(t1, 1e-3))))
     t0 = time.perf_counter()
     orc.render(cam, args.width, args.height, args.depth, 1, frames, precision=32)
     dt = time.perf_counter() - t0
     v = frames / dt
     log(f"cpu baseline: {frames} frames of {args.width}x{args.height} in {dt:.2f}s on {threads} threads "
         f"-> {v:.3f} spp/s")
-    return dict(value=v, unit="spp/s", cores=threads, kind="port",
+    return dict(value=v, unit="spp/s", cores=threads, kind="port", host_cpus=os.cpu_count(),
+                affinity_cpus=affinity, core_limit=16,
                 sample=f"oracle FP32 restatement (C++, std::thread), {frames} whole {args.width}x{args.height} "
                        f"frames (1 sample per pixel each, depth {args.depth}) of the same scene and camera")
 
@@ -528,8 +583,9 @@ def run_traversal(args, pike_amd, scenes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=256)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=8, help="timed steps; one step = one wavefront pass of the "
+                                                          "autotuned samples per pixel (S), i.e. S spp")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes before the timed ones")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=8)
@@ -644,7 +700,9 @@ def main():
         "config": {"workload": f"plain wavefront path tracing, {res['triangles']:,} triangles "
                                f"({args.bvh.upper()} wide BVH{', 200 instanced icospheres' if args.instanced else ''}), "
                                f"2048x1024 env (2D CDF importance sampling), "
-                               f"{args.width}x{args.height}, depth {args.depth}, 1 spp per step",
+                               f"{args.width}x{args.height}, depth {args.depth}; one step = one wavefront pass of "
+                               f"{res['samples_per_step']} samples per pixel",
+                   "samples_per_step": res["samples_per_step"],
                    "width": args.width, "height": args.height, "depth": args.depth, "triangles": res["triangles"],
                    "bvh": args.bvh, "collapse": args.collapse, "instanced": args.instanced,
                    "scheduler": res["scheduler"]},

@@ -14,13 +14,13 @@
  * the repository root):
  *   context            render/pike.js:92-118 (initWebGpu: adapter + device)
  *   scene upload       render/passes/trace_pass.js:220-263 (initBVHBuffer),
- *                      render/passes/shade_hit_pass.js:58-81,293-368 (atlas,
+ *                      render/passes/shade_hit_pass.js:58-81,92-167 (atlas,
  *                      attributes, materials, env tables),
  *                      render/env_sampler.js:7-157 (EnvironmentGenerator)
  *   BVH build          render/bvh.js:7-108 (BVH), :110-275 (Node)
  *   frame / passes     render/passes/camera_pass.js:3-77 (RenderState),
  *                      :79-221 (CameraPass), trace_pass.js:265-276,
- *                      trace_shadow_pass.js:42-52, shade_hit_pass.js:369-382,
+ *                      trace_shadow_pass.js:42-52, shade_hit_pass.js:169-181,
  *                      shade_miss_pass.js:68-80, accumulate_pass.js:75-85,
  *                      render/pike.js:120-162 (tick)
  */
@@ -138,7 +138,7 @@ int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_c
                       uint32_t* num_cus);
 /* Pass scheduler of frames (no reference counterpart: the
  * reference always runs its passes as wavefront compute dispatches,
- * pathtracer.js:201-266).  PK_SCHED_WAVEFRONT: raygen, then per bounce
+ * render/pike.js:135-155).  PK_SCHED_WAVEFRONT: raygen, then per bounce
  * trace_closest / shade / trace_shadow over queues in device memory.
  * PK_SCHED_PATHS: one persistent megakernel in which each lane carries a
  * path through every bounce in registers.  Both give bit-identical color

@@ -27,6 +27,45 @@ namespace {
 
 constexpr double kInf = std::numeric_limits<double>::infinity();
 
+// Planted mistakes for the tests that show each closed-form pin has teeth
+// (or_set_mutation; 0 = the faithful restatement).  Each OR_MUT_* names one
+// wrong variant of one line below; the test suite checks that the pin aimed
+// at it fails while it is planted.
+int g_mut = 0;
+inline bool mut(int k) { return g_mut == k; }
+enum {
+  OR_MUT_G1_SWAP = 1,          // Smith G1 takes the microfacet normal's slope instead of w's
+  OR_MUT_LAMBERT_INV_PI,       // evalLambert without 1/pi
+  OR_MUT_ONB_SIGN,             // branchlessONB: b1.z sign flipped
+  OR_MUT_JITTER_ORDER,         // createPrimaryRay draws the y jitter first
+  OR_MUT_D_PI,                 // GGX D without pi
+  OR_MUT_SCHLICK_POW,          // Schlick (1-cos)^4
+  OR_MUT_PH_BALANCE,           // balance heuristic instead of power heuristic
+  OR_MUT_LAMBERT_SAMPLE,       // cosine sampling with r = u instead of sqrt(u)
+  OR_MUT_VNDF_BLEND,           // sampleGGX: s = 0.5 (1 - vh.z)
+  OR_MUT_ENV_PDF_SIN,          // envPdf without the 1/sin(phi) Jacobian
+  OR_MUT_ENV_CENTER,           // sampleEnv at the texel corner, not its centre
+  OR_MUT_ACC_DENOM,            // accumulate divides by samples + 2
+  OR_MUT_NORMAL_MAP,           // normal map xy not scaled by 2
+  OR_MUT_ORIGIN_SIGN,          // hit origin pushed past the surface
+  OR_MUT_ASPECT,               // camera without the W/H aspect factor
+  OR_MUT_SPEC_PDF4,            // specularPdf without the 1/4 Jacobian
+  OR_MUT_ENV_COLOR_V,          // envColor v from acos(-y)
+  OR_MUT_TEX_HALF,             // bilinear lookup without the half-texel offset
+  OR_MUT_EMISSION,             // emission scale 1 instead of 50
+  OR_MUT_NEE_SPEC_WEIGHT,      // NEE specular MIS weight against the Lambert pdf
+  OR_MUT_BOUNCE_WEIGHT,        // bounce ray without its MIS weight
+  OR_MUT_EVAL_SPEC_COS,        // evalSpecular divides by wi.z instead of wo.z
+  OR_MUT_SAMPLE_ENV_SIN,       // sampleEnv pdf without 1/sin(phi)
+  OR_MUT_METAL_CHANNEL,        // metallic read from G instead of B
+  OR_MUT_ALPHA_SQUARE,         // GGX alpha = roughness instead of roughness^2
+  OR_MUT_DISTORTION,           // radial distortion r (1 - k r^2)
+  OR_MUT_G1_SIGN,              // G1 without the back-facing test
+  OR_MUT_FRESNEL_METAL,        // f = mix(schlick, 1, 1 - metallic)
+  OR_MUT_THETA_SIGN,           // envColor rotates by -env_theta
+  OR_MUT_COUNT
+};
+
 // ---------------------------------------------------------------------------
 // PCG hash, restated from camera.wgsl:46-56 (identical in shade_hit.wgsl:153-163)
 // ---------------------------------------------------------------------------
@@ -1012,7 +1051,8 @@ inline int wrap(int i, int n) {
 // textureSampleLevel with a linear filter and repeat addressing, level 0,
 // of the sampling atlas
 template <class T> void sample_tex(const or_scene* s, int layer, T u, T v, T out[4]) {
-  T x = u * (T)s->tw - T(0.5), y = v * (T)s->th - T(0.5);
+  const T half = mut(OR_MUT_TEX_HALF) ? T(0) : T(0.5);
+  T x = u * (T)s->tw - half, y = v * (T)s->th - half;
   T fx0 = M<T>::floor(x), fy0 = M<T>::floor(y);
   T fx = x - fx0, fy = y - fy0;
   int x0 = wrap((int)fx0, (int)s->tw), x1 = wrap((int)fx0 + 1, (int)s->tw);
@@ -1098,8 +1138,8 @@ template <> double env_pdf_tex<double>(const or_scene* s, size_t i) { return s->
 template <class T, class U> V3<U> env_color(const Ctx<T>& c, V3<T> dir) {
   const or_scene* s = c.s;
   if (s->efmt == PK_ENV_NONE) return v3(U(0), U(0), U(0));
-  T u = c.env_theta + M<T>::atan2(dir.z, dir.x) / M<T>::tau();
-  T v = M<T>::acos(dir.y) * M<T>::inv_pi();
+  T u = (mut(OR_MUT_THETA_SIGN) ? -c.env_theta : c.env_theta) + M<T>::atan2(dir.z, dir.x) / M<T>::tau();
+  T v = M<T>::acos(mut(OR_MUT_ENV_COLOR_V) ? -dir.y : dir.y) * M<T>::inv_pi();
   int ix = wrap((int)M<T>::floor(u * (T)s->ew), (int)s->ew);
   int iy = wrap((int)M<T>::floor(v * (T)s->eh), (int)s->eh);
   return env_radiance<T, U>(c, (size_t)iy * s->ew + (size_t)ix);
@@ -1113,7 +1153,7 @@ template <class T> T env_pdf(const Ctx<T>& c, V3<T> dir) {
   ix = std::min(std::max(ix, 0), (int)s->ew - 1);
   iy = std::min(std::max(iy, 0), (int)s->eh - 1);
   T phi = v * M<T>::pi();
-  T sin_phi = M<T>::sin(phi);
+  T sin_phi = mut(OR_MUT_ENV_PDF_SIN) ? T(1) : M<T>::sin(phi);
   return env_pdf_tex<T>(s, (size_t)iy * s->ew + (size_t)ix) / sin_phi;
 }
 
@@ -1137,7 +1177,7 @@ template <class T> M3<T> onb(V3<T> n) {
   T sign = n.z > T(0) ? T(1) : T(-1);
   T a = T(-1) / (sign + n.z);
   T b = n.x * n.y * a;
-  V3<T> b1 = v3(T(1) + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  V3<T> b1 = v3(T(1) + sign * n.x * n.x * a, sign * b, mut(OR_MUT_ONB_SIGN) ? sign * n.x : -sign * n.x);
   V3<T> b2 = v3(b, sign + n.y * n.y * a, -n.y);
   return M3<T>{b1, b2, n};
 }
@@ -1145,33 +1185,36 @@ template <class T, class U> U ggx_d(V3<T> m, U au, U av) {
   U auv = au * av;
   U tx = m.x / au, ty = m.y / av;
   U e = tx * tx + ty * ty + m.z * m.z;
-  return U(1) / (M<T>::pi() * auv * e * e);
+  return U(1) / ((mut(OR_MUT_D_PI) ? T(1) : M<T>::pi()) * auv * e * e);
 }
 template <class T, class U> U ggx_g1(V3<T> w, V3<T> m, U au, U av) {
-  U ax = w.x * au, ay = w.y * av;
+  const V3<T> ws = mut(OR_MUT_G1_SWAP) ? m : w;
+  U ax = ws.x * au, ay = ws.y * av;
   U axy2 = ax * ax + ay * ay;
-  U tan_theta = axy2 / (w.z * w.z);
+  U tan_theta = axy2 / (ws.z * ws.z);
   U r = U(2) / (U(1) + usqrt(U(1) + tan_theta));
+  if (mut(OR_MUT_G1_SIGN)) return r;
   return dot(w, m) * dot(w, v3(T(0), T(0), T(1))) <= T(0) ? U(0) : r;
 }
 template <class T> T lambert_pdf(V3<T> wi) { return mx(wi.z, T(0.0000001f)) * M<T>::inv_pi(); }
 template <class T> T specular_pdf(const SI<T>& si, V3<T> m) {
   return mx(T(0.0000001f), ggx_d<T, T>(m, si.alpha, si.alpha) * ggx_g1<T, T>(si.wo, m, si.alpha, si.alpha) /
-                               (T(4) * si.wo.z));
+                               ((mut(OR_MUT_SPEC_PDF4) ? T(1) : T(4)) * si.wo.z));
 }
 template <class T, class U> V3<U> eval_lambert(const EV<U>& e, const Sample<T>& s) {
   T k = mx(T(0.0000001f), s.wi.z);
-  return v3(e.base.x * M<T>::inv_pi() * k / s.pdf, e.base.y * M<T>::inv_pi() * k / s.pdf,
-            e.base.z * M<T>::inv_pi() * k / s.pdf);
+  const T ip = mut(OR_MUT_LAMBERT_INV_PI) ? T(1) : M<T>::inv_pi();
+  return v3(e.base.x * ip * k / s.pdf, e.base.y * ip * k / s.pdf, e.base.z * ip * k / s.pdf);
 }
 template <class T, class U> V3<U> eval_specular(const SI<T>& si, const EV<U>& e, const Sample<T>& s) {
   V3<T> h = normalize(si.wo + s.wi);
   U d = ggx_d<T, U>(h, e.alpha, e.alpha);
   U g = ggx_g1<T, U>(s.wi, h, e.alpha, e.alpha) * ggx_g1<T, U>(si.wo, h, e.alpha, e.alpha);
-  U k = mx(d * g / (T(4) * si.wo.z * s.pdf), U(0));
+  U k = mx(d * g / (T(4) * (mut(OR_MUT_EVAL_SPEC_COS) ? s.wi.z : si.wo.z) * s.pdf), U(0));
   return e.spec * k;
 }
 template <class T> T power_heuristic(T a, T b) {
+  if (mut(OR_MUT_PH_BALANCE)) return a / (a + b);
   T a2 = a * a;
   return a2 / (a2 + b * b);
 }
@@ -1180,7 +1223,7 @@ template <class T> T schlick(T cos_theta, T ior) {
   r0 *= r0;
   T tmp = T(1) - cos_theta;
   T tmp2 = tmp * tmp;
-  return r0 + (T(1) - r0) * tmp2 * tmp2 * tmp;
+  return r0 + (T(1) - r0) * tmp2 * tmp2 * (mut(OR_MUT_SCHLICK_POW) ? T(1) : tmp);
 }
 // sampleGGX (:295-316), Heitz 2018 VNDF
 template <class T> V3<T> sample_ggx(const SI<T>& si, Rng& rng) {
@@ -1193,13 +1236,13 @@ template <class T> V3<T> sample_ggx(const SI<T>& si, Rng& rng) {
   T phi = M<T>::tau() * rng.rand<T>();
   T t1 = r * M<T>::cos(phi);
   T t2 = r * M<T>::sin(phi);
-  T sv = T(0.5) * (T(1) + vh.z);
+  T sv = T(0.5) * (mut(OR_MUT_VNDF_BLEND) ? T(1) - vh.z : T(1) + vh.z);
   t2 = (T(1) - sv) * M<T>::sqrt(T(1) - t1 * t1) + sv * t2;
   V3<T> nh = t1v * t1 + t2v * t2 + vh * M<T>::sqrt(mx(T(0), T(1) - t1 * t1 - t2 * t2));
   return normalize(v3(au * nh.x, av * nh.y, mx(T(0), nh.z)));
 }
 template <class T> Sample<T> sample_lambert(Rng& rng) {
-  T r = M<T>::sqrt(rng.rand<T>());
+  T r = mut(OR_MUT_LAMBERT_SAMPLE) ? rng.rand<T>() : M<T>::sqrt(rng.rand<T>());
   T phi = M<T>::tau() * rng.rand<T>();
   T x = r * M<T>::cos(phi);
   T y = r * M<T>::sin(phi);
@@ -1235,13 +1278,15 @@ template <class T> Sample<T> sample_env(const Ctx<T>& c, const M3<T>& basis, Rng
     cy = s->ecoords[ci][1];
   }
   const std::array<int32_t, 2> coord = {cx, cy};
-  T u = -c.env_theta + ((T(0.5) + (T)coord[0]) / (T)s->ew);
-  T v = (T(0.5) + (T)coord[1]) / (T)s->eh;
+  const T half = mut(OR_MUT_ENV_CENTER) ? T(0) : T(0.5);
+  T u = -c.env_theta + ((half + (T)coord[0]) / (T)s->ew);
+  T v = (half + (T)coord[1]) / (T)s->eh;
   T theta = u * M<T>::tau();
   T phi = v * M<T>::pi();
   T sin_phi = M<T>::sin(phi);
   V3<T> dir = v3(M<T>::cos(theta) * sin_phi, M<T>::cos(phi), M<T>::sin(theta) * sin_phi);
-  T pdf = env_pdf_tex<T>(s, (size_t)coord[1] * s->ew + (size_t)coord[0]) / sin_phi;
+  T pdf = env_pdf_tex<T>(s, (size_t)coord[1] * s->ew + (size_t)coord[0]) /
+          (mut(OR_MUT_SAMPLE_ENV_SIN) ? T(1) : sin_phi);
   return Sample<T>{vec_mat(dir, basis), pdf};
 }
 
@@ -1307,18 +1352,19 @@ template <class T, class U> SI<T> surface(const Ctx<T>& c, V3<T> ro, V3<T> rd, c
   const int32_t* mi = &s->mat[(size_t)mat_id * 4];
   T nm[4], dc[4], mr[4], em[4];
   sample_tex<T>(s, mi[2], uvx, uvy, nm);
-  V3<T> map_normal = v3((nm[0] - T(0.5)) * T(2), (nm[1] - T(0.5)) * T(2), (nm[2] - T(0)) * T(1));
+  const T nms = mut(OR_MUT_NORMAL_MAP) ? T(1) : T(2);
+  V3<T> map_normal = v3((nm[0] - T(0.5)) * nms, (nm[1] - T(0.5)) * nms, (nm[2] - T(0)) * T(1));
   SI<T> si;
   si.normal = normalize(mat_vec(M3<T>{tangent, bitangent, normal}, map_normal));
   const T eps40 = T(0.0000001f) * T(40);
-  si.origin = ro + rd * (h.t - eps40);
+  si.origin = ro + rd * (mut(OR_MUT_ORIGIN_SIGN) ? h.t + eps40 : h.t - eps40);
   si.basis = onb(si.normal);
   si.wo = vec_mat(-rd, si.basis);
   sample_tex<T>(s, mi[0], uvx, uvy, dc);
   si.opacity = dc[3];
   sample_tex<T>(s, mi[1], uvx, uvy, mr);
-  si.metallic = mr[2];
-  si.alpha = mr[1] * mr[1];
+  si.metallic = mut(OR_MUT_METAL_CHANNEL) ? mr[1] : mr[2];
+  si.alpha = mut(OR_MUT_ALPHA_SQUARE) ? mr[1] : mr[1] * mr[1];
   sample_tex<T>(s, mi[3], uvx, uvy, em);
   si.emission = v3(em[0], em[1], em[2]);
   U dce[4], mre[4];
@@ -1340,10 +1386,11 @@ template <class T> Ray<T> primary_ray(const pk_camera& cam, uint32_t px, uint32_
   T resx = (T)W, resy = (T)H;
   T jx = rng.rand<T>();
   T jy = rng.rand<T>();
-  T ux = (T(2) * (((T)px + jx) / resx) - T(1)) * (resx / resy);
+  if (mut(OR_MUT_JITTER_ORDER)) std::swap(jx, jy);
+  T ux = (T(2) * (((T)px + jx) / resx) - T(1)) * (mut(OR_MUT_ASPECT) ? T(1) : resx / resy);
   T uy = (T(2) * (((T)py + jy) / resy) - T(1)) * T(-1);
   T rd = M<T>::sqrt(ux * ux + uy * uy);
-  T ru = rd * (T(1) + k * rd * rd);
+  T ru = rd * (mut(OR_MUT_DISTORTION) ? T(1) - k * rd * rd : T(1) + k * rd * rd);
   T l2 = M<T>::sqrt(ux * ux + uy * uy);
   ux = ru * (ux / l2);
   uy = ru * (uy / l2);
@@ -1407,11 +1454,13 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
     }
     if (si.emission.x * T(1) + si.emission.y * T(1) + si.emission.z * T(1) > T(0)) {
       T att = mx(dot(si.normal, mat_vec(si.basis, si.wo)), T(0));
-      add(v3(T(50), T(50), T(50)) * si.emission * thr * att);
+      const T scale = mut(OR_MUT_EMISSION) ? T(1) : T(50);
+      add(v3(scale, scale, scale) * si.emission * thr * att);
       return;
     }
     V3<T> m = sample_ggx<T>(si, sr);
-    T f = mix(schlick(mx(dot(si.wo, m), T(0)), T(1.5)), T(1), si.metallic);
+    T f = mix(schlick(mx(dot(si.wo, m), T(0)), T(1.5)), T(1),
+              mut(OR_MUT_FRESNEL_METAL) ? T(1) - si.metallic : si.metallic);
     Sample<T> bs;
     V3<U> bsdf;
     if (sr.rand<T>() > f) {
@@ -1427,7 +1476,7 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
     V3<U> next_thr;
     if (bs.wi.z > T(0)) {
       V3<T> dir = mat_vec(si.basis, bs.wi);
-      T weight = c.sample_env ? power_heuristic(bs.pdf, env_pdf<T>(c, dir)) : T(1);
+      T weight = c.sample_env && !mut(OR_MUT_BOUNCE_WEIGHT) ? power_heuristic(bs.pdf, env_pdf<T>(c, dir)) : T(1);
       next = Ray<T>{si.origin, dir};
       next_thr = bsdf * thr * weight;
       bounce = true;
@@ -1440,7 +1489,7 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
         V3<U> el = eval_lambert<T, U>(ev, es);
         V3<U> scale = v3((T(1) - f) * el.x * lw, (T(1) - f) * el.y * lw, (T(1) - f) * el.z * lw);
         V3<T> hh = normalize(si.wo + es.wi);
-        T sw = power_heuristic(es.pdf, specular_pdf(si, hh));
+        T sw = power_heuristic(es.pdf, mut(OR_MUT_NEE_SPEC_WEIGHT) ? lambert_pdf(es.wi) : specular_pdf(si, hh));
         V3<U> evs = eval_specular<T, U>(si, ev, es);
         scale = scale + v3(f * evs.x * sw, f * evs.y * sw, f * evs.z * sw);
         V3<T> sdir = mat_vec(si.basis, es.wi);
@@ -1454,6 +1503,19 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
     ray = next;
     thr = next_thr;
   }
+}
+
+// accumulate.wgsl:26-30: acc = (max(c, 0) + acc * s) / (s + 1), s = the
+// frame's sample index; the accumulator is an f32 buffer (FP64 evaluates the
+// update in double and rounds once).
+template <class T> float accumulate(T c, float acc, uint32_t sample) {
+  const uint32_t den = sample + (mut(OR_MUT_ACC_DENOM) ? 2u : 1u);
+  if (sizeof(T) == 4) {
+    const float col = mx((float)c, 0.0f);
+    return (col + acc * (float)sample) / (float)den;
+  }
+  const double col = mx((double)c, 0.0);
+  return (float)((col + (double)acc * (double)sample) / (double)den);
 }
 
 template <class T>
@@ -1486,19 +1548,7 @@ int render_t(const or_scene* s, const pk_camera* cam, const or_render_params* p,
       hits[i] += o.hits;
       // accumulate.wgsl:26-30 in f32 (the accumulator is an f32 buffer)
       float* a = &accum[(size_t)i * 4];
-      if (sizeof(T) == 4) {
-        float sf = (float)sample;
-        for (int k = 0; k < 3; ++k) {
-          float col = mx((float)o.c[k], 0.0f);
-          a[k] = (col + a[k] * sf) / (float)(sample + 1);
-        }
-      } else {
-        double sd = (double)sample;
-        for (int k = 0; k < 3; ++k) {
-          double col = mx((double)o.c[k], 0.0);
-          a[k] = (float)((col + (double)a[k] * sd) / (double)(sample + 1));
-        }
-      }
+      for (int k = 0; k < 3; ++k) a[k] = accumulate<T>((T)o.c[k], a[k], sample);
       a[3] = 1.0f;
       if (color_out && f + 1 == p->num_frames)
         for (int k = 0; k < 4; ++k) color_out[(size_t)i * 4 + k] = (float)o.c[k];
@@ -1568,9 +1618,178 @@ int gradient_t(const or_scene* s, const pk_camera* cam, const or_render_params* 
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Probes: single functions of the restatement evaluated on given inputs, for
+// the closed-form tests (or_probe).  T = double is the FP64 oracle, float
+// the FP32 restatement; both instantiate the same templates.
+// ---------------------------------------------------------------------------
+template <class T> V3<T> in3(const double* p) { return v3((T)p[0], (T)p[1], (T)p[2]); }
+template <class T> void out3(double* o, V3<T> v) {
+  o[0] = (double)v.x;
+  o[1] = (double)v.y;
+  o[2] = (double)v.z;
+}
+
+template <class T> int probe_one(const or_scene* s, int fn, const double* in, double* out) {
+  Ctx<T> c;
+  c.s = s;
+  c.nodes = nullptr;
+  c.env_theta = T(0);
+  c.sample_env = s && s->efmt != PK_ENV_NONE;
+  c.seeds = nullptr;
+  auto need_scene = [&]() { return s != nullptr; };
+  switch (fn) {
+    case OR_PROBE_GGX_D: out[0] = (double)ggx_d<T, T>(in3<T>(in), (T)in[3], (T)in[3]); return 0;
+    case OR_PROBE_GGX_G1: out[0] = (double)ggx_g1<T, T>(in3<T>(in), in3<T>(in + 3), (T)in[6], (T)in[6]); return 0;
+    case OR_PROBE_LAMBERT_PDF: out[0] = (double)lambert_pdf<T>(in3<T>(in)); return 0;
+    case OR_PROBE_SPECULAR_PDF: {
+      SI<T> si{};
+      si.wo = in3<T>(in);
+      si.alpha = (T)in[6];
+      out[0] = (double)specular_pdf<T>(si, in3<T>(in + 3));
+      return 0;
+    }
+    case OR_PROBE_EVAL_LAMBERT: {
+      EV<T> e{};
+      e.base = in3<T>(in);
+      out3(out, eval_lambert<T, T>(e, Sample<T>{in3<T>(in + 3), (T)in[6]}));
+      return 0;
+    }
+    case OR_PROBE_EVAL_SPECULAR: {
+      SI<T> si{};
+      si.wo = in3<T>(in);
+      EV<T> e{};
+      e.alpha = (T)in[7];
+      e.spec = in3<T>(in + 8);
+      out3(out, eval_specular<T, T>(si, e, Sample<T>{in3<T>(in + 3), (T)in[6]}));
+      return 0;
+    }
+    case OR_PROBE_POWER_HEURISTIC: out[0] = (double)power_heuristic<T>((T)in[0], (T)in[1]); return 0;
+    case OR_PROBE_SCHLICK: out[0] = (double)schlick<T>((T)in[0], (T)in[1]); return 0;
+    case OR_PROBE_ONB: {
+      const M3<T> b = onb<T>(in3<T>(in));
+      out3(out, b.c0);
+      out3(out + 3, b.c1);
+      out3(out + 6, b.c2);
+      return 0;
+    }
+    case OR_PROBE_SAMPLE_GGX: {
+      SI<T> si{};
+      si.wo = in3<T>(in);
+      si.alpha = (T)in[3];
+      Rng rng{(uint32_t)in[4]};
+      out3(out, sample_ggx<T>(si, rng));
+      return 0;
+    }
+    case OR_PROBE_SAMPLE_LAMBERT: {
+      Rng rng{(uint32_t)in[0]};
+      const Sample<T> smp = sample_lambert<T>(rng);
+      out3(out, smp.wi);
+      out[3] = (double)smp.pdf;
+      return 0;
+    }
+    case OR_PROBE_ENV_COLOR: {
+      if (!need_scene()) return -1;
+      c.env_theta = (T)in[3];
+      out3(out, env_color<T, T>(c, in3<T>(in)));
+      return 0;
+    }
+    case OR_PROBE_ENV_PDF: {
+      if (!need_scene() || s->efmt == PK_ENV_NONE) return -1;
+      c.env_theta = (T)in[3];
+      out[0] = (double)env_pdf<T>(c, in3<T>(in));
+      return 0;
+    }
+    case OR_PROBE_SAMPLE_ENV: {
+      if (!need_scene() || s->efmt == PK_ENV_NONE) return -1;
+      c.env_theta = (T)in[1];
+      Rng rng{(uint32_t)in[0]};
+      const M3<T> id{v3(T(1), T(0), T(0)), v3(T(0), T(1), T(0)), v3(T(0), T(0), T(1))};
+      const Sample<T> smp = sample_env<T>(c, id, rng);
+      out3(out, smp.wi);
+      out[3] = (double)smp.pdf;
+      return 0;
+    }
+    case OR_PROBE_TEXTURE: {
+      if (!need_scene() || s->tw == 0) return -1;
+      T r[4];
+      sample_tex<T>(s, (int)in[0], (T)in[1], (T)in[2], r);
+      for (int k = 0; k < 4; ++k) out[k] = (double)r[k];
+      return 0;
+    }
+    case OR_PROBE_PRIMARY_RAY: {
+      pk_camera cam{};
+      for (int k = 0; k < 3; ++k) {
+        cam.origin[k] = (float)in[k];
+        cam.dir[k] = (float)in[3 + k];
+      }
+      cam.fov = (float)in[6];
+      cam.focal_depth = (float)in[7];
+      cam.aperture = (float)in[8];
+      cam.distortion = (float)in[9];
+      cam.bokeh = (float)in[10];
+      const uint32_t px = (uint32_t)in[11], py = (uint32_t)in[12], W = (uint32_t)in[13], H = (uint32_t)in[14];
+      const uint32_t sample = (uint32_t)in[15];
+      // the camera's seed and discarded first hash (camera.wgsl:86-87)
+      Rng rng;
+      rng.seed = (px * 1973u + py * 9277u + sample * 26699u) | 1u;
+      rng.seed = rng.hash();
+      const Ray<T> r = primary_ray<T>(cam, px, py, W, H, rng);
+      out3(out, r.o);
+      out3(out + 3, r.d);
+      return 0;
+    }
+    case OR_PROBE_SURFACE: {
+      if (!need_scene()) return -1;
+      c.nodes = sizeof(T) == 4 ? &s->cnodes32 : &s->cnodes64;
+      const V3<T> o = in3<T>(in), d = in3<T>(in + 3);
+      const HitT<T> h = closest<T>(c, o, d);
+      for (int k = 0; k < OR_PROBE_SURFACE_OUT; ++k) out[k] = 0;
+      out[0] = h.id < 0 ? -1.0 : (double)h.t;
+      out[1] = (double)h.id;
+      if (h.id < 0) return 0;
+      EV<T> ev;
+      const SI<T> si = surface<T, T>(c, o, d, h, &ev);
+      out3(out + 2, si.normal);
+      out3(out + 5, si.origin);
+      out3(out + 8, si.wo);
+      out3(out + 11, si.basis.c0);
+      out3(out + 14, si.basis.c1);
+      out3(out + 17, si.basis.c2);
+      out[20] = (double)si.metallic;
+      out[21] = (double)si.opacity;
+      out[22] = (double)si.alpha;
+      out3(out + 23, si.emission);
+      out3(out + 26, ev.base);
+      out3(out + 29, ev.spec);
+      out[32] = (double)ev.alpha;
+      out[33] = (double)(h.bary & 0xffffu) / 65535.0;
+      out[34] = (double)(h.bary >> 16) / 65535.0;
+      return 0;
+    }
+    case OR_PROBE_ACCUMULATE: out[0] = (double)accumulate<T>((T)in[0], (float)in[1], (uint32_t)in[2]); return 0;
+    default: return -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C API
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int or_probe(const or_scene* s, int fn, int precision, const double* in, int64_t in_stride, int64_t n, double* out,
+             int64_t out_stride) {
+  if (fn < 0 || fn >= OR_PROBE_COUNT || n < 0) return -1;
+  if (fn == OR_PROBE_SURFACE && s && (s->instanced ? !s->built : s->cnodes64.empty())) return -1;
+  for (int64_t i = 0; i < n; ++i) {
+    const int rc = precision == 32 ? probe_one<float>(s, fn, in + i * in_stride, out + i * out_stride)
+                                   : probe_one<double>(s, fn, in + i * in_stride, out + i * out_stride);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+void or_set_mutation(int k) { g_mut = (k > 0 && k < OR_MUT_COUNT) ? k : 0; }
+int or_mutation_count(void) { return OR_MUT_COUNT; }
 
 or_scene* or_scene_create(const pk_scene_desc* d) {
   auto* s = new or_scene();

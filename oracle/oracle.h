@@ -89,6 +89,65 @@ int or_gradient(const or_scene* s, const pk_camera* cam, const or_render_params*
 /* pk_math contract probes for tests */
 void or_math_eval(int fn, const float* x, const float* y, float* out, int64_t n);
 
+/* Single functions of the shading / camera / accumulate restatement
+ * (shade_hit.wgsl:166-361, shade_miss.wgsl:37-43, camera.wgsl:58-73,
+ * accumulate.wgsl:26-30) evaluated on n inputs, for the closed-form tests.
+ * precision 64: the FP64 oracle, 32: the FP32 restatement.  Inputs per item
+ * (doubles, in_stride apart) -> outputs (out_stride apart):
+ *   GGX_D            m.xyz, alpha                                  -> D
+ *   GGX_G1           w.xyz, m.xyz, alpha                           -> G1(w, m)
+ *   LAMBERT_PDF      wi.xyz                                        -> pdf
+ *   SPECULAR_PDF     wo.xyz, m.xyz, alpha                          -> pdf of the reflected direction
+ *   EVAL_LAMBERT     base.rgb, wi.xyz, pdf                         -> rgb
+ *   EVAL_SPECULAR    wo.xyz, wi.xyz, pdf, alpha, spec.rgb          -> rgb
+ *   POWER_HEURISTIC  a, b                                          -> weight
+ *   SCHLICK          cos, ior                                      -> F
+ *   ONB              n.xyz                                         -> b1.xyz, b2.xyz, n.xyz (columns)
+ *   SAMPLE_GGX       wo.xyz, alpha, rng seed                       -> m.xyz
+ *   SAMPLE_LAMBERT   rng seed                                      -> wi.xyz, pdf
+ *   ENV_COLOR        dir.xyz, env_theta     (scene)                -> rgb
+ *   ENV_PDF          dir.xyz, env_theta     (scene)                -> pdf
+ *   SAMPLE_ENV       rng seed, env_theta    (scene; identity frame)-> wi.xyz, pdf
+ *   TEXTURE          layer, u, v            (scene; sampling atlas)-> rgba
+ *   PRIMARY_RAY      origin.xyz, dir.xyz, fov, focal depth, aperture, distortion, bokeh,
+ *                    px, py, W, H, sample                          -> origin.xyz, dir.xyz
+ *   SURFACE          o.xyz, d.xyz           (scene with BVH)       -> OR_PROBE_SURFACE_OUT values:
+ *                    [0] t (-1: miss), [1] id, [2..4] shading normal, [5..7] origin,
+ *                    [8..10] wo (local), [11..19] basis columns, [20] metallic,
+ *                    [21] opacity, [22] alpha, [23..25] emission, [26..28] base,
+ *                    [29..31] spec, [32] evaluation alpha, [33..34] barycentric u, v
+ *   ACCUMULATE       color, accumulator, sample index              -> new accumulator
+ * Returns 0, or -1 on an unknown function, a missing scene or bad input. */
+enum {
+  OR_PROBE_GGX_D = 0,
+  OR_PROBE_GGX_G1,
+  OR_PROBE_LAMBERT_PDF,
+  OR_PROBE_SPECULAR_PDF,
+  OR_PROBE_EVAL_LAMBERT,
+  OR_PROBE_EVAL_SPECULAR,
+  OR_PROBE_POWER_HEURISTIC,
+  OR_PROBE_SCHLICK,
+  OR_PROBE_ONB,
+  OR_PROBE_SAMPLE_GGX,
+  OR_PROBE_SAMPLE_LAMBERT,
+  OR_PROBE_ENV_COLOR,
+  OR_PROBE_ENV_PDF,
+  OR_PROBE_SAMPLE_ENV,
+  OR_PROBE_TEXTURE,
+  OR_PROBE_PRIMARY_RAY,
+  OR_PROBE_SURFACE,
+  OR_PROBE_ACCUMULATE,
+  OR_PROBE_COUNT
+};
+#define OR_PROBE_SURFACE_OUT 35
+int or_probe(const or_scene* s, int fn, int precision, const double* in, int64_t in_stride, int64_t n, double* out,
+             int64_t out_stride);
+/* Test-only planted mistakes (one wrong line of the restatement each, see
+ * OR_MUT_* in oracle.cpp): the pin tests show that each closed-form check
+ * fails while its mutation is planted.  0 restores the faithful code. */
+void or_set_mutation(int k);
+int or_mutation_count(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,10 @@ def load():
     lib.or_scene_write_params.argtypes = [vp, i32, i64, vp, i64, i32]
     lib.or_gradient.restype = i32
     lib.or_gradient.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp]
+    lib.or_probe.restype = i32
+    lib.or_probe.argtypes = [vp, i32, i32, vp, i64, i64, vp, i64]
+    lib.or_set_mutation.argtypes = [i32]
+    lib.or_mutation_count.restype = i32
     _lib = lib
     return lib
 
@@ -75,6 +79,40 @@ def math_eval(fn, x, y=None):
     out = np.empty_like(x)
     load().or_math_eval(MATH_FNS[fn], _p(x), _p(y), _p(out), x.size)
     return out
+
+
+PROBES = ["ggx_d", "ggx_g1", "lambert_pdf", "specular_pdf", "eval_lambert", "eval_specular", "power_heuristic",
+          "schlick", "onb", "sample_ggx", "sample_lambert", "env_color", "env_pdf", "sample_env", "texture",
+          "primary_ray", "surface", "accumulate"]
+PROBE_OUT = {"eval_lambert": 3, "eval_specular": 3, "onb": 9, "sample_ggx": 3, "sample_lambert": 4, "env_color": 3,
+             "sample_env": 4, "texture": 4, "primary_ray": 6, "surface": 35}
+
+
+def probe(fn, inputs, precision=64, scene=None):
+    """Evaluates one restated function (oracle.h or_probe) on each row of
+    `inputs`; returns [n, outputs] float64."""
+    x = np.ascontiguousarray(np.atleast_2d(np.asarray(inputs, np.float64)))
+    nout = PROBE_OUT.get(fn, 1)
+    out = np.zeros((x.shape[0], nout), np.float64)
+    h = scene.h if scene is not None else None
+    rc = load().or_probe(h, PROBES.index(fn), precision, _p(x), x.shape[1], x.shape[0], _p(out), nout)
+    if rc != 0:
+        raise ValueError(f"or_probe({fn}) failed")
+    return out
+
+
+MUTATIONS = ["g1_swap", "lambert_inv_pi", "onb_sign", "jitter_order", "d_pi", "schlick_pow", "ph_balance",
+             "lambert_sample", "vndf_blend", "env_pdf_sin", "env_center", "acc_denom", "normal_map", "origin_sign",
+             "aspect", "spec_pdf4", "env_color_v", "tex_half", "emission", "nee_spec_weight", "bounce_weight",
+             "eval_spec_cos", "sample_env_sin", "metal_channel", "alpha_square", "distortion", "g1_sign",
+             "fresnel_metal", "theta_sign"]
+
+
+def set_mutation(name):
+    """Plants one of the test-only mistakes of oracle.cpp (None: none)."""
+    lib = load()
+    assert lib.or_mutation_count() == len(MUTATIONS) + 1
+    lib.or_set_mutation(0 if name is None else MUTATIONS.index(name) + 1)
 
 
 class OracleScene:
