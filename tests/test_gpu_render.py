@@ -106,11 +106,13 @@ def test_empty_scene_region_is_env(ctx):
     np.testing.assert_array_equal(img[..., :3], np.ones_like(img[..., :3]))
 
 
-@pytest.mark.parametrize("samples,frames", [(4, 8), (3, 7), (16, 5)])
+@pytest.mark.parametrize("samples,frames", [(4, 8), (3, 7), (16, 5), (40, 45), (64, 70)])
 def test_samples_per_pass_equals_one_sample_frames(ctx, samples, frames):
     """A pass of S samples per pixel (one wavefront of W*H*S paths) folds its
     samples in order: the image equals one-sample frames bit for bit, also
-    when the last pass is partial, and equals the FP32 oracle."""
+    when the last pass is partial, and equals the FP32 oracle.  S above 32
+    makes k_accumulate fold more than one LDS chunk of 32 samples per pixel,
+    with a partial last chunk."""
     env = scenes.lognormal_sun_env(64, 32, seed=5, sigma=1.0, sun_deg=6.0, sun_scale=30.0)
     data = scenes.mixed_scene(7, 4000, env=env)
     data.camera = scenes.camera((0.5, 0.5, 1.4), (0.5, 0.05, 0.45), 1.0)

@@ -25,7 +25,8 @@ def _render(ctx, data, bvh, sched, W, H, depth, spp, per_pass, first=0, graph=Tr
     return out
 
 
-@pytest.mark.parametrize("depth,spp,per_pass,graph", [(1, 3, 1, False), (4, 8, 4, True), (8, 6, 4, True)])
+@pytest.mark.parametrize("depth,spp,per_pass,graph", [(1, 3, 1, False), (4, 8, 4, True), (8, 6, 4, True),
+                                                     (4, 45, 40, True)])
 def test_paths_scheduler_is_bit_identical(ctx, depth, spp, per_pass, graph):
     data = scenes.config1(seed=5)
     bvh = pike_amd.BVH(data.positions, data.triangles)
