@@ -136,6 +136,22 @@ int pk_sync(pk_ctx* ctx);
 int pk_ctx_set_tuning(pk_ctx* ctx, int kernel_class, uint32_t waves_per_cu, uint32_t refill_min);
 int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_cu, uint32_t* refill_min,
                       uint32_t* num_cus);
+/* Traversal stack entries per ray, in [1, 64] (default 64: 12 in LDS, the
+ * rest in a per-wave global spill area).  The reference's stack holds 32
+ * entries and has no overflow check (trace.wgsl:5-9, 134-146).  A ray that
+ * needs more entries than the limit skips the subtrees it cannot push; the
+ * kernels end normally and report it: pk_read_counters().stack_overflow,
+ * PK_E_OVERFLOW from pk_trace_rays / pk_ctx_trace_stats.  Lower limits cap
+ * the work per ray (and let tests force the overflow path). */
+int pk_ctx_set_stack_limit(pk_ctx* ctx, uint32_t entries);
+int pk_ctx_get_stack_limit(const pk_ctx* ctx, uint32_t* entries);
+/* Device memory cap of the context's scenes, frames and their scratch (no
+ * reference counterpart: WebGPU reports allocation failure through its
+ * validation errors only).  An allocation that would pass `bytes` fails with
+ * PK_E_OOM and leaves the call's objects unchanged; 0 removes the cap.
+ * pk_ctx_memory_used gives the bytes currently allocated. */
+int pk_ctx_set_memory_limit(pk_ctx* ctx, uint64_t bytes);
+int pk_ctx_memory_used(const pk_ctx* ctx, uint64_t* bytes);
 /* Pass scheduler of frames (no reference counterpart: the
  * reference always runs its passes as wavefront compute dispatches,
  * render/pike.js:135-155).  PK_SCHED_WAVEFRONT: raygen, then per bounce

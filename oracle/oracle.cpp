@@ -1505,16 +1505,17 @@ void render_pixel(const Ctx<T>& c, const pk_camera& cam, uint32_t px, uint32_t p
   }
 }
 
-// accumulate.wgsl:26-30: acc = (max(c, 0) + acc * s) / (s + 1), s = the
+// accumulate.wgsl:26-31: acc = (max(c, 0) + acc * s) / (s + 1), s = the
 // frame's sample index; the accumulator is an f32 buffer (FP64 evaluates the
 // update in double and rounds once).
 template <class T> float accumulate(T c, float acc, uint32_t sample) {
   const uint32_t den = sample + (mut(OR_MUT_ACC_DENOM) ? 2u : 1u);
+  // max(color, 0) with the GPU max's NaN rule: a NaN color adds 0
   if (sizeof(T) == 4) {
-    const float col = mx((float)c, 0.0f);
+    const float col = (float)c > 0.0f ? (float)c : 0.0f;
     return (col + acc * (float)sample) / (float)den;
   }
-  const double col = mx((double)c, 0.0);
+  const double col = (double)c > 0.0 ? (double)c : 0.0;
   return (float)((col + (double)acc * (double)sample) / (double)den);
 }
 
@@ -1596,12 +1597,16 @@ int gradient_t(const or_scene* s, const pk_camera* cam, const or_render_params* 
       for (uint32_t f = 0; f < p->num_frames; ++f) {
         PixelOut<D> o;
         render_pixel<T, D>(c, *cam, px, py, W, H, p->first_sample + f, p->max_depth, o);
+        // accumulate drops a NaN channel of a sample (adds 0), so it
+        // contributes neither value nor gradient
+        double gk[3];
+        for (int ch = 0; ch < 3; ++ch) gk[ch] = o.c[ch].v == o.c[ch].v ? (double)g[ch] : 0.0;
         for (int k = 0; k < cnt; ++k)
-          part[(size_t)i * cnt + k] += ((double)g[0] * (double)o.c[0].g[k] + (double)g[1] * (double)o.c[1].g[k] +
-                                        (double)g[2] * (double)o.c[2].g[k]) *
+          part[(size_t)i * cnt + k] += (gk[0] * (double)o.c[0].g[k] + gk[1] * (double)o.c[1].g[k] +
+                                        gk[2] * (double)o.c[2].g[k]) *
                                        inv_n;
-        val[i] += ((double)g[0] * (double)o.c[0].v + (double)g[1] * (double)o.c[1].v +
-                   (double)g[2] * (double)o.c[2].v) *
+        val[i] += ((gk[0] != 0.0 ? gk[0] * (double)o.c[0].v : 0.0) + (gk[1] != 0.0 ? gk[1] * (double)o.c[1].v : 0.0) +
+                   (gk[2] != 0.0 ? gk[2] * (double)o.c[2].v : 0.0)) *
                   inv_n;
       }
     });

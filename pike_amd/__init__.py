@@ -91,6 +91,25 @@ class Context:
             out[name] = dict(waves_per_cu=w.value, refill_min=r.value)
         return out
 
+    def set_stack_limit(self, entries):
+        """Traversal stack entries per ray (1..64, default 64); rays needing
+        more are flagged as overflowed and skip what they cannot push."""
+        _check(lib().pk_ctx_set_stack_limit(self.h, entries), "pk_ctx_set_stack_limit")
+
+    def stack_limit(self):
+        v = ctypes.c_uint32()
+        _check(lib().pk_ctx_get_stack_limit(self.h, ctypes.byref(v)), "pk_ctx_get_stack_limit")
+        return v.value
+
+    def set_memory_limit(self, nbytes):
+        """Cap on the device memory of this context's scenes and frames (0: none)."""
+        _check(lib().pk_ctx_set_memory_limit(self.h, int(nbytes)), "pk_ctx_set_memory_limit")
+
+    def memory_used(self):
+        v = ctypes.c_uint64()
+        _check(lib().pk_ctx_memory_used(self.h, ctypes.byref(v)), "pk_ctx_memory_used")
+        return v.value
+
     def autotune_rays(self, scene, rays_ptr, n, any_hit, ids_ptr, t_ptr, bary_ptr):
         ms = (ctypes.c_float * 2)()
         _check(lib().pk_autotune_rays(self.h, scene.h, rays_ptr, n, int(any_hit), ids_ptr, t_ptr, bary_ptr, ms),

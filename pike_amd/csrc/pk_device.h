@@ -44,6 +44,9 @@ struct DevScene {
   const uint32_t* inst_leaf_base;  // ascending, one per instance
   uint32_t num_inst;               // 0: a single-level BVH
   uint32_t world_leaves;           // leaves below this are the scene's own triangles
+  // traversal stack entries per ray (<= kStackMax), set per launch from the
+  // context; a ray that needs more flags an overflow
+  uint32_t stack_limit;
 };
 
 struct Hit {
@@ -102,6 +105,7 @@ struct Stack {
   float* lds_t;         // [kStackLds][64]
   uint2* spill;         // [kSpillPerLane][64] for this wave
   int lane;
+  int limit;            // entries a ray may hold, in [1, kStackMax]
 };
 
 // State of one ray's traversal, kept in registers between steps.  With
@@ -163,17 +167,19 @@ __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
   r.pmask = 0;
 }
 
+// A push beyond the limit drops the entry (its subtree is not visited) and
+// flags the ray; the traversal still ends normally.
 __device__ inline void stack_push(const Stack& stk, RayState& r, uint32_t node, float t) {
   const int lane = stk.lane;
-  if (r.sp < kStackLds) {
+  if (r.sp >= stk.limit) {
+    r.overflow = true;
+  } else if (r.sp < kStackLds) {
     stk.lds_node[r.sp * 64 + lane] = node;
     stk.lds_t[r.sp * 64 + lane] = t;
     ++r.sp;
-  } else if (r.sp < kStackMax) {
+  } else {
     stk.spill[(r.sp - kStackLds) * 64 + lane] = make_uint2(node, __float_as_uint(t));
     ++r.sp;
-  } else {
-    r.overflow = true;
   }
 }
 
@@ -298,7 +304,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
       return r.sp == 0;
     }
     if (r.node & kInstChild) {
-      if (r.sp >= kStackMax) {  // no room for the way back: skip it, flagged
+      if (r.sp >= stk.limit) {  // no room for the way back: skip it, flagged
         r.overflow = true;
         r.have = false;
         return r.sp == 0;

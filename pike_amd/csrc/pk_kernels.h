@@ -108,6 +108,7 @@ struct TraceLaunch {
   uint32_t refill;
   bool count;
   uint2* spill;  // grid * kSpillPerLane * 64 entries
+  uint32_t stack_limit;  // traversal stack entries per ray, in [1, kStackMax]
 };
 
 void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, const TraceLaunch& L, hipStream_t s);
@@ -138,7 +139,7 @@ struct AdjointArgs {
   float* vtx;             // grid * depth * kAdjVtxWords * 64 floats
   uint32_t* overflow;
 };
-void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, hipStream_t s);
+void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, uint32_t stack_limit, hipStream_t s);
 void launch_loss_l2(const float4* img, const float4* target, float4* grad, double* loss, uint32_t n, hipStream_t s);
 void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats, uint32_t* overflow,

@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevSce
   }
   const uint32_t n = F.cnt[ray_counter(in)];
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
   const float4* qo = F.qo[in];
@@ -821,7 +821,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScen
   const uint32_t n = F.cnt[kCntShadow];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, overflow = 0, pid = 0;
   trace_loop<true, kCount, kInst>(
@@ -877,7 +877,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
   const FrameParams P = *F.params;
   const uint32_t ns = kBatch ? F.S : pass_samples(F, P);
   const uint32_t B = F.W * F.H;
@@ -1073,9 +1073,9 @@ __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
         const float sm = (float)(P.sample + c0 + k);
         const float s1 = (float)(P.sample + c0 + k + 1u);
         const float4 c = tile[t * kAccChunk + k];
-        a.x = (pkm::max_(c.x, 0.0f) + a.x * sm) / s1;
-        a.y = (pkm::max_(c.y, 0.0f) + a.y * sm) / s1;
-        a.z = (pkm::max_(c.z, 0.0f) + a.z * sm) / s1;
+        a.x = (pkm::max0_(c.x) + a.x * sm) / s1;
+        a.y = (pkm::max0_(c.y) + a.y * sm) / s1;
+        a.z = (pkm::max0_(c.z) + a.z * sm) / s1;
       }
     }
     __syncthreads();
@@ -1133,6 +1133,10 @@ constexpr int kScatterPeel = 4;
 template <int C>
 __device__ inline void wave_scatter_add(float* dst, uint32_t key, bool valid, const float* v, int c0) {
   const int lane = (int)__lane_id();
+  // a path that turned non-finite (its sample is dropped by accumulate's
+  // NaN rule) scatters nothing from the point where it did
+#pragma unroll
+  for (int c = 0; c < C; ++c) valid = valid && pkm::abs_(v[c]) < __builtin_inff();
   unsigned long long pending = __ballot(valid);
   for (int round = 0; pending; ++round) {
     const bool mine_pending = (pending >> lane) & 1ull;
@@ -1184,7 +1188,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
   float* const vtx = A.vtx + (size_t)blockIdx.x * A.depth * kAdjVtxWords * 64 + lane;
   const uint32_t B = A.W * A.H;
   const uint32_t total = B * A.num_frames;
@@ -1405,7 +1409,7 @@ __global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_rays(DevScene 
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, spill_all + (size_t)blockIdx.x * kSpillPerLane * 64, lane};
+  const Stack stk{lds_node, lds_t, spill_all + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
   trace_loop<kAnyHit, kCount, kInst>(
@@ -1452,7 +1456,15 @@ void launch_raygen(const FrameDev& F, hipStream_t s) {
     hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, F);
   }
 }
-void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
+// the scene as one launch sees it: the context's stack limit stamped in
+static inline DevScene with_limit(const DevScene& S0, uint32_t stack_limit) {
+  DevScene S = S0;
+  S.stack_limit = stack_limit;
+  return S;
+}
+
+void launch_trace_closest(const DevScene& S0, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
+  const DevScene S = with_limit(S0, L.stack_limit);
   FrameDev F = F0;
   F.spill = L.spill;
   const bool inst = S.num_inst != 0;
@@ -1464,7 +1476,8 @@ void launch_trace_closest(const DevScene& S, const FrameDev& F0, int bounce, con
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
   hipLaunchKernelGGL(k_shade, dim3(cdiv(F.hit_cap, kShadeWaves * 64)), dim3(kShadeWaves * 64), 0, s, S, F, bounce);
 }
-void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
+void launch_trace_shadow(const DevScene& S0, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
+  const DevScene S = with_limit(S0, L.stack_limit);
   FrameDev F = F0;
   F.spill = L.spill;
   const bool inst = S.num_inst != 0;
@@ -1473,7 +1486,9 @@ void launch_trace_shadow(const DevScene& S, const FrameDev& F0, const TraceLaunc
   else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
   else hipLaunchKernelGGL((k_trace_shadow<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
 }
-void launch_render_paths(const DevScene& S, const FrameDev& F0, uint32_t depth, const TraceLaunch& L, hipStream_t s) {
+void launch_render_paths(const DevScene& S0, const FrameDev& F0, uint32_t depth, const TraceLaunch& L,
+                         hipStream_t s) {
+  const DevScene S = with_limit(S0, L.stack_limit);
   FrameDev F = F0;
   F.spill = L.spill;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, F.cnt + kCntWorkTrace, 0u, 1u);
@@ -1494,16 +1509,18 @@ void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s) {
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_copy_f4, dim3(min(cdiv(n, 256), 8192u)), dim3(256), 0, s, dst, src, n);
 }
-void launch_adjoint(const DevScene& S, const AdjointArgs& A, uint32_t grid, hipStream_t s) {
+void launch_adjoint(const DevScene& S0, const AdjointArgs& A, uint32_t grid, uint32_t stack_limit, hipStream_t s) {
+  const DevScene S = with_limit(S0, stack_limit);
   if (S.num_inst) hipLaunchKernelGGL(k_adjoint<true>, dim3(grid), dim3(64), 0, s, S, A);
   else hipLaunchKernelGGL(k_adjoint<false>, dim3(grid), dim3(64), 0, s, S, A);
 }
 void launch_loss_l2(const float4* img, const float4* target, float4* grad, double* loss, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_loss_l2, dim3(cdiv(n, 256)), dim3(256), 0, s, img, target, grad, loss, n);
 }
-void launch_trace_rays(const DevScene& S, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
+void launch_trace_rays(const DevScene& S0, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,
                        uint32_t* overflow, const TraceLaunch& L, hipStream_t s) {
+  const DevScene S = with_limit(S0, L.stack_limit);
 #define PK_LAUNCH_RAYS(A, C, I)                                                                                    \
   hipLaunchKernelGGL((k_trace_rays<A, C, I>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor, spill, \
                      stats, overflow, L.refill)

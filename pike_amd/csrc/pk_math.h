@@ -41,6 +41,11 @@ PK_HD float abs_(float x) { return u2f(f2u(x) & 0x7fffffffu); }
 PK_HD float min_(float a, float b) { return b < a ? b : a; }
 PK_HD float max_(float a, float b) { return a < b ? b : a; }
 PK_HD float clamp_(float x, float lo, float hi) { return min_(max_(x, lo), hi); }
+// accumulate.wgsl:29's max(color, 0) with the GPU max's NaN rule (the
+// operand that is not a NaN): a sample whose color is NaN adds 0 instead of
+// poisoning the running mean.  Equal to max_(c, 0) for every other c (up to
+// the sign of a zero, which the running-mean update does not see).
+PK_HD float max0_(float c) { return c > 0.0f ? c : 0.0f; }
 PK_HD float copysign_(float m, float s) { return u2f((f2u(m) & 0x7fffffffu) | (f2u(s) & 0x80000000u)); }
 
 constexpr float kPi = 0x1.921fb6p+1f;        // 3.14159274

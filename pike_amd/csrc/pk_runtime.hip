@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "pk_internal.h"
@@ -35,6 +36,11 @@ struct pk_ctx {
     uint32_t refill;  // idle lanes that trigger a wave-level ray refill
   } shape[PK_TUNE_CLASSES];
   uint32_t tuning_gen = 0;  // bumped by pk_ctx_set_tuning; captured graphs keep the old shape
+  uint32_t stack_limit = pk::kStackMax;  // traversal stack entries per ray
+  // device memory of this context's scenes, frames and their scratch:
+  // bytes in use, the cap (0: none) and each allocation's size
+  size_t mem_used = 0, mem_cap = 0;
+  std::unordered_map<void*, size_t> mem_sizes;
   int sched = PK_SCHED_WAVEFRONT;
   uint32_t* cursors = nullptr;  // per-launch work cursors for ray queries
   uint32_t ncursors = 0;
@@ -44,10 +50,17 @@ struct pk_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+// Device allocations of one owner (a scene, a frame, one call's
+// temporaries), counted against the owning context's memory cap.
+struct AllocList {
+  pk_ctx* ctx = nullptr;
+  std::vector<void*> ptrs;
+};
+
 struct pk_scene {
   pk_ctx* ctx = nullptr;
   pk::DevScene dev{};
-  std::vector<void*> allocs;
+  AllocList allocs;
   int64_t env_info[4] = {0, 0, 0, 0};
   std::vector<uint32_t> roots{0};  // wide-BVH root of each merged sub-scene
   float* tex_eval = nullptr;       // evaluation atlas, allocated by the first parameter write
@@ -58,8 +71,8 @@ struct pk_frame {
   pk_ctx* ctx = nullptr;
   uint32_t W = 0, H = 0;
   pk::FrameDev dev{};
-  std::vector<void*> allocs;
-  std::vector<void*> path_allocs;  // per-path queues, sized W * H * samples per pass
+  AllocList allocs;
+  AllocList path_allocs;  // per-path queues, sized W * H * samples per pass
   pk_camera cam{};
   bool has_camera = false;
   float theta = 0;
@@ -101,34 +114,56 @@ static_assert(kAdjointWavesPerCu <= kMaxWavesPerCu, "adjoint grid exceeds the sp
 static_assert(4 * pk::kMegaWavesPerSimd <= kMaxWavesPerCu, "path megakernel grid exceeds the spill area");
 
 pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
-  return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill};
+  return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill,
+                         c->stack_limit};
 }
 pk::TraceLaunch trace_launch(const pk_frame* f, int cls) { return shape_launch(f->ctx, cls, f->count_nodes); }
 int closest_class(uint32_t bounce) { return bounce == 0 ? PK_TUNE_PRIMARY : PK_TUNE_SECONDARY; }
 
-template <class T> int dev_alloc(std::vector<void*>& list, T** p, size_t count) {
+template <class T> int dev_alloc(AllocList& list, T** p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    pk_set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes failed: " +
-                 hipGetErrorString(e));
+  const size_t bytes = count * sizeof(T);
+  pk_ctx* c = list.ctx;
+  if (c && c->mem_cap && c->mem_used + bytes > c->mem_cap) {
+    pk_set_error("device memory cap of " + std::to_string(c->mem_cap) + " bytes reached: " +
+                 std::to_string(c->mem_used) + " in use, " + std::to_string(bytes) + " requested");
     return PK_E_OOM;
   }
-  list.push_back(*p);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    pk_set_error(std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    *p = nullptr;
+    return PK_E_OOM;
+  }
+  list.ptrs.push_back(*p);
+  if (c) {
+    c->mem_used += bytes;
+    c->mem_sizes[*p] = bytes;
+  }
   return PK_OK;
 }
 
-template <class T> int upload(std::vector<void*>& list, T** p, const void* src, size_t count, hipStream_t s) {
+template <class T> int upload(AllocList& list, T** p, const void* src, size_t count, hipStream_t s) {
   int rc = dev_alloc(list, p, count);
   if (rc) return rc;
   if (src && count) PK_HIP(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
   return PK_OK;
 }
 
-void free_all(std::vector<void*>& list) {
-  for (void* p : list) (void)hipFree(p);
-  list.clear();
+void free_all(AllocList& list) {
+  pk_ctx* c = list.ctx;
+  for (void* p : list.ptrs) {
+    (void)hipFree(p);
+    if (c) {
+      auto it = c->mem_sizes.find(p);
+      if (it != c->mem_sizes.end()) {
+        c->mem_used -= it->second;
+        c->mem_sizes.erase(it);
+      }
+    }
+  }
+  list.ptrs.clear();
 }
 
 int ensure_cursors(pk_ctx* c, uint32_t n) {
@@ -271,6 +306,44 @@ int pk_ctx_get_tuning(const pk_ctx* c, int cls, uint32_t* waves_per_cu, uint32_t
   return PK_OK;
 }
 
+int pk_ctx_set_stack_limit(pk_ctx* c, uint32_t entries) {
+  if (!c || entries < 1 || entries > (uint32_t)pk::kStackMax) {
+    pk_set_error("pk_ctx_set_stack_limit: null context or entries outside [1, " + std::to_string(pk::kStackMax) +
+                 "]");
+    return PK_E_INVALID;
+  }
+  if (entries != c->stack_limit) ++c->tuning_gen;  // captured graphs hold the old limit
+  c->stack_limit = entries;
+  return PK_OK;
+}
+
+int pk_ctx_get_stack_limit(const pk_ctx* c, uint32_t* entries) {
+  if (!c || !entries) {
+    pk_set_error("pk_ctx_get_stack_limit: null argument");
+    return PK_E_INVALID;
+  }
+  *entries = c->stack_limit;
+  return PK_OK;
+}
+
+int pk_ctx_set_memory_limit(pk_ctx* c, uint64_t bytes) {
+  if (!c) {
+    pk_set_error("pk_ctx_set_memory_limit: null context");
+    return PK_E_INVALID;
+  }
+  c->mem_cap = (size_t)bytes;
+  return PK_OK;
+}
+
+int pk_ctx_memory_used(const pk_ctx* c, uint64_t* bytes) {
+  if (!c || !bytes) {
+    pk_set_error("pk_ctx_memory_used: null argument");
+    return PK_E_INVALID;
+  }
+  *bytes = (uint64_t)c->mem_used;
+  return PK_OK;
+}
+
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
@@ -396,6 +469,7 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
   PK_HIP(hipSetDevice(c->device));
   auto* s = new pk_scene();
   s->ctx = c;
+  s->allocs.ctx = c;
   pk::DevScene& D = s->dev;
   hipStream_t st = c->stream;
   auto fail = [&](int code) {
@@ -545,28 +619,33 @@ int pk_scene_env_info(const pk_scene* s, int64_t* info4) {
 // frame
 // ---------------------------------------------------------------------------
 // Per-path queue buffers for `paths` paths (replacing any previous ones).
-static int alloc_paths(pk_frame* f, size_t paths) {
-  free_all(f->path_allocs);
-  pk::FrameDev& F = f->dev;
-  int rc;
+// Allocates per-path queues for `paths` paths into `list` and the pointers
+// of F (the frame's current queues are not touched); on failure everything
+// allocated here is freed again.
+static int alloc_paths_into(pk_frame* f, size_t paths, AllocList& list, pk::FrameDev& F) {
+  int rc = PK_OK;
+  auto fail = [&](int code) {
+    free_all(list);
+    return code;
+  };
   for (int q = 0; q < 2; ++q) {
-    if ((rc = dev_alloc(f->path_allocs, &F.qo[q], paths))) return rc;
-    if ((rc = dev_alloc(f->path_allocs, &F.qd[q], paths))) return rc;
+    if ((rc = dev_alloc(list, &F.qo[q], paths))) return fail(rc);
+    if ((rc = dev_alloc(list, &F.qd[q], paths))) return fail(rc);
   }
-  if ((rc = dev_alloc(f->path_allocs, &F.thr, paths))) return rc;
+  if ((rc = dev_alloc(list, &F.thr, paths))) return fail(rc);
   // the hit queue also holds the holes at the end of each trace_closest
   // wave's last run (fewer than kHitRun per wave of the largest grid)
   const size_t hit_cap = paths + (size_t)f->ctx->num_cus * kMaxWavesPerCu * pk::kHitRun;
   if (hit_cap >= (1ull << 32)) {
     pk_set_error("frame: hit queue beyond 2^32 records");
-    return PK_E_LIMIT;
+    return fail(PK_E_LIMIT);
   }
-  if ((rc = dev_alloc(f->path_allocs, &F.hits, hit_cap))) return rc;
+  if ((rc = dev_alloc(list, &F.hits, hit_cap))) return fail(rc);
   F.hit_cap = (uint32_t)hit_cap;
-  if ((rc = dev_alloc(f->path_allocs, &F.so, paths))) return rc;
-  if ((rc = dev_alloc(f->path_allocs, &F.sd, paths))) return rc;
-  if ((rc = dev_alloc(f->path_allocs, &F.sthr, paths))) return rc;
-  if ((rc = dev_alloc(f->path_allocs, &F.color, paths))) return rc;
+  if ((rc = dev_alloc(list, &F.so, paths))) return fail(rc);
+  if ((rc = dev_alloc(list, &F.sd, paths))) return fail(rc);
+  if ((rc = dev_alloc(list, &F.sthr, paths))) return fail(rc);
+  if ((rc = dev_alloc(list, &F.color, paths))) return fail(rc);
   pk::launch_fill_f4(F.color, make_float4(0, 0, 0, 0), (uint32_t)paths, f->ctx->stream);
   return PK_OK;
 }
@@ -584,6 +663,8 @@ int pk_frame_create(pk_ctx* c, uint32_t W, uint32_t H, pk_frame** out) {
   PK_HIP(hipSetDevice(c->device));
   auto* f = new pk_frame();
   f->ctx = c;
+  f->allocs.ctx = c;
+  f->path_allocs.ctx = c;
   f->W = W;
   f->H = H;
   const size_t B = (size_t)W * H;
@@ -598,7 +679,7 @@ int pk_frame_create(pk_ctx* c, uint32_t W, uint32_t H, pk_frame** out) {
     return code;
   };
   F.S = 1;
-  if ((rc = alloc_paths(f, B))) return fail(rc);
+  if ((rc = alloc_paths_into(f, B, f->path_allocs, F))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.accum, B))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.cnt, pk::kNumCounters))) return fail(rc);
   if ((rc = dev_alloc(f->allocs, &F.stats, pk::kNumStats))) return fail(rc);
@@ -657,15 +738,17 @@ int pk_frame_set_samples_per_pass(pk_frame* f, uint32_t samples) {
   if (samples == f->dev.S) return PK_OK;
   PK_HIP(hipSetDevice(f->ctx->device));
   PK_HIP(hipStreamSynchronize(f->ctx->stream));
+  // the new queues first: on failure the frame keeps its queues and its
+  // samples per pass
+  AllocList fresh{f->ctx, {}};
+  pk::FrameDev F = f->dev;
+  int rc = alloc_paths_into(f, (size_t)paths, fresh, F);
+  if (rc) return rc;
   drop_graph(f);
-  f->dev.S = samples;
-  int rc = alloc_paths(f, (size_t)paths);
-  if (rc) {
-    // keep the frame usable: fall back to one sample per pass
-    f->dev.S = 1;
-    int rc1 = alloc_paths(f, (size_t)f->W * f->H);
-    return rc1 ? rc1 : rc;
-  }
+  free_all(f->path_allocs);
+  f->path_allocs = std::move(fresh);
+  F.S = samples;
+  f->dev = F;
   PK_HIP(hipStreamSynchronize(f->ctx->stream));
   return PK_OK;
 }
@@ -980,7 +1063,7 @@ int pk_trace_rays(pk_ctx* c, pk_scene* s, const float* rays, int64_t n, int any_
     return PK_E_INVALID;
   }
   PK_HIP(hipSetDevice(c->device));
-  std::vector<void*> tmp;
+  AllocList tmp{c, {}};
   float4* d_rays;
   int32_t* d_ids;
   float* d_t;
@@ -1501,7 +1584,7 @@ int pk_adjoint(pk_frame* f, pk_scene* s, const pk_render_params* p, const float*
     f->adj_vtx_n = nvtx;
   }
   A.vtx = f->adj_vtx;
-  pk::launch_adjoint(s->dev, A, grid, st);
+  pk::launch_adjoint(s->dev, A, grid, f->ctx->stack_limit, st);
   PK_HIP(hipGetLastError());
   if (!g->on_device) {
     PK_HIP(hipStreamSynchronize(st));
