@@ -683,7 +683,7 @@ def main():
         extra["batching_32_scenes_64_views"] = run_batching(args, pike_amd, scenes, bctx)
     if not args.no_diff:
         extra["differentiation_c5"] = run_diff(args, pike_amd, scenes)
-    roof = {k: res["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
+    roof = {k: v for k, v in res["roofline"].items() if k not in ("bytes_per_launch", "launch_ms")}
     line = {
         "metric": "spp/s at 1920x1080 (plain wavefront, depth 8)",
         "value": res["spp_s"],
