@@ -356,6 +356,20 @@ int pk_trace_rays(pk_ctx* ctx, pk_scene* scene, const float* rays, int64_t n, in
  * per-ray node/triangle totals to the context's traversal counters. */
 int pk_trace_rays_device(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int64_t n, int any_hit,
                          int32_t* d_ids, float* d_t, uint32_t* d_bary, int count_nodes);
+/* Allocation-free variant of pk_trace_rays_device (no counterpart in the
+ * reference).  It is for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The kernels' work cursor and traversal stack spill area live in
+ * a file-scope device array of the library (54.5 MB per device, held until
+ * the process ends) instead of context memory; the call allocates nothing,
+ * does not block and launches on the scene's context stream.  The launch
+ * uses the context's ray-query shape with at most 2048 one-wave blocks (the
+ * stacks the array holds).  Same limits (n < 2^31), results and overflow
+ * reporting (pk_ctx_trace_stats) as pk_trace_rays_device, without node
+ * counting.  Calls that overlap on one device share that array and race:
+ * order them on one stream. */
+int pk_trace_rays_noalloc(pk_scene* scene, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids, float* d_t,
+                          uint32_t* d_bary);
 /* Traversal statistics of the last counted pk_trace_rays_device call:
  * [0] nodes visited, [1] triangles tested. */
 int pk_ctx_trace_stats(pk_ctx* ctx, uint64_t* stats2);

@@ -148,6 +148,7 @@ SIGNATURES = {
     "pk_ctx_get_scheduler": (c_int, [c_void_p, ctypes.POINTER(c_int)]),
     "pk_ctx_get_tuning": (c_int, [c_void_p, c_int, ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
                                   ctypes.POINTER(c_uint32)]),
+    "pk_trace_rays_noalloc": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]),
     "pk_ctx_set_stack_limit": (c_int, [c_void_p, c_uint32]),
     "pk_ctx_get_stack_limit": (c_int, [c_void_p, ctypes.POINTER(c_uint32)]),
     "pk_ctx_set_memory_limit": (c_int, [c_void_p, ctypes.c_uint64]),

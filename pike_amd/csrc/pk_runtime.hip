@@ -3,8 +3,10 @@
 // through one captured hipGraph) and standalone ray queries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -14,7 +16,43 @@
 
 namespace {
 thread_local std::string g_err;
+
+// Scratch of pk_trace_rays_noalloc: the work cursor (first 256 bytes) and
+// the traversal stack spill area of up to kNoAllocMaxWaves one-wave blocks
+// (8 per CU on a 256-CU MI355X), 54.5 MB, held by every device the library
+// runs on until the process ends.  Host code never uses the array's name as
+// an address: pk_ctx_create resolves each device's copy once with
+// hipGetSymbolAddress, and only that address is passed to kernels.
+constexpr uint32_t kNoAllocMaxWaves = 2048;
+constexpr size_t kNoAllocCursorBytes = 256;
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocCursorBytes + (size_t)kNoAllocMaxWaves * pk::kSpillPerLane * 64 * sizeof(uint2);
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), resolved once.
+int scratch_address(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    g_err = "device index beyond the scratch table";
+    return PK_E_LIMIT;
+  }
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+      return PK_E_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return PK_OK;
 }
+}  // namespace
 
 void pk_set_error(const std::string& msg) { g_err = msg; }
 
@@ -48,6 +86,7 @@ struct pk_ctx {
   unsigned long long* stats = nullptr;
   uint32_t* overflow = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  unsigned char* scratch = nullptr;  // this device's pk_trace_rays_noalloc scratch
 };
 
 // Device allocations of one owner (a scene, a frame, one call's
@@ -228,6 +267,7 @@ int pk_ctx_create(int device, pk_ctx** out) {
     return PK_E_HIP;
   }
   int rc = ensure_cursors(c, 64);
+  if (!rc) rc = scratch_address(device, &c->scratch);
   if (rc) {
     pk_ctx_destroy(c);
     return rc;
@@ -1032,6 +1072,46 @@ int pk_trace_rays_device(pk_ctx* c, pk_scene* s, const float* d_rays, int64_t n,
   pk::launch_fill_u32((uint32_t*)c->stats, 0u, 4, c->stream);
   pk::launch_fill_u32(c->overflow, 0u, 1, c->stream);
   return trace_rays_launch(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, count_nodes, 0);
+}
+
+// pk_trace_rays_device with its work cursor and traversal stack spill area
+// in the file-scope scratch array (this device's copy, resolved by
+// pk_ctx_create) instead of context memory: no allocation, no blocking call.
+// The grid is capped at the kNoAllocMaxWaves one-wave blocks the scratch
+// holds stacks for.  Overflow is reported through pk_ctx_trace_stats, as
+// after pk_trace_rays_device.  Calls that overlap on one device race on the
+// shared scratch.
+int pk_trace_rays_noalloc(pk_scene* s, const float* d_rays, int64_t n, int any_hit, int32_t* d_ids, float* d_t,
+                          uint32_t* d_bary) {
+  if (!s) {
+    pk_set_error("pk_trace_rays_noalloc: null scene");
+    return PK_E_INVALID;
+  }
+  pk_ctx* c = s->ctx;
+  int rc = check_rays(c, s, n, "pk_trace_rays_noalloc");
+  if (rc) return rc;
+  if (n == 0) return PK_OK;
+  if (!d_rays || !d_ids || !d_t || !d_bary) {
+    pk_set_error("pk_trace_rays_noalloc: null buffer");
+    return PK_E_INVALID;
+  }
+  if (!c->scratch) {
+    pk_set_error("pk_trace_rays_noalloc: the context has no scratch address");
+    return PK_E_STATE;
+  }
+  PK_HIP(hipSetDevice(c->device));
+  uint32_t* cursor = reinterpret_cast<uint32_t*>(c->scratch);
+  uint2* spill = reinterpret_cast<uint2*>(c->scratch + kNoAllocCursorBytes);
+  pk::TraceLaunch L = shape_launch(c, any_hit ? PK_TUNE_RAYS_ANY : PK_TUNE_RAYS_CLOSEST, false);
+  L.grid = std::min(L.grid, kNoAllocMaxWaves);
+  L.spill = spill;
+  // cleared by kernels (no memset nodes if a caller captures this into a graph)
+  pk::launch_fill_u32(cursor, 0u, 1, c->stream);
+  pk::launch_fill_u32(c->overflow, 0u, 1, c->stream);
+  pk::launch_trace_rays(s->dev, (const float4*)d_rays, (uint32_t)n, any_hit, d_ids, d_t, d_bary, cursor, spill,
+                        c->stats, c->overflow, L, c->stream);
+  PK_HIP(hipGetLastError());
+  return PK_OK;
 }
 
 int pk_ctx_trace_stats(pk_ctx* c, uint64_t* stats2) {
