@@ -17,8 +17,11 @@ using pkm::v3;
 struct DevScene {
   const WNode* nodes;
   const float4* tris;        // TriRecord as 3 float4, leaf order
-  const int4* tri_index;     // {i0, i1, i2, mat}, leaf order
-  const uint4* attrs;        // quantized vertex attributes
+  // Shading record per leaf-order triangle, 64 bytes: the quantized
+  // attributes {tangent, bitangent, normal, uv} of its vertices i0, i1, i2
+  // and {material, 0, 0, 0}, so a hit reads one contiguous half line
+  // instead of an index record and three vertex records.
+  const uint4* shade_rec;
   const int4* materials;     // {diffMap, metRoughMap, normMap, emitMap}
   const void* tex;
   uint32_t tw, th, tl, tfmt;

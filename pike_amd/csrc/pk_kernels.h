@@ -121,6 +121,9 @@ void launch_accumulate(const FrameDev& F, hipStream_t s);
 // 4 * kMegaWavesPerSimd one-wave blocks per CU (the spill area's size).
 constexpr int kMegaWavesPerSimd = 4;
 void launch_render_paths(const DevScene& S, const FrameDev& F, uint32_t depth, const TraceLaunch& L, hipStream_t s);
+// DevScene::shade_rec from leaf-order {i0, i1, i2, mat} and vertex attributes
+// (every index already checked against the attribute count).
+void launch_build_shade_records(const int4* tri, const uint4* attrs, uint4* rec, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_fill_f4(float4* p, float4 v, uint32_t n, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, uint32_t n, hipStream_t s);

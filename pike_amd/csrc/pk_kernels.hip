@@ -469,11 +469,6 @@ __device__ inline v3 sample_ggx(const SI& si, uint32_t* seed) {
   return pkm::normalize(pkm::mk(au * nh.x, av * nh.y, pkm::max_(0.0f, nh.z)));
 }
 
-__device__ inline v3 attr_vec(const DevScene& S, int vi, int word) {
-  uint4 a = S.attrs[vi];
-  uint32_t w = word == 0 ? a.x : (word == 1 ? a.y : a.z);
-  return pkm::unpack4x8snorm_xyz(w);
-}
 
 // Largest i in [0, n) with cdf[i] <= u (cdf[0] = 0 <= u < 1), searched
 // between the cutpoint guides of the cells around k = (int)(u * n): the
@@ -580,24 +575,25 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
     inst = S.inst + lo;
     rec = leaf - inst->leaf_delta;
   }
-  int4 tri = S.tri_index[rec];
-  if (inst && inst->material >= 0) tri.w = inst->material;
+  const uint4* R = S.shade_rec + 4 * (size_t)rec;
+  const uint4 a0 = R[0], a1 = R[1], a2 = R[2];
+  const int mat = (inst && inst->material >= 0) ? inst->material : (int)R[3].x;
   const float bu = pkm::unpack_unorm16(bary_mask & 0xffffu), bv = pkm::unpack_unorm16(bary_mask >> 16);
   const v3 bary = pkm::mk(1.0f - bu - bv, bu, bv);
   v3 tng, btg, nrm;
   {
     pkm::m3 m;
-    m.c0 = attr_vec(S, tri.x, 0);
-    m.c1 = attr_vec(S, tri.y, 0);
-    m.c2 = attr_vec(S, tri.z, 0);
+    m.c0 = pkm::unpack4x8snorm_xyz(a0.x);
+    m.c1 = pkm::unpack4x8snorm_xyz(a1.x);
+    m.c2 = pkm::unpack4x8snorm_xyz(a2.x);
     tng = pkm::normalize(pkm::mat_mul(m, bary));
-    m.c0 = attr_vec(S, tri.x, 1);
-    m.c1 = attr_vec(S, tri.y, 1);
-    m.c2 = attr_vec(S, tri.z, 1);
+    m.c0 = pkm::unpack4x8snorm_xyz(a0.y);
+    m.c1 = pkm::unpack4x8snorm_xyz(a1.y);
+    m.c2 = pkm::unpack4x8snorm_xyz(a2.y);
     btg = pkm::normalize(pkm::mat_mul(m, bary));
-    m.c0 = attr_vec(S, tri.x, 2);
-    m.c1 = attr_vec(S, tri.y, 2);
-    m.c2 = attr_vec(S, tri.z, 2);
+    m.c0 = pkm::unpack4x8snorm_xyz(a0.z);
+    m.c1 = pkm::unpack4x8snorm_xyz(a1.z);
+    m.c2 = pkm::unpack4x8snorm_xyz(a2.z);
     nrm = pkm::normalize(pkm::mat_mul(m, bary));
   }
   if (inst) {
@@ -615,12 +611,12 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
                                  (A[1] * nrm.x + A[5] * nrm.y) + A[9] * nrm.z,
                                  (A[2] * nrm.x + A[6] * nrm.y) + A[10] * nrm.z));
   }
-  const uint32_t w0 = S.attrs[tri.x].w, w1 = S.attrs[tri.y].w, w2 = S.attrs[tri.z].w;
+  const uint32_t w0 = a0.w, w1 = a1.w, w2 = a2.w;
   const float uvx = pkm::unpack_snorm16(w0 & 0xffffu) * bary.x + pkm::unpack_snorm16(w1 & 0xffffu) * bary.y +
                     pkm::unpack_snorm16(w2 & 0xffffu) * bary.z;
   const float uvy = pkm::unpack_snorm16(w0 >> 16) * bary.x + pkm::unpack_snorm16(w1 >> 16) * bary.y +
                     pkm::unpack_snorm16(w2 >> 16) * bary.z;
-  const int4 mi = S.materials[tri.w];
+  const int4 mi = S.materials[mat];
   const float4 nm = sample_tex(S, S.tex, mi.z, uvx, uvy);
   const v3 map_normal = pkm::mk((nm.x - 0.5f) * 2.0f, (nm.y - 0.5f) * 2.0f, (nm.z - 0.0f) * 1.0f);
   SI si;
@@ -1099,6 +1095,20 @@ __global__ void k_advance(FrameDev F) {
   F.stats[kStatFrames] += (unsigned long long)(F.views ? 1u : pass_samples(F, P));
   F.params->sample = P.sample + F.S;
 }
+// Shading records (DevScene::shade_rec) from the leaf-order {i0, i1, i2,
+// mat} records and the vertex attributes, one record per thread.
+__global__ void __launch_bounds__(256) k_build_shade_records(const int4* tri, const uint4* attrs, uint4* rec,
+                                                             uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const int4 t = tri[i];
+  uint4* r = rec + 4 * (size_t)i;
+  r[0] = attrs[t.x];
+  r[1] = attrs[t.y];
+  r[2] = attrs[t.z];
+  r[3] = make_uint4((uint32_t)t.w, 0u, 0u, 0u);
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1499,6 +1509,9 @@ void launch_render_paths(const DevScene& S0, const FrameDev& F0, uint32_t depth,
 }
 void launch_accumulate(const FrameDev& F, hipStream_t s) {
   hipLaunchKernelGGL(k_accumulate, dim3(cdiv(F.W * F.H, kAccPix)), dim3(256), 0, s, F);
+}
+void launch_build_shade_records(const int4* tri, const uint4* attrs, uint4* rec, uint32_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_build_shade_records, dim3(cdiv(n, 256)), dim3(256), 0, s, tri, attrs, rec, n);
 }
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);

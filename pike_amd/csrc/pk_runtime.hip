@@ -512,8 +512,10 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
   s->allocs.ctx = c;
   pk::DevScene& D = s->dev;
   hipStream_t st = c->stream;
+  AllocList tmp{c, {}};  // the index records and attributes the shading records are built from
   auto fail = [&](int code) {
     (void)hipStreamSynchronize(st);
+    free_all(tmp);
     free_all(s->allocs);
     delete s;
     return code;
@@ -523,10 +525,9 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
   int4* tidx;
   if ((rc = upload(s->allocs, &nodes, bvh->wnodes.data(), bvh->wnodes.size(), st))) return fail(rc);
   if ((rc = upload(s->allocs, &tris, bvh->wtris.data(), bvh->wtris.size() * 3, st))) return fail(rc);
-  if ((rc = upload(s->allocs, &tidx, bvh->wtri_index.data(), bvh->wtri_index.size() / 4, st))) return fail(rc);
+  if ((rc = upload(tmp, &tidx, bvh->wtri_index.data(), bvh->wtri_index.size() / 4, st))) return fail(rc);
   D.nodes = nodes;
   D.tris = tris;
-  D.tri_index = tidx;
   std::vector<uint32_t> all_attr;
   const void* attr_src = d->attributes;
   size_t attr_count = d->num_vertices;
@@ -539,9 +540,21 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     attr_src = all_attr.data();
     attr_count = all_attr.size() / 4;
   }
+  // every vertex index of a leaf record must name an attribute record
+  for (size_t i = 0; i + 3 < bvh->wtri_index.size(); i += 4)
+    for (int v = 0; v < 3; ++v)
+      if (bvh->wtri_index[i + v] < 0 || (size_t)bvh->wtri_index[i + v] >= attr_count) {
+        pk_set_error("pk_scene_upload: triangle vertex index beyond the vertex attributes");
+        return fail(PK_E_INVALID);
+      }
   uint4* attrs;
-  if ((rc = upload(s->allocs, &attrs, attr_src, attr_count, st))) return fail(rc);
-  D.attrs = attrs;
+  if ((rc = upload(tmp, &attrs, attr_src, attr_count, st))) return fail(rc);
+  const size_t nrec = bvh->wtri_index.size() / 4;
+  uint4* srec;
+  if ((rc = dev_alloc(s->allocs, &srec, 4 * nrec))) return fail(rc);
+  pk::launch_build_shade_records(tidx, attrs, srec, (uint32_t)nrec, st);
+  PK_HIP(hipGetLastError());
+  D.shade_rec = srec;
   D.inst = nullptr;
   D.inst_leaf_base = nullptr;
   D.num_inst = 0;
@@ -633,6 +646,7 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     pk_set_error(std::string("pk_scene_upload: ") + hipGetErrorString(e));
     return fail(PK_E_HIP);
   }
+  free_all(tmp);
   *out = s;
   return PK_OK;
 }
