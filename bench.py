@@ -543,6 +543,18 @@ def run_traversal(args, pike_amd, scenes):
     kinds = {"sbvh": [True], "sah": [False], "both": [True, False]}[args.soup_bvh]
     ctx = pike_amd.Context(0)
     out = {}
+    # full-size check, outside the timed region: hit ids, t and bary of a
+    # ray subset against the oracle's canonical closest hit (argmin over
+    # (t, triangle id) with the reference's f32 intersector) and occlusion
+    # against its any-hit answer
+    orc = None
+    if not args.no_cpu and args.soup_check_rays > 0:
+        from oracle.oracle import OracleScene
+
+        t0 = time.time()
+        orc = OracleScene(data, threads=threads)
+        orc.build_bvh()
+        log(f"oracle soup BVH ({time.time() - t0:.1f}s)")
     for spatial in kinds:
         kind = "sbvh" if spatial else "sah"
         t0 = time.time()
@@ -555,6 +567,7 @@ def run_traversal(args, pike_amd, scenes):
         rays = scenes.incoherent_rays(args.soup_rays, bvh.normalizer(), seed=33)
         n = rays.shape[0]
         d_rays = pike_amd.DeviceBuffer.from_array(ctx, rays)
+        check = rays[:min(n, args.soup_check_rays)].copy() if orc is not None else None
         del rays
         ids = pike_amd.DeviceBuffer(ctx, 4 * n)
         t = pike_amd.DeviceBuffer(ctx, 4 * n)
@@ -573,6 +586,18 @@ def run_traversal(args, pike_amd, scenes):
             hit_rate = float((ids.read(np.int32, n) >= 0).mean())
             res[name] = dict(mrays_s=n / (ms * 1e-3) / 1e6, ms=ms, nodes_per_ray=nodes / n, tris_per_ray=tris / n,
                              gbs=byts / (ms * 1e-3) / 1e9, hit_rate=hit_rate)
+            if check is not None:
+                m = check.shape[0]
+                t0 = time.time()
+                ids_o, t_o, b_o, _ = orc.trace(check, "canonical", any_hit=anyh)
+                ids_g = ids.read(np.int32, m)
+                if anyh:  # occlusion: the id an any-hit query reports is order-dependent
+                    bad = int(((ids_g >= 0) != (ids_o >= 0)).sum())
+                else:
+                    bad = int(((ids_g != ids_o) | (t.read(np.uint32, m) != t_o.view(np.uint32)) |
+                               (b.read(np.uint32, m) != b_o)).sum())
+                res[name]["oracle_check"] = dict(rays=m, mismatches=bad, compared="occlusion" if anyh else
+                                                 "id, t bits, bary", cpu_s=time.time() - t0)
             log(f"traversal {kind} {name}: {res[name]}")
         out[kind] = res
         for obj in (scene, bvh, d_rays, ids, t, b):
@@ -617,6 +642,8 @@ def main():
                     help="pass scheduler of the single-view render legs: per-bounce wavefront kernels, the path "
                          "megakernel, or the faster of the two by timing (auto)")
     ap.add_argument("--trace-iters", type=int, default=5)
+    ap.add_argument("--soup-check-rays", type=int, default=1_000_000,
+                    help="rays of the soup checked against the oracle's canonical hits (0: none)")
     ap.add_argument("--spp-per-pass", type=int, default=1,
                     help="samples per pixel traced per wavefront pass of the 1080p render (without autotuning)")
     ap.add_argument("--max-spp-per-pass", type=int, default=128, help="autotuner's largest samples per pass")

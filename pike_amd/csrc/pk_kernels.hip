@@ -1032,23 +1032,25 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
 // mean equals S one-sample frames bit for bit.
 // ---------------------------------------------------------------------------
 // A pixel's S color slots are adjacent, so one thread per pixel reading its
-// own would stride S * 16 bytes across the wave: a block stages kAccPix
-// pixels' slots in LDS, kAccChunk samples at a time, with coalesced loads,
-// and one wave folds them in sample order.
-constexpr uint32_t kAccPix = 64, kAccChunk = 32;
+// own would stride S * 16 bytes across the wave: a block stages its kAccPix
+// pixels' slots in LDS, kAccChunk samples at a time, with loads that read
+// each pixel's chunk contiguously, and every thread folds its own pixel in
+// sample order.  A pixel's row in LDS is padded by one slot (kAccRow), so
+// the fold's 16-lane phases of 16-byte reads fall on distinct banks.
+constexpr uint32_t kAccPix = 256, kAccChunk = 8, kAccRow = kAccChunk + 1;
 __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
-  __shared__ float4 tile[kAccPix * kAccChunk];
+  __shared__ float4 tile[kAccPix * kAccRow];
   const uint32_t B = F.W * F.H, S = F.S;
   const uint32_t pix0 = blockIdx.x * kAccPix;
   const uint32_t t = threadIdx.x;
   const FrameParams P = *F.params;
-  // the folding wave's pixel and its sample count this pass (batch: a view's
+  // this thread's pixel and its sample count this pass (batch: a view's
   // pixels fold the round's samples below its spp; views whose budget is
   // spent keep their mean)
   const uint32_t idx = pix0 + t;
   uint32_t ns = 0;
   float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (t < kAccPix && idx < B) {
+  if (idx < B) {
     ns = pass_samples(F, P);
     if (F.views) {
       const uint32_t spp = F.views[view_of(F, idx)].spp;
@@ -1060,23 +1062,21 @@ __global__ void __launch_bounds__(256) k_accumulate(FrameDev F) {
   for (uint32_t c0 = 0; c0 < nmax; c0 += kAccChunk) {
     for (uint32_t e = t; e < kAccPix * kAccChunk; e += blockDim.x) {
       const uint32_t pl = e / kAccChunk, k = e - pl * kAccChunk;
-      if (pix0 + pl < B && c0 + k < nmax) tile[e] = F.color[(size_t)(pix0 + pl) * S + c0 + k];
+      if (pix0 + pl < B && c0 + k < nmax) tile[pl * kAccRow + k] = F.color[(size_t)(pix0 + pl) * S + c0 + k];
     }
     __syncthreads();
-    if (t < kAccPix) {
-      const uint32_t kend = ns > c0 ? min(kAccChunk, ns - c0) : 0u;
-      for (uint32_t k = 0; k < kend; ++k) {
-        const float sm = (float)(P.sample + c0 + k);
-        const float s1 = (float)(P.sample + c0 + k + 1u);
-        const float4 c = tile[t * kAccChunk + k];
-        a.x = (pkm::max0_(c.x) + a.x * sm) / s1;
-        a.y = (pkm::max0_(c.y) + a.y * sm) / s1;
-        a.z = (pkm::max0_(c.z) + a.z * sm) / s1;
-      }
+    const uint32_t kend = ns > c0 ? min(kAccChunk, ns - c0) : 0u;
+    for (uint32_t k = 0; k < kend; ++k) {
+      const float sm = (float)(P.sample + c0 + k);
+      const float s1 = (float)(P.sample + c0 + k + 1u);
+      const float4 c = tile[t * kAccRow + k];
+      a.x = (pkm::max0_(c.x) + a.x * sm) / s1;
+      a.y = (pkm::max0_(c.y) + a.y * sm) / s1;
+      a.z = (pkm::max0_(c.z) + a.z * sm) / s1;
     }
     __syncthreads();
   }
-  if (t < kAccPix && idx < B && ns > 0) {
+  if (idx < B && ns > 0) {
     a.w = 1.0f;
     F.accum[idx] = a;
   }
