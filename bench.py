@@ -85,6 +85,14 @@ def measured_counters(kernel):
     return {k: v for k, v in summary_row(COUNTER_SUMMARY, kernel).items() if isinstance(v, float)}
 
 
+def apply_shapes(ctx, shapes):
+    """Launch shapes {class: {waves_per_cu, refill_min[, stack_lds,
+    waves_per_block, leaf_phase16]}} (an autotune record) onto a context."""
+    for cls, sh in shapes.items():
+        ctx.set_tuning(cls, sh["waves_per_cu"], sh["refill_min"], sh.get("stack_lds"), sh.get("waves_per_block"),
+                       sh.get("leaf_phase16"))
+
+
 def run_render(args, pike_amd, scenes):
     import numpy as np
 
@@ -110,9 +118,7 @@ def run_render(args, pike_amd, scenes):
         if "samples_per_pass" in cfg:
             args.spp_per_pass = int(cfg["samples_per_pass"])
             frame.set_samples_per_pass(args.spp_per_pass)
-        for cls, shape in cfg.get("shapes", cfg).items():
-            if isinstance(shape, dict):
-                ctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+        apply_shapes(ctx, {k: v for k, v in cfg.get("shapes", cfg).items() if isinstance(v, dict)})
         if args.scheduler == "auto" and "scheduler" in cfg:
             ctx.set_scheduler(cfg["scheduler"])
         tuned = dict(shapes=ctx.tuning(), samples_per_pass=frame.samples_per_pass, source="--tuning")
@@ -697,16 +703,14 @@ def main():
     if not args.no_instanced:
         ictx = pike_amd.Context(0)
         if res["autotune"]:
-            for cls, shape in res["autotune"]["shapes"].items():
-                ictx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+            apply_shapes(ictx, res["autotune"]["shapes"])
         extra["instanced_c2"] = run_instanced(args, pike_amd, scenes, ictx, max(1, args.spp_per_pass))
         extra["instanced_c2"]["baked_layout_mb"] = (bvh.info()["wide_nodes"] * 128 +
                                                     bvh.info()["leaf_triangles"] * 48) / 1e6
     if not args.no_batch:
         bctx = pike_amd.Context(0)
         if res["autotune"]:
-            for cls, shape in res["autotune"]["shapes"].items():
-                bctx.set_tuning(cls, shape["waves_per_cu"], shape["refill_min"])
+            apply_shapes(bctx, res["autotune"]["shapes"])
         extra["batching_32_scenes_64_views"] = run_batching(args, pike_amd, scenes, bctx)
     if not args.no_diff:
         extra["differentiation_c5"] = run_diff(args, pike_amd, scenes)

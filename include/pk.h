@@ -136,6 +136,31 @@ int pk_sync(pk_ctx* ctx);
 int pk_ctx_set_tuning(pk_ctx* ctx, int kernel_class, uint32_t waves_per_cu, uint32_t refill_min);
 int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_cu, uint32_t* refill_min,
                       uint32_t* num_cus);
+/* The whole launch shape of a kernel class (no reference counterpart; the
+ * reference's trace shaders have a fixed 128-thread workgroup and a 24+8
+ * entry stack, trace.wgsl:5-9, 191-213).  pk_ctx_set_tuning changes the
+ * first two fields and keeps the others.
+ *   waves_per_cu     persistent traversal waves per CU, at most what the LDS
+ *                    of the stack parts allows (8 entries: 40, 12: 26 with 1
+ *                    or 2 waves per block and 24 with 4, 16: 20)
+ *   refill_min       idle lanes that trigger a wave's ray refill, [1, 64]
+ *   stack_lds        traversal stack entries per lane kept in LDS (8, 12 or
+ *                    16; the rest of the stack_limit entries spill to global
+ *                    memory); the counting and instanced variants use 12
+ *   waves_per_block  waves of one block (1, 2 or 4), each independent
+ *   leaf_phase16     the wave runs its deferred leaf tests together once
+ *                    this many 16ths of its tracing lanes hold some, [1, 16]
+ * The path megakernel class (PK_TUNE_PATHS) takes stack_lds 12 and 1 wave
+ * per block.  Every choice traces the same hits and renders the same image. */
+typedef struct pk_launch_shape {
+  uint32_t waves_per_cu;
+  uint32_t refill_min;
+  uint32_t stack_lds;
+  uint32_t waves_per_block;
+  uint32_t leaf_phase16;
+} pk_launch_shape;
+int pk_ctx_set_launch(pk_ctx* ctx, int kernel_class, const pk_launch_shape* shape);
+int pk_ctx_get_launch(const pk_ctx* ctx, int kernel_class, pk_launch_shape* shape);
 /* Traversal stack entries per ray, in [1, 64] (default 64: 12 in LDS, the
  * rest in a per-wave global spill area).  The reference's stack holds 32
  * entries and has no overflow check (trace.wgsl:5-9, 134-146).  A ray that
@@ -165,13 +190,15 @@ int pk_ctx_memory_used(const pk_ctx* ctx, uint64_t* bytes);
 #define PK_SCHED_PATHS 1
 int pk_ctx_set_scheduler(pk_ctx* ctx, int scheduler);
 int pk_ctx_get_scheduler(const pk_ctx* ctx, int* scheduler);
-/* Autotuner: coordinate search over (waves_per_cu, refill_min) for each
- * render kernel class (refill_min in {8, 16, ..., 64} at 24 waves per CU,
- * then waves_per_cu in {4, 8, ..., 24} at the best threshold), timing that
- * class's kernels with HIP events over
- * `frames` profiled frames of this scene, camera and resolution; keeps the
- * fastest shape per class in the context.  ms2 receives the traversal
- * milliseconds per frame before and after. */
+/* Autotuner: coordinate search over the launch shape of each render kernel
+ * class (refill_min in {8, 16, ..., 64} at 24 waves per CU, then
+ * waves_per_cu in {4, 8, ..., 24} at the best threshold, then leaf_phase16
+ * in {4, 6, ..., 12}, then stack_lds 8 / 12 / 16 at the best width and at
+ * that depth's widest launch, then 2 and 4 waves per block), timing that
+ * class's kernels with HIP events over `frames` profiled frames of this
+ * scene, camera and resolution; keeps the fastest shape per class in the
+ * context.  ms2 receives the traversal milliseconds per frame before and
+ * after. */
 int pk_autotune_render(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t frames, float* ms2);
 /* Queue capacity: chooses the frame's samples per pass (1, 2, 4, ... up to
  * max_samples, below 2^28 paths) by timing max_samples samples of
@@ -360,7 +387,7 @@ int pk_trace_rays_device(pk_ctx* ctx, pk_scene* scene, const float* d_rays, int6
  * reference).  It is for callers that cannot allocate device memory: code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc.  The kernels' work cursor and traversal stack spill area live in
- * a file-scope device array of the library (54.5 MB per device, held until
+ * a file-scope device array of the library (58.7 MB per device, held until
  * the process ends) instead of context memory; the call allocates nothing,
  * does not block and launches on the scene's context stream.  The launch
  * uses the context's ray-query shape with at most 2048 one-wave blocks (the

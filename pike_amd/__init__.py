@@ -78,17 +78,28 @@ class Context:
         _check(lib().pk_ctx_get_scheduler(self.h, ctypes.byref(v)), "pk_ctx_get_scheduler")
         return {c: n for n, c in self.SCHEDULERS.items()}[v.value]
 
-    def set_tuning(self, kernel_class, waves_per_cu, refill_min):
+    def set_tuning(self, kernel_class, waves_per_cu, refill_min, stack_lds=None, waves_per_block=None,
+                   leaf_phase16=None):
+        """Launch shape of a kernel class; the fields left None keep their value."""
         cls = self.TUNE_CLASSES[kernel_class] if isinstance(kernel_class, str) else kernel_class
-        _check(lib().pk_ctx_set_tuning(self.h, cls, waves_per_cu, refill_min), "pk_ctx_set_tuning")
+        sh = _ffi.LaunchShape()
+        _check(lib().pk_ctx_get_launch(self.h, cls, ctypes.byref(sh)), "pk_ctx_get_launch")
+        sh.waves_per_cu, sh.refill_min = waves_per_cu, refill_min
+        if stack_lds is not None:
+            sh.stack_lds = stack_lds
+        if waves_per_block is not None:
+            sh.waves_per_block = waves_per_block
+        if leaf_phase16 is not None:
+            sh.leaf_phase16 = leaf_phase16
+        _check(lib().pk_ctx_set_launch(self.h, cls, ctypes.byref(sh)), "pk_ctx_set_launch")
 
     def tuning(self):
         out = {}
         for name, cls in self.TUNE_CLASSES.items():
-            w, r, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-            _check(lib().pk_ctx_get_tuning(self.h, cls, ctypes.byref(w), ctypes.byref(r), ctypes.byref(n)),
-                   "pk_ctx_get_tuning")
-            out[name] = dict(waves_per_cu=w.value, refill_min=r.value)
+            sh = _ffi.LaunchShape()
+            _check(lib().pk_ctx_get_launch(self.h, cls, ctypes.byref(sh)), "pk_ctx_get_launch")
+            out[name] = dict(waves_per_cu=sh.waves_per_cu, refill_min=sh.refill_min, stack_lds=sh.stack_lds,
+                             waves_per_block=sh.waves_per_block, leaf_phase16=sh.leaf_phase16, node_width=8)
         return out
 
     def set_stack_limit(self, entries):

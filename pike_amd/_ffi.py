@@ -79,6 +79,11 @@ class SceneDesc(ctypes.Structure):
     ]
 
 
+class LaunchShape(ctypes.Structure):
+    _fields_ = [("waves_per_cu", c_uint32), ("refill_min", c_uint32), ("stack_lds", c_uint32),
+                ("waves_per_block", c_uint32), ("leaf_phase16", c_uint32)]
+
+
 class Camera(ctypes.Structure):
     _fields_ = [
         ("origin", c_float * 3),
@@ -149,6 +154,8 @@ SIGNATURES = {
     "pk_ctx_get_tuning": (c_int, [c_void_p, c_int, ctypes.POINTER(c_uint32), ctypes.POINTER(c_uint32),
                                   ctypes.POINTER(c_uint32)]),
     "pk_trace_rays_noalloc": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]),
+    "pk_ctx_set_launch": (c_int, [c_void_p, c_int, ctypes.POINTER(LaunchShape)]),
+    "pk_ctx_get_launch": (c_int, [c_void_p, c_int, ctypes.POINTER(LaunchShape)]),
     "pk_ctx_set_stack_limit": (c_int, [c_void_p, c_uint32]),
     "pk_ctx_get_stack_limit": (c_int, [c_void_p, ctypes.POINTER(c_uint32)]),
     "pk_ctx_set_memory_limit": (c_int, [c_void_p, ctypes.c_uint64]),

@@ -50,6 +50,9 @@ struct DevScene {
   // traversal stack entries per ray (<= kStackMax), set per launch from the
   // context; a ray that needs more flags an overflow
   uint32_t stack_limit;
+  // leaf phase trigger, in 16ths of the wave's tracing lanes that hold
+  // deferred leaves (8: half), set per launch from the kernel class's shape
+  uint32_t leaf16;
 };
 
 struct Hit {
@@ -83,13 +86,19 @@ __device__ inline float intersect_tri(const float4* tris, uint32_t k, v3 o, v3 d
   return dist > eps ? dist : kMaxT;
 }
 
-// Per-lane traversal stack: kStackLds entries in LDS ([entry][lane], so the
+// Per-lane traversal stack: kLds entries in LDS ([entry][lane], so the
 // 64 lanes of a wave hit 64 distinct banks), the rest spilled to a per-wave
 // region in global memory.  An entry is a node index and the entry distance
-// of its box (for culling at pop time).
+// of its box (for culling at pop time).  The frame and ray-query traversal
+// kernels come in 8, 12 and 16 LDS entries per lane (4, 6 and 8 KB per wave:
+// deeper stacks spill less, shallower ones let more waves share a CU's
+// 160 KB); the path megakernel, the adjoint and the counting and instanced
+// variants use kStackLds.  Every wave's spill region is kSpillStride entries
+// per lane, enough for the shallowest LDS part.
 constexpr int kStackLds = 12;
+constexpr int kStackLdsMin = 8;
 constexpr int kStackMax = 64;
-constexpr int kSpillPerLane = kStackMax - kStackLds;
+constexpr int kSpillStride = kStackMax - kStackLdsMin;
 // Occupancy target of the traversal kernels: 6 waves per SIMD lets the
 // compiler keep the traversal state in 80 VGPRs without scratch spills; 24
 // one-wave blocks per CU then use 24 * 6 KB = 144 KB of the 160 KB LDS.
@@ -104,9 +113,9 @@ struct TraverseStats {
 };
 
 struct Stack {
-  uint32_t* lds_node;   // [kStackLds][64]
-  float* lds_t;         // [kStackLds][64]
-  uint2* spill;         // [kSpillPerLane][64] for this wave
+  uint32_t* lds_node;   // [kLds][64]
+  float* lds_t;         // [kLds][64]
+  uint2* spill;         // [kSpillStride][64] for this wave
   int lane;
   int limit;            // entries a ray may hold, in [1, kStackMax]
 };
@@ -172,16 +181,17 @@ __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
 
 // A push beyond the limit drops the entry (its subtree is not visited) and
 // flags the ray; the traversal still ends normally.
+template <int kLds>
 __device__ inline void stack_push(const Stack& stk, RayState& r, uint32_t node, float t) {
   const int lane = stk.lane;
   if (r.sp >= stk.limit) {
     r.overflow = true;
-  } else if (r.sp < kStackLds) {
+  } else if (r.sp < kLds) {
     stk.lds_node[r.sp * 64 + lane] = node;
     stk.lds_t[r.sp * 64 + lane] = t;
     ++r.sp;
   } else {
-    stk.spill[(r.sp - kStackLds) * 64 + lane] = make_uint2(node, __float_as_uint(t));
+    stk.spill[(r.sp - kLds) * 64 + lane] = make_uint2(node, __float_as_uint(t));
     ++r.sp;
   }
 }
@@ -250,19 +260,19 @@ __device__ inline bool test_leaves(const DevScene& S, RayState& r, const uint32_
 // are the same parameter in both spaces (the map is affine and the direction
 // is not renormalized), so tbest and the culling bound carry over.
 // Returns true when the ray is finished.
-template <bool kAnyHit, bool kCount, bool kInst = false>
+template <bool kAnyHit, bool kCount, bool kInst = false, int kLds = kStackLds>
 __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
   const int lane = stk.lane;
   // Leaf phase.  A node visit defers its hit leaf children; the lane then
-  // waits while the others visit nodes, until at least half the wave's
-  // lanes hold deferred leaves (or no lane has a node left to visit), and
-  // those lanes intersect them together: on incoherent rays the triangle
-  // loop would otherwise run with a few lanes of the wave active.
+  // waits while the others visit nodes, until at least S.leaf16 / 16 of the
+  // wave's tracing lanes hold deferred leaves (or no lane has a node left to
+  // visit), and those lanes intersect them together: on incoherent rays the
+  // triangle loop would otherwise run with a few lanes of the wave active.
   const bool pend = r.pmask != 0u;
   const unsigned long long pm = __ballot(pend);
   if (pm) {
     const unsigned long long inner = __ballot(!pend && (r.have || r.sp > 0));
-    if (inner == 0ull || 2 * __popcll(pm) >= __popcll(__ballot(true))) {
+    if (inner == 0ull || 16u * (uint32_t)__popcll(pm) >= S.leaf16 * (uint32_t)__popcll(__ballot(true))) {
       if (pend) {
         const uint4* lp = reinterpret_cast<const uint4*>(S.nodes + r.pnode);
         const uint4 C1 = lp[1], C2 = lp[2], C3 = lp[3];
@@ -280,11 +290,11 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
       --r.sp;
       uint32_t n;
       float tn;
-      if (r.sp < kStackLds) {
+      if (r.sp < kLds) {
         n = stk.lds_node[r.sp * 64 + lane];
         tn = stk.lds_t[r.sp * 64 + lane];
       } else {
-        const uint2 e = stk.spill[(r.sp - kStackLds) * 64 + lane];
+        const uint2 e = stk.spill[(r.sp - kLds) * 64 + lane];
         n = e.x;
         tn = __uint_as_float(e.y);
       }
@@ -312,7 +322,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
         r.have = false;
         return r.sp == 0;
       }
-      stack_push(stk, r, kReturnMarker, 0.0f);
+      stack_push<kLds>(stk, r, kReturnMarker, 0.0f);
       const float4* ip = reinterpret_cast<const float4*>(S.inst + (r.node & ~kInstChild));
       const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
       const uint4 ref = reinterpret_cast<const uint4*>(ip)[3];
@@ -389,7 +399,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     while (im) {
       const int c = 31 - __builtin_clz(im);
       im &= ~(1u << c);
-      stack_push(stk, r, select8(child, (uint32_t)c), 0.0f);
+      stack_push<kLds>(stk, r, select8(child, (uint32_t)c), 0.0f);
     }
     return false;
   }
@@ -417,7 +427,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     if (i < m) {
       const uint32_t nx = select8(child, k[i] & 7u);
       const float t = __uint_as_float(k[i] & ~7u);  // rounded down: conservative
-      stack_push(stk, r, nx, t);
+      stack_push<kLds>(stk, r, nx, t);
     }
   }
   return false;

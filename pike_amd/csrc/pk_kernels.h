@@ -103,12 +103,18 @@ void launch_begin(const FrameDev& F, uint32_t sample, float theta, const pk_came
 void launch_advance(const FrameDev& F, hipStream_t s);
 void launch_raygen(const FrameDev& F, hipStream_t s);
 // persistent trace launch shape: blocks of one wave, lane refill threshold
+// grid counts waves (grid / waves_per_block blocks); the frame and ray-query
+// traversal kernels come in stack_lds = 8, 12 or 16 LDS entries and 1, 2 or
+// 4 waves per block (the counting and instanced variants: 12 and 1)
 struct TraceLaunch {
   uint32_t grid;
   uint32_t refill;
   bool count;
-  uint2* spill;  // grid * kSpillPerLane * 64 entries
+  uint2* spill;  // grid * kSpillStride * 64 entries
   uint32_t stack_limit;  // traversal stack entries per ray, in [1, kStackMax]
+  uint32_t stack_lds;
+  uint32_t waves_per_block;
+  uint32_t leaf16;       // leaf phase trigger in 16ths of the tracing lanes
 };
 
 void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, const TraceLaunch& L, hipStream_t s);
@@ -138,7 +144,7 @@ struct AdjointArgs {
   float inv_frames, theta;
   pk_camera cam;
   uint32_t* cursor;       // zeroed before the launch
-  uint2* spill;           // grid * kSpillPerLane * 64 entries
+  uint2* spill;           // grid * kSpillStride * 64 entries
   float* vtx;             // grid * depth * kAdjVtxWords * 64 floats
   uint32_t* overflow;
 };

@@ -16,6 +16,8 @@
 //   * counters are reset by the kernel that runs before their consumer, so a
 //     whole frame is kernels only and can be captured in one hipGraph.
 #include "pk_device.h"
+#include <type_traits>
+
 #include "pk_kernels.h"
 
 namespace pk {
@@ -210,7 +212,7 @@ __device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v
 // loop once the queue is exhausted and its own lanes are done.
 constexpr uint32_t kClaimChunk = 256;  // slots per cursor atomic (>= 64)
 
-template <bool kAnyHit, bool kCount, bool kInst, class Load, class Finish, class Flush>
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, class Load, class Finish, class Flush>
 __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* cursor, uint32_t refill_min,
                                   const Stack& stk, TraverseStats* st, uint32_t* overflow, Load load,
                                   Finish finish, Flush flush) {
@@ -225,7 +227,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = false;
   while (true) {
-    if (active && ray_step<kAnyHit, kCount, kInst>(S, r, stk, st)) {
+    if (active && ray_step<kAnyHit, kCount, kInst, kLds>(S, r, stk, st)) {
       active = false;
       done = true;
       if (r.overflow) *overflow = 1;
@@ -277,26 +279,35 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   }
 }
 
-template <bool kCount, bool kInst>
-__global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_closest(DevScene S, FrameDev F, int bounce, uint32_t refill_min) {
-  __shared__ uint32_t lds_node[kStackLds * 64];
-  __shared__ float lds_t[kStackLds * 64];
-  const int lane = (int)threadIdx.x;
+// The traversal kernels run kWpb independent waves per block (the block
+// size is a launch-shape choice of the autotuner), each with its own LDS
+// stack part of kLds entries per lane and its own spill region.
+#define PK_TRACE_WAVE_SETUP                                                                                       \
+  __shared__ uint32_t lds_node[kWpb][kLds * 64];                                                                 \
+  __shared__ float lds_t[kWpb][kLds * 64];                                                                       \
+  const int lane = (int)(threadIdx.x & 63u);                                                                     \
+  const int wave = kWpb == 1 ? 0 : (int)(threadIdx.x >> 6);                                                      \
+  const size_t wave_id = (size_t)blockIdx.x * kWpb + (size_t)wave
+
+template <bool kCount, bool kInst, int kLds, int kWpb>
+__global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_closest(DevScene S, FrameDev F, int bounce,
+                                                                                uint32_t refill_min) {
+  PK_TRACE_WAVE_SETUP;
   const int in = bounce & 1;
-  if (blockIdx.x == 0 && lane == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     F.cnt[ray_counter(in ^ 1)] = 0;
     F.cnt[kCntShadow] = 0;
   }
   const uint32_t n = F.cnt[ray_counter(in)];
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
   const float4* qo = F.qo[in];
   const float4* qd = F.qd[in];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   uint32_t res_lo = 0, res_hi = 0;  // this wave's unused run of hit slots (wave-uniform)
-  trace_loop<false, kCount, kInst>(
+  trace_loop<false, kCount, kInst, kLds>(
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = qo[slot], d4 = qd[slot];
@@ -805,22 +816,21 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
 // ---------------------------------------------------------------------------
 // trace_shadow.wgsl main (:192-207) + shade_miss for unoccluded shadow rays
 // ---------------------------------------------------------------------------
-template <bool kCount, bool kInst>
-__global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_shadow(DevScene S, FrameDev F, uint32_t refill_min) {
-  __shared__ uint32_t lds_node[kStackLds * 64];
-  __shared__ float lds_t[kStackLds * 64];
-  const int lane = (int)threadIdx.x;
-  if (blockIdx.x == 0 && lane == 0) {
+template <bool kCount, bool kInst, int kLds, int kWpb>
+__global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_shadow(DevScene S, FrameDev F,
+                                                                               uint32_t refill_min) {
+  PK_TRACE_WAVE_SETUP;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     F.cnt[kCntHits] = 0;
     F.cnt[kCntWorkTrace] = 0;
   }
   const uint32_t n = F.cnt[kCntShadow];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, overflow = 0, pid = 0;
-  trace_loop<true, kCount, kInst>(
+  trace_loop<true, kCount, kInst, kLds>(
       S, n, &F.cnt[kCntWorkShadow], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = F.so[slot], d4 = F.sd[slot];
@@ -873,7 +883,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit};
   const FrameParams P = *F.params;
   const uint32_t ns = kBatch ? F.S : pass_samples(F, P);
   const uint32_t B = F.W * F.H;
@@ -1198,7 +1208,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit};
   float* const vtx = A.vtx + (size_t)blockIdx.x * A.depth * kAdjVtxWords * 64 + lane;
   const uint32_t B = A.W * A.H;
   const uint32_t total = B * A.num_frames;
@@ -1411,18 +1421,19 @@ __global__ void __launch_bounds__(256) k_loss_l2(const float4* img, const float4
 // ---------------------------------------------------------------------------
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
 // ---------------------------------------------------------------------------
-template <bool kAnyHit, bool kCount, bool kInst>
-__global__ void __launch_bounds__(64, kTraceWavesPerSimd) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int32_t* ids,
-                                                       float* tout, uint32_t* bary, uint32_t* cursor,
-                                                       uint2* spill_all, unsigned long long* stats,
-                                                       uint32_t* overflow_flag, uint32_t refill_min) {
-  __shared__ uint32_t lds_node[kStackLds * 64];
-  __shared__ float lds_t[kStackLds * 64];
-  const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, spill_all + (size_t)blockIdx.x * kSpillPerLane * 64, lane, (int)S.stack_limit};
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kWpb>
+__global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_rays(DevScene S, const float4* rays,
+                                                                             uint32_t n, int32_t* ids, float* tout,
+                                                                             uint32_t* bary, uint32_t* cursor,
+                                                                             uint2* spill_all,
+                                                                             unsigned long long* stats,
+                                                                             uint32_t* overflow_flag,
+                                                                             uint32_t refill_min) {
+  PK_TRACE_WAVE_SETUP;
+  const Stack stk{lds_node[wave], lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
-  trace_loop<kAnyHit, kCount, kInst>(
+  trace_loop<kAnyHit, kCount, kInst, kLds>(
       S, n, cursor, refill_min, stk, &st, &overflow,
       [&](uint32_t i, RayState& r) {
         const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
@@ -1466,39 +1477,76 @@ void launch_raygen(const FrameDev& F, hipStream_t s) {
     hipLaunchKernelGGL(k_raygen, dim3(cdiv(F.W * F.H * F.S, 256)), dim3(256), 0, s, F);
   }
 }
-// the scene as one launch sees it: the context's stack limit stamped in
-static inline DevScene with_limit(const DevScene& S0, uint32_t stack_limit) {
+// the scene as one launch sees it: the context's stack limit and the
+// class's leaf phase trigger stamped in
+static inline DevScene with_limit(const DevScene& S0, uint32_t stack_limit, uint32_t leaf16 = 8u) {
   DevScene S = S0;
   S.stack_limit = stack_limit;
+  S.leaf16 = leaf16;
   return S;
 }
 
+// Calls f(integral_constant kLds, integral_constant kWpb) for the launch
+// shape's variant; unsupported combinations take 12 entries, one wave.
+template <class Fn>
+static inline void with_variant(const TraceLaunch& L, Fn f) {
+  using std::integral_constant;
+  const uint32_t w = L.waves_per_block;
+  switch (L.stack_lds) {
+    case 8:
+      if (w == 2) f(integral_constant<int, 8>(), integral_constant<int, 2>());
+      else if (w == 4) f(integral_constant<int, 8>(), integral_constant<int, 4>());
+      else f(integral_constant<int, 8>(), integral_constant<int, 1>());
+      break;
+    case 16:
+      if (w == 2) f(integral_constant<int, 16>(), integral_constant<int, 2>());
+      else if (w == 4) f(integral_constant<int, 16>(), integral_constant<int, 4>());
+      else f(integral_constant<int, 16>(), integral_constant<int, 1>());
+      break;
+    default:
+      if (w == 2) f(integral_constant<int, 12>(), integral_constant<int, 2>());
+      else if (w == 4) f(integral_constant<int, 12>(), integral_constant<int, 4>());
+      else f(integral_constant<int, 12>(), integral_constant<int, 1>());
+      break;
+  }
+}
+
 void launch_trace_closest(const DevScene& S0, const FrameDev& F0, int bounce, const TraceLaunch& L, hipStream_t s) {
-  const DevScene S = with_limit(S0, L.stack_limit);
+  const DevScene S = with_limit(S0, L.stack_limit, L.leaf16);
   FrameDev F = F0;
   F.spill = L.spill;
   const bool inst = S.num_inst != 0;
-  if (L.count && inst) hipLaunchKernelGGL((k_trace_closest<true, true>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
-  else if (L.count) hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
-  else if (inst) hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
-  else hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  if (L.count && inst) hipLaunchKernelGGL((k_trace_closest<true, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else if (L.count) hipLaunchKernelGGL((k_trace_closest<true, false, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else if (inst) hipLaunchKernelGGL((k_trace_closest<false, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else
+    with_variant(L, [&](auto lds, auto wpb) {
+      constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
+      hipLaunchKernelGGL((k_trace_closest<false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S, F,
+                         bounce, L.refill);
+    });
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
   hipLaunchKernelGGL(k_shade, dim3(cdiv(F.hit_cap, kShadeWaves * 64)), dim3(kShadeWaves * 64), 0, s, S, F, bounce);
 }
 void launch_trace_shadow(const DevScene& S0, const FrameDev& F0, const TraceLaunch& L, hipStream_t s) {
-  const DevScene S = with_limit(S0, L.stack_limit);
+  const DevScene S = with_limit(S0, L.stack_limit, L.leaf16);
   FrameDev F = F0;
   F.spill = L.spill;
   const bool inst = S.num_inst != 0;
-  if (L.count && inst) hipLaunchKernelGGL((k_trace_shadow<true, true>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
-  else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
-  else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
-  else hipLaunchKernelGGL((k_trace_shadow<false, false>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  if (L.count && inst) hipLaunchKernelGGL((k_trace_shadow<true, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  else
+    with_variant(L, [&](auto lds, auto wpb) {
+      constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
+      hipLaunchKernelGGL((k_trace_shadow<false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S, F,
+                         L.refill);
+    });
 }
 void launch_render_paths(const DevScene& S0, const FrameDev& F0, uint32_t depth, const TraceLaunch& L,
                          hipStream_t s) {
-  const DevScene S = with_limit(S0, L.stack_limit);
+  const DevScene S = with_limit(S0, L.stack_limit, L.leaf16);
   FrameDev F = F0;
   F.spill = L.spill;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, F.cnt + kCntWorkTrace, 0u, 1u);
@@ -1533,17 +1581,29 @@ void launch_loss_l2(const float4* img, const float4* target, float4* grad, doubl
 void launch_trace_rays(const DevScene& S0, const float4* rays, uint32_t n, int any_hit, int32_t* ids, float* t,
                        uint32_t* bary, uint32_t* cursor, uint2* spill, unsigned long long* stats,
                        uint32_t* overflow, const TraceLaunch& L, hipStream_t s) {
-  const DevScene S = with_limit(S0, L.stack_limit);
+  const DevScene S = with_limit(S0, L.stack_limit, L.leaf16);
 #define PK_LAUNCH_RAYS(A, C, I)                                                                                    \
-  hipLaunchKernelGGL((k_trace_rays<A, C, I>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary, cursor, spill, \
-                     stats, overflow, L.refill)
+  hipLaunchKernelGGL((k_trace_rays<A, C, I, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary,   \
+                     cursor, spill, stats, overflow, L.refill)
   const int variant = (any_hit ? 4 : 0) | (L.count ? 2 : 0) | (S.num_inst ? 1 : 0);
   switch (variant) {
-    case 0: PK_LAUNCH_RAYS(false, false, false); break;
+    case 0:
+      with_variant(L, [&](auto lds, auto wpb) {
+        constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
+        hipLaunchKernelGGL((k_trace_rays<false, false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S,
+                           rays, n, ids, t, bary, cursor, spill, stats, overflow, L.refill);
+      });
+      break;
     case 1: PK_LAUNCH_RAYS(false, false, true); break;
     case 2: PK_LAUNCH_RAYS(false, true, false); break;
     case 3: PK_LAUNCH_RAYS(false, true, true); break;
-    case 4: PK_LAUNCH_RAYS(true, false, false); break;
+    case 4:
+      with_variant(L, [&](auto lds, auto wpb) {
+        constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
+        hipLaunchKernelGGL((k_trace_rays<true, false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S,
+                           rays, n, ids, t, bary, cursor, spill, stats, overflow, L.refill);
+      });
+      break;
     case 5: PK_LAUNCH_RAYS(true, false, true); break;
     case 6: PK_LAUNCH_RAYS(true, true, false); break;
     default: PK_LAUNCH_RAYS(true, true, true); break;

@@ -19,14 +19,14 @@ thread_local std::string g_err;
 
 // Scratch of pk_trace_rays_noalloc: the work cursor (first 256 bytes) and
 // the traversal stack spill area of up to kNoAllocMaxWaves one-wave blocks
-// (8 per CU on a 256-CU MI355X), 54.5 MB, held by every device the library
+// (8 per CU on a 256-CU MI355X), 58.7 MB, held by every device the library
 // runs on until the process ends.  Host code never uses the array's name as
 // an address: pk_ctx_create resolves each device's copy once with
 // hipGetSymbolAddress, and only that address is passed to kernels.
 constexpr uint32_t kNoAllocMaxWaves = 2048;
 constexpr size_t kNoAllocCursorBytes = 256;
 constexpr size_t kNoAllocScratchBytes =
-    kNoAllocCursorBytes + (size_t)kNoAllocMaxWaves * pk::kSpillPerLane * 64 * sizeof(uint2);
+    kNoAllocCursorBytes + (size_t)kNoAllocMaxWaves * pk::kSpillStride * 64 * sizeof(uint2);
 __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -70,8 +70,11 @@ struct pk_ctx {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   struct Shape {
-    uint32_t waves;   // one-wave blocks per CU
+    uint32_t waves;   // persistent traversal waves per CU
     uint32_t refill;  // idle lanes that trigger a wave-level ray refill
+    uint32_t lds = pk::kStackLds;  // LDS stack entries per lane (8, 12, 16)
+    uint32_t wpb = 1;              // waves per block (1, 2, 4)
+    uint32_t leaf16 = 8;           // leaf phase trigger, 16ths of the tracing lanes
   } shape[PK_TUNE_CLASSES];
   uint32_t tuning_gen = 0;  // bumped by pk_ctx_set_tuning; captured graphs keep the old shape
   uint32_t stack_limit = pk::kStackMax;  // traversal stack entries per ray
@@ -141,9 +144,17 @@ struct pk_frame {
 
 namespace {
 
-// One-wave traversal blocks per CU at most: kTraceWavesPerSimd waves on each
-// of the 4 SIMDs (the register and LDS budget of the traversal kernels).
-constexpr uint32_t kMaxWavesPerCu = 4 * pk::kTraceWavesPerSimd;
+// Traversal waves per CU at most: the LDS of 40 waves with 8-entry stack
+// parts (4 KB each) fills a CU's 160 KB (the kernels' 40 VGPRs would allow
+// 48); the spill area and the hit queue's slack are sized for it.
+constexpr uint32_t kMaxWavesPerCu = 40;
+constexpr uint32_t kLdsBytesPerCu = 160u * 1024u;
+// waves per CU that a shape's LDS stack parts fit: whole blocks of wpb waves
+// of lds entries x 64 lanes x 8 bytes
+uint32_t lds_max_waves(uint32_t lds, uint32_t wpb) {
+  const uint32_t per_block = lds * 64u * 8u * wpb;
+  return std::min(kMaxWavesPerCu, kLdsBytesPerCu / per_block * wpb);
+}
 // One-wave adjoint blocks per CU: k_adjoint is compiled for 4 waves per SIMD
 // (128 VGPRs, the rest spilled to scratch), which measured 1.35x faster on
 // the 512x512 differentiation workload than the 2 waves per SIMD its
@@ -153,8 +164,10 @@ static_assert(kAdjointWavesPerCu <= kMaxWavesPerCu, "adjoint grid exceeds the sp
 static_assert(4 * pk::kMegaWavesPerSimd <= kMaxWavesPerCu, "path megakernel grid exceeds the spill area");
 
 pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
-  return pk::TraceLaunch{(uint32_t)c->num_cus * c->shape[cls].waves, c->shape[cls].refill, count, c->spill,
-                         c->stack_limit};
+  const pk_ctx::Shape& sh = c->shape[cls];
+  // whole blocks: the grid (in waves) is a multiple of the waves per block
+  const uint32_t grid = std::max(sh.wpb, (uint32_t)c->num_cus * sh.waves / sh.wpb * sh.wpb);
+  return pk::TraceLaunch{grid, sh.refill, count, c->spill, c->stack_limit, sh.lds, sh.wpb, sh.leaf16};
 }
 pk::TraceLaunch trace_launch(const pk_frame* f, int cls) { return shape_launch(f->ctx, cls, f->count_nodes); }
 int closest_class(uint32_t bounce) { return bounce == 0 ? PK_TUNE_PRIMARY : PK_TUNE_SECONDARY; }
@@ -251,12 +264,12 @@ int pk_ctx_create(int device, pk_ctx** out) {
   c->shape[PK_TUNE_PRIMARY] = {16, 64};
   c->shape[PK_TUNE_SECONDARY] = {16, 64};
   c->shape[PK_TUNE_SHADOW] = {8, 64};
-  c->shape[PK_TUNE_RAYS_CLOSEST] = {kMaxWavesPerCu, 16};
-  c->shape[PK_TUNE_RAYS_ANY] = {kMaxWavesPerCu, 24};
+  c->shape[PK_TUNE_RAYS_CLOSEST] = {24, 16};
+  c->shape[PK_TUNE_RAYS_ANY] = {24, 24};
   c->shape[PK_TUNE_PATHS] = {4 * pk::kMegaWavesPerSimd, 64};
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess)
-    e = hipMalloc((void**)&c->spill, (size_t)c->num_cus * kMaxWavesPerCu * pk::kSpillPerLane * 64 * sizeof(uint2));
+    e = hipMalloc((void**)&c->spill, (size_t)c->num_cus * kMaxWavesPerCu * pk::kSpillStride * 64 * sizeof(uint2));
   if (e == hipSuccess) e = hipMalloc((void**)&c->stats, 8 * sizeof(unsigned long long));
   if (e == hipSuccess) e = hipMalloc((void**)&c->overflow, 4 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -300,20 +313,52 @@ int pk_sync(pk_ctx* c) {
   return PK_OK;
 }
 
+// largest waves_per_cu a class allows with the given stack part and block
+static uint32_t class_max_waves(int cls, uint32_t lds, uint32_t wpb) {
+  return cls == PK_TUNE_PATHS ? 4u * pk::kMegaWavesPerSimd : lds_max_waves(lds, wpb);
+}
+
+int pk_ctx_set_launch(pk_ctx* c, int cls, const pk_launch_shape* sh) {
+  if (!c || !sh || cls < 0 || cls >= PK_TUNE_CLASSES) {
+    pk_set_error("pk_ctx_set_launch: null argument or unknown kernel class");
+    return PK_E_INVALID;
+  }
+  const bool lds_ok = sh->stack_lds == 8 || sh->stack_lds == 12 || sh->stack_lds == 16;
+  const bool wpb_ok = sh->waves_per_block == 1 || sh->waves_per_block == 2 || sh->waves_per_block == 4;
+  if (!lds_ok || !wpb_ok || (cls == PK_TUNE_PATHS && (sh->stack_lds != 12 || sh->waves_per_block != 1))) {
+    pk_set_error("pk_ctx_set_launch: stack_lds must be 8, 12 or 16 and waves_per_block 1, 2 or 4 (the path "
+                 "megakernel: 12 and 1)");
+    return PK_E_INVALID;
+  }
+  const uint32_t wmax = class_max_waves(cls, sh->stack_lds, sh->waves_per_block);
+  if (sh->waves_per_cu < sh->waves_per_block || sh->waves_per_cu > wmax || sh->refill_min < 1 ||
+      sh->refill_min > 64 || sh->leaf_phase16 < 1 || sh->leaf_phase16 > 16) {
+    pk_set_error("pk_ctx_set_launch: waves_per_cu must be in [waves_per_block, " + std::to_string(wmax) +
+                 "] for this stack part and block, refill_min in [1, 64], leaf_phase16 in [1, 16]");
+    return PK_E_LIMIT;
+  }
+  c->shape[cls] = {sh->waves_per_cu, sh->refill_min, sh->stack_lds, sh->waves_per_block, sh->leaf_phase16};
+  c->tuning_gen++;
+  return PK_OK;
+}
+
+int pk_ctx_get_launch(const pk_ctx* c, int cls, pk_launch_shape* sh) {
+  if (!c || !sh || cls < 0 || cls >= PK_TUNE_CLASSES) {
+    pk_set_error("pk_ctx_get_launch: null argument or unknown kernel class");
+    return PK_E_INVALID;
+  }
+  const pk_ctx::Shape& s = c->shape[cls];
+  *sh = pk_launch_shape{s.waves, s.refill, s.lds, s.wpb, s.leaf16};
+  return PK_OK;
+}
+
 int pk_ctx_set_tuning(pk_ctx* c, int cls, uint32_t waves_per_cu, uint32_t refill_min) {
   if (!c || cls < 0 || cls >= PK_TUNE_CLASSES) {
     pk_set_error("pk_ctx_set_tuning: null context or unknown kernel class");
     return PK_E_INVALID;
   }
-  const uint32_t wmax = cls == PK_TUNE_PATHS ? 4u * pk::kMegaWavesPerSimd : kMaxWavesPerCu;
-  if (waves_per_cu < 1 || waves_per_cu > wmax || refill_min < 1 || refill_min > 64) {
-    pk_set_error("pk_ctx_set_tuning: waves_per_cu must be in [1, " + std::to_string(wmax) +
-                 "] and refill_min in [1, 64]");
-    return PK_E_LIMIT;
-  }
-  c->shape[cls] = {waves_per_cu, refill_min};
-  c->tuning_gen++;
-  return PK_OK;
+  pk_launch_shape sh{waves_per_cu, refill_min, c->shape[cls].lds, c->shape[cls].wpb, c->shape[cls].leaf16};
+  return pk_ctx_set_launch(c, cls, &sh);
 }
 
 int pk_ctx_set_scheduler(pk_ctx* c, int sched) {
@@ -1701,6 +1746,41 @@ namespace {
 
 const uint32_t kTuneWaves[] = {4, 8, 12, 16, 20, 24};
 const uint32_t kTuneRefill[] = {8, 16, 24, 32, 40, 48, 56, 64};
+const uint32_t kTuneLeaf16[] = {4, 6, 8, 10, 12};
+
+// The launch-shape dimensions after the refill threshold and the width:
+// the leaf phase trigger, then the LDS stack part (each depth at the best
+// width and at its own widest launch), then the waves per block.  `trial`
+// times one shape and keeps the fastest in `best`.
+template <class Trial>
+int tune_shape_dims(pk_ctx::Shape& best, Trial trial) {
+  int rc;
+  for (uint32_t l : kTuneLeaf16) {
+    pk_ctx::Shape t = best;
+    t.leaf16 = l;
+    if ((rc = trial(t))) return rc;
+  }
+  const pk_ctx::Shape base = best;
+  for (uint32_t lds : {8u, 12u, 16u}) {
+    if (lds == base.lds) continue;
+    const uint32_t wmax = lds_max_waves(lds, 1);
+    for (uint32_t w : {std::min(base.waves, wmax), wmax}) {
+      pk_ctx::Shape t = base;
+      t.lds = lds;
+      t.wpb = 1;
+      t.waves = w;
+      if ((rc = trial(t))) return rc;
+    }
+  }
+  const pk_ctx::Shape base2 = best;
+  for (uint32_t wpb : {2u, 4u}) {
+    pk_ctx::Shape t = base2;
+    t.wpb = wpb;
+    t.waves = std::max(wpb, std::min(base2.waves, lds_max_waves(base2.lds, wpb)) / wpb * wpb);
+    if ((rc = trial(t))) return rc;
+  }
+  return PK_OK;
+}
 
 // Milliseconds per frame spent in each render traversal class.
 int time_render_classes(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t frames, double ms[3]) {
@@ -1765,8 +1845,8 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
   for (int cls = PK_TUNE_PRIMARY; cls <= PK_TUNE_SHADOW; ++cls) {
     pk_ctx::Shape best = c->shape[cls];
     double best_ms = base[cls];
-    auto trial = [&](uint32_t w, uint32_t r) -> int {
-      c->shape[cls] = {w, r};
+    auto trial_shape = [&](const pk_ctx::Shape& sh) -> int {
+      c->shape[cls] = sh;
       double ms[3];
       const int e = time_render_classes(f, s, depth, frames, ms);
       if (!e && ms[cls] < best_ms) {
@@ -1775,11 +1855,18 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
       }
       return e;
     };
+    auto trial = [&](uint32_t w, uint32_t r) -> int {
+      pk_ctx::Shape t = best;
+      t.waves = std::min(w, lds_max_waves(t.lds, t.wpb)) / t.wpb * t.wpb;
+      t.refill = r;
+      return trial_shape(t);
+    };
     for (uint32_t r : kTuneRefill)
-      if ((rc = trial(kMaxWavesPerCu, r))) return rc;
+      if ((rc = trial(24, r))) return rc;
     const uint32_t r_best = best.refill;
     for (uint32_t w : kTuneWaves)
       if ((rc = trial(w, r_best))) return rc;
+    if ((rc = tune_shape_dims(best, trial_shape))) return rc;
     c->shape[cls] = best;
   }
   double after[3];
@@ -1897,16 +1984,24 @@ extern "C" int pk_autotune_rays(pk_ctx* c, pk_scene* s, const float* d_rays, int
   if ((rc = pk_time_trace_rays(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, 2, &before))) return rc;
   pk_ctx::Shape best = c->shape[cls];
   float best_ms = before;
+  auto trial_shape = [&](const pk_ctx::Shape& sh) -> int {
+    c->shape[cls] = sh;
+    float ms = 0;
+    const int e = pk_time_trace_rays(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, 2, &ms);
+    if (!e && ms < best_ms) {
+      best_ms = ms;
+      best = c->shape[cls];
+    }
+    return e;
+  };
   for (uint32_t w : kTuneWaves)
     for (uint32_t r : {1u, 8u, 16u, 24u, 32u, 48u, 64u}) {
-      c->shape[cls] = {w, r};
-      float ms = 0;
-      if ((rc = pk_time_trace_rays(c, s, d_rays, n, any_hit, d_ids, d_t, d_bary, 2, &ms))) return rc;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = c->shape[cls];
-      }
+      pk_ctx::Shape t = best;
+      t.waves = std::min(w, lds_max_waves(t.lds, t.wpb)) / t.wpb * t.wpb;
+      t.refill = r;
+      if ((rc = trial_shape(t))) return rc;
     }
+  if ((rc = tune_shape_dims(best, trial_shape))) return rc;
   c->shape[cls] = best;
   c->tuning_gen++;
   if (ms2) {
