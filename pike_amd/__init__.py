@@ -246,7 +246,8 @@ class BVH:
     WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "u1", 3), ("nchild", "u1"), ("child", "<u4", 8),
                                 ("meta", "u1", 8), ("qlox", "u1", 8), ("qhix", "u1", 8), ("qloy", "u1", 8),
                                 ("qhiy", "u1", 8), ("qloz", "u1", 8), ("qhiz", "u1", 8), ("leafmask", "u1"),
-                                ("pad8", "u1", 3), ("pad", "<u4", 5)])
+                                ("pad8", "u1", 3), ("ibase", "<u4"), ("tbase", "<u4"),
+                                ("pad", "<u4", 3)])
     TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("id", "<u4"), ("e1", "<f4", 3), ("pad0", "<u4"), ("e2", "<f4", 3),
                           ("pad1", "<u4")])
 

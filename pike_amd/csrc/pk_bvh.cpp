@@ -660,6 +660,7 @@ void quantize_node(pk::WNode& w, const FBox* cb, int nc) {
   w.ex = (uint8_t)(ex[0] + 127);
   w.ey = (uint8_t)(ex[1] + 127);
   w.ez = (uint8_t)(ex[2] + 127);
+  pk::wnode_link(w);
 }
 
 }  // namespace

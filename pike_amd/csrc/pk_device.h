@@ -55,13 +55,6 @@ struct DevScene {
   uint32_t leaf16;
 };
 
-struct Hit {
-  float t;
-  uint32_t id;    // original triangle id (0xffffffff: none)
-  uint32_t leaf;  // leaf-order triangle index
-  uint32_t bary;
-};
-
 __device__ inline v3 ld3(float4 a) { return pkm::mk(a.x, a.y, a.z); }
 
 // Möller–Trumbore exactly as trace.wgsl:97-114, with e1/e2 precomputed.
@@ -99,10 +92,14 @@ constexpr int kStackLds = 12;
 constexpr int kStackLdsMin = 8;
 constexpr int kStackMax = 64;
 constexpr int kSpillStride = kStackMax - kStackLdsMin;
-// Occupancy target of the traversal kernels: 6 waves per SIMD lets the
-// compiler keep the traversal state in 80 VGPRs without scratch spills; 24
-// one-wave blocks per CU then use 24 * 6 KB = 144 KB of the 160 KB LDS.
+// Occupancy target of the traversal kernels.  With the 8-entry LDS part,
+// 8 waves per SIMD (64 VGPRs): 32 one-wave blocks per CU use 32 * 4 KB of
+// the 160 KB LDS, and the single-level kernels keep their traversal state in
+// 64 VGPRs (the closest-hit kernel reloads two loop invariants of its miss
+// shading from scratch).  The 12- and 16-entry parts fit at most 26 and 20
+// waves per CU in LDS, so they target 6 waves per SIMD (80 VGPRs).
 constexpr int kTraceWavesPerSimd = 6;
+constexpr int trace_waves_per_simd(int lds) { return lds <= kStackLdsMin ? 8 : kTraceWavesPerSimd; }
 
 // Culling bound: a box entered beyond the best hit by more than the
 // intersector's rounding cannot hold a better (t, id) hit.
@@ -128,7 +125,11 @@ struct Stack {
 struct RayState {
   v3 o, d;
   float invx, invy, invz;
-  float tbest, cull;
+  float tbest;
+  // best hit: leaf-order triangle record (0xffffffff: none), packed
+  // barycentrics and, for the instanced traversal only, the baked triangle
+  // id (its record indices are virtual); the single-level traversal reads the
+  // id back from the record (ray_hit_id), so it holds no register for it
   uint32_t best_id, best_leaf, best_bary;
   uint32_t node;
   int sp;
@@ -165,7 +166,6 @@ __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
   r.d = d;
   ray_set_inv(r);
   r.tbest = kMaxT;
-  r.cull = kAnyHit ? kMaxT * 2.0f : cull_bound(kMaxT);
   r.best_id = 0xffffffffu;
   r.best_leaf = 0xffffffffu;
   r.best_bary = 0;
@@ -177,6 +177,22 @@ __device__ inline void ray_init(RayState& r, v3 o, v3 d) {
   r.leaf_delta = 0;
   r.pnode = 0;
   r.pmask = 0;
+}
+
+// Culling bound of a ray, recomputed from tbest where it is used (one fma)
+// rather than held in a register; any-hit rays never tighten it.
+template <bool kAnyHit>
+__device__ inline float ray_cull(const RayState& r) {
+  return kAnyHit ? kMaxT * 2.0f : cull_bound(r.tbest);
+}
+
+__device__ inline bool ray_missed(const RayState& r) { return r.best_leaf == 0xffffffffu; }
+
+// Original triangle id of the best hit (0xffffffff: none).
+template <bool kInst>
+__device__ inline uint32_t ray_hit_id(const DevScene& S, const RayState& r) {
+  if (kInst) return r.best_id;
+  return ray_missed(r) ? 0xffffffffu : __float_as_uint(S.tris[3 * (size_t)r.best_leaf].w);
 }
 
 // A push beyond the limit drops the entry (its subtree is not visited) and
@@ -212,13 +228,23 @@ __device__ inline uint32_t select8(const uint32_t* c, uint32_t s) {
   return b2 ? d1 : d0;
 }
 
-// Intersect the triangles of the leaf children in `lm` (child words and
-// leaf counts of one node), one loop over all of them so the wave iterates
-// as often as its busiest lane has triangles.  Returns true for an any-hit
-// ray that found its hit.
-template <bool kAnyHit, bool kCount, bool kInst>
-__device__ inline bool test_leaves(const DevScene& S, RayState& r, const uint32_t* child, uint32_t meta0,
-                                   uint32_t meta1, uint32_t lm, TraverseStats* st) {
+// First triangle record of leaf child c (c < 8) of a linked node: tbase
+// plus the leaf counts of the slots below c (internal slots count 0), summed
+// bytewise with v_sad_u8 against zero.
+__device__ inline uint32_t leaf_first(uint32_t tbase, uint32_t meta0, uint32_t meta1, uint32_t c) {
+  const uint32_t b = 8u * c;
+  const uint32_t lo = b >= 32u ? meta0 : meta0 & ((1u << b) - 1u);
+  const uint32_t hi = b <= 32u ? 0u : meta1 & ((1u << (b - 32u)) - 1u);
+  return __builtin_amdgcn_sad_u8(lo, 0u, __builtin_amdgcn_sad_u8(hi, 0u, tbase));
+}
+
+// Intersect the triangles of the leaf children in `lm` of one node (first(c)
+// gives leaf child c's first triangle record, meta0/meta1 the leaf counts),
+// one loop over all of them so the wave iterates as often as its busiest
+// lane has triangles.  Returns true for an any-hit ray that found its hit.
+template <bool kAnyHit, bool kCount, bool kInst, class First>
+__device__ inline bool test_leaves(const DevScene& S, RayState& r, First first, uint32_t meta0, uint32_t meta1,
+                                   uint32_t lm, TraverseStats* st) {
   const float eps = kAnyHit ? kEpsShadow : kEpsTrace;
   uint32_t tri = 0, tend = 0;
   while (true) {
@@ -226,20 +252,20 @@ __device__ inline bool test_leaves(const DevScene& S, RayState& r, const uint32_
       if (lm == 0) break;
       const int c = __builtin_ctz(lm);
       lm &= lm - 1;
-      tri = select8(child, (uint32_t)c);
+      tri = first((uint32_t)c);
       tend = tri + (((c < 4 ? meta0 : meta1) >> ((c & 3) * 8)) & 0xffu);
     }
     if (kCount) st->tris++;
     uint32_t id, bary = 0;
     const float t = intersect_tri(S.tris, tri, r.o, r.d, eps, &id, &bary);
     if (kInst) id += r.id_delta;
-    if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < r.best_id))) {
+    // t == tbest < kMaxT: a best hit exists, so ray_hit_id reads a record
+    if (t < kMaxT && (t < r.tbest || (t == r.tbest && id < ray_hit_id<kInst>(S, r)))) {
       r.tbest = t;
-      r.best_id = id;
+      if (kInst) r.best_id = id;
       r.best_leaf = kInst ? tri + r.leaf_delta : tri;
       r.best_bary = bary;
       if (kAnyHit) return true;
-      r.cull = cull_bound(t);
     }
     ++tri;
   }
@@ -275,17 +301,28 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     if (inner == 0ull || 16u * (uint32_t)__popcll(pm) >= S.leaf16 * (uint32_t)__popcll(__ballot(true))) {
       if (pend) {
         const uint4* lp = reinterpret_cast<const uint4*>(S.nodes + r.pnode);
-        const uint4 C1 = lp[1], C2 = lp[2], C3 = lp[3];
-        const uint32_t pchild[8] = {C1.x, C1.y, C1.z, C1.w, C2.x, C2.y, C2.z, C2.w};
         const uint32_t lm = r.pmask;
         r.pmask = 0;
-        if (test_leaves<kAnyHit, kCount, kInst>(S, r, pchild, C3.x, C3.y, lm, st)) return true;
+        if (kInst) {
+          const uint4 C1 = lp[1], C2 = lp[2], C3 = lp[3];
+          const uint32_t pchild[8] = {C1.x, C1.y, C1.z, C1.w, C2.x, C2.y, C2.z, C2.w};
+          if (test_leaves<kAnyHit, kCount, kInst>(
+                  S, r, [&](uint32_t c) { return select8(pchild, c); }, C3.x, C3.y, lm, st))
+            return true;
+        } else {
+          const uint2 M = reinterpret_cast<const uint2*>(lp)[6];          // meta, bytes 48..55
+          const uint32_t tbase = reinterpret_cast<const uint32_t*>(lp)[28];  // byte 112
+          if (test_leaves<kAnyHit, kCount, kInst>(
+                  S, r, [&](uint32_t c) { return leaf_first(tbase, M.x, M.y, c); }, M.x, M.y, lm, st))
+            return true;
+        }
       }
       return !r.have && r.sp == 0 && r.pmask == 0u;
     }
   }
   if (pend) return false;  // waits for the leaf phase
   if (!r.have) {
+    const float cull = ray_cull<kAnyHit>(r);
     while (r.sp > 0) {
       --r.sp;
       uint32_t n;
@@ -298,7 +335,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
         n = e.x;
         tn = __uint_as_float(e.y);
       }
-      if (tn <= r.cull) {
+      if (tn <= cull) {
         r.node = n;
         r.have = true;
         break;
@@ -341,7 +378,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   if (kCount) st->nodes++;
   const uint32_t visited = r.node;
   const uint4* np = reinterpret_cast<const uint4*>(S.nodes + visited);
-  const uint4 L0 = np[0], L1 = np[1], L2 = np[2], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
+  const uint4 L0 = np[0], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
   const float ox = __uint_as_float(L0.x), oy = __uint_as_float(L0.y), oz = __uint_as_float(L0.z);
   const uint32_t ew = L0.w;
   const int nchild = (int)(ew >> 24);
@@ -358,7 +395,20 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   const uint32_t fy0 = negy ? L4.z : L5.x, fy1 = negy ? L4.w : L5.y;
   const uint32_t nz0 = negz ? L6.x : L5.z, nz1 = negz ? L6.y : L5.w;
   const uint32_t fz0 = negz ? L5.z : L6.x, fz1 = negz ? L5.w : L6.y;
-  const uint32_t child[8] = {L1.x, L1.y, L1.z, L1.w, L2.x, L2.y, L2.z, L2.w};
+  // child node of an internal slot: the instanced traversal selects among
+  // the child words (placement children break the runs), the single-level
+  // one counts the internal slots below it from ibase (L6.w)
+  uint32_t child[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (kInst) {
+    const uint4 L1 = np[1], L2 = np[2];
+    child[0] = L1.x, child[1] = L1.y, child[2] = L1.z, child[3] = L1.w;
+    child[4] = L2.x, child[5] = L2.y, child[6] = L2.z, child[7] = L2.w;
+  }
+  const uint32_t inner = ~L6.z & ((1u << nchild) - 1u) & 0xffu;
+  auto node_of = [&](uint32_t slot) {
+    return kInst ? select8(child, slot) : L6.w + (uint32_t)__popc(inner & ((1u << slot) - 1u));
+  };
+  const float cull = ray_cull<kAnyHit>(r);
   float tn[8];
   uint32_t hitmask = 0;
 #pragma unroll
@@ -376,7 +426,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     const float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
     const float tfar = fminf(fminf(tfx, tfy), tfz);
     tn[c] = tnear;
-    const bool h = tnear <= tfar && tnear <= r.cull;
+    const bool h = tnear <= tfar && tnear <= cull;
     hitmask |= h ? (1u << c) : 0u;
   }
   hitmask &= (1u << nchild) - 1u;  // the slots past the node's children
@@ -393,13 +443,13 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     // lowest visited next and the rest pushed so that lower slots pop first
     uint32_t im = hitmask & ~leafmask;
     if (im == 0) return r.sp == 0 && r.pmask == 0u;
-    r.node = select8(child, (uint32_t)__builtin_ctz(im));
+    r.node = node_of((uint32_t)__builtin_ctz(im));
     r.have = true;
     im &= im - 1u;
     while (im) {
       const int c = 31 - __builtin_clz(im);
       im &= ~(1u << c);
-      stack_push<kLds>(stk, r, select8(child, (uint32_t)c), 0.0f);
+      stack_push<kLds>(stk, r, node_of((uint32_t)c), 0.0f);
     }
     return false;
   }
@@ -409,7 +459,7 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   int m = 0;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const bool live = ((hitmask & ~leafmask) >> c) & 1u && tn[c] <= r.cull;
+    const bool live = ((hitmask & ~leafmask) >> c) & 1u && tn[c] <= cull;
     k[c] = live ? ((__float_as_uint(tn[c]) & ~7u) | (uint32_t)c) : 0xffffffffu;
     m += live ? 1 : 0;
   }
@@ -420,26 +470,17 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   cswap(k[2], k[4]); cswap(k[3], k[5]);
   cswap(k[1], k[4]); cswap(k[3], k[6]);
   cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
-  r.node = select8(child, k[0] & 7u);
+  r.node = node_of(k[0] & 7u);
   r.have = true;
 #pragma unroll
   for (int i = 7; i >= 1; --i) {
     if (i < m) {
-      const uint32_t nx = select8(child, k[i] & 7u);
+      const uint32_t nx = node_of(k[i] & 7u);
       const float t = __uint_as_float(k[i] & ~7u);  // rounded down: conservative
       stack_push<kLds>(stk, r, nx, t);
     }
   }
   return false;
-}
-
-__device__ inline Hit ray_result(const RayState& r) {
-  Hit h;
-  h.t = r.tbest;
-  h.id = r.best_id;
-  h.leaf = r.best_leaf;
-  h.bary = r.best_bary;
-  return h;
 }
 
 // ---------------------------------------------------------------------------

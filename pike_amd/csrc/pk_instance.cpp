@@ -574,7 +574,7 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
     root[m] = noff;
     toff[m] = to;
     for (pk::WNode w : b.wnodes) {
-      for (int i = 0; i < w.nchild; ++i) w.child[i] += w.meta[i] ? to : noff;
+      pk::wnode_rebase(w, noff, to);
       bvh->wnodes.push_back(w);
     }
     bvh->wtris.insert(bvh->wtris.end(), b.wtris.begin(), b.wtris.end());

@@ -218,7 +218,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
                                   Finish finish, Flush flush) {
   const int lane = stk.lane;
   RayState r;
-  r.best_id = 0xffffffffu;
+  r.best_leaf = 0xffffffffu;
   bool active = false, done = false, exhausted = false;
   uint32_t slot = 0;
   // slots this wave has claimed and not yet started (wave-uniform); claims
@@ -290,7 +290,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   const size_t wave_id = (size_t)blockIdx.x * kWpb + (size_t)wave
 
 template <bool kCount, bool kInst, int kLds, int kWpb>
-__global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_closest(DevScene S, FrameDev F, int bounce,
+__global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_closest(DevScene S, FrameDev F, int bounce,
                                                                                 uint32_t refill_min) {
   PK_TRACE_WAVE_SETUP;
   const int in = bounce & 1;
@@ -302,7 +302,9 @@ __global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_closest
   const float theta = F.params->env_theta;
   const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
-  uint32_t traced = 0, hits = 0, overflow = 0, pid = 0;
+  // traced/hits are wave-uniform (counted from ballots at each flush, which
+  // every finished ray passes once), so they cost no per-lane registers
+  uint32_t traced = 0, hits = 0, overflow = 0;
   const float4* qo = F.qo[in];
   const float4* qd = F.qd[in];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
@@ -311,23 +313,24 @@ __global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_closest
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = qo[slot], d4 = qd[slot];
-        pid = __float_as_uint(o4.w);
         ray_init<false>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
         if (F.views) r.node = F.views[__float_as_uint(d4.w)].root;
       },
-      [&](uint32_t, const RayState& r) {
-        traced++;
-        if (r.best_id == 0xffffffffu) {  // miss: shade_miss.wgsl envColor x throughput
+      [&](uint32_t slot, const RayState& r) {
+        if (ray_missed(r)) {  // miss: shade_miss.wgsl envColor x throughput
+          const uint32_t pid = __float_as_uint(qo[slot].w);  // reread: no register held through the traversal
           const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.thr[pid];
           add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
       },
       [&](bool done, uint32_t slot, const RayState& r) {
+        traced += (uint32_t)__popcll(__ballot(done));
         // hit records go to this wave's reserved run of the hit queue, in
         // the order the wave finishes them (its claimed chunk's pixel order)
-        const bool is_hit = done && r.best_id != 0xffffffffu;
+        const bool is_hit = done && !ray_missed(r);
         const unsigned long long m = __ballot(is_hit);
         const uint32_t cnt = (uint32_t)__popcll(m);
+        hits += cnt;
         if (cnt == 0) return;
         const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         const uint32_t rem = res_hi - res_lo;
@@ -342,15 +345,10 @@ __global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_closest
         } else {
           res_lo += cnt;
         }
-        if (is_hit) {
-          F.hits[at] = make_uint4(slot, r.best_leaf, __float_as_uint(r.tbest), r.best_bary);
-          hits++;
-        }
+        if (is_hit) F.hits[at] = make_uint4(slot, r.best_leaf, __float_as_uint(r.tbest), r.best_bary);
       });
   // the unused end of the wave's run: holes k_shade skips
   for (uint32_t i = res_lo + (uint32_t)lane; i < res_hi; i += 64u) F.hits[i] = make_uint4(0xffffffffu, 0u, 0u, 0u);
-  traced = wave_sum(traced);
-  hits = wave_sum(hits);
   if (kCount) {
     st.nodes = wave_sum(st.nodes);
     st.tris = wave_sum(st.tris);
@@ -817,7 +815,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
 // trace_shadow.wgsl main (:192-207) + shade_miss for unoccluded shadow rays
 // ---------------------------------------------------------------------------
 template <bool kCount, bool kInst, int kLds, int kWpb>
-__global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_shadow(DevScene S, FrameDev F,
+__global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_shadow(DevScene S, FrameDev F,
                                                                                uint32_t refill_min) {
   PK_TRACE_WAVE_SETUP;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -829,24 +827,22 @@ __global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_shadow(
   const float theta = F.params->env_theta;
   const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
-  uint32_t traced = 0, overflow = 0, pid = 0;
+  uint32_t traced = 0, overflow = 0;  // traced: wave-uniform, counted at each flush
   trace_loop<true, kCount, kInst, kLds>(
       S, n, &F.cnt[kCntWorkShadow], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = F.so[slot], d4 = F.sd[slot];
-        pid = __float_as_uint(o4.w);
         ray_init<true>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
         if (F.views) r.node = F.views[__float_as_uint(d4.w)].root;
       },
       [&](uint32_t slot, const RayState& r) {
-        traced++;
-        if (r.best_id == 0xffffffffu) {  // unoccluded: env radiance x NEE throughput
+        if (ray_missed(r)) {  // unoccluded: env radiance x NEE throughput
+          const uint32_t pid = __float_as_uint(F.so[slot].w);
           const float4 t4 = F.sthr[slot];
           add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
       },
-      [&](bool, uint32_t, const RayState&) {});
-  traced = wave_sum(traced);
+      [&](bool done, uint32_t, const RayState&) { traced += (uint32_t)__popcll(__ballot(done)); });
   if (kCount) {
     st.nodes = wave_sum(st.nodes);
     st.tris = wave_sum(st.tris);
@@ -958,7 +954,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
     if (__ballot(has) == 0ull) break;
     // closest hit (trace.wgsl) and miss shading (shade_miss.wgsl)
     RayState r;
-    r.best_id = 0xffffffffu;
+    r.best_leaf = 0xffffffffu;
     r.d = d;
     if (has) {
       ray_init<false>(r, o, d);
@@ -972,7 +968,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
     ShadeOut so;
     so.emit = so.scatter = so.bounce = so.shadow = false;
     if (has) {
-      if (r.best_id == 0xffffffffu) {
+      if (ray_missed(r)) {
         const v3 c = pkm::mul(env_color(S, P.env_theta, r.d), thr);
         L.x += c.x;
         L.y += c.y;
@@ -1000,7 +996,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
       }
       if (q.overflow) overflow = 1;
       shadows++;
-      if (q.best_id == 0xffffffffu) {
+      if (ray_missed(q)) {
         const v3 c = pkm::mul(env_color(S, P.env_theta, q.d), so.s_thr);
         L.x += c.x;
         L.y += c.y;
@@ -1244,9 +1240,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       v3 c = zero;  // this vertex's contribution (at most one of the three)
       // closest hit (trace.wgsl) and miss shading (shade_miss.wgsl)
       RayState r;
-      r.best_id = 0xffffffffu;
       r.tbest = kMaxT;
-      r.best_leaf = 0;
+      r.best_leaf = 0xffffffffu;
       r.best_bary = 0;
       if (alive) {
         ray_init<false>(r, o, d);
@@ -1254,7 +1249,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         }
         if (r.overflow) overflow = 1;
       }
-      const bool miss = alive && r.best_id == 0xffffffffu;
+      const bool miss = alive && ray_missed(r);
       if (miss) {
         c = pkm::mul(env_color(S, A.theta, d), thr);
         alive = false;
@@ -1287,7 +1282,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
         }
         if (q.overflow) overflow = 1;
-        vis = q.best_id == 0xffffffffu;
+        vis = ray_missed(q);
       }
       v3 E = zero;
       if (vis) {
@@ -1422,7 +1417,7 @@ __global__ void __launch_bounds__(256) k_loss_l2(const float4* img, const float4
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
 // ---------------------------------------------------------------------------
 template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kWpb>
-__global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_rays(DevScene S, const float4* rays,
+__global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_rays(DevScene S, const float4* rays,
                                                                              uint32_t n, int32_t* ids, float* tout,
                                                                              uint32_t* bary, uint32_t* cursor,
                                                                              uint2* spill_all,
@@ -1440,7 +1435,7 @@ __global__ void __launch_bounds__(64 * kWpb, kTraceWavesPerSimd) k_trace_rays(De
         ray_init<kAnyHit>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
       },
       [&](uint32_t i, const RayState& r) {
-        ids[i] = (int32_t)r.best_id;
+        ids[i] = (int32_t)ray_hit_id<kInst>(S, r);
         tout[i] = r.tbest;
         bary[i] = r.best_bary;
       },

@@ -29,6 +29,15 @@ constexpr double kBoxMargin = 1.0 / 131072.0;
 // meta[i] = k > 0 a leaf of k triangles starting at triangle child[i];
 // leafmask bit i = (meta[i] != 0), so traversal reads it with one byte
 // extract instead of testing eight bytes.
+//
+// The builders lay a node's children out in slot order: its internal
+// children are the consecutive nodes ibase, ibase + 1, ... and its leaf
+// children's triangle ranges follow each other from record tbase.  So the
+// single-level traversal finds child i from the header alone -- node
+// ibase + (internal slots below i), triangles tbase + (leaf counts below i)
+// -- without loading or selecting among the eight child words, which stay
+// for the instanced traversal (whose placement children break the runs)
+// and for host-side consumers.
 struct alignas(128) WNode {
   float ox, oy, oz;       // 0
   uint8_t ex, ey, ez;     // 12: biased exponents
@@ -43,9 +52,50 @@ struct alignas(128) WNode {
   uint8_t qhiz[8];        // 96
   uint8_t leafmask;       // 104
   uint8_t pad8[3];        // 105
-  uint32_t pad[5];        // 108..127
+  uint32_t ibase;         // 108: first internal child (0 without one)
+  uint32_t tbase;         // 112: first triangle record of the first leaf child (0 without one)
+  uint32_t pad[3];        // 116..127
 };
 static_assert(sizeof(WNode) == 128, "wide node must be one 128-byte line");
+
+// Sets ibase/tbase from the child words (placement children skipped).
+inline void wnode_link(WNode& w) {
+  bool have_i = false, have_t = false;
+  w.ibase = 0;
+  w.tbase = 0;
+  for (int c = 0; c < w.nchild; ++c) {
+    if (w.meta[c]) {
+      if (!have_t) w.tbase = w.child[c];
+      have_t = true;
+    } else if (!(w.child[c] & 0x80000000u)) {
+      if (!have_i) w.ibase = w.child[c];
+      have_i = true;
+    }
+  }
+}
+
+// True when the child words follow the slot-order runs from ibase/tbase.
+inline bool wnode_linked(const WNode& w) {
+  uint32_t ni = w.ibase, nt = w.tbase;
+  for (int c = 0; c < w.nchild; ++c) {
+    if (w.meta[c]) {
+      if (w.child[c] != nt) return false;
+      nt += w.meta[c];
+    } else {
+      if (w.child[c] != ni) return false;
+      ++ni;
+    }
+  }
+  return true;
+}
+
+// Moves a node into a merged layout whose nodes start at noff and triangle
+// records at toff (single-level BVHs: no placement children).
+inline void wnode_rebase(WNode& w, uint32_t noff, uint32_t toff) {
+  for (int c = 0; c < w.nchild; ++c) w.child[c] += w.meta[c] ? toff : noff;
+  w.ibase += noff;
+  w.tbase += toff;
+}
 
 struct TriRecord {        // 48 bytes
   float v0x, v0y, v0z;

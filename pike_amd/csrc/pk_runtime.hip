@@ -259,13 +259,16 @@ int pk_ctx_create(int device, pk_ctx** out) {
   auto* c = new pk_ctx();
   c->device = device;
   c->num_cus = prop.multiProcessorCount;
-  // Defaults measured on MI355X: coherent camera and shadow rays keep whole
-  // waves together (refill 64); incoherent ray queries refill early.
-  c->shape[PK_TUNE_PRIMARY] = {16, 64};
-  c->shape[PK_TUNE_SECONDARY] = {16, 64};
-  c->shape[PK_TUNE_SHADOW] = {8, 64};
-  c->shape[PK_TUNE_RAYS_CLOSEST] = {24, 16};
-  c->shape[PK_TUNE_RAYS_ANY] = {24, 24};
+  // Defaults measured on MI355X (1080p, 3.1M triangles, depth 8): the
+  // 8-entry LDS stack part at 32 waves per CU (the traversal kernels fit 64
+  // VGPRs, 8 waves per SIMD); camera rays keep whole waves together (refill
+  // 64), bounce rays refill at half a wave, shadow rays and incoherent ray
+  // queries early.
+  c->shape[PK_TUNE_PRIMARY] = {32, 64, 8, 1, 8};
+  c->shape[PK_TUNE_SECONDARY] = {32, 32, 8, 1, 8};
+  c->shape[PK_TUNE_SHADOW] = {32, 8, 8, 1, 8};
+  c->shape[PK_TUNE_RAYS_CLOSEST] = {32, 16, 8, 1, 8};
+  c->shape[PK_TUNE_RAYS_ANY] = {32, 24, 8, 1, 8};
   c->shape[PK_TUNE_PATHS] = {4 * pk::kMegaWavesPerSimd, 64};
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess)
@@ -535,6 +538,14 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
     for (size_t i = 0; i < bvh->inst.size(); ++i)
       if (bvh->inst[i].material >= (int32_t)nm) {
         pk_set_error("pk_scene_upload: instance material " + std::to_string(bvh->inst[i].material) + " out of range");
+        return PK_E_INVALID;
+      }
+  } else {
+    // the single-level traversal finds children from ibase/tbase alone
+    for (size_t i = 0; i < bvh->wnodes.size(); ++i)
+      if (!pk::wnode_linked(bvh->wnodes[i])) {
+        pk_set_error("pk_scene_upload: wide node " + std::to_string(i) +
+                     " does not keep its children in slot-order runs from ibase/tbase");
         return PK_E_INVALID;
       }
   }
@@ -1354,7 +1365,7 @@ int pk_scene_upload_batch(pk_ctx* c, const pk_scene_desc* descs, const pk_bvh* c
     const uint32_t noff = (uint32_t)merged.wnodes.size(), toff = (uint32_t)merged.wtris.size();
     roots.push_back(noff);
     for (pk::WNode w : b->wnodes) {
-      for (int i = 0; i < w.nchild; ++i) w.child[i] += w.meta[i] ? toff : noff;
+      pk::wnode_rebase(w, noff, toff);
       merged.wnodes.push_back(w);
     }
     merged.wtris.insert(merged.wtris.end(), b->wtris.begin(), b->wtris.end());
@@ -1744,7 +1755,7 @@ int pk_adjoint(pk_frame* f, pk_scene* s, const pk_render_params* p, const float*
 // ---------------------------------------------------------------------------
 namespace {
 
-const uint32_t kTuneWaves[] = {4, 8, 12, 16, 20, 24};
+const uint32_t kTuneWaves[] = {8, 12, 16, 20, 24, 28, 32, 40};
 const uint32_t kTuneRefill[] = {8, 16, 24, 32, 40, 48, 56, 64};
 const uint32_t kTuneLeaf16[] = {4, 6, 8, 10, 12};
 
@@ -1862,7 +1873,7 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
       return trial_shape(t);
     };
     for (uint32_t r : kTuneRefill)
-      if ((rc = trial(24, r))) return rc;
+      if ((rc = trial(best.waves, r))) return rc;
     const uint32_t r_best = best.refill;
     for (uint32_t w : kTuneWaves)
       if ((rc = trial(w, r_best))) return rc;

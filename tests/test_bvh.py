@@ -125,6 +125,19 @@ def test_wide_layout_is_conservative_and_complete():
     bits = (nodes["meta"] != 0) & (np.arange(8)[None, :] < nodes["nchild"][:, None])
     np.testing.assert_array_equal(nodes["leafmask"], (bits * (1 << np.arange(8))).sum(1))
     assert (nodes["leafmask"] != 0).any() and (nodes["leafmask"] != 255).any()
+    # slot-order runs: internal children are the nodes ibase, ibase + 1, ...
+    # and leaf children's triangle ranges follow each other from tbase, so the
+    # single-level traversal finds child c as ibase + (internal slots below
+    # c) or tbase + (leaf counts below c)
+    for n in nodes:
+        k = int(n["nchild"])
+        internal = [c for c in range(k) if n["meta"][c] == 0]
+        leaves = [c for c in range(k) if n["meta"][c] != 0]
+        assert [int(n["child"][c]) for c in internal] == [int(n["ibase"]) + i for i in range(len(internal))]
+        first = int(n["tbase"])
+        for c in leaves:
+            assert int(n["child"][c]) == first
+            first += int(n["meta"][c])
     _, _, verts, _ = b.export_reference()
     V = verts[:, :3].astype(np.float64)
     seen = np.zeros(len(tris), np.int64)
