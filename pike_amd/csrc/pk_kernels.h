@@ -107,7 +107,11 @@ void launch_raygen(const FrameDev& F, hipStream_t s);
 // traversal kernels come in stack_lds = 8, 12 or 16 LDS entries and 1, 2 or
 // 4 waves per block (the counting and instanced variants: 12 and 1)
 struct TraceLaunch {
-  uint32_t grid;
+  uint32_t grid;        // waves of the shape's variant (a multiple of waves_per_block)
+  // waves of the counting and instanced variants (kStackLds entries, one
+  // wave per block, 6 waves per SIMD): the shape's waves per CU capped at the
+  // 24 those keep resident, so no wave of a persistent launch waits for a slot
+  uint32_t fixed_grid;
   uint32_t refill;
   bool count;
   uint2* spill;  // grid * kSpillStride * 64 entries

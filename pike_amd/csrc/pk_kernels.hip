@@ -1511,9 +1511,9 @@ void launch_trace_closest(const DevScene& S0, const FrameDev& F0, int bounce, co
   FrameDev F = F0;
   F.spill = L.spill;
   const bool inst = S.num_inst != 0;
-  if (L.count && inst) hipLaunchKernelGGL((k_trace_closest<true, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
-  else if (L.count) hipLaunchKernelGGL((k_trace_closest<true, false, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
-  else if (inst) hipLaunchKernelGGL((k_trace_closest<false, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  if (L.count && inst) hipLaunchKernelGGL((k_trace_closest<true, true, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else if (L.count) hipLaunchKernelGGL((k_trace_closest<true, false, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, bounce, L.refill);
+  else if (inst) hipLaunchKernelGGL((k_trace_closest<false, true, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, bounce, L.refill);
   else
     with_variant(L, [&](auto lds, auto wpb) {
       constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
@@ -1529,9 +1529,9 @@ void launch_trace_shadow(const DevScene& S0, const FrameDev& F0, const TraceLaun
   FrameDev F = F0;
   F.spill = L.spill;
   const bool inst = S.num_inst != 0;
-  if (L.count && inst) hipLaunchKernelGGL((k_trace_shadow<true, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
-  else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
-  else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, F, L.refill);
+  if (L.count && inst) hipLaunchKernelGGL((k_trace_shadow<true, true, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, L.refill);
+  else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, L.refill);
+  else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, L.refill);
   else
     with_variant(L, [&](auto lds, auto wpb) {
       constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
@@ -1578,7 +1578,7 @@ void launch_trace_rays(const DevScene& S0, const float4* rays, uint32_t n, int a
                        uint32_t* overflow, const TraceLaunch& L, hipStream_t s) {
   const DevScene S = with_limit(S0, L.stack_limit, L.leaf16);
 #define PK_LAUNCH_RAYS(A, C, I)                                                                                    \
-  hipLaunchKernelGGL((k_trace_rays<A, C, I, kStackLds, 1>), dim3(L.grid), dim3(64), 0, s, S, rays, n, ids, t, bary,   \
+  hipLaunchKernelGGL((k_trace_rays<A, C, I, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, rays, n, ids, t, bary,   \
                      cursor, spill, stats, overflow, L.refill)
   const int variant = (any_hit ? 4 : 0) | (L.count ? 2 : 0) | (S.num_inst ? 1 : 0);
   switch (variant) {

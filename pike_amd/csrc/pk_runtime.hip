@@ -167,7 +167,8 @@ pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
   const pk_ctx::Shape& sh = c->shape[cls];
   // whole blocks: the grid (in waves) is a multiple of the waves per block
   const uint32_t grid = std::max(sh.wpb, (uint32_t)c->num_cus * sh.waves / sh.wpb * sh.wpb);
-  return pk::TraceLaunch{grid, sh.refill, count, c->spill, c->stack_limit, sh.lds, sh.wpb, sh.leaf16};
+  const uint32_t fixed_grid = (uint32_t)c->num_cus * std::min(sh.waves, 4u * pk::kTraceWavesPerSimd);
+  return pk::TraceLaunch{grid, fixed_grid, sh.refill, count, c->spill, c->stack_limit, sh.lds, sh.wpb, sh.leaf16};
 }
 pk::TraceLaunch trace_launch(const pk_frame* f, int cls) { return shape_launch(f->ctx, cls, f->count_nodes); }
 int closest_class(uint32_t bounce) { return bounce == 0 ? PK_TUNE_PRIMARY : PK_TUNE_SECONDARY; }
@@ -1174,6 +1175,7 @@ int pk_trace_rays_noalloc(pk_scene* s, const float* d_rays, int64_t n, int any_h
   uint2* spill = reinterpret_cast<uint2*>(c->scratch + kNoAllocCursorBytes);
   pk::TraceLaunch L = shape_launch(c, any_hit ? PK_TUNE_RAYS_ANY : PK_TUNE_RAYS_CLOSEST, false);
   L.grid = std::min(L.grid, kNoAllocMaxWaves);
+  L.fixed_grid = std::min(L.fixed_grid, kNoAllocMaxWaves);
   L.spill = spill;
   // cleared by kernels (no memset nodes if a caller captures this into a graph)
   pk::launch_fill_u32(cursor, 0u, 1, c->stream);
