@@ -87,10 +87,11 @@ def measured_counters(kernel):
 
 def apply_shapes(ctx, shapes):
     """Launch shapes {class: {waves_per_cu, refill_min[, stack_lds,
-    waves_per_block, leaf_phase16]}} (an autotune record) onto a context."""
+    waves_per_block, leaf_phase16, node_width]}} (an autotune record) onto a
+    context."""
     for cls, sh in shapes.items():
         ctx.set_tuning(cls, sh["waves_per_cu"], sh["refill_min"], sh.get("stack_lds"), sh.get("waves_per_block"),
-                       sh.get("leaf_phase16"))
+                       sh.get("leaf_phase16"), sh.get("node_width"))
 
 
 def run_render(args, pike_amd, scenes):

@@ -150,14 +150,20 @@ int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_c
  *   waves_per_block  waves of one block (1, 2 or 4), each independent
  *   leaf_phase16     the wave runs its deferred leaf tests together once
  *                    this many 16ths of its tracing lanes hold some, [1, 16]
- * The path megakernel class (PK_TUNE_PATHS) takes stack_lds 12 and 1 wave
- * per block.  Every choice traces the same hits and renders the same image. */
+ *   node_width       BVH nodes the traversal walks: 8 (128-byte nodes) or 4
+ *                    (the 64-byte layout pk_bvh_export_wide4 describes; with
+ *                    stack_lds 8 or 12 and 1 wave per block); instanced
+ *                    scenes and the counting variants walk 8-wide nodes
+ * The path megakernel class (PK_TUNE_PATHS) takes stack_lds 12, 1 wave per
+ * block and node_width 8.  Every choice traces the same hits and renders the
+ * same image. */
 typedef struct pk_launch_shape {
   uint32_t waves_per_cu;
   uint32_t refill_min;
   uint32_t stack_lds;
   uint32_t waves_per_block;
   uint32_t leaf_phase16;
+  uint32_t node_width;
 } pk_launch_shape;
 int pk_ctx_set_launch(pk_ctx* ctx, int kernel_class, const pk_launch_shape* shape);
 int pk_ctx_get_launch(const pk_ctx* ctx, int kernel_class, pk_launch_shape* shape);
@@ -260,6 +266,11 @@ int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, 
  * CPU traversal cost model): wide_nodes * 128 bytes, leaf_triangles * 48
  * bytes of triangle records, and leaf_triangles * 4 words {i0,i1,i2,mat}. */
 int pk_bvh_export_wide(const pk_bvh* bvh, void* nodes128, void* tris48, int32_t* tri_index4);
+/* The 4-wide layout derived from a single-level BVH's wide layout (no
+ * reference counterpart; the layout the node_width 4 traversal reads):
+ * *count 64-byte nodes.  With nodes64 null, only *count is set; otherwise
+ * *count is the capacity of nodes64 in nodes on entry. */
+int pk_bvh_export_wide4(const pk_bvh* bvh, void* nodes64, int64_t* count);
 /* The normalizer (center xyz, scale) of any BVH, instanced ones included. */
 int pk_bvh_normalizer(const pk_bvh* bvh, double* normalizer4);
 

@@ -79,7 +79,7 @@ class Context:
         return {c: n for n, c in self.SCHEDULERS.items()}[v.value]
 
     def set_tuning(self, kernel_class, waves_per_cu, refill_min, stack_lds=None, waves_per_block=None,
-                   leaf_phase16=None):
+                   leaf_phase16=None, node_width=None):
         """Launch shape of a kernel class; the fields left None keep their value."""
         cls = self.TUNE_CLASSES[kernel_class] if isinstance(kernel_class, str) else kernel_class
         sh = _ffi.LaunchShape()
@@ -91,6 +91,8 @@ class Context:
             sh.waves_per_block = waves_per_block
         if leaf_phase16 is not None:
             sh.leaf_phase16 = leaf_phase16
+        if node_width is not None:
+            sh.node_width = node_width
         _check(lib().pk_ctx_set_launch(self.h, cls, ctypes.byref(sh)), "pk_ctx_set_launch")
 
     def tuning(self):
@@ -99,7 +101,8 @@ class Context:
             sh = _ffi.LaunchShape()
             _check(lib().pk_ctx_get_launch(self.h, cls, ctypes.byref(sh)), "pk_ctx_get_launch")
             out[name] = dict(waves_per_cu=sh.waves_per_cu, refill_min=sh.refill_min, stack_lds=sh.stack_lds,
-                             waves_per_block=sh.waves_per_block, leaf_phase16=sh.leaf_phase16, node_width=8)
+                             waves_per_block=sh.waves_per_block, leaf_phase16=sh.leaf_phase16,
+                             node_width=sh.node_width)
         return out
 
     def set_stack_limit(self, entries):
@@ -248,6 +251,10 @@ class BVH:
                                 ("qhiy", "u1", 8), ("qloz", "u1", 8), ("qhiz", "u1", 8), ("leafmask", "u1"),
                                 ("pad8", "u1", 3), ("ibase", "<u4"), ("tbase", "<u4"),
                                 ("pad", "<u4", 3)])
+    WIDE4_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "u1", 3), ("nchild", "u1"), ("meta", "u1", 4),
+                                 ("qlox", "u1", 4), ("qhix", "u1", 4), ("qloy", "u1", 4), ("qhiy", "u1", 4),
+                                 ("qloz", "u1", 4), ("qhiz", "u1", 4), ("leafmask", "u1"), ("pad8", "u1", 3),
+                                 ("ibase", "<u4"), ("tbase", "<u4"), ("pad", "<u4", 2)])
     TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("id", "<u4"), ("e1", "<f4", 3), ("pad0", "<u4"), ("e2", "<f4", 3),
                           ("pad1", "<u4")])
 
@@ -260,6 +267,16 @@ class BVH:
         assert nodes.itemsize == 128 and tris.itemsize == 48
         _check(lib().pk_bvh_export_wide(self.h, _ptr(nodes), _ptr(tris), _ptr(tidx)), "pk_bvh_export_wide")
         return nodes, tris, tidx
+
+    def export_wide4(self):
+        """Nodes of the 64-byte 4-wide layout derived from the 128-byte one
+        (same triangle records)."""
+        n = ctypes.c_int64(0)
+        _check(lib().pk_bvh_export_wide4(self.h, None, ctypes.byref(n)), "pk_bvh_export_wide4")
+        nodes = np.zeros(n.value, self.WIDE4_NODE_DTYPE)
+        assert nodes.itemsize == 64
+        _check(lib().pk_bvh_export_wide4(self.h, _ptr(nodes), ctypes.byref(n)), "pk_bvh_export_wide4")
+        return nodes
 
     def normalizer(self):
         norm = np.zeros(4, np.float64)

@@ -81,7 +81,7 @@ class SceneDesc(ctypes.Structure):
 
 class LaunchShape(ctypes.Structure):
     _fields_ = [("waves_per_cu", c_uint32), ("refill_min", c_uint32), ("stack_lds", c_uint32),
-                ("waves_per_block", c_uint32), ("leaf_phase16", c_uint32)]
+                ("waves_per_block", c_uint32), ("leaf_phase16", c_uint32), ("node_width", c_uint32)]
 
 
 class Camera(ctypes.Structure):
@@ -174,6 +174,7 @@ SIGNATURES = {
     "pk_bvh_info": (c_int, [c_void_p, c_void_p]),
     "pk_bvh_export_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pk_bvh_export_wide": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "pk_bvh_export_wide4": (c_int, [c_void_p, c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "pk_bvh_normalizer": (c_int, [c_void_p, c_void_p]),
     "pk_bvh_build_instanced": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_int, c_uint32,
                                        ctypes.POINTER(c_void_p)]),

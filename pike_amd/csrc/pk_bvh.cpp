@@ -667,6 +667,95 @@ void quantize_node(pk::WNode& w, const FBox* cb, int nc) {
 
 void pk_quantize_node(pk::WNode& w, const FBox* cb, int nc) { quantize_node(w, cb, nc); }
 
+namespace {
+// Slots [b, e) of 8-wide node w as a 4-wide node (e - b <= 4) in w's frame.
+pk::WNode4 wide4_part(const pk::WNode& w, int b, int e) {
+  pk::WNode4 n;
+  std::memset(&n, 0, sizeof(n));
+  n.ox = w.ox, n.oy = w.oy, n.oz = w.oz;
+  n.ex = w.ex, n.ey = w.ey, n.ez = w.ez;
+  n.nchild = (uint8_t)(e - b);
+  uint32_t ib = w.ibase, tb = w.tbase;
+  for (int c = 0; c < b; ++c) {
+    if (w.meta[c]) tb += w.meta[c];
+    else ++ib;
+  }
+  n.ibase = ib;
+  n.tbase = tb;
+  for (int i = 0; i < 4; ++i) {
+    const int c = b + i;
+    const bool on = c < e;
+    n.meta[i] = on ? w.meta[c] : 0;
+    n.qlox[i] = on ? w.qlox[c] : 255, n.qhix[i] = on ? w.qhix[c] : 0;
+    n.qloy[i] = on ? w.qloy[c] : 255, n.qhiy[i] = on ? w.qhiy[c] : 0;
+    n.qloz[i] = on ? w.qloz[c] : 255, n.qhiz[i] = on ? w.qhiz[c] : 0;
+    if (on && w.meta[c]) n.leafmask |= (uint8_t)(1u << i);
+  }
+  return n;
+}
+
+// Surface area (in quantization steps of w's frame) of the box around
+// slots [b, e).
+double wide4_area(const pk::WNode& w, int b, int e) {
+  const double sx = std::ldexp(1.0, (int)w.ex - 127), sy = std::ldexp(1.0, (int)w.ey - 127),
+               sz = std::ldexp(1.0, (int)w.ez - 127);
+  int lo[3] = {255, 255, 255}, hi[3] = {0, 0, 0};
+  for (int c = b; c < e; ++c) {
+    lo[0] = std::min(lo[0], (int)w.qlox[c]), hi[0] = std::max(hi[0], (int)w.qhix[c]);
+    lo[1] = std::min(lo[1], (int)w.qloy[c]), hi[1] = std::max(hi[1], (int)w.qhiy[c]);
+    lo[2] = std::min(lo[2], (int)w.qloz[c]), hi[2] = std::max(hi[2], (int)w.qhiz[c]);
+  }
+  const double dx = (hi[0] - lo[0]) * sx, dy = (hi[1] - lo[1]) * sy, dz = (hi[2] - lo[2]) * sz;
+  return dx * dy + dy * dz + dz * dx;
+}
+}  // namespace
+
+void pk_make_wide4(const std::vector<pk::WNode>& n8, std::vector<pk::WNode4>& n4) {
+  n4.assign(n8.size(), pk::WNode4{});
+  for (size_t i = 0; i < n8.size(); ++i) {
+    const pk::WNode& w = n8[i];
+    const int n = w.nchild;
+    if (n <= 4) {
+      n4[i] = wide4_part(w, 0, n);
+      continue;
+    }
+    // one cut s (slots [0, s) and [s, n), at most four each) with the least
+    // summed surface area: a ray that enters the node visits each sub-node
+    // in proportion to its area
+    int best_s = 4;
+    double best = INFINITY;
+    for (int s = n - 4; s <= 4; ++s) {
+      const double a = wide4_area(w, 0, s) + wide4_area(w, s, n);
+      if (a < best) {
+        best = a;
+        best_s = s;
+      }
+    }
+    const uint32_t sub = (uint32_t)n4.size();
+    pk::WNode4 root;
+    std::memset(&root, 0, sizeof(root));
+    root.ox = w.ox, root.oy = w.oy, root.oz = w.oz;
+    root.ex = w.ex, root.ey = w.ey, root.ez = w.ez;
+    root.nchild = 2;
+    root.ibase = sub;
+    const int cut[3] = {0, best_s, n};
+    for (int h = 0; h < 4; ++h) {
+      uint8_t lx = 255, hx = 0, ly = 255, hy = 0, lz = 255, hz = 0;
+      if (h < 2)
+        for (int c = cut[h]; c < cut[h + 1]; ++c) {
+          lx = std::min(lx, w.qlox[c]), hx = std::max(hx, w.qhix[c]);
+          ly = std::min(ly, w.qloy[c]), hy = std::max(hy, w.qhiy[c]);
+          lz = std::min(lz, w.qloz[c]), hz = std::max(hz, w.qhiz[c]);
+        }
+      root.qlox[h] = lx, root.qhix[h] = hx, root.qloy[h] = ly, root.qhiy[h] = hy;
+      root.qloz[h] = lz, root.qhiz[h] = hz;
+    }
+    n4[i] = root;
+    n4.push_back(wide4_part(w, 0, best_s));
+    n4.push_back(wide4_part(w, best_s, n));
+  }
+}
+
 void pk_bvh_make_wide(pk_bvh* bvh);
 
 extern "C" {
@@ -883,6 +972,28 @@ int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, 
     for (int k = 0; k < 3; ++k) norm4[k] = bvh->norm_center[k];
     norm4[3] = bvh->norm_scale;
   }
+  return PK_OK;
+}
+
+int pk_bvh_export_wide4(const pk_bvh* bvh, void* nodes64, int64_t* count) {
+  if (!bvh || !count) {
+    pk_set_error("pk_bvh_export_wide4: null argument");
+    return PK_E_INVALID;
+  }
+  if (bvh->instanced) {
+    pk_set_error("pk_bvh_export_wide4: an instanced BVH has no 4-wide layout");
+    return PK_E_INVALID;
+  }
+  std::vector<pk::WNode4> n4;
+  pk_make_wide4(bvh->wnodes, n4);
+  if (nodes64) {
+    if (*count < (int64_t)n4.size()) {
+      pk_set_error("pk_bvh_export_wide4: output holds fewer than " + std::to_string(n4.size()) + " nodes");
+      return PK_E_LIMIT;
+    }
+    std::memcpy(nodes64, n4.data(), n4.size() * sizeof(pk::WNode4));
+  }
+  *count = (int64_t)n4.size();
   return PK_OK;
 }
 

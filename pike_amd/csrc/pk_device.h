@@ -16,6 +16,7 @@ using pkm::v3;
 
 struct DevScene {
   const WNode* nodes;
+  const WNode4* nodes4;      // the derived 4-wide layout (single-level scenes; else null)
   const float4* tris;        // TriRecord as 3 float4, leaf order
   // Shading record per leaf-order triangle, 64 bytes: the quantized
   // attributes {tangent, bitangent, normal, uv} of its vertices i0, i1, i2
@@ -285,9 +286,12 @@ __device__ inline bool test_leaves(const DevScene& S, RayState& r, First first, 
 // on at the mesh's root; popping the marker maps the ray back.  Distances
 // are the same parameter in both spaces (the map is affine and the direction
 // is not renormalized), so tbest and the culling bound carry over.
+// kW is the node width the single-level traversal walks: 8 (WNode) or 4
+// (WNode4, the derived layout); the instanced traversal walks 8-wide nodes.
 // Returns true when the ray is finished.
-template <bool kAnyHit, bool kCount, bool kInst = false, int kLds = kStackLds>
+template <bool kAnyHit, bool kCount, bool kInst = false, int kLds = kStackLds, int kW = 8>
 __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
+  static_assert(kW == 8 || (kW == 4 && !kInst), "4-wide nodes are single-level only");
   const int lane = stk.lane;
   // Leaf phase.  A node visit defers its hit leaf children; the lane then
   // waits while the others visit nodes, until at least S.leaf16 / 16 of the
@@ -303,7 +307,13 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
         const uint4* lp = reinterpret_cast<const uint4*>(S.nodes + r.pnode);
         const uint32_t lm = r.pmask;
         r.pmask = 0;
-        if (kInst) {
+        if (kW == 4) {
+          const uint32_t* w4 = reinterpret_cast<const uint32_t*>(S.nodes4 + r.pnode);
+          const uint32_t meta = w4[4], tbase = w4[13];  // bytes 16..19 and 52
+          if (test_leaves<kAnyHit, kCount, kInst>(
+                  S, r, [&](uint32_t c) { return leaf_first(tbase, meta, 0u, c); }, meta, 0u, lm, st))
+            return true;
+        } else if (kInst) {
           const uint4 C1 = lp[1], C2 = lp[2], C3 = lp[3];
           const uint32_t pchild[8] = {C1.x, C1.y, C1.z, C1.w, C2.x, C2.y, C2.z, C2.w};
           if (test_leaves<kAnyHit, kCount, kInst>(
@@ -377,8 +387,29 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   r.have = false;
   if (kCount) st->nodes++;
   const uint32_t visited = r.node;
-  const uint4* np = reinterpret_cast<const uint4*>(S.nodes + visited);
-  const uint4 L0 = np[0], L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
+  // header (origin, exponents, child count) and, per axis and ray direction
+  // sign, the near and far plane bytes (kW / 4 words each), leaf mask and ibase
+  constexpr int kWords = kW / 4;
+  const uint4* np = kW == 8 ? reinterpret_cast<const uint4*>(S.nodes + visited)
+                            : reinterpret_cast<const uint4*>(S.nodes4 + visited);
+  const uint4 L0 = np[0];
+  uint32_t qlx[kWords], qhx[kWords], qly[kWords], qhy[kWords], qlz[kWords], qhz[kWords];
+  uint32_t leafmask, ibase;
+  uint4 L1i = make_uint4(0u, 0u, 0u, 0u), L2i = L1i;  // child words (instanced traversal)
+  if constexpr (kW == 8) {
+    if constexpr (kInst) L1i = np[1], L2i = np[2];
+    const uint4 L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
+    qlx[0] = L3.z, qlx[1] = L3.w, qhx[0] = L4.x, qhx[1] = L4.y;
+    qly[0] = L4.z, qly[1] = L4.w, qhy[0] = L5.x, qhy[1] = L5.y;
+    qlz[0] = L5.z, qlz[1] = L5.w, qhz[0] = L6.x, qhz[1] = L6.y;
+    leafmask = L6.z & 0xffu;
+    ibase = L6.w;
+  } else {
+    const uint4 L1 = np[1], L2 = np[2];
+    qlx[0] = L1.y, qhx[0] = L1.z, qly[0] = L1.w, qhy[0] = L2.x, qlz[0] = L2.y, qhz[0] = L2.z;
+    leafmask = L2.w & 0xffu;
+    ibase = reinterpret_cast<const uint32_t*>(np)[12];
+  }
   const float ox = __uint_as_float(L0.x), oy = __uint_as_float(L0.y), oz = __uint_as_float(L0.z);
   const uint32_t ew = L0.w;
   const int nchild = (int)(ew >> 24);
@@ -389,34 +420,34 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   const float bx = (ox - r.o.x) * r.invx, by = (oy - r.o.y) * r.invy, bz = (oz - r.o.z) * r.invz;
   const bool negx = r.invx < 0.0f, negy = r.invy < 0.0f, negz = r.invz < 0.0f;
   // near/far plane bytes per axis by ray direction sign
-  const uint32_t nx0 = negx ? L4.x : L3.z, nx1 = negx ? L4.y : L3.w;
-  const uint32_t fx0 = negx ? L3.z : L4.x, fx1 = negx ? L3.w : L4.y;
-  const uint32_t ny0 = negy ? L5.x : L4.z, ny1 = negy ? L5.y : L4.w;
-  const uint32_t fy0 = negy ? L4.z : L5.x, fy1 = negy ? L4.w : L5.y;
-  const uint32_t nz0 = negz ? L6.x : L5.z, nz1 = negz ? L6.y : L5.w;
-  const uint32_t fz0 = negz ? L5.z : L6.x, fz1 = negz ? L5.w : L6.y;
+  uint32_t nxw[kWords], fxw[kWords], nyw[kWords], fyw[kWords], nzw[kWords], fzw[kWords];
+#pragma unroll
+  for (int i = 0; i < kWords; ++i) {
+    nxw[i] = negx ? qhx[i] : qlx[i], fxw[i] = negx ? qlx[i] : qhx[i];
+    nyw[i] = negy ? qhy[i] : qly[i], fyw[i] = negy ? qly[i] : qhy[i];
+    nzw[i] = negz ? qhz[i] : qlz[i], fzw[i] = negz ? qlz[i] : qhz[i];
+  }
   // child node of an internal slot: the instanced traversal selects among
   // the child words (placement children break the runs), the single-level
-  // one counts the internal slots below it from ibase (L6.w)
+  // one counts the internal slots below it from ibase
   uint32_t child[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (kInst) {
-    const uint4 L1 = np[1], L2 = np[2];
-    child[0] = L1.x, child[1] = L1.y, child[2] = L1.z, child[3] = L1.w;
-    child[4] = L2.x, child[5] = L2.y, child[6] = L2.z, child[7] = L2.w;
+  if constexpr (kInst) {
+    child[0] = L1i.x, child[1] = L1i.y, child[2] = L1i.z, child[3] = L1i.w;
+    child[4] = L2i.x, child[5] = L2i.y, child[6] = L2i.z, child[7] = L2i.w;
   }
-  const uint32_t inner = ~L6.z & ((1u << nchild) - 1u) & 0xffu;
+  const uint32_t inner = ~leafmask & ((1u << nchild) - 1u) & 0xffu;
   auto node_of = [&](uint32_t slot) {
-    return kInst ? select8(child, slot) : L6.w + (uint32_t)__popc(inner & ((1u << slot) - 1u));
+    return kInst ? select8(child, slot) : ibase + (uint32_t)__popc(inner & ((1u << slot) - 1u));
   };
   const float cull = ray_cull<kAnyHit>(r);
-  float tn[8];
+  float tn[kW];
   uint32_t hitmask = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
+  for (int c = 0; c < kW; ++c) {
     const int sh = (c & 3) * 8;
-    const uint32_t wnx = c < 4 ? nx0 : nx1, wfx = c < 4 ? fx0 : fx1;
-    const uint32_t wny = c < 4 ? ny0 : ny1, wfy = c < 4 ? fy0 : fy1;
-    const uint32_t wnz = c < 4 ? nz0 : nz1, wfz = c < 4 ? fz0 : fz1;
+    const uint32_t wnx = nxw[c >> 2], wfx = fxw[c >> 2];
+    const uint32_t wny = nyw[c >> 2], wfy = fyw[c >> 2];
+    const uint32_t wnz = nzw[c >> 2], wfz = fzw[c >> 2];
     const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bx);
     const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, by);
     const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bz);
@@ -430,7 +461,6 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     hitmask |= h ? (1u << c) : 0u;
   }
   hitmask &= (1u << nchild) - 1u;  // the slots past the node's children
-  const uint32_t leafmask = L6.z & 0xffu;
   // hit leaf children: intersected in the next leaf phase
   const uint32_t lmask = hitmask & leafmask;
   if (lmask) {
@@ -455,25 +485,31 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
   }
   // internal children, sorted near-to-far: key = entry distance bits (>= 0,
   // so order-preserving) with the slot in the low 3 bits
-  uint32_t k[8];
+  uint32_t k[kW];
   int m = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
+  for (int c = 0; c < kW; ++c) {
     const bool live = ((hitmask & ~leafmask) >> c) & 1u && tn[c] <= cull;
     k[c] = live ? ((__float_as_uint(tn[c]) & ~7u) | (uint32_t)c) : 0xffffffffu;
     m += live ? 1 : 0;
   }
   if (m == 0) return r.sp == 0 && r.pmask == 0u;
-  cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
-  cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
-  cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
-  cswap(k[2], k[4]); cswap(k[3], k[5]);
-  cswap(k[1], k[4]); cswap(k[3], k[6]);
-  cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
+  if constexpr (kW == 8) {
+    cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
+    cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
+    cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
+    cswap(k[2], k[4]); cswap(k[3], k[5]);
+    cswap(k[1], k[4]); cswap(k[3], k[6]);
+    cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
+  } else {
+    cswap(k[0], k[1]); cswap(k[2], k[3]);
+    cswap(k[0], k[2]); cswap(k[1], k[3]);
+    cswap(k[1], k[2]);
+  }
   r.node = node_of(k[0] & 7u);
   r.have = true;
 #pragma unroll
-  for (int i = 7; i >= 1; --i) {
+  for (int i = kW - 1; i >= 1; --i) {
     if (i < m) {
       const uint32_t nx = node_of(k[i] & 7u);
       const float t = __uint_as_float(k[i] & ~7u);  // rounded down: conservative

@@ -57,6 +57,10 @@ struct pk_bvh {
 // Conservative 8-bit quantization of wide-node child boxes (pk_bvh.cpp).
 void pk_quantize_node(pk::WNode& w, const FBox* cb, int nc);
 
+// The 4-wide layout of a linked single-level 8-wide layout (pk_bvh.cpp):
+// node i < n8.size() stands for 8-wide node i, sub-nodes follow.
+void pk_make_wide4(const std::vector<pk::WNode>& n8, std::vector<pk::WNode4>& n4);
+
 // Environment importance-sampling tables (render/env_sampler.js).
 struct EnvTables {
   uint32_t width = 0, height = 0;

@@ -119,6 +119,7 @@ struct TraceLaunch {
   uint32_t stack_lds;
   uint32_t waves_per_block;
   uint32_t leaf16;       // leaf phase trigger in 16ths of the tracing lanes
+  uint32_t node_width;   // 8 (WNode) or 4 (WNode4, single-level scenes)
 };
 
 void launch_trace_closest(const DevScene& S, const FrameDev& F, int bounce, const TraceLaunch& L, hipStream_t s);

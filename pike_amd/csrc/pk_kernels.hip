@@ -212,7 +212,7 @@ __device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v
 // loop once the queue is exhausted and its own lanes are done.
 constexpr uint32_t kClaimChunk = 256;  // slots per cursor atomic (>= 64)
 
-template <bool kAnyHit, bool kCount, bool kInst, int kLds, class Load, class Finish, class Flush>
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kNW, class Load, class Finish, class Flush>
 __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* cursor, uint32_t refill_min,
                                   const Stack& stk, TraverseStats* st, uint32_t* overflow, Load load,
                                   Finish finish, Flush flush) {
@@ -227,7 +227,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = false;
   while (true) {
-    if (active && ray_step<kAnyHit, kCount, kInst, kLds>(S, r, stk, st)) {
+    if (active && ray_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st)) {
       active = false;
       done = true;
       if (r.overflow) *overflow = 1;
@@ -289,7 +289,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   const int wave = kWpb == 1 ? 0 : (int)(threadIdx.x >> 6);                                                      \
   const size_t wave_id = (size_t)blockIdx.x * kWpb + (size_t)wave
 
-template <bool kCount, bool kInst, int kLds, int kWpb>
+template <bool kCount, bool kInst, int kLds, int kWpb, int kNW = 8>
 __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_closest(DevScene S, FrameDev F, int bounce,
                                                                                 uint32_t refill_min) {
   PK_TRACE_WAVE_SETUP;
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   const float4* qd = F.qd[in];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   uint32_t res_lo = 0, res_hi = 0;  // this wave's unused run of hit slots (wave-uniform)
-  trace_loop<false, kCount, kInst, kLds>(
+  trace_loop<false, kCount, kInst, kLds, kNW>(
       S, n, &F.cnt[kCntWorkTrace], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = qo[slot], d4 = qd[slot];
@@ -814,7 +814,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
 // ---------------------------------------------------------------------------
 // trace_shadow.wgsl main (:192-207) + shade_miss for unoccluded shadow rays
 // ---------------------------------------------------------------------------
-template <bool kCount, bool kInst, int kLds, int kWpb>
+template <bool kCount, bool kInst, int kLds, int kWpb, int kNW = 8>
 __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_shadow(DevScene S, FrameDev F,
                                                                                uint32_t refill_min) {
   PK_TRACE_WAVE_SETUP;
@@ -828,7 +828,7 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, overflow = 0;  // traced: wave-uniform, counted at each flush
-  trace_loop<true, kCount, kInst, kLds>(
+  trace_loop<true, kCount, kInst, kLds, kNW>(
       S, n, &F.cnt[kCntWorkShadow], refill_min, stk, &st, &overflow,
       [&](uint32_t slot, RayState& r) {
         const float4 o4 = F.so[slot], d4 = F.sd[slot];
@@ -1416,7 +1416,7 @@ __global__ void __launch_bounds__(256) k_loss_l2(const float4* img, const float4
 // ---------------------------------------------------------------------------
 // Standalone ray queries (traversal microbenchmark, pk_trace_rays)
 // ---------------------------------------------------------------------------
-template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kWpb>
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kWpb, int kNW = 8>
 __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_rays(DevScene S, const float4* rays,
                                                                              uint32_t n, int32_t* ids, float* tout,
                                                                              uint32_t* bary, uint32_t* cursor,
@@ -1428,7 +1428,7 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   const Stack stk{lds_node[wave], lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
-  trace_loop<kAnyHit, kCount, kInst, kLds>(
+  trace_loop<kAnyHit, kCount, kInst, kLds, kNW>(
       S, n, cursor, refill_min, stk, &st, &overflow,
       [&](uint32_t i, RayState& r) {
         const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
@@ -1481,27 +1481,35 @@ static inline DevScene with_limit(const DevScene& S0, uint32_t stack_limit, uint
   return S;
 }
 
-// Calls f(integral_constant kLds, integral_constant kWpb) for the launch
-// shape's variant; unsupported combinations take 12 entries, one wave.
+// Calls f(kLds, kWpb, kNW) (integral constants) for the launch shape's
+// variant: 8-wide nodes with 8, 12 or 16 LDS entries and 1, 2 or 4 waves per
+// block, 4-wide nodes with 8 or 12 entries and one wave per block (the
+// context validates shapes; anything else takes 12 entries, one wave).
+template <int v>
+using ic = std::integral_constant<int, v>;
 template <class Fn>
 static inline void with_variant(const TraceLaunch& L, Fn f) {
-  using std::integral_constant;
   const uint32_t w = L.waves_per_block;
+  if (L.node_width == 4) {
+    if (L.stack_lds == 8) f(ic<8>(), ic<1>(), ic<4>());
+    else f(ic<12>(), ic<1>(), ic<4>());
+    return;
+  }
   switch (L.stack_lds) {
     case 8:
-      if (w == 2) f(integral_constant<int, 8>(), integral_constant<int, 2>());
-      else if (w == 4) f(integral_constant<int, 8>(), integral_constant<int, 4>());
-      else f(integral_constant<int, 8>(), integral_constant<int, 1>());
+      if (w == 2) f(ic<8>(), ic<2>(), ic<8>());
+      else if (w == 4) f(ic<8>(), ic<4>(), ic<8>());
+      else f(ic<8>(), ic<1>(), ic<8>());
       break;
     case 16:
-      if (w == 2) f(integral_constant<int, 16>(), integral_constant<int, 2>());
-      else if (w == 4) f(integral_constant<int, 16>(), integral_constant<int, 4>());
-      else f(integral_constant<int, 16>(), integral_constant<int, 1>());
+      if (w == 2) f(ic<16>(), ic<2>(), ic<8>());
+      else if (w == 4) f(ic<16>(), ic<4>(), ic<8>());
+      else f(ic<16>(), ic<1>(), ic<8>());
       break;
     default:
-      if (w == 2) f(integral_constant<int, 12>(), integral_constant<int, 2>());
-      else if (w == 4) f(integral_constant<int, 12>(), integral_constant<int, 4>());
-      else f(integral_constant<int, 12>(), integral_constant<int, 1>());
+      if (w == 2) f(ic<12>(), ic<2>(), ic<8>());
+      else if (w == 4) f(ic<12>(), ic<4>(), ic<8>());
+      else f(ic<12>(), ic<1>(), ic<8>());
       break;
   }
 }
@@ -1515,10 +1523,10 @@ void launch_trace_closest(const DevScene& S0, const FrameDev& F0, int bounce, co
   else if (L.count) hipLaunchKernelGGL((k_trace_closest<true, false, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, bounce, L.refill);
   else if (inst) hipLaunchKernelGGL((k_trace_closest<false, true, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, bounce, L.refill);
   else
-    with_variant(L, [&](auto lds, auto wpb) {
-      constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
-      hipLaunchKernelGGL((k_trace_closest<false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S, F,
-                         bounce, L.refill);
+    with_variant(L, [&](auto lds, auto wpb, auto nw) {
+      constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value, kN = decltype(nw)::value;
+      hipLaunchKernelGGL((k_trace_closest<false, false, kL, kW, kN>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S,
+                         F, bounce, L.refill);
     });
 }
 void launch_shade(const DevScene& S, const FrameDev& F, int bounce, hipStream_t s) {
@@ -1533,10 +1541,10 @@ void launch_trace_shadow(const DevScene& S0, const FrameDev& F0, const TraceLaun
   else if (L.count) hipLaunchKernelGGL((k_trace_shadow<true, false, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, L.refill);
   else if (inst) hipLaunchKernelGGL((k_trace_shadow<false, true, kStackLds, 1>), dim3(L.fixed_grid), dim3(64), 0, s, S, F, L.refill);
   else
-    with_variant(L, [&](auto lds, auto wpb) {
-      constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
-      hipLaunchKernelGGL((k_trace_shadow<false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S, F,
-                         L.refill);
+    with_variant(L, [&](auto lds, auto wpb, auto nw) {
+      constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value, kN = decltype(nw)::value;
+      hipLaunchKernelGGL((k_trace_shadow<false, false, kL, kW, kN>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S,
+                         F, L.refill);
     });
 }
 void launch_render_paths(const DevScene& S0, const FrameDev& F0, uint32_t depth, const TraceLaunch& L,
@@ -1583,20 +1591,20 @@ void launch_trace_rays(const DevScene& S0, const float4* rays, uint32_t n, int a
   const int variant = (any_hit ? 4 : 0) | (L.count ? 2 : 0) | (S.num_inst ? 1 : 0);
   switch (variant) {
     case 0:
-      with_variant(L, [&](auto lds, auto wpb) {
-        constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
-        hipLaunchKernelGGL((k_trace_rays<false, false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S,
-                           rays, n, ids, t, bary, cursor, spill, stats, overflow, L.refill);
+      with_variant(L, [&](auto lds, auto wpb, auto nw) {
+        constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value, kN = decltype(nw)::value;
+        hipLaunchKernelGGL((k_trace_rays<false, false, false, kL, kW, kN>), dim3(L.grid / kW), dim3(64 * kW), 0,
+                           s, S, rays, n, ids, t, bary, cursor, spill, stats, overflow, L.refill);
       });
       break;
     case 1: PK_LAUNCH_RAYS(false, false, true); break;
     case 2: PK_LAUNCH_RAYS(false, true, false); break;
     case 3: PK_LAUNCH_RAYS(false, true, true); break;
     case 4:
-      with_variant(L, [&](auto lds, auto wpb) {
-        constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value;
-        hipLaunchKernelGGL((k_trace_rays<true, false, false, kL, kW>), dim3(L.grid / kW), dim3(64 * kW), 0, s, S,
-                           rays, n, ids, t, bary, cursor, spill, stats, overflow, L.refill);
+      with_variant(L, [&](auto lds, auto wpb, auto nw) {
+        constexpr int kL = decltype(lds)::value, kW = decltype(wpb)::value, kN = decltype(nw)::value;
+        hipLaunchKernelGGL((k_trace_rays<true, false, false, kL, kW, kN>), dim3(L.grid / kW), dim3(64 * kW), 0,
+                           s, S, rays, n, ids, t, bary, cursor, spill, stats, overflow, L.refill);
       });
       break;
     case 5: PK_LAUNCH_RAYS(true, false, true); break;

@@ -97,6 +97,33 @@ inline void wnode_rebase(WNode& w, uint32_t noff, uint32_t toff) {
   w.tbase += toff;
 }
 
+// 64-byte 4-wide node of a single-level BVH: the same fields as WNode for
+// four children, without child words (children come from ibase/tbase only).
+// The 4-wide layout is derived from the 8-wide one and shares its triangle
+// and shading records: node i (< the 8-wide node count) stands for 8-wide
+// node i -- its children when it has at most four, else two sub-nodes that
+// split its slots at one cut -- and the sub-nodes follow after.  Every box is
+// the 8-wide node's quantized box (a sub-node's box is the byte-wise union
+// of its slots' boxes in the same frame), so nothing is quantized again.
+struct alignas(64) WNode4 {
+  float ox, oy, oz;       // 0
+  uint8_t ex, ey, ez;     // 12
+  uint8_t nchild;         // 15
+  uint8_t meta[4];        // 16
+  uint8_t qlox[4];        // 20
+  uint8_t qhix[4];        // 24
+  uint8_t qloy[4];        // 28
+  uint8_t qhiy[4];        // 32
+  uint8_t qloz[4];        // 36
+  uint8_t qhiz[4];        // 40
+  uint8_t leafmask;       // 44
+  uint8_t pad8[3];        // 45
+  uint32_t ibase;         // 48
+  uint32_t tbase;         // 52
+  uint32_t pad[2];        // 56..63
+};
+static_assert(sizeof(WNode4) == 64, "4-wide node must be 64 bytes");
+
 struct TriRecord {        // 48 bytes
   float v0x, v0y, v0z;
   uint32_t id;            // original scene.indices position

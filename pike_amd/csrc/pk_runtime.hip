@@ -75,6 +75,7 @@ struct pk_ctx {
     uint32_t lds = pk::kStackLds;  // LDS stack entries per lane (8, 12, 16)
     uint32_t wpb = 1;              // waves per block (1, 2, 4)
     uint32_t leaf16 = 8;           // leaf phase trigger, 16ths of the tracing lanes
+    uint32_t width = 8;            // BVH node width (8, or 4 with 8/12 entries and one wave per block)
   } shape[PK_TUNE_CLASSES];
   uint32_t tuning_gen = 0;  // bumped by pk_ctx_set_tuning; captured graphs keep the old shape
   uint32_t stack_limit = pk::kStackMax;  // traversal stack entries per ray
@@ -168,7 +169,8 @@ pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
   // whole blocks: the grid (in waves) is a multiple of the waves per block
   const uint32_t grid = std::max(sh.wpb, (uint32_t)c->num_cus * sh.waves / sh.wpb * sh.wpb);
   const uint32_t fixed_grid = (uint32_t)c->num_cus * std::min(sh.waves, 4u * pk::kTraceWavesPerSimd);
-  return pk::TraceLaunch{grid, fixed_grid, sh.refill, count, c->spill, c->stack_limit, sh.lds, sh.wpb, sh.leaf16};
+  return pk::TraceLaunch{grid,   fixed_grid, sh.refill, count,     c->spill,
+                         c->stack_limit, sh.lds, sh.wpb, sh.leaf16, sh.width};
 }
 pk::TraceLaunch trace_launch(const pk_frame* f, int cls) { return shape_launch(f->ctx, cls, f->count_nodes); }
 int closest_class(uint32_t bounce) { return bounce == 0 ? PK_TUNE_PRIMARY : PK_TUNE_SECONDARY; }
@@ -329,9 +331,12 @@ int pk_ctx_set_launch(pk_ctx* c, int cls, const pk_launch_shape* sh) {
   }
   const bool lds_ok = sh->stack_lds == 8 || sh->stack_lds == 12 || sh->stack_lds == 16;
   const bool wpb_ok = sh->waves_per_block == 1 || sh->waves_per_block == 2 || sh->waves_per_block == 4;
-  if (!lds_ok || !wpb_ok || (cls == PK_TUNE_PATHS && (sh->stack_lds != 12 || sh->waves_per_block != 1))) {
-    pk_set_error("pk_ctx_set_launch: stack_lds must be 8, 12 or 16 and waves_per_block 1, 2 or 4 (the path "
-                 "megakernel: 12 and 1)");
+  const bool width_ok = sh->node_width == 8 ||
+                        (sh->node_width == 4 && sh->stack_lds != 16 && sh->waves_per_block == 1);
+  if (!lds_ok || !wpb_ok || !width_ok ||
+      (cls == PK_TUNE_PATHS && (sh->stack_lds != 12 || sh->waves_per_block != 1 || sh->node_width != 8))) {
+    pk_set_error("pk_ctx_set_launch: stack_lds must be 8, 12 or 16, waves_per_block 1, 2 or 4 and node_width 8, "
+                 "or 4 with stack_lds 8 or 12 and 1 wave per block (the path megakernel: 12, 1 and 8)");
     return PK_E_INVALID;
   }
   const uint32_t wmax = class_max_waves(cls, sh->stack_lds, sh->waves_per_block);
@@ -341,7 +346,8 @@ int pk_ctx_set_launch(pk_ctx* c, int cls, const pk_launch_shape* sh) {
                  "] for this stack part and block, refill_min in [1, 64], leaf_phase16 in [1, 16]");
     return PK_E_LIMIT;
   }
-  c->shape[cls] = {sh->waves_per_cu, sh->refill_min, sh->stack_lds, sh->waves_per_block, sh->leaf_phase16};
+  c->shape[cls] = {sh->waves_per_cu, sh->refill_min, sh->stack_lds, sh->waves_per_block, sh->leaf_phase16,
+                   sh->node_width};
   c->tuning_gen++;
   return PK_OK;
 }
@@ -352,7 +358,7 @@ int pk_ctx_get_launch(const pk_ctx* c, int cls, pk_launch_shape* sh) {
     return PK_E_INVALID;
   }
   const pk_ctx::Shape& s = c->shape[cls];
-  *sh = pk_launch_shape{s.waves, s.refill, s.lds, s.wpb, s.leaf16};
+  *sh = pk_launch_shape{s.waves, s.refill, s.lds, s.wpb, s.leaf16, s.width};
   return PK_OK;
 }
 
@@ -361,7 +367,8 @@ int pk_ctx_set_tuning(pk_ctx* c, int cls, uint32_t waves_per_cu, uint32_t refill
     pk_set_error("pk_ctx_set_tuning: null context or unknown kernel class");
     return PK_E_INVALID;
   }
-  pk_launch_shape sh{waves_per_cu, refill_min, c->shape[cls].lds, c->shape[cls].wpb, c->shape[cls].leaf16};
+  pk_launch_shape sh{waves_per_cu, refill_min, c->shape[cls].lds, c->shape[cls].wpb, c->shape[cls].leaf16,
+                     c->shape[cls].width};
   return pk_ctx_set_launch(c, cls, &sh);
 }
 
@@ -585,6 +592,16 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
   if ((rc = upload(tmp, &tidx, bvh->wtri_index.data(), bvh->wtri_index.size() / 4, st))) return fail(rc);
   D.nodes = nodes;
   D.tris = tris;
+  // the 4-wide layout of a single-level BVH (node_width 4 shapes walk it);
+  // n4 stays alive until the stream synchronizes below
+  std::vector<pk::WNode4> n4;
+  D.nodes4 = nullptr;
+  if (!bvh->instanced) {
+    pk_make_wide4(bvh->wnodes, n4);
+    pk::WNode4* nodes4;
+    if ((rc = upload(s->allocs, &nodes4, n4.data(), n4.size(), st))) return fail(rc);
+    D.nodes4 = nodes4;
+  }
   std::vector<uint32_t> all_attr;
   const void* attr_src = d->attributes;
   size_t attr_count = d->num_vertices;
@@ -1761,12 +1778,13 @@ const uint32_t kTuneWaves[] = {8, 12, 16, 20, 24, 28, 32, 40};
 const uint32_t kTuneRefill[] = {8, 16, 24, 32, 40, 48, 56, 64};
 const uint32_t kTuneLeaf16[] = {4, 6, 8, 10, 12};
 
-// The launch-shape dimensions after the refill threshold and the width:
-// the leaf phase trigger, then the LDS stack part (each depth at the best
-// width and at its own widest launch), then the waves per block.  `trial`
-// times one shape and keeps the fastest in `best`.
+// The launch-shape dimensions after the refill threshold and the launch
+// width: the leaf phase trigger, then the LDS stack part (each depth at the
+// best launch width and at its own widest launch), then the waves per block,
+// then (single-level scenes) the 4-wide node layout.  `trial` times one
+// shape and keeps the fastest in `best`.
 template <class Trial>
-int tune_shape_dims(pk_ctx::Shape& best, Trial trial) {
+int tune_shape_dims(pk_ctx::Shape& best, bool with_width4, Trial trial) {
   int rc;
   for (uint32_t l : kTuneLeaf16) {
     pk_ctx::Shape t = best;
@@ -1791,6 +1809,18 @@ int tune_shape_dims(pk_ctx::Shape& best, Trial trial) {
     t.wpb = wpb;
     t.waves = std::max(wpb, std::min(base2.waves, lds_max_waves(base2.lds, wpb)) / wpb * wpb);
     if ((rc = trial(t))) return rc;
+  }
+  if (with_width4) {
+    // 4-wide nodes: one wave per block, the 8- or 12-entry stack part
+    const pk_ctx::Shape base3 = best;
+    for (uint32_t lds : {8u, 12u}) {
+      pk_ctx::Shape t = base3;
+      t.width = 4;
+      t.wpb = 1;
+      t.lds = lds;
+      t.waves = std::min(base3.waves, lds_max_waves(lds, 1));
+      if ((rc = trial(t))) return rc;
+    }
   }
   return PK_OK;
 }
@@ -1879,7 +1909,7 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
     const uint32_t r_best = best.refill;
     for (uint32_t w : kTuneWaves)
       if ((rc = trial(w, r_best))) return rc;
-    if ((rc = tune_shape_dims(best, trial_shape))) return rc;
+    if ((rc = tune_shape_dims(best, s->dev.nodes4 != nullptr, trial_shape))) return rc;
     c->shape[cls] = best;
   }
   double after[3];
@@ -2014,7 +2044,7 @@ extern "C" int pk_autotune_rays(pk_ctx* c, pk_scene* s, const float* d_rays, int
       t.refill = r;
       if ((rc = trial_shape(t))) return rc;
     }
-  if ((rc = tune_shape_dims(best, trial_shape))) return rc;
+  if ((rc = tune_shape_dims(best, s->dev.nodes4 != nullptr, trial_shape))) return rc;
   c->shape[cls] = best;
   c->tuning_gen++;
   if (ms2) {

@@ -175,6 +175,75 @@ def test_wide_layout_is_conservative_and_complete():
     assert np.all(nodes["nchild"] >= 1) and np.all(nodes["nchild"] <= 8)
 
 
+@pytest.mark.parametrize("spatial", [False, True])
+def test_wide4_layout_is_conservative_and_complete(spatial):
+    """The 4-wide layout: node i stands for 8-wide node i (its children, or
+    two sub-nodes splitting its slots at one cut), every box holds its
+    subtree, each triangle record is reached exactly once, and children come
+    from ibase/tbase alone."""
+    data = scenes.config1(seed=6)
+    b = pike_amd.BVH(data.positions, data.triangles, spatial_splits=spatial, sah_collapse=True)
+    n8, tris, tidx = b.export_wide()
+    n4 = b.export_wide4()
+    split = n8["nchild"] > 4
+    assert len(n4) == len(n8) + 2 * int(split.sum()) and split.any() and (~split).any()
+    assert np.all(n4["nchild"] >= 1) and np.all(n4["nchild"] <= 4)
+    bits = (n4["meta"] != 0) & (np.arange(4)[None, :] < n4["nchild"][:, None])
+    np.testing.assert_array_equal(n4["leafmask"], (bits * (1 << np.arange(4))).sum(1))
+    # nodes without a split keep the 8-wide node's slots, frame and runs
+    for i in np.flatnonzero(~split)[:200]:
+        k = int(n8["nchild"][i])
+        assert n4["nchild"][i] == k and n4["ibase"][i] == n8["ibase"][i] and n4["tbase"][i] == n8["tbase"][i]
+        for f in ("meta", "qlox", "qhix", "qloy", "qhiy", "qloz", "qhiz"):
+            np.testing.assert_array_equal(n4[f][i][:k], n8[f][i][:k])
+
+    def kids(n):
+        out, ni, ti = [], int(n["ibase"]), int(n["tbase"])
+        for c in range(int(n["nchild"])):
+            if n["meta"][c]:
+                out.append(("leaf", ti, int(n["meta"][c])))
+                ti += int(n["meta"][c])
+            else:
+                out.append(("node", ni, 0))
+                ni += 1
+        return out
+
+    _, _, verts, _ = b.export_reference()
+    V = verts[:, :3].astype(np.float64)
+    seen = np.zeros(len(tris), np.int64)
+
+    def subtree_tris(ni):
+        out, st = [], [ni]
+        while st:
+            for kind, x, cnt in kids(n4[st.pop()]):
+                if kind == "leaf":
+                    out.append(np.arange(x, x + cnt))
+                else:
+                    st.append(x)
+        return np.concatenate(out)
+
+    st = [0]
+    visited = 0
+    while st:
+        ni = st.pop()
+        visited += 1
+        n = n4[ni]
+        lo, hi = _decode_child_boxes(n)
+        for c, (kind, x, cnt) in enumerate(kids(n)):
+            if kind == "leaf":
+                seen[x:x + cnt] += 1
+                ts = np.arange(x, x + cnt)
+            else:
+                st.append(x)
+                ts = subtree_tris(x) if ni % 7 == 0 else None
+            # (spatial splits clip references: their vertices may leave the box)
+            if ts is not None and not spatial:
+                pts = V[tidx[ts, :3].ravel()]
+                assert np.all(pts >= lo[c] + 1e-6) and np.all(pts <= hi[c] - 1e-6)
+    assert visited == len(n4)
+    assert np.all(seen == 1)
+
+
 def test_bvh_build_rejects_bad_input():
     pos = np.zeros((3, 3))
     with pytest.raises(pike_amd.PikeError) as e:
