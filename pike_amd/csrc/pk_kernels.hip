@@ -755,6 +755,10 @@ __global__ void __launch_bounds__(kShadeWaves * 64) k_shade(DevScene S, FrameDev
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
   const uint32_t nh = F.cnt[kCntHits];  // hit records and holes
+  // The grid covers the queue's capacity (fixed when the pass is captured);
+  // most bounces fill a small part of it, and the blocks past the records
+  // leave at once (block-uniform: no thread of theirs reaches a barrier).
+  if (blockIdx.x * blockDim.x >= nh) return;
   const FrameParams P = *F.params;
   const uint32_t ns = F.views ? F.S : pass_samples(F, P);
   ShadeOut o;
