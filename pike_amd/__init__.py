@@ -249,7 +249,7 @@ class BVH:
     WIDE_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "u1", 3), ("nchild", "u1"), ("child", "<u4", 8),
                                 ("meta", "u1", 8), ("qlox", "u1", 8), ("qhix", "u1", 8), ("qloy", "u1", 8),
                                 ("qhiy", "u1", 8), ("qloz", "u1", 8), ("qhiz", "u1", 8), ("leafmask", "u1"),
-                                ("pad8", "u1", 3), ("ibase", "<u4"), ("tbase", "<u4"),
+                                ("instmask", "u1"), ("pad8", "u1", 2), ("ibase", "<u4"), ("tbase", "<u4"),
                                 ("pad", "<u4", 3)])
     WIDE4_NODE_DTYPE = np.dtype([("origin", "<f4", 3), ("exp", "u1", 3), ("nchild", "u1"), ("meta", "u1", 4),
                                  ("qlox", "u1", 4), ("qhix", "u1", 4), ("qloy", "u1", 4), ("qhiy", "u1", 4),

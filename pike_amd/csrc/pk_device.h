@@ -219,16 +219,6 @@ __device__ inline void cswap(uint32_t& a, uint32_t& b) {
   b = hi;
 }
 
-// c[s] for a per-lane slot s in [0, 8): a 3-level select tree on the slot's
-// bits (7 conditional moves) instead of a compare-and-move per slot
-__device__ inline uint32_t select8(const uint32_t* c, uint32_t s) {
-  const bool b0 = (s & 1u) != 0u, b1 = (s & 2u) != 0u, b2 = (s & 4u) != 0u;
-  const uint32_t a0 = b0 ? c[1] : c[0], a1 = b0 ? c[3] : c[2];
-  const uint32_t a2 = b0 ? c[5] : c[4], a3 = b0 ? c[7] : c[6];
-  const uint32_t d0 = b1 ? a1 : a0, d1 = b1 ? a3 : a2;
-  return b2 ? d1 : d0;
-}
-
 // First triangle record of leaf child c (c < 8) of a linked node: tbase
 // plus the leaf counts of the slots below c (internal slots count 0), summed
 // bytewise with v_sad_u8 against zero.
@@ -313,12 +303,6 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
           if (test_leaves<kAnyHit, kCount, kInst>(
                   S, r, [&](uint32_t c) { return leaf_first(tbase, meta, 0u, c); }, meta, 0u, lm, st))
             return true;
-        } else if (kInst) {
-          const uint4 C1 = lp[1], C2 = lp[2], C3 = lp[3];
-          const uint32_t pchild[8] = {C1.x, C1.y, C1.z, C1.w, C2.x, C2.y, C2.z, C2.w};
-          if (test_leaves<kAnyHit, kCount, kInst>(
-                  S, r, [&](uint32_t c) { return select8(pchild, c); }, C3.x, C3.y, lm, st))
-            return true;
         } else {
           const uint2 M = reinterpret_cast<const uint2*>(lp)[6];          // meta, bytes 48..55
           const uint32_t tbase = reinterpret_cast<const uint32_t*>(lp)[28];  // byte 112
@@ -394,15 +378,14 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
                             : reinterpret_cast<const uint4*>(S.nodes4 + visited);
   const uint4 L0 = np[0];
   uint32_t qlx[kWords], qhx[kWords], qly[kWords], qhy[kWords], qlz[kWords], qhz[kWords];
-  uint32_t leafmask, ibase;
-  uint4 L1i = make_uint4(0u, 0u, 0u, 0u), L2i = L1i;  // child words (instanced traversal)
+  uint32_t leafmask, ibase, instmask = 0;
   if constexpr (kW == 8) {
-    if constexpr (kInst) L1i = np[1], L2i = np[2];
     const uint4 L3 = np[3], L4 = np[4], L5 = np[5], L6 = np[6];
     qlx[0] = L3.z, qlx[1] = L3.w, qhx[0] = L4.x, qhx[1] = L4.y;
     qly[0] = L4.z, qly[1] = L4.w, qhy[0] = L5.x, qhy[1] = L5.y;
     qlz[0] = L5.z, qlz[1] = L5.w, qhz[0] = L6.x, qhz[1] = L6.y;
     leafmask = L6.z & 0xffu;
+    if constexpr (kInst) instmask = (L6.z >> 8) & 0xffu;
     ibase = L6.w;
   } else {
     const uint4 L1 = np[1], L2 = np[2];
@@ -427,17 +410,14 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     nyw[i] = negy ? qhy[i] : qly[i], fyw[i] = negy ? qly[i] : qhy[i];
     nzw[i] = negz ? qhz[i] : qlz[i], fzw[i] = negz ? qlz[i] : qhz[i];
   }
-  // child node of an internal slot: the instanced traversal selects among
-  // the child words (placement children break the runs), the single-level
-  // one counts the internal slots below it from ibase
-  uint32_t child[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if constexpr (kInst) {
-    child[0] = L1i.x, child[1] = L1i.y, child[2] = L1i.z, child[3] = L1i.w;
-    child[4] = L2i.x, child[5] = L2i.y, child[6] = L2i.z, child[7] = L2i.w;
-  }
-  const uint32_t inner = ~leafmask & ((1u << nchild) - 1u) & 0xffu;
+  // child node of an internal slot: ibase + the internal slots below it; a
+  // placement child of an instanced top level (outside the runs) reads its
+  // child word
+  const uint32_t inner = ~leafmask & ~instmask & ((1u << nchild) - 1u) & 0xffu;
   auto node_of = [&](uint32_t slot) {
-    return kInst ? select8(child, slot) : ibase + (uint32_t)__popc(inner & ((1u << slot) - 1u));
+    if (kInst && ((instmask >> slot) & 1u))
+      return reinterpret_cast<const uint32_t*>(S.nodes + visited)[4u + slot];
+    return ibase + (uint32_t)__popc(inner & ((1u << slot) - 1u));
   };
   const float cull = ray_cull<kAnyHit>(r);
   float tn[kW];

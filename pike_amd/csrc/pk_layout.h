@@ -33,11 +33,12 @@ constexpr double kBoxMargin = 1.0 / 131072.0;
 // The builders lay a node's children out in slot order: its internal
 // children are the consecutive nodes ibase, ibase + 1, ... and its leaf
 // children's triangle ranges follow each other from record tbase.  So the
-// single-level traversal finds child i from the header alone -- node
-// ibase + (internal slots below i), triangles tbase + (leaf counts below i)
-// -- without loading or selecting among the eight child words, which stay
-// for the instanced traversal (whose placement children break the runs)
-// and for host-side consumers.
+// traversal finds child i from the header alone -- node ibase + (internal
+// slots below i), triangles tbase + (leaf counts below i) -- without loading
+// or selecting among the eight child words.  Placement children of an
+// instanced top level (instmask bit i) are outside the runs: the traversal
+// reads their child word, the only one it needs.  The child words stay for
+// that and for host-side consumers.
 struct alignas(128) WNode {
   float ox, oy, oz;       // 0
   uint8_t ex, ey, ez;     // 12: biased exponents
@@ -51,42 +52,50 @@ struct alignas(128) WNode {
   uint8_t qloz[8];        // 88
   uint8_t qhiz[8];        // 96
   uint8_t leafmask;       // 104
-  uint8_t pad8[3];        // 105
+  uint8_t instmask;       // 105: placement children (instanced top level)
+  uint8_t pad8[2];        // 106
   uint32_t ibase;         // 108: first internal child (0 without one)
   uint32_t tbase;         // 112: first triangle record of the first leaf child (0 without one)
   uint32_t pad[3];        // 116..127
 };
 static_assert(sizeof(WNode) == 128, "wide node must be one 128-byte line");
 
-// Sets ibase/tbase from the child words (placement children skipped).
+// Sets ibase/tbase and instmask from the child words.
 inline void wnode_link(WNode& w) {
   bool have_i = false, have_t = false;
   w.ibase = 0;
   w.tbase = 0;
+  w.instmask = 0;
   for (int c = 0; c < w.nchild; ++c) {
     if (w.meta[c]) {
       if (!have_t) w.tbase = w.child[c];
       have_t = true;
-    } else if (!(w.child[c] & 0x80000000u)) {
+    } else if (w.child[c] & 0x80000000u) {
+      w.instmask |= (uint8_t)(1u << c);
+    } else {
       if (!have_i) w.ibase = w.child[c];
       have_i = true;
     }
   }
 }
 
-// True when the child words follow the slot-order runs from ibase/tbase.
+// True when the child words follow the slot-order runs from ibase/tbase and
+// instmask marks exactly the placement children.
 inline bool wnode_linked(const WNode& w) {
   uint32_t ni = w.ibase, nt = w.tbase;
+  uint32_t im = 0;
   for (int c = 0; c < w.nchild; ++c) {
     if (w.meta[c]) {
       if (w.child[c] != nt) return false;
       nt += w.meta[c];
+    } else if (w.child[c] & 0x80000000u) {
+      im |= 1u << c;
     } else {
       if (w.child[c] != ni) return false;
       ++ni;
     }
   }
-  return true;
+  return im == w.instmask;
 }
 
 // Moves a node into a merged layout whose nodes start at noff and triangle

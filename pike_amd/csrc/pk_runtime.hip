@@ -548,8 +548,9 @@ int pk_scene_upload(pk_ctx* c, const pk_scene_desc* d, const pk_bvh* bvh, pk_sce
         pk_set_error("pk_scene_upload: instance material " + std::to_string(bvh->inst[i].material) + " out of range");
         return PK_E_INVALID;
       }
-  } else {
-    // the single-level traversal finds children from ibase/tbase alone
+  }
+  {
+    // the traversal finds children from ibase/tbase (and instmask) alone
     for (size_t i = 0; i < bvh->wnodes.size(); ++i)
       if (!pk::wnode_linked(bvh->wnodes[i])) {
         pk_set_error("pk_scene_upload: wide node " + std::to_string(i) +
