@@ -534,10 +534,16 @@ __device__ inline float env_pdf(const DevScene& S, float theta, v3 dir) {
   return S.env_pdf[(size_t)iy * S.ew + ix] / sin_phi;
 }
 
-__device__ inline float texel(const DevScene& S, const void* tex, int layer, int x, int y, int c) {
-  size_t i = (((size_t)layer * S.th + (size_t)y) * S.tw + (size_t)x) * 4 + c;
-  if (S.tfmt == 0) return pkm::unpack_unorm8(((const uint8_t*)tex)[i]);
-  return ((const float*)tex)[i];
+// The four channels of one texel with one load: an RGBA8 texel is one
+// little-endian word (channel c in byte c), an RGBA32F texel one float4.
+__device__ inline float4 texel4(const DevScene& S, const void* tex, int layer, int x, int y) {
+  const size_t i = ((size_t)layer * S.th + (size_t)y) * S.tw + (size_t)x;
+  if (S.tfmt == 0) {
+    const uint32_t w = ((const uint32_t*)tex)[i];
+    return make_float4(pkm::unpack_unorm8(w & 0xffu), pkm::unpack_unorm8((w >> 8) & 0xffu),
+                       pkm::unpack_unorm8((w >> 16) & 0xffu), pkm::unpack_unorm8(w >> 24));
+  }
+  return ((const float4*)tex)[i];
 }
 
 // Bilinear footprint of one texture lookup (the same index arithmetic as
@@ -572,11 +578,14 @@ __device__ inline float4 sample_tex(const DevScene& S, const void* tex, int laye
   float fx = x - fx0, fy = y - fy0;
   int x0 = wrapi((int)fx0, (int)S.tw), x1 = wrapi((int)fx0 + 1, (int)S.tw);
   int y0 = wrapi((int)fy0, (int)S.th), y1 = wrapi((int)fy0 + 1, (int)S.th);
+  const float4 A = texel4(S, tex, layer, x0, y0), B = texel4(S, tex, layer, x1, y0);
+  const float4 C = texel4(S, tex, layer, x0, y1), D = texel4(S, tex, layer, x1, y1);
+  const float ta[4] = {A.x, A.y, A.z, A.w}, tb[4] = {B.x, B.y, B.z, B.w};
+  const float tc[4] = {C.x, C.y, C.z, C.w}, td[4] = {D.x, D.y, D.z, D.w};
   float r[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    float a = texel(S, tex, layer, x0, y0, c), b = texel(S, tex, layer, x1, y0, c);
-    float cc = texel(S, tex, layer, x0, y1, c), dd = texel(S, tex, layer, x1, y1, c);
+    const float a = ta[c], b = tb[c], cc = tc[c], dd = td[c];
     float top = a + (b - a) * fx;
     float bot = cc + (dd - cc) * fx;
     r[c] = top + (bot - top) * fy;
