@@ -48,6 +48,7 @@ struct FrameParams {
   float env_theta;
   pk_camera cam;
   uint32_t limit;   // samples >= limit are not traced (end of a pk_render run)
+  float cbx[3], cby[3];  // the camera's basis (camera_basis), formed once by k_begin
 };
 
 // One camera view of a batch: its pixels occupy [offset, offset + W*H) of

@@ -81,8 +81,19 @@ __device__ inline void block_append2(uint32_t* c0, bool p0, uint32_t* c1, bool p
 // ---------------------------------------------------------------------------
 // camera.wgsl main (:75-92) / createPrimaryRay (:58-73)
 // ---------------------------------------------------------------------------
-__device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t sample,
-                                  const pk_camera& cam, v3* o_out, v3* d_out) {
+// The image plane's axes scaled by tan(fov / 2): uniform over a frame.
+__device__ inline void camera_basis(const pk_camera& cam, v3* bx, v3* by) {
+  const v3 up = pkm::mk(0.0f, 1.0f, 0.0f);
+  const v3 dir = pkm::mk(cam.dir[0], cam.dir[1], cam.dir[2]);
+  const float tf = pkm::tanf_(cam.fov * 0.5f);
+  *bx = pkm::scale(pkm::normalize(pkm::cross(dir, up)), tf);
+  *by = pkm::scale(pkm::normalize(pkm::cross(*bx, dir)), tf);
+}
+
+// createPrimaryRay for pixel (px, py) and `sample`, given the camera's
+// basis (camera_basis).
+__device__ inline void camera_ray_b(uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t sample,
+                                    const pk_camera& cam, v3 bx, v3 by, v3* o_out, v3* d_out) {
   uint32_t seed = (px * 1973u + py * 9277u + sample * 26699u) | 1u;
   seed = pkm::pcg_next(&seed);
   const float k = cam.distortion;
@@ -96,12 +107,8 @@ __device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t
   const float l2 = pkm::sqrt_(ux * ux + uy * uy);
   ux = ru * (ux / l2);
   uy = ru * (uy / l2);
-  const v3 up = pkm::mk(0.0f, 1.0f, 0.0f);
   const v3 dir = pkm::mk(cam.dir[0], cam.dir[1], cam.dir[2]);
   const v3 org = pkm::mk(cam.origin[0], cam.origin[1], cam.origin[2]);
-  const float tf = pkm::tanf_(cam.fov * 0.5f);
-  const v3 bx = pkm::scale(pkm::normalize(pkm::cross(dir, up)), tf);
-  const v3 by = pkm::scale(pkm::normalize(pkm::cross(bx, dir)), tf);
   const float theta = pkm::pcg_rand(&seed) * pkm::kTau;
   const float pw = pkm::powf_(pkm::pcg_rand(&seed), cam.bokeh);
   float st, ct;
@@ -110,6 +117,13 @@ __device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t
   const v3 screen = pkm::add(pkm::add(pkm::add(pkm::scale(bx, ux), pkm::scale(by, uy)), dir), org);
   *d_out = pkm::normalize(pkm::sub(pkm::add(screen, pkm::scale(dof, cam.focal_depth)), pkm::add(org, dof)));
   *o_out = pkm::add(dof, org);
+}
+
+__device__ inline void camera_ray(uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t sample,
+                                  const pk_camera& cam, v3* o_out, v3* d_out) {
+  v3 bx, by;
+  camera_basis(cam, &bx, &by);
+  camera_ray_b(px, py, W, H, sample, cam, bx, by, o_out, d_out);
 }
 
 // One thread per path of the pass: path idx = pixel * ns + s traces sample
@@ -131,7 +145,8 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
   // queues and in the color buffer)
   const uint32_t pixel = idx / ns, s = idx - pixel * ns;
   v3 o, d;
-  camera_ray(pixel % W, pixel / W, W, H, P.sample + s, P.cam, &o, &d);
+  camera_ray_b(pixel % W, pixel / W, W, H, P.sample + s, P.cam, pkm::mk(P.cbx[0], P.cbx[1], P.cbx[2]),
+               pkm::mk(P.cby[0], P.cby[1], P.cby[2]), &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
   // (camera rays carry throughput 1: bounce 0 does not read F.thr)
@@ -938,7 +953,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
             const uint32_t s = p - pixel * ns;
             sample = P.sample + s;
             slot = color_slot(F, ns, p);
-            camera_ray(pixel % F.W, pixel / F.W, F.W, F.H, sample, P.cam, &o, &d);
+            camera_ray_b(pixel % F.W, pixel / F.W, F.W, F.H, sample, P.cam, pkm::mk(P.cbx[0], P.cbx[1], P.cbx[2]),
+                         pkm::mk(P.cby[0], P.cby[1], P.cby[2]), &o, &d);
           }
           b = 0;
           thr = pkm::mk(1.0f, 1.0f, 1.0f);
@@ -1101,6 +1117,10 @@ __global__ void k_begin(FrameDev F, uint32_t sample, float theta, pk_camera cam,
   F.params->env_theta = theta;
   F.params->cam = cam;
   F.params->limit = limit;
+  v3 bx, by;
+  camera_basis(cam, &bx, &by);
+  F.params->cbx[0] = bx.x, F.params->cbx[1] = bx.y, F.params->cbx[2] = bx.z;
+  F.params->cby[0] = by.x, F.params->cby[1] = by.y, F.params->cby[2] = by.z;
 }
 __global__ void k_advance(FrameDev F) {
   const FrameParams P = *F.params;
