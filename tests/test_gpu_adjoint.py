@@ -141,7 +141,7 @@ def test_adjoint_matches_dual_number_gradients(ctx, c5):
 
 def test_adjoint_matches_central_differences(ctx, c5):
     """Central differences of J = sum(d_image * image) on the GPU forward
-    (eps = 1e-3, common seeds): |fd - adjoint| <= 2% |fd| + the FD's own
+    (eps = 1e-3, common seeds): |fd - adjoint| <= 0.2% |fd| + the FD's own
     rounding bound (the pixels a perturbation changes carry f32 running-mean
     rounding of up to 2 ulp per accumulated frame; pixels it does not change
     are bit-identical and cancel exactly)."""
@@ -177,9 +177,11 @@ def test_adjoint_matches_central_differences(ctx, c5):
     rel = err / np.abs(fd)
     print("adjoint vs central FD: median rel", np.median(rel), "within 2%:", int((rel <= 0.02).sum()), "of",
           len(rel), "max err / rounding bound", (err / noise).max())
-    bad = err > 0.02 * np.abs(fd) + noise
+    # observed: every error below 0.16 of the FD's rounding bound, 63 of 64
+    # within 2% outright (the rest are texels whose FD is mostly rounding)
+    bad = err > 0.002 * np.abs(fd) + noise
     assert not bad.any(), np.column_stack([g, fd, noise])[bad]
-    assert (rel <= 0.02).sum() >= 48, np.column_stack([g, fd, noise])[rel > 0.02]
+    assert (rel <= 0.02).sum() >= 60, np.column_stack([g, fd, noise])[rel > 0.02]
 
 
 def test_device_api_loss_and_accumulate(ctx, c5):
