@@ -127,6 +127,29 @@ def test_batch_width4_equals_width8(shaped):
         np.testing.assert_array_equal(out[4][i].view(np.uint32), out[8][i].view(np.uint32), err_msg=f"view {i}")
 
 
+def test_instanced_scene_under_every_width(shaped):
+    """Instanced scenes walk 8-wide nodes in the instanced kernels (12 LDS
+    entries, one wave per block) whatever the classes' shapes ask; hits and
+    frames stay the oracle's."""
+    data = scenes.instanced_scene(7, num_instances=20)
+    bvh = pike_amd.BVH.for_scene(data)
+    scene = pike_amd.Scene(shaped, data, bvh)
+    orc = OracleScene(data, threads=16)
+    orc.build_bvh()
+    cam = scenes.normalized_camera(data.camera, bvh.normalizer())
+    rays = scenes.incoherent_rays(30_000, bvh.normalizer(), seed=71)
+    ids_o, t_o, b_o, _ = orc.trace(rays, "canonical")
+    ref, _, _ = orc.render(cam, 40, 32, 3, 0, 2, precision=32)
+    for shape in [(4, 8, 1, 8), (8, 16, 4, 12)]:
+        _apply(shaped, TRACE_CLASSES + ("rays_closest", "rays_any"), shape)
+        ids, t, bary = shaped.trace_rays(scene, rays)
+        _assert_same(ids, t, bary, ids_o, t_o, b_o)
+        fr = pike_amd.Frame(shaped, 40, 32)
+        fr.set_camera(cam)
+        fr.render(scene, max_depth=3, first_sample=0, num_frames=2, use_graph=True)
+        np.testing.assert_array_equal(fr.read_image().view(np.uint32), ref.view(np.uint32))
+
+
 def test_invalid_shapes_are_refused(shaped):
     bad = [dict(node_width=4, stack_lds=16, waves_per_block=1),   # 4-wide: 8 or 12 LDS entries
            dict(node_width=4, stack_lds=8, waves_per_block=2),    # 4-wide: one wave per block
