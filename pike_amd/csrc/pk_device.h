@@ -573,6 +573,10 @@ __device__ inline Footprint tex_footprint(const DevScene& S, float u, float v) {
 
 // textureSampleLevel, linear filter, repeat, level 0, from atlas `tex`
 __device__ inline float4 sample_tex(const DevScene& S, const void* tex, int layer, float u, float v) {
+  // 1x1 RGBA8 layers (constant-color materials): the four taps are one
+  // texel, and for its unorm8 values (finite, >= +0) the blend below
+  // returns that texel bit for bit (a + (a - a) * f = a), so it is read once.
+  if (S.tw == 1 && S.th == 1 && S.tfmt == 0) return texel4(S, tex, layer, 0, 0);
   float x = u * (float)S.tw - 0.5f, y = v * (float)S.th - 0.5f;
   float fx0 = floorf(x), fy0 = floorf(y);
   float fx = x - fx0, fy = y - fy0;
