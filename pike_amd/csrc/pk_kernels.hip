@@ -765,9 +765,9 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
 
 constexpr int kShadeWaves = 4;  // k_shade blocks: 256 threads
 
-// 5 waves per SIMD (96 VGPRs, 20 bytes of scratch per lane): 1 ms per
-// 128-sample 1080p pass faster than the 100 VGPRs of 4 waves.
-__global__ void __launch_bounds__(kShadeWaves * 64, 5) k_shade(DevScene S, FrameDev F, int bounce) {
+// 7 waves per SIMD (70 VGPRs, no scratch); 8 (64 VGPRs) spills 32 bytes per
+// lane and measured 10% slower.
+__global__ void __launch_bounds__(kShadeWaves * 64, 7) k_shade(DevScene S, FrameDev F, int bounce) {
   const int in = bounce & 1;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx == 0) F.cnt[kCntWorkShadow] = 0;
