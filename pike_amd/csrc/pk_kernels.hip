@@ -331,15 +331,17 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
         ray_init<false>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
         if (F.views) r.node = F.views[__float_as_uint(d4.w)].root;
       },
-      [&](uint32_t slot, const RayState& r) {
-        if (ray_missed(r)) {  // miss: shade_miss.wgsl envColor x throughput
+      [&](uint32_t, const RayState&) {},
+      [&](bool done, uint32_t slot, const RayState& r) {
+        traced += (uint32_t)__popcll(__ballot(done));
+        // misses are shaded here, by every lane that finished one since the
+        // last refill together, not in the step each one finished in (where
+        // the wave would run the environment lookup for a lane or two)
+        if (done && ray_missed(r)) {  // shade_miss.wgsl envColor x throughput
           const uint32_t pid = __float_as_uint(qo[slot].w);  // reread: no register held through the traversal
           const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.thr[pid];
           add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
-      },
-      [&](bool done, uint32_t slot, const RayState& r) {
-        traced += (uint32_t)__popcll(__ballot(done));
         // hit records go to this wave's reserved run of the hit queue, in
         // the order the wave finishes them (its claimed chunk's pixel order)
         const bool is_hit = done && !ray_missed(r);
@@ -856,14 +858,17 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
         ray_init<true>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
         if (F.views) r.node = F.views[__float_as_uint(d4.w)].root;
       },
-      [&](uint32_t slot, const RayState& r) {
-        if (ray_missed(r)) {  // unoccluded: env radiance x NEE throughput
+      [&](uint32_t, const RayState&) {},
+      [&](bool done, uint32_t slot, const RayState& r) {
+        traced += (uint32_t)__popcll(__ballot(done));
+        // unoccluded rays are shaded at the refill, together (as misses are
+        // in k_trace_closest): env radiance x NEE throughput
+        if (done && ray_missed(r)) {
           const uint32_t pid = __float_as_uint(F.so[slot].w);
           const float4 t4 = F.sthr[slot];
           add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
-      },
-      [&](bool done, uint32_t, const RayState&) { traced += (uint32_t)__popcll(__ballot(done)); });
+      });
   if (kCount) {
     st.nodes = wave_sum(st.nodes);
     st.tris = wave_sum(st.tris);
@@ -1460,12 +1465,14 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
         const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
         ray_init<kAnyHit>(r, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z));
       },
-      [&](uint32_t i, const RayState& r) {
-        ids[i] = (int32_t)ray_hit_id<kInst>(S, r);
-        tout[i] = r.tbest;
-        bary[i] = r.best_bary;
-      },
-      [&](bool, uint32_t, const RayState&) {});
+      [&](uint32_t, const RayState&) {},
+      [&](bool done, uint32_t i, const RayState& r) {  // results written at the refill, together
+        if (done) {
+          ids[i] = (int32_t)ray_hit_id<kInst>(S, r);
+          tout[i] = r.tbest;
+          bary[i] = r.best_bary;
+        }
+      });
   if (kCount) {
     st.nodes = wave_sum(st.nodes);
     st.tris = wave_sum(st.tris);
