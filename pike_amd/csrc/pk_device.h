@@ -94,11 +94,12 @@ constexpr int kStackLdsMin = 8;
 constexpr int kStackMax = 64;
 constexpr int kSpillStride = kStackMax - kStackLdsMin;
 // Occupancy target of the traversal kernels.  With the 8-entry LDS part,
-// 8 waves per SIMD (64 VGPRs): 32 one-wave blocks per CU use 32 * 4 KB of
-// the 160 KB LDS, and the single-level kernels keep their traversal state in
-// 64 VGPRs (the closest-hit kernel reloads two loop invariants of its miss
-// shading from scratch).  The 12- and 16-entry parts fit at most 26 and 20
-// waves per CU in LDS, so they target 6 waves per SIMD (80 VGPRs).
+// 8 waves per SIMD (64 VGPRs): 32 one-wave blocks per CU use 32 * 5 KB (the
+// 4 KB stack part and the cooperative leaf phase's 1 KB) of the 160 KB LDS,
+// and the single-level kernels keep their traversal state in 64 VGPRs (a few
+// loop invariants are reloaded from scratch).  The 12- and 16-entry parts
+// fit at most 22 and 17 waves per CU in LDS, so they target 6 waves per SIMD
+// (80 VGPRs).
 constexpr int kTraceWavesPerSimd = 6;
 constexpr int trace_waves_per_simd(int lds) { return lds <= kStackLdsMin ? 8 : kTraceWavesPerSimd; }
 
@@ -116,6 +117,10 @@ struct Stack {
   uint2* spill;         // [kSpillStride][64] for this wave
   int lane;
   int limit;            // entries a ray may hold, in [1, kStackMax]
+  // [128] per wave in LDS for a cooperative leaf phase (the wave-level trace
+  // loops; null elsewhere): [lane] each lane's best (t bits, triangle id),
+  // [64 + lane] the record and barycentrics of that hit
+  unsigned long long* key;
 };
 
 // State of one ray's traversal, kept in registers between steps.  With
@@ -263,6 +268,130 @@ __device__ inline bool test_leaves(const DevScene& S, RayState& r, First first, 
   return false;
 }
 
+// Bytes of the slots in the 4-bit mask m4 set to 0xff, the others 0.
+__device__ inline uint32_t slot_bytes(uint32_t m4) { return ((m4 * 0x00204081u) & 0x01010101u) * 0xffu; }
+
+// Lanes below this one in the 64-bit mask m.
+__device__ inline uint32_t lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Cooperative leaf phase of the single-level traversal, run by all 64 lanes
+// of the wave together (idle lanes included).  The triangles of the leaf
+// children each pending lane deferred are numbered lane by lane (in slot
+// order within a lane) and dealt out to the wave 64 at a time: lane w of a
+// round tests item base + w against its owner's ray.  Every candidate hit
+// takes the minimum of its owner's key in LDS, (t bits, triangle id) as one
+// 64-bit word — for t > 0 the order of the canonical (t, id) comparison —
+// and the lane holding the minimum after the round writes its record and
+// barycentrics beside it.  The tested set is exactly the one the lane-by-
+// lane loop tests, and the minimum over it does not depend on how it is
+// dealt, so the hit is the same bit for bit; lanes stay busy instead of
+// waiting for the lane with the most triangles.  Any-hit rays take the
+// least hit of the phase (the same whichever way it is dealt).  `pend`
+// marks the lanes whose ray holds deferred leaves; they leave with none.
+// Returns true on a pending any-hit lane that found a hit.
+template <bool kAnyHit, bool kCount, int kW>
+__device__ inline bool coop_leaves(const DevScene& S, RayState& r, bool pend, const Stack& stk, TraverseStats* st) {
+  const uint32_t lane = (uint32_t)stk.lane;
+  unsigned long long* const key = stk.key;
+  unsigned long long* const aux = stk.key + 64;
+  const unsigned long long init =
+      kAnyHit ? ~0ull : ((unsigned long long)__float_as_uint(r.tbest) << 32) | 0xffffffffull;
+  // the pending lane's node: its leaf-count bytes and first triangle record
+  uint32_t m0 = 0, m1 = 0, tb = 0, lm = 0;
+  if (pend) {
+    lm = r.pmask;
+    r.pmask = 0;
+    if (kW == 4) {
+      const uint32_t* w4 = reinterpret_cast<const uint32_t*>(S.nodes4 + r.pnode);
+      m0 = w4[4];   // bytes 16..19
+      tb = w4[13];  // byte 52
+    } else {
+      const uint2* w8 = reinterpret_cast<const uint2*>(S.nodes + r.pnode);
+      const uint2 M = w8[6];  // bytes 48..55
+      m0 = M.x;
+      m1 = M.y;
+      tb = reinterpret_cast<const uint32_t*>(w8)[28];  // byte 112
+    }
+    key[lane] = init;
+  }
+  // the pending children's counts (other bytes zeroed) and their sum
+  const uint32_t p0 = m0 & slot_bytes(lm & 0xfu), p1 = m1 & slot_bytes(lm >> 4);
+  const uint32_t n = __builtin_amdgcn_sad_u8(p0, 0u, __builtin_amdgcn_sad_u8(p1, 0u, 0u));
+  // E: items of the lanes below (exclusive prefix), T: items of the wave,
+  // summed bit plane by bit plane of n
+  uint32_t E = 0, T = 0;
+  for (uint32_t b = 0; __ballot((n >> b) != 0u); ++b) {
+    const unsigned long long bm = __ballot((n >> b) & 1u);
+    E += lanes_below(bm) << b;
+    T += (uint32_t)__popcll(bm) << b;
+  }
+  const float eps = kAnyHit ? kEpsShadow : kEpsTrace;
+  for (uint32_t base = 0; base < T; base += 64u) {
+    const uint32_t w = base + lane;
+    // owner: the last lane whose first item is at most w (a lane without
+    // items shares its E with the next lane, so the last one has items)
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const uint32_t e = (uint32_t)__shfl((int)E, o + step);
+      o = e <= w ? o + step : o;
+    }
+    const uint32_t q0 = w - (uint32_t)__shfl((int)E, o);
+    const uint32_t op0 = (uint32_t)__shfl((int)p0, o), op1 = (uint32_t)__shfl((int)p1, o);
+    // the owner's pending child holding item q0: halve the slot range by
+    // pending counts (the four low slots, then two, then one)
+    uint32_t q = q0, c = 0, word = op0;
+    const uint32_t h4 = __builtin_amdgcn_sad_u8(op0, 0u, 0u);
+    if (q >= h4) {
+      q -= h4;
+      c = 4;
+      word = op1;
+    }
+    const uint32_t h2 = __builtin_amdgcn_sad_u8(word & 0xffffu, 0u, 0u);
+    if (q >= h2) {
+      q -= h2;
+      c += 2;
+      word >>= 16;
+    }
+    if (q >= (word & 0xffu)) {
+      q -= word & 0xffu;
+      c += 1;
+    }
+    const uint32_t om0 = (uint32_t)__shfl((int)m0, o), om1 = (uint32_t)__shfl((int)m1, o);
+    const uint32_t rec = leaf_first((uint32_t)__shfl((int)tb, o), om0, om1, c) + q;
+    const v3 ro = pkm::mk(__shfl(r.o.x, o), __shfl(r.o.y, o), __shfl(r.o.z, o));
+    const v3 rd = pkm::mk(__shfl(r.d.x, o), __shfl(r.d.y, o), __shfl(r.d.z, o));
+    if (w < T) {
+      if (kCount) st->tris++;
+      uint32_t id, bary = 0;
+      const float t = intersect_tri(S.tris, rec, ro, rd, eps, &id, &bary);
+      if (t < kMaxT) {
+        const unsigned long long mine = ((unsigned long long)__float_as_uint(t) << 32) | (unsigned long long)id;
+        atomicMin(&key[o], mine);
+        // (a wave's LDS operations complete in order: every lane's minimum
+        // of this round is in)
+        if (key[o] == mine) aux[o] = ((unsigned long long)bary << 32) | rec;
+      }
+    }
+  }
+  if (!pend) return false;
+  const unsigned long long k = key[lane];
+  if (k == init) return false;
+  const float t = __uint_as_float((uint32_t)(k >> 32));
+  // a hit at the best distance replaces the best only with a lower id
+  if (!kAnyHit && t == r.tbest && !ray_missed(r) && (uint32_t)k >= ray_hit_id<false>(S, r)) return false;
+  const unsigned long long a = aux[lane];
+  r.tbest = t;
+  r.best_leaf = (uint32_t)a;
+  r.best_bary = (uint32_t)(a >> 32);
+  return kAnyHit;
+}
+
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kW>
+__device__ inline bool node_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st);
+
 // One traversal step, wave-cooperative (every lane of the wave that still
 // traces calls it together): either a leaf phase, in which the lanes holding
 // deferred leaf children intersect their triangles, or a node step, in
@@ -282,7 +411,6 @@ __device__ inline bool test_leaves(const DevScene& S, RayState& r, First first, 
 template <bool kAnyHit, bool kCount, bool kInst = false, int kLds = kStackLds, int kW = 8>
 __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
   static_assert(kW == 8 || (kW == 4 && !kInst), "4-wide nodes are single-level only");
-  const int lane = stk.lane;
   // Leaf phase.  A node visit defers its hit leaf children; the lane then
   // waits while the others visit nodes, until at least S.leaf16 / 16 of the
   // wave's tracing lanes hold deferred leaves (or no lane has a node left to
@@ -315,6 +443,17 @@ __device__ inline bool ray_step(const DevScene& S, RayState& r, const Stack& stk
     }
   }
   if (pend) return false;  // waits for the leaf phase
+  return node_step<kAnyHit, kCount, kInst, kLds, kW>(S, r, stk, st);
+}
+
+// The node step of a traversal step: pop the next live stack entry unless a
+// node is at hand, then visit it (box-test its children, defer the hit leaf
+// children, continue with the nearest hit internal child and push the others
+// far-to-near).  The ray holds no deferred leaves.  Returns true when the
+// ray is finished.
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kW>
+__device__ inline bool node_step(const DevScene& S, RayState& r, const Stack& stk, TraverseStats* st) {
+  const int lane = stk.lane;
   if (!r.have) {
     const float cull = ray_cull<kAnyHit>(r);
     while (r.sp > 0) {

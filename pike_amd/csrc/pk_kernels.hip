@@ -242,7 +242,28 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = false;
   while (true) {
-    if (active && ray_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st)) {
+    bool fin = false;
+    if constexpr (!kInst) {
+      // the leaf phase runs on the whole wave (coop_leaves), the node steps
+      // on the tracing lanes without deferred leaves; the trigger is
+      // ray_step's
+      const bool pend = active && r.pmask != 0u;
+      const unsigned long long pm = __ballot(pend);
+      bool leaf = false;
+      if (pm) {
+        const unsigned long long inner = __ballot(active && !pend && (r.have || r.sp > 0));
+        leaf = inner == 0ull || 16u * (uint32_t)__popcll(pm) >= S.leaf16 * (uint32_t)__popcll(__ballot(active));
+      }
+      if (leaf) {
+        const bool hit = coop_leaves<kAnyHit, kCount, kNW>(S, r, pend, stk, st);
+        fin = active && (hit || (!r.have && r.sp == 0));
+      } else if (active && !pend) {
+        fin = node_step<kAnyHit, kCount, false, kLds, kNW>(S, r, stk, st);
+      }
+    } else {
+      fin = active && ray_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
+    }
+    if (fin) {
       active = false;
       done = true;
       if (r.overflow) *overflow = 1;
@@ -300,6 +321,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
 #define PK_TRACE_WAVE_SETUP                                                                                       \
   __shared__ uint32_t lds_node[kWpb][kLds * 64];                                                                 \
   __shared__ float lds_t[kWpb][kLds * 64];                                                                       \
+  __shared__ unsigned long long lds_key[kWpb][128];                                                              \
   const int lane = (int)(threadIdx.x & 63u);                                                                     \
   const int wave = kWpb == 1 ? 0 : (int)(threadIdx.x >> 6);                                                      \
   const size_t wave_id = (size_t)blockIdx.x * kWpb + (size_t)wave
@@ -315,7 +337,8 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   }
   const uint32_t n = F.cnt[ray_counter(in)];
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
+                  lds_key[wave]};
   TraverseStats st = {0, 0};
   // traced/hits are wave-uniform (counted from ballots at each flush, which
   // every finished ray passes once), so they cost no per-lane registers
@@ -848,7 +871,8 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   const uint32_t n = F.cnt[kCntShadow];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
+                  lds_key[wave]};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, overflow = 0;  // traced: wave-uniform, counted at each flush
   trace_loop<true, kCount, kInst, kLds, kNW>(
@@ -905,7 +929,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
+                  nullptr};
   const FrameParams P = *F.params;
   const uint32_t ns = kBatch ? F.S : pass_samples(F, P);
   const uint32_t B = F.W * F.H;
@@ -1235,7 +1260,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
+                  nullptr};
   float* const vtx = A.vtx + (size_t)blockIdx.x * A.depth * kAdjVtxWords * 64 + lane;
   const uint32_t B = A.W * A.H;
   const uint32_t total = B * A.num_frames;
@@ -1456,7 +1482,8 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
                                                                              uint32_t* overflow_flag,
                                                                              uint32_t refill_min) {
   PK_TRACE_WAVE_SETUP;
-  const Stack stk{lds_node[wave], lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit};
+  const Stack stk{lds_node[wave], lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
+                  lds_key[wave]};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
   trace_loop<kAnyHit, kCount, kInst, kLds, kNW>(

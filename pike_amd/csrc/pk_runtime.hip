@@ -145,15 +145,17 @@ struct pk_frame {
 
 namespace {
 
-// Traversal waves per CU at most: the LDS of 40 waves with 8-entry stack
-// parts (4 KB each) fills a CU's 160 KB (the kernels' 40 VGPRs would allow
-// 48); the spill area and the hit queue's slack are sized for it.
+// Waves per CU at most of a persistent traversal launch; the spill area and
+// the hit queue's slack are sized for it.  The wavefront kernels' LDS (an
+// 8-entry stack part and the leaf phase's 1 KB per wave) fits 32 per CU, the
+// path megakernel and the adjoint launch fewer.
 constexpr uint32_t kMaxWavesPerCu = 40;
 constexpr uint32_t kLdsBytesPerCu = 160u * 1024u;
-// waves per CU that a shape's LDS stack parts fit: whole blocks of wpb waves
-// of lds entries x 64 lanes x 8 bytes
+// waves per CU that a shape's LDS fits: whole blocks of wpb waves, each with
+// its stack part (lds entries x 64 lanes x 8 bytes) and the leaf phase's
+// 128 words of 8 bytes
 uint32_t lds_max_waves(uint32_t lds, uint32_t wpb) {
-  const uint32_t per_block = lds * 64u * 8u * wpb;
+  const uint32_t per_block = (lds * 64u * 8u + 128u * 8u) * wpb;
   return std::min(kMaxWavesPerCu, kLdsBytesPerCu / per_block * wpb);
 }
 // One-wave adjoint blocks per CU: k_adjoint is compiled for 4 waves per SIMD
@@ -1777,7 +1779,7 @@ namespace {
 
 const uint32_t kTuneWaves[] = {8, 12, 16, 20, 24, 28, 32, 40};
 const uint32_t kTuneRefill[] = {8, 16, 24, 32, 40, 48, 56, 64};
-const uint32_t kTuneLeaf16[] = {4, 6, 8, 10, 12};
+const uint32_t kTuneLeaf16[] = {2, 4, 6, 8, 10, 12, 14};
 
 // The launch-shape dimensions after the refill threshold and the launch
 // width: the leaf phase trigger, then the LDS stack part (each depth at the

@@ -22,8 +22,10 @@ TRACE_CLASSES = ("primary", "secondary", "shadow")
 
 
 def _waves(lds, wpb):
-    """A launch width every stack part allows (20 waves per CU fit 16 LDS entries)."""
-    return (20 if lds == 16 else 24) // wpb * wpb
+    """A launch width every block of the stack part allows (a wave's LDS holds
+    its stack part and the leaf phase's 1 KB: 16 waves per CU fit the
+    16-entry part in blocks of up to 4 waves, 20 the 12-entry part)."""
+    return (16 if lds == 16 else 20) // wpb * wpb
 
 
 @pytest.fixture
