@@ -111,16 +111,22 @@ struct TraverseStats {
   uint32_t nodes, tris;
 };
 
+// Pointers into LDS (address space 3): accesses through them are ds_
+// operations, never flat ones that have to test which memory they reach.
+using lds_u32 = __attribute__((address_space(3))) uint32_t*;
+using lds_f32 = __attribute__((address_space(3))) float*;
+using lds_u64 = __attribute__((address_space(3))) unsigned long long*;
+
 struct Stack {
-  uint32_t* lds_node;   // [kLds][64]
-  float* lds_t;         // [kLds][64]
+  lds_u32 lds_node;     // [kLds][64]
+  lds_f32 lds_t;        // [kLds][64]
   uint2* spill;         // [kSpillStride][64] for this wave
   int lane;
   int limit;            // entries a ray may hold, in [1, kStackMax]
   // [128] per wave in LDS for a cooperative leaf phase (the wave-level trace
   // loops; null elsewhere): [lane] each lane's best (t bits, triangle id),
   // [64 + lane] the record and barycentrics of that hit
-  unsigned long long* key;
+  lds_u64 key;
 };
 
 // State of one ray's traversal, kept in registers between steps.  With
@@ -294,8 +300,8 @@ __device__ inline uint32_t lanes_below(unsigned long long m) {
 template <bool kAnyHit, bool kCount, int kW>
 __device__ inline bool coop_leaves(const DevScene& S, RayState& r, bool pend, const Stack& stk, TraverseStats* st) {
   const uint32_t lane = (uint32_t)stk.lane;
-  unsigned long long* const key = stk.key;
-  unsigned long long* const aux = stk.key + 64;
+  const lds_u64 key = stk.key;
+  const lds_u64 aux = stk.key + 64;
   const unsigned long long init =
       kAnyHit ? ~0ull : ((unsigned long long)__float_as_uint(r.tbest) << 32) | 0xffffffffull;
   // the pending lane's node: its leaf-count bytes and first triangle record
@@ -369,7 +375,7 @@ __device__ inline bool coop_leaves(const DevScene& S, RayState& r, bool pend, co
       const float t = intersect_tri(S.tris, rec, ro, rd, eps, &id, &bary);
       if (t < kMaxT) {
         const unsigned long long mine = ((unsigned long long)__float_as_uint(t) << 32) | (unsigned long long)id;
-        atomicMin(&key[o], mine);
+        __hip_atomic_fetch_min(key + o, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         // (a wave's LDS operations complete in order: every lane's minimum
         // of this round is in)
         if (key[o] == mine) aux[o] = ((unsigned long long)bary << 32) | rec;
@@ -604,15 +610,14 @@ __device__ inline bool node_step(const DevScene& S, RayState& r, const Stack& st
   }
   // internal children, sorted near-to-far: key = entry distance bits (>= 0,
   // so order-preserving) with the slot in the low 3 bits
+  // (a hit child's entry distance is within the culling bound already)
+  const uint32_t imask = hitmask & ~leafmask;
+  if (imask == 0u) return r.sp == 0 && r.pmask == 0u;
+  const int m = __popc(imask);
   uint32_t k[kW];
-  int m = 0;
 #pragma unroll
-  for (int c = 0; c < kW; ++c) {
-    const bool live = ((hitmask & ~leafmask) >> c) & 1u && tn[c] <= cull;
-    k[c] = live ? ((__float_as_uint(tn[c]) & ~7u) | (uint32_t)c) : 0xffffffffu;
-    m += live ? 1 : 0;
-  }
-  if (m == 0) return r.sp == 0 && r.pmask == 0u;
+  for (int c = 0; c < kW; ++c)
+    k[c] = (imask >> c) & 1u ? ((__float_as_uint(tn[c]) & ~7u) | (uint32_t)c) : 0xffffffffu;
   if constexpr (kW == 8) {
     cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
     cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);

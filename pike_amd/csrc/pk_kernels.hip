@@ -337,8 +337,8 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   }
   const uint32_t n = F.cnt[ray_counter(in)];
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
-                  lds_key[wave]};
+  const Stack stk{(lds_u32)lds_node[wave], (lds_f32)lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
+                  (lds_u64)lds_key[wave]};
   TraverseStats st = {0, 0};
   // traced/hits are wave-uniform (counted from ballots at each flush, which
   // every finished ray passes once), so they cost no per-lane registers
@@ -871,8 +871,8 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
   const uint32_t n = F.cnt[kCntShadow];
   const uint32_t ns = F.views ? F.S : pass_samples(F, *F.params);
   const float theta = F.params->env_theta;
-  const Stack stk{lds_node[wave], lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
-                  lds_key[wave]};
+  const Stack stk{(lds_u32)lds_node[wave], (lds_f32)lds_t[wave], F.spill + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
+                  (lds_u64)lds_key[wave]};
   TraverseStats st = {0, 0};
   uint32_t traced = 0, overflow = 0;  // traced: wave-uniform, counted at each flush
   trace_loop<true, kCount, kInst, kLds, kNW>(
@@ -929,7 +929,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, F.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
+  const Stack stk{(lds_u32)lds_node, (lds_f32)lds_t, F.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
                   nullptr};
   const FrameParams P = *F.params;
   const uint32_t ns = kBatch ? F.S : pass_samples(F, P);
@@ -1260,7 +1260,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
-  const Stack stk{lds_node, lds_t, A.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
+  const Stack stk{(lds_u32)lds_node, (lds_f32)lds_t, A.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
                   nullptr};
   float* const vtx = A.vtx + (size_t)blockIdx.x * A.depth * kAdjVtxWords * 64 + lane;
   const uint32_t B = A.W * A.H;
@@ -1482,8 +1482,8 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
                                                                              uint32_t* overflow_flag,
                                                                              uint32_t refill_min) {
   PK_TRACE_WAVE_SETUP;
-  const Stack stk{lds_node[wave], lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
-                  lds_key[wave]};
+  const Stack stk{(lds_u32)lds_node[wave], (lds_f32)lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
+                  (lds_u64)lds_key[wave]};
   TraverseStats st = {0, 0};
   uint32_t overflow = 0;
   trace_loop<kAnyHit, kCount, kInst, kLds, kNW>(
