@@ -282,8 +282,8 @@ __device__ inline uint32_t lanes_below(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Cooperative leaf phase of the single-level traversal, run by all 64 lanes
-// of the wave together (idle lanes included).  The triangles of the leaf
+// Cooperative leaf phase of the traversal kernels, run by all 64 lanes of
+// the wave together (idle lanes included).  The triangles of the leaf
 // children each pending lane deferred are numbered lane by lane (in slot
 // order within a lane) and dealt out to the wave 64 at a time: lane w of a
 // round tests item base + w against its owner's ray.  Every candidate hit
@@ -296,8 +296,10 @@ __device__ inline uint32_t lanes_below(unsigned long long m) {
 // waiting for the lane with the most triangles.  Any-hit rays take the
 // least hit of the phase (the same whichever way it is dealt).  `pend`
 // marks the lanes whose ray holds deferred leaves; they leave with none.
-// Returns true on a pending any-hit lane that found a hit.
-template <bool kAnyHit, bool kCount, int kW>
+// With instancing an owner's ray, id and leaf deltas are those of the mesh
+// it waits in (a lane with deferred leaves visits no node), and keys carry
+// baked ids.  Returns true on a pending any-hit lane that found a hit.
+template <bool kAnyHit, bool kCount, bool kInst, int kW>
 __device__ inline bool coop_leaves(const DevScene& S, RayState& r, bool pend, const Stack& stk, TraverseStats* st) {
   const uint32_t lane = (uint32_t)stk.lane;
   const lds_u64 key = stk.key;
@@ -369,16 +371,22 @@ __device__ inline bool coop_leaves(const DevScene& S, RayState& r, bool pend, co
     const uint32_t rec = leaf_first((uint32_t)__shfl((int)tb, o), om0, om1, c) + q;
     const v3 ro = pkm::mk(__shfl(r.o.x, o), __shfl(r.o.y, o), __shfl(r.o.z, o));
     const v3 rd = pkm::mk(__shfl(r.d.x, o), __shfl(r.d.y, o), __shfl(r.d.z, o));
+    uint32_t id_delta = 0, leaf_delta = 0;
+    if (kInst) {
+      id_delta = (uint32_t)__shfl((int)r.id_delta, o);
+      leaf_delta = (uint32_t)__shfl((int)r.leaf_delta, o);
+    }
     if (w < T) {
       if (kCount) st->tris++;
       uint32_t id, bary = 0;
       const float t = intersect_tri(S.tris, rec, ro, rd, eps, &id, &bary);
       if (t < kMaxT) {
-        const unsigned long long mine = ((unsigned long long)__float_as_uint(t) << 32) | (unsigned long long)id;
+        const unsigned long long mine =
+            ((unsigned long long)__float_as_uint(t) << 32) | (unsigned long long)(id + id_delta);
         __hip_atomic_fetch_min(key + o, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         // (a wave's LDS operations complete in order: every lane's minimum
         // of this round is in)
-        if (key[o] == mine) aux[o] = ((unsigned long long)bary << 32) | rec;
+        if (key[o] == mine) aux[o] = ((unsigned long long)bary << 32) | (rec + leaf_delta);
       }
     }
   }
@@ -387,8 +395,9 @@ __device__ inline bool coop_leaves(const DevScene& S, RayState& r, bool pend, co
   if (k == init) return false;
   const float t = __uint_as_float((uint32_t)(k >> 32));
   // a hit at the best distance replaces the best only with a lower id
-  if (!kAnyHit && t == r.tbest && !ray_missed(r) && (uint32_t)k >= ray_hit_id<false>(S, r)) return false;
+  if (!kAnyHit && t == r.tbest && !ray_missed(r) && (uint32_t)k >= ray_hit_id<kInst>(S, r)) return false;
   const unsigned long long a = aux[lane];
+  if (kInst) r.best_id = (uint32_t)k;
   r.tbest = t;
   r.best_leaf = (uint32_t)a;
   r.best_bary = (uint32_t)(a >> 32);

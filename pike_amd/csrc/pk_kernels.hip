@@ -242,11 +242,14 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = false;
   while (true) {
+    // the leaf phase runs on the whole wave (coop_leaves), the node steps on
+    // the tracing lanes without deferred leaves; the trigger is ray_step's.
+    // Instanced any-hit rays keep ray_step's lane-by-lane leaf phase: their
+    // kernel has no registers to spare for the cooperative one (measured
+    // 8% slower on the instanced 1080p workload).
+    constexpr bool kCoop = !(kInst && kAnyHit);
     bool fin = false;
-    if constexpr (!kInst) {
-      // the leaf phase runs on the whole wave (coop_leaves), the node steps
-      // on the tracing lanes without deferred leaves; the trigger is
-      // ray_step's
+    if constexpr (kCoop) {
       const bool pend = active && r.pmask != 0u;
       const unsigned long long pm = __ballot(pend);
       bool leaf = false;
@@ -255,10 +258,10 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
         leaf = inner == 0ull || 16u * (uint32_t)__popcll(pm) >= S.leaf16 * (uint32_t)__popcll(__ballot(active));
       }
       if (leaf) {
-        const bool hit = coop_leaves<kAnyHit, kCount, kNW>(S, r, pend, stk, st);
+        const bool hit = coop_leaves<kAnyHit, kCount, kInst, kNW>(S, r, pend, stk, st);
         fin = active && (hit || (!r.have && r.sp == 0));
       } else if (active && !pend) {
-        fin = node_step<kAnyHit, kCount, false, kLds, kNW>(S, r, stk, st);
+        fin = node_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
       }
     } else {
       fin = active && ray_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
@@ -317,11 +320,12 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
 
 // The traversal kernels run kWpb independent waves per block (the block
 // size is a launch-shape choice of the autotuner), each with its own LDS
-// stack part of kLds entries per lane and its own spill region.
-#define PK_TRACE_WAVE_SETUP                                                                                       \
+// stack part of kLds entries per lane, its own spill region and, with the
+// cooperative leaf phase (kCoop), its 128 words of leaf-phase keys.
+#define PK_TRACE_WAVE_SETUP(kCoop)                                                                                \
   __shared__ uint32_t lds_node[kWpb][kLds * 64];                                                                 \
   __shared__ float lds_t[kWpb][kLds * 64];                                                                       \
-  __shared__ unsigned long long lds_key[kWpb][128];                                                              \
+  __shared__ unsigned long long lds_key[kWpb][(kCoop) ? 128 : 1];                                                \
   const int lane = (int)(threadIdx.x & 63u);                                                                     \
   const int wave = kWpb == 1 ? 0 : (int)(threadIdx.x >> 6);                                                      \
   const size_t wave_id = (size_t)blockIdx.x * kWpb + (size_t)wave
@@ -329,7 +333,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
 template <bool kCount, bool kInst, int kLds, int kWpb, int kNW = 8>
 __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_closest(DevScene S, FrameDev F, int bounce,
                                                                                 uint32_t refill_min) {
-  PK_TRACE_WAVE_SETUP;
+  PK_TRACE_WAVE_SETUP(true);
   const int in = bounce & 1;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     F.cnt[ray_counter(in ^ 1)] = 0;
@@ -863,7 +867,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64, 7) k_shade(DevScene S, Frame
 template <bool kCount, bool kInst, int kLds, int kWpb, int kNW = 8>
 __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace_shadow(DevScene S, FrameDev F,
                                                                                uint32_t refill_min) {
-  PK_TRACE_WAVE_SETUP;
+  PK_TRACE_WAVE_SETUP(!kInst);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     F.cnt[kCntHits] = 0;
     F.cnt[kCntWorkTrace] = 0;
@@ -1481,7 +1485,7 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
                                                                              unsigned long long* stats,
                                                                              uint32_t* overflow_flag,
                                                                              uint32_t refill_min) {
-  PK_TRACE_WAVE_SETUP;
+  PK_TRACE_WAVE_SETUP(!(kInst && kAnyHit));
   const Stack stk{(lds_u32)lds_node[wave], (lds_f32)lds_t[wave], spill_all + wave_id * kSpillStride * 64, lane, (int)S.stack_limit,
                   (lds_u64)lds_key[wave]};
   TraverseStats st = {0, 0};

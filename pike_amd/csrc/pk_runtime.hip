@@ -170,7 +170,8 @@ pk::TraceLaunch shape_launch(const pk_ctx* c, int cls, bool count) {
   const pk_ctx::Shape& sh = c->shape[cls];
   // whole blocks: the grid (in waves) is a multiple of the waves per block
   const uint32_t grid = std::max(sh.wpb, (uint32_t)c->num_cus * sh.waves / sh.wpb * sh.wpb);
-  const uint32_t fixed_grid = (uint32_t)c->num_cus * std::min(sh.waves, 4u * pk::kTraceWavesPerSimd);
+  const uint32_t fixed_grid =
+      (uint32_t)c->num_cus * std::min({sh.waves, 4u * pk::kTraceWavesPerSimd, lds_max_waves(pk::kStackLds, 1)});
   return pk::TraceLaunch{grid,   fixed_grid, sh.refill, count,     c->spill,
                          c->stack_limit, sh.lds, sh.wpb, sh.leaf16, sh.width};
 }
