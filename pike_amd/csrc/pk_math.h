@@ -121,19 +121,22 @@ PK_HD float atan_poly(float z) {
   p = fma_(p, z2, -1.0f / 3.0f);
   return fma_(p * z2, z, z);
 }
-// atan(a) for a >= 0.
-PK_HD float atan_pos(float a) {
+// atan(x) for x in [0, 1]: arguments above tan(pi/12) are reduced by
+// atan(x) = pi/6 + atan((x sqrt3 - 1) / (x + sqrt3)).  The reduced value is
+// selected rather than branched to, so a wave forms it once whichever of its
+// lanes need it.
+PK_HD float atan_unit(float x) {
   constexpr float kTanPi12 = 0x1.126146p-2f;  // 0.267949194 = 2 - sqrt3
   constexpr float kSqrt3 = 0x1.bb67aep+0f;
   constexpr float kPi6 = 0x1.0c1524p-1f;
+  const bool red = x > kTanPi12;
+  const float xr = (x * kSqrt3 - 1.0f) / (x + kSqrt3);
+  return (red ? kPi6 : 0.0f) + atan_poly(red ? xr : x);
+}
+// atan(a) for a >= 0.
+PK_HD float atan_pos(float a) {
   bool inv = a > 1.0f;
-  float x = inv ? 1.0f / a : a;
-  float base = 0.0f;
-  if (x > kTanPi12) {
-    x = (x * kSqrt3 - 1.0f) / (x + kSqrt3);
-    base = kPi6;
-  }
-  float r = base + atan_poly(x);
+  float r = atan_unit(inv ? 1.0f / a : a);
   return inv ? kHalfPi - r : r;
 }
 PK_HD float atanf_(float x) {
@@ -141,16 +144,19 @@ PK_HD float atanf_(float x) {
   return copysign_(r, x);
 }
 // atan2(y, x) in [-pi, pi]; atan2(+-0, x>=0) = +-0, atan2(y, 0) = +-pi/2.
+// The smaller magnitude over the larger (never above 1) goes through one
+// division and one atan_unit, then the octant's reflections: the same
+// operations as atan_pos(ay / ax) for ay <= ax and pi/2 - atan_pos(ax / ay)
+// otherwise, without a wave evaluating both.
 PK_HD float atan2f_(float y, float x) {
   float ay = abs_(y), ax = abs_(x);
   float r;
   if (ax == 0.0f && ay == 0.0f) {
     r = (f2u(x) >> 31) ? kPi : 0.0f;
-  } else if (ay <= ax) {
-    r = atan_pos(ay / ax);
-    if (f2u(x) >> 31) r = kPi - r;
   } else {
-    r = kHalfPi - atan_pos(ax / ay);
+    const bool steep = !(ay <= ax);
+    r = atan_unit((steep ? ax : ay) / (steep ? ay : ax));
+    if (steep) r = kHalfPi - r;
     if (f2u(x) >> 31) r = kPi - r;
   }
   return copysign_(r, y);
