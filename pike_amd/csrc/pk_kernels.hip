@@ -774,9 +774,20 @@ __device__ inline void shade_hit(const DevScene& S, float theta, v3 ro, v3 rd, f
       const float lw = power_heuristic(epdf, lambert_pdf(ewi));
       const v3 el = eval_lambert(si, ewi, epdf);
       v3 sc = pkm::mk((1.0f - f) * el.x * lw, (1.0f - f) * el.y * lw, (1.0f - f) * el.z * lw);
+      // specular_pdf(si, hh) and eval_specular(si, ewi, epdf) share the
+      // half vector, and also D and G1(wo) unless the evaluation roughness
+      // differs from the sampling one (a scene-wide, wave-uniform test)
       const v3 hh = pkm::normalize(pkm::add(si.wo, ewi));
-      const float sw = power_heuristic(epdf, specular_pdf(si, hh));
-      const v3 ev = eval_specular(si, ewi, epdf);
+      const float d_s = ggx_d(hh, si.alpha, si.alpha);
+      const float g1o_s = ggx_g1(si.wo, hh, si.alpha, si.alpha);
+      const float sw = power_heuristic(epdf, pkm::max_(kEpsShadow, d_s * g1o_s / (4.0f * si.wo.z)));
+      float d_e = d_s, g1o_e = g1o_s;
+      if (split) {
+        d_e = ggx_d(hh, si.alpha_e, si.alpha_e);
+        g1o_e = ggx_g1(si.wo, hh, si.alpha_e, si.alpha_e);
+      }
+      const v3 ev = pkm::scale(
+          si.spec_e, pkm::max_(d_e * (ggx_g1(ewi, hh, si.alpha_e, si.alpha_e) * g1o_e) / (4.0f * si.wo.z * epdf), 0.0f));
       sc = pkm::add(sc, pkm::mk(f * ev.x * sw, f * ev.y * sw, f * ev.z * sw));
       o.s_o = si.origin;
       o.s_d = pkm::mat_mul(si.basis, ewi);
