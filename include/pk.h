@@ -122,10 +122,10 @@ int pk_ctx_create(int device, pk_ctx** out);
 int pk_ctx_destroy(pk_ctx* ctx);
 int pk_sync(pk_ctx* ctx);
 /* Launch shape of the persistent traversal kernels, per kernel class (no
- * reference counterpart; chosen by the autotuner): waves_per_cu in [1, 24]
- * one-wave blocks per CU, refill_min in [1, 64] idle lanes that trigger a
- * wave's ray refill (64 = refill only when the whole wave is idle, which
- * keeps coherent rays together). */
+ * reference counterpart; chosen by the autotuner): waves_per_cu persistent
+ * waves per CU (at most what the class's LDS allows, below), refill_min in
+ * [1, 64] idle lanes that trigger a wave's ray refill (64 = refill only when
+ * the whole wave is idle, which keeps coherent rays together). */
 #define PK_TUNE_PRIMARY 0      /* closest hit, bounce 0 (camera rays) */
 #define PK_TUNE_SECONDARY 1    /* closest hit, bounces >= 1 */
 #define PK_TUNE_SHADOW 2       /* any hit, NEE shadow rays */
@@ -141,8 +141,10 @@ int pk_ctx_get_tuning(const pk_ctx* ctx, int kernel_class, uint32_t* waves_per_c
  * entry stack, trace.wgsl:5-9, 191-213).  pk_ctx_set_tuning changes the
  * first two fields and keeps the others.
  *   waves_per_cu     persistent traversal waves per CU, at most what the LDS
- *                    of the stack parts allows (8 entries: 40, 12: 26 with 1
- *                    or 2 waves per block and 24 with 4, 16: 20)
+ *                    allows: each wave holds its stack part and 1 KB for the
+ *                    cooperative leaf phase (8 entries: 32; 12: 22 with 1 or
+ *                    2 waves per block, 20 with 4; 16: 17 with 1, 16 with 2
+ *                    or 4)
  *   refill_min       idle lanes that trigger a wave's ray refill, [1, 64]
  *   stack_lds        traversal stack entries per lane kept in LDS (8, 12 or
  *                    16; the rest of the stack_limit entries spill to global
@@ -197,10 +199,11 @@ int pk_ctx_memory_used(const pk_ctx* ctx, uint64_t* bytes);
 int pk_ctx_set_scheduler(pk_ctx* ctx, int scheduler);
 int pk_ctx_get_scheduler(const pk_ctx* ctx, int* scheduler);
 /* Autotuner: coordinate search over the launch shape of each render kernel
- * class (refill_min in {8, 16, ..., 64} at 24 waves per CU, then
- * waves_per_cu in {4, 8, ..., 24} at the best threshold, then leaf_phase16
- * in {4, 6, ..., 12}, then stack_lds 8 / 12 / 16 at the best width and at
- * that depth's widest launch, then 2 and 4 waves per block), timing that
+ * class (refill_min in {8, 16, ..., 64} at the current width, then
+ * waves_per_cu in {8, 12, ..., 32, 40} (capped by the LDS) at the best
+ * threshold, then leaf_phase16 in {2, 4, ..., 14}, then stack_lds 8 / 12 / 16
+ * at the best width and at that depth's widest launch, then 2 and 4 waves
+ * per block, then for single-level scenes the 4-wide nodes), timing that
  * class's kernels with HIP events over `frames` profiled frames of this
  * scene, camera and resolution; keeps the fastest shape per class in the
  * context.  ms2 receives the traversal milliseconds per frame before and
