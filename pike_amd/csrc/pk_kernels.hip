@@ -227,6 +227,35 @@ __device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v
 // loop once the queue is exhausted and its own lanes are done.
 constexpr uint32_t kClaimChunk = 256;  // slots per cursor atomic (>= 64)
 
+// One wave-wide traversal step for the lanes whose `active` ray still
+// traces: the leaf phase runs on the whole wave (coop_leaves), the node
+// steps on the active lanes without deferred leaves, with ray_step's
+// trigger.  Instanced any-hit rays keep ray_step's lane-by-lane leaf phase:
+// their kernels have no registers to spare for the cooperative one
+// (measured 8% slower on the instanced 1080p workload).  Every lane of the
+// wave calls it together; returns true on an active lane whose ray is
+// finished.
+template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kNW>
+__device__ inline bool wave_step(const DevScene& S, RayState& r, bool active, const Stack& stk, TraverseStats* st) {
+  if constexpr (kInst && kAnyHit) {
+    return active && ray_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
+  } else {
+    const bool pend = active && r.pmask != 0u;
+    const unsigned long long pm = __ballot(pend);
+    bool leaf = false;
+    if (pm) {
+      const unsigned long long inner = __ballot(active && !pend && (r.have || r.sp > 0));
+      leaf = inner == 0ull || 16u * (uint32_t)__popcll(pm) >= S.leaf16 * (uint32_t)__popcll(__ballot(active));
+    }
+    if (leaf) {
+      const bool hit = coop_leaves<kAnyHit, kCount, kInst, kNW>(S, r, pend, stk, st);
+      return active && (hit || (!r.have && r.sp == 0));
+    }
+    if (active && !pend) return node_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
+    return false;
+  }
+}
+
 template <bool kAnyHit, bool kCount, bool kInst, int kLds, int kNW, class Load, class Finish, class Flush>
 __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* cursor, uint32_t refill_min,
                                   const Stack& stk, TraverseStats* st, uint32_t* overflow, Load load,
@@ -242,31 +271,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = false;
   while (true) {
-    // the leaf phase runs on the whole wave (coop_leaves), the node steps on
-    // the tracing lanes without deferred leaves; the trigger is ray_step's.
-    // Instanced any-hit rays keep ray_step's lane-by-lane leaf phase: their
-    // kernel has no registers to spare for the cooperative one (measured
-    // 8% slower on the instanced 1080p workload).
-    constexpr bool kCoop = !(kInst && kAnyHit);
-    bool fin = false;
-    if constexpr (kCoop) {
-      const bool pend = active && r.pmask != 0u;
-      const unsigned long long pm = __ballot(pend);
-      bool leaf = false;
-      if (pm) {
-        const unsigned long long inner = __ballot(active && !pend && (r.have || r.sp > 0));
-        leaf = inner == 0ull || 16u * (uint32_t)__popcll(pm) >= S.leaf16 * (uint32_t)__popcll(__ballot(active));
-      }
-      if (leaf) {
-        const bool hit = coop_leaves<kAnyHit, kCount, kInst, kNW>(S, r, pend, stk, st);
-        fin = active && (hit || (!r.have && r.sp == 0));
-      } else if (active && !pend) {
-        fin = node_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
-      }
-    } else {
-      fin = active && ray_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, stk, st);
-    }
-    if (fin) {
+    if (wave_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, active, stk, st)) {
       active = false;
       done = true;
       if (r.overflow) *overflow = 1;
@@ -944,8 +949,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
+  __shared__ unsigned long long lds_key[128];
   const Stack stk{(lds_u32)lds_node, (lds_f32)lds_t, F.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
-                  nullptr};
+                  (lds_u64)lds_key};
   const FrameParams P = *F.params;
   const uint32_t ns = kBatch ? F.S : pass_samples(F, P);
   const uint32_t B = F.W * F.H;
@@ -1026,8 +1032,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
     if (has) {
       ray_init<false>(r, o, d);
       if (kBatch) r.node = root;
-      while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
-      }
+    }
+    for (bool tr = has; __ballot(tr);)
+      if (wave_step<false, false, kInst, kStackLds, 8>(S, r, tr, stk, &st)) tr = false;
+    if (has) {
       if (r.overflow) overflow = 1;
       traced++;
     }
@@ -1055,12 +1063,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMegaWa
       }
     }
     // NEE shadow ray (trace_shadow.wgsl), unoccluded: env radiance x NEE throughput
+    RayState q;
+    q.best_leaf = 0xffffffffu;
     if (so.shadow) {
-      RayState q;
       ray_init<true>(q, so.s_o, so.s_d);
       if (kBatch) q.node = root;
-      while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
-      }
+    }
+    for (bool tr = so.shadow; __ballot(tr);)
+      if (wave_step<true, false, kInst, kStackLds, 8>(S, q, tr, stk, &st)) tr = false;
+    if (so.shadow) {
       if (q.overflow) overflow = 1;
       shadows++;
       if (ray_missed(q)) {
@@ -1275,8 +1286,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   __shared__ uint32_t lds_node[kStackLds * 64];
   __shared__ float lds_t[kStackLds * 64];
   const int lane = (int)threadIdx.x;
+  __shared__ unsigned long long lds_key[128];
   const Stack stk{(lds_u32)lds_node, (lds_f32)lds_t, A.spill + (size_t)blockIdx.x * kSpillStride * 64, lane, (int)S.stack_limit,
-                  nullptr};
+                  (lds_u64)lds_key};
   float* const vtx = A.vtx + (size_t)blockIdx.x * A.depth * kAdjVtxWords * 64 + lane;
   const uint32_t B = A.W * A.H;
   const uint32_t total = B * A.num_frames;
@@ -1315,12 +1327,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       r.tbest = kMaxT;
       r.best_leaf = 0xffffffffu;
       r.best_bary = 0;
-      if (alive) {
-        ray_init<false>(r, o, d);
-        while (!ray_step<false, false, kInst>(S, r, stk, &st)) {
-        }
-        if (r.overflow) overflow = 1;
-      }
+      if (alive) ray_init<false>(r, o, d);
+      for (bool tr = alive; __ballot(tr);)
+        if (wave_step<false, false, kInst, kStackLds, 8>(S, r, tr, stk, &st)) tr = false;
+      if (alive && r.overflow) overflow = 1;
       const bool miss = alive && ray_missed(r);
       if (miss) {
         c = pkm::mul(env_color(S, A.theta, d), thr);
@@ -1348,13 +1358,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       }
       // NEE shadow ray (trace_shadow.wgsl) and its environment term
       bool vis = false;
-      if (hit && so.shadow) {
+      {
         RayState q;
-        ray_init<true>(q, so.s_o, so.s_d);
-        while (!ray_step<true, false, kInst>(S, q, stk, &st)) {
+        q.best_leaf = 0xffffffffu;
+        const bool sh = hit && so.shadow;
+        if (sh) ray_init<true>(q, so.s_o, so.s_d);
+        for (bool tr = sh; __ballot(tr);)
+          if (wave_step<true, false, kInst, kStackLds, 8>(S, q, tr, stk, &st)) tr = false;
+        if (sh) {
+          if (q.overflow) overflow = 1;
+          vis = ray_missed(q);
         }
-        if (q.overflow) overflow = 1;
-        vis = ray_missed(q);
       }
       v3 E = zero;
       if (vis) {
