@@ -157,3 +157,64 @@ def test_overlapping_boxes_deep_stacks(ctx, refill):
     _assert_same(ids, t, bary, ids_o, t_o, b_o)
     occ_o, _, _, _ = orc.trace(rays, "canonical", any_hit=True)
     np.testing.assert_array_equal(occ >= 0, occ_o >= 0)
+
+
+@pytest.mark.parametrize("leaf16", [1, 16])
+def test_stacked_duplicates_large_leaves(ctx, leaf16):
+    """Stacks of identical triangles (equal t for every copy, so the lowest
+    id must win) that no split separates: leaves of up to 255 triangles and
+    leaf phases whose items outnumber the wave's 64 lanes several times, one
+    lane's items spanning several rounds.  Closest hits equal the canonical
+    ones (ties resolved by id) and occlusion the oracle's, with the leaf
+    phase triggered by a single deferring lane or only by all of them."""
+    rng = np.random.default_rng(21)
+    pos, tri = [], []
+    stacks = []  # centroid of each stack
+    for k in (1, 7, 33, 64, 130, 255, 300):
+        c = rng.uniform(0.1, 0.9, 3)
+        v = c + rng.uniform(-0.04, 0.04, (3, 3))
+        stacks.append(v.mean(0))
+        for _ in range(k):
+            base = len(pos)
+            pos.extend(v)
+            tri.append((base, base + 1, base + 2, 0))
+    bg = rng.uniform(0.0, 1.0, (600, 3))
+    for i in range(200):
+        base = len(pos)
+        pos.extend(bg[3 * i:3 * i + 3])
+        tri.append((base, base + 1, base + 2, 0))
+    from .test_bvh import _data
+    data = _data(np.asarray(pos, np.float64), np.asarray(tri, np.int32))
+    bvh, scene, orc = _setup(ctx, data)
+    assert bvh.info()["largest_leaf"] >= 64
+    # rays through the stacks' centroids from random origins, and incoherent ones
+    cents = np.asarray(stacks)
+    n = 30_000
+    org = rng.uniform(-0.5, 1.5, (n, 3))
+    tgt = cents[rng.integers(0, len(cents), n)] + rng.normal(0, 0.01, (n, 3))
+    norm = bvh.normalizer()
+    o = (org - norm[:3]) * norm[3]
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays = np.concatenate([rays, scenes.incoherent_rays(10_000, norm, seed=22)])
+    old = ctx.tuning()
+    try:
+        for cls in ("rays_closest", "rays_any"):
+            sh = old[cls]
+            ctx.set_tuning(cls, sh["waves_per_cu"], sh["refill_min"], sh["stack_lds"], sh["waves_per_block"], leaf16,
+                           sh["node_width"])
+        ids, t, bary = ctx.trace_rays(scene, rays)
+        occ, _, _ = ctx.trace_rays(scene, rays, any_hit=True)
+    finally:
+        for cls in ("rays_closest", "rays_any"):
+            sh = old[cls]
+            ctx.set_tuning(cls, sh["waves_per_cu"], sh["refill_min"], sh["stack_lds"], sh["waves_per_block"],
+                           sh["leaf_phase16"], sh["node_width"])
+    ids_o, t_o, b_o, _ = orc.trace(rays, "canonical")
+    assert (ids_o >= 0).mean() > 0.5
+    _assert_same(ids, t, bary, ids_o, t_o, b_o)
+    occ_o, _, _, _ = orc.trace(rays, "canonical", any_hit=True)
+    np.testing.assert_array_equal(occ >= 0, occ_o >= 0)
