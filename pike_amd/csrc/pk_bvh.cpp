@@ -11,8 +11,10 @@
 //   * leaf iff best cost > N (bvh.js:137), leaf order = refs on the best axis;
 //   * children inherit each axis order by stable filtering (bvh.js:80-107).
 // The implementation is range-based (three index arrays partitioned in
-// place, O(N) scratch per level) instead of bvh.js's per-node copies, and
-// builds independent subtrees on threads; the tree is the same.
+// place, O(N) scratch per level) instead of bvh.js's per-node copies; it
+// splits a large node's passes (unions, sweeps, binning, partitions) over
+// threads and hands subtrees to threads as they free up.  The tree is the
+// same for every thread count.
 #include <algorithm>
 #include <iterator>
 #include <cfloat>
@@ -22,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <future>
 #include <memory>
 #include <mutex>
@@ -54,6 +57,66 @@ inline double box_area(const Box& b) {
   double zl = std::max(b.hi[2] - b.lo[2], 0.0);
   return (xl * yl + xl * zl + yl * zl) * 2;
 }
+
+// fn(c) for c in [0, parts), part 0 on the calling thread, the others on
+// threads of their own; an exception from any part is rethrown after all end.
+template <class F> void run_parts(int parts, F&& fn) {
+  if (parts <= 1) {
+    fn(0);
+    return;
+  }
+  std::vector<std::exception_ptr> err(parts);
+  auto guarded = [&](int c) {
+    try {
+      fn(c);
+    } catch (...) {
+      err[c] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> ts;
+  ts.reserve(parts - 1);
+  for (int c = 1; c < parts; ++c) ts.emplace_back(guarded, c);
+  guarded(0);
+  for (auto& t : ts) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+// first item of part c when n items are cut into `parts` consecutive parts
+inline size_t part_begin(size_t n, int parts, int c) { return n * (size_t)c / (size_t)parts; }
+
+// A node's passes over its references run on up to `width` threads when it
+// holds at least two grains of them.  min and max are exact, so unions taken
+// part by part are the serial unions, and sweeps cut into parts keep the
+// serial first minimum: the tree does not depend on the thread count.
+constexpr size_t kParGrain = 1u << 15;
+inline int parts_for(int width, size_t n) {
+  return width <= 1 || n < 2 * kParGrain ? 1 : (int)std::min<size_t>((size_t)width, n / kParGrain);
+}
+
+// Subtree tasks: a split builds its left subtree on a thread of its own while
+// fewer than `threads` threads are working; a thread that waits for the half
+// it handed off does not count as working.  Which thread builds a subtree
+// changes node numbering only, never the tree.
+struct Tasks {
+  int threads = 1;
+  std::atomic<int> working{1};
+  template <class L, class R> void fork(size_t n, L&& left, R&& right) {
+    if (n <= 32768 || working.load(std::memory_order_relaxed) >= threads) {
+      left();
+      right();
+      return;
+    }
+    working.fetch_add(1);
+    auto fut = std::async(std::launch::async, [&]() {
+      left();
+      working.fetch_sub(1);
+    });
+    right();
+    working.fetch_sub(1);
+    fut.get();
+    working.fetch_add(1);
+  }
+};
 
 // Chunked store with lock-free allocation and stable addresses for the
 // threaded builds (nodes; SBVH references).
@@ -97,6 +160,97 @@ template <class T> class Pool {
 };
 using NodePool = Pool<BNode>;
 
+// The object-split sweep (bvh.js:184-208) over n boxes in sweep order:
+// cost_i = 0.5 + A(front_i)/A(P)*(i+1) + A(back_i)/A(P)*(n-1-i), first strict
+// minimum.  Part c of `parts` starts from the union of the boxes before it
+// and the union of the boxes after it, so each cost is the serial one.
+template <class BoxOf>
+void sweep_parts(size_t n, BoxOf box_of, double parent_area, int parts, double* best_cost, uint32_t* best_index) {
+  if (parts <= 1) {
+    // small sweeps reuse one buffer per thread; large ones take their own
+    thread_local std::vector<double> small;
+    std::vector<double> large;
+    if (n <= kParGrain && small.size() < n) small.resize(kParGrain);
+    if (n > kParGrain) large.resize(n);
+    double* suf = n <= kParGrain ? small.data() : large.data();
+    Box bb;
+    box_empty(bb);
+    for (size_t i = n; i-- > 0;) {
+      box_add(bb, box_of(i));
+      suf[i] = box_area(bb);
+    }
+    Box bf;
+    box_empty(bf);
+    double bc = INFINITY;
+    uint32_t bi = 0;
+    for (size_t i = 0; i < n; ++i) {
+      box_add(bf, box_of(i));
+      const double cst = 0.5 + (box_area(bf) / parent_area) * (double)(i + 1) + (suf[i] / parent_area) * (double)(n - 1 - i);
+      if (cst < bc) {
+        bc = cst;
+        bi = (uint32_t)i;
+      }
+    }
+    *best_cost = bc;
+    *best_index = bi;
+    return;
+  }
+  std::vector<Box> before(parts), after(parts);
+  for (int c = 0; c < parts; ++c) {
+    box_empty(before[c]);
+    box_empty(after[c]);
+  }
+  if (parts > 1) {
+    std::vector<Box> tot(parts);
+    run_parts(parts, [&](int c) {
+      box_empty(tot[c]);
+      for (size_t i = part_begin(n, parts, c), e = part_begin(n, parts, c + 1); i < e; ++i) box_add(tot[c], box_of(i));
+    });
+    for (int c = 1; c < parts; ++c) {
+      before[c] = before[c - 1];
+      box_add(before[c], tot[c - 1]);
+    }
+    for (int c = parts - 1; c-- > 0;) {
+      after[c] = after[c + 1];
+      box_add(after[c], tot[c + 1]);
+    }
+  }
+  std::vector<double> cost(parts);
+  std::vector<uint32_t> index(parts);
+  run_parts(parts, [&](int c) {
+    const size_t b = part_begin(n, parts, c), e = part_begin(n, parts, c + 1);
+    std::vector<double> suf(e - b);
+    Box bb = after[c];
+    for (size_t i = e; i-- > b;) {
+      box_add(bb, box_of(i));
+      suf[i - b] = box_area(bb);  // union of boxes i..n-1
+    }
+    Box bf = before[c];
+    double bc = INFINITY;
+    uint32_t bi = 0;
+    for (size_t i = b; i < e; ++i) {
+      box_add(bf, box_of(i));
+      const double cst =
+          0.5 + (box_area(bf) / parent_area) * (double)(i + 1) + (suf[i - b] / parent_area) * (double)(n - 1 - i);
+      if (cst < bc) {
+        bc = cst;
+        bi = (uint32_t)i;
+      }
+    }
+    cost[c] = bc;
+    index[c] = bi;
+  });
+  double bc = INFINITY;
+  uint32_t bi = 0;
+  for (int c = 0; c < parts; ++c)
+    if (cost[c] < bc) {
+      bc = cost[c];
+      bi = index[c];
+    }
+  *best_cost = bc;
+  *best_index = bi;
+}
+
 struct Build {
   const std::vector<Box>* ref_box;
   uint32_t* axis[3];           // three index arrays of length nt
@@ -105,71 +259,48 @@ struct Build {
   NodePool* pool;
   std::atomic<int> degenerate{0};
   std::atomic<int> max_depth{0};
+  Tasks tasks;
 
-  void sweep(int a, uint32_t b, uint32_t n, double parent_area, double* best_cost, uint32_t* best_index,
-             std::vector<double>& suf) {
-    const uint32_t* refs = axis[a] + b;
-    const std::vector<Box>& rb = *ref_box;
-    Box bb;
-    box_empty(bb);
-    for (uint32_t i = n; i-- > 0;) {
-      box_add(bb, rb[refs[i]]);
-      suf[i] = box_area(bb);  // union of refs[i..n-1]
-    }
-    Box bf;
-    box_empty(bf);
-    double bc = INFINITY;
-    uint32_t bi = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-      box_add(bf, rb[refs[i]]);
-      double saf = box_area(bf);
-      double cost = 0.5 + (saf / parent_area) * (double)(i + 1) + (suf[i] / parent_area) * (double)(n - 1 - i);
-      if (cost < bc) {
-        bc = cost;
-        bi = i;
-      }
-    }
-    *best_cost = bc;
-    *best_index = bi;
-  }
-
-  // Builds the subtree over [b, b+n) of all three arrays into node `ni`.
-  void node(uint32_t ni, uint32_t b, uint32_t n, int depth, int par_depth) {
+  // Builds the subtree over [b, b+n) of all three arrays into node `ni`;
+  // width = threads for this node's own passes.  Returns the subtree's node
+  // count (kept in an internal node's `begin` until the compaction).
+  uint32_t node(uint32_t ni, uint32_t b, uint32_t n, int depth, int width) {
     int md = max_depth.load();
     while (depth > md && !max_depth.compare_exchange_weak(md, depth)) {
     }
     const std::vector<Box>& rb = *ref_box;
+    const int parts = parts_for(width, n);
     Box bounds;
-    box_empty(bounds);
-    for (uint32_t i = 0; i < n; ++i) box_add(bounds, rb[axis[0][b + i]]);
+    {
+      std::vector<Box> u(parts);
+      run_parts(parts, [&](int c) {
+        box_empty(u[c]);
+        for (size_t i = part_begin(n, parts, c), e = part_begin(n, parts, c + 1); i < e; ++i)
+          box_add(u[c], rb[axis[0][b + i]]);
+      });
+      box_empty(bounds);
+      for (const Box& x : u) box_add(bounds, x);
+    }
     BNode& nd = pool->at(ni);
     for (int k = 0; k < 3; ++k) {
       nd.lo[k] = bounds.lo[k];
       nd.hi[k] = bounds.hi[k];
     }
     const double parent_area = box_area(bounds);
-    double cost[3];
-    uint32_t index[3];
-    if (n > (1u << 20)) {
-      std::vector<double> s0(n), s1(n), s2(n);
-      std::thread t1([&]() { sweep(1, b, n, parent_area, &cost[1], &index[1], s1); });
-      std::thread t2([&]() { sweep(2, b, n, parent_area, &cost[2], &index[2], s2); });
-      sweep(0, b, n, parent_area, &cost[0], &index[0], s0);
-      t1.join();
-      t2.join();
-    } else {
-      std::vector<double> suf(n);
-      for (int a = 0; a < 3; ++a) sweep(a, b, n, parent_area, &cost[a], &index[a], suf);
-    }
     double best_cost = INFINITY;
     int best_axis = -1;
     uint32_t best_index = 0;
-    for (int a = 0; a < 3; ++a)
-      if (cost[a] < best_cost) {
-        best_cost = cost[a];
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t* refs = axis[a] + b;
+      double c;
+      uint32_t i;
+      sweep_parts(n, [&](size_t j) -> const Box& { return rb[refs[j]]; }, parent_area, parts, &c, &i);
+      if (c < best_cost) {
+        best_cost = c;
         best_axis = a;
-        best_index = index[a];
+        best_index = i;
       }
+    }
     if (best_axis < 0 || best_cost > (double)n) {
       if (best_axis < 0) {
         degenerate.store(1);
@@ -180,16 +311,25 @@ struct Build {
       nd.begin = b;
       nd.count = n;
       std::memcpy(leaf_out + b, axis[best_axis] + b, (size_t)n * 4);
-      return;
+      return 1;
     }
     nd.split_axis = (uint8_t)best_axis;
     const uint32_t nl = best_index + 1;
     uint32_t* split = axis[best_axis] + b;
-    for (uint32_t i = 0; i < nl; ++i) in_left[split[i]] = 1;
-    std::vector<uint32_t> tmp(n - nl);
-    for (int a = 0; a < 3; ++a) {
-      if (a == best_axis) continue;
-      uint32_t* arr = axis[a] + b;
+    auto set_left = [&](uint8_t v) {
+      run_parts(parts, [&](int c) {
+        for (size_t i = part_begin(nl, parts, c), e = part_begin(nl, parts, c + 1); i < e; ++i) in_left[split[i]] = v;
+      });
+    };
+    set_left(1);
+    // the other two arrays, partitioned stably (on two threads when the
+    // node's passes are split)
+    int other[2], no = 0;
+    for (int a = 0; a < 3; ++a)
+      if (a != best_axis) other[no++] = a;
+    auto partition = [&](int c) {
+      uint32_t* arr = axis[other[c]] + b;
+      std::vector<uint32_t> tmp(n - nl);
       uint32_t li = 0, ri = 0;
       for (uint32_t i = 0; i < n; ++i) {
         uint32_t r = arr[i];
@@ -197,23 +337,22 @@ struct Build {
         else tmp[ri++] = r;
       }
       std::memcpy(arr + nl, tmp.data(), (size_t)ri * 4);
-    }
-    for (uint32_t i = 0; i < nl; ++i) in_left[split[i]] = 0;
-    std::vector<uint32_t>().swap(tmp);
+    };
+    if (parts > 1) run_parts(2, partition);
+    else
+      for (int c = 0; c < 2; ++c) partition(c);
+    set_left(0);
     uint32_t l = pool->alloc(), r = pool->alloc();
     {
       BNode& p = pool->at(ni);
       p.left = (int32_t)l;
       p.right = (int32_t)r;
     }
-    if (par_depth > 0 && n > 32768) {
-      auto fut = std::async(std::launch::async, [&]() { node(l, b, nl, depth + 1, par_depth - 1); });
-      node(r, b + nl, n - nl, depth + 1, par_depth - 1);
-      fut.get();
-    } else {
-      node(l, b, nl, depth + 1, 0);
-      node(r, b + nl, n - nl, depth + 1, 0);
-    }
+    const int cw = std::max(1, width / 2);
+    uint32_t sl = 0, sr = 0;
+    tasks.fork(n, [&]() { sl = node(l, b, nl, depth + 1, cw); }, [&]() { sr = node(r, b + nl, n - nl, depth + 1, cw); });
+    pool->at(ni).begin = 1 + sl + sr;
+    return 1 + sl + sr;
   }
 };
 
@@ -261,36 +400,48 @@ struct SBuild {
   std::atomic<int> degenerate{0};
   std::atomic<int> max_depth{0};
   std::atomic<int64_t> spatial{0};
+  Tasks tasks;
 
-  static double area_of(const Box& b) { return box_area(b); }
+  // a reference's fragments: parts[part + i - first] is its fragment in slab i
+  struct Cut {
+    uint32_t ref;
+    int first, last;
+    size_t part;
+  };
+  // the slab binning of a run of references: per reference its fragments,
+  // per slab the bounds and the references entering and leaving there
+  struct Bins {
+    std::vector<Cut> cuts;
+    std::vector<Box> parts;
+    Box slab_box[kSpatialBins];
+    uint32_t enter[kSpatialBins] = {0}, leave[kSpatialBins] = {0};
+  };
 
-  template <class BoxOf>
-  static void sweep_boxes(uint32_t n, BoxOf box_of, double parent_area, double* best_cost, uint32_t* best_index) {
-    std::vector<double> suf(n);
-    Box bb;
-    box_empty(bb);
-    for (uint32_t i = n; i-- > 0;) {
-      box_add(bb, box_of(i));
-      suf[i] = box_area(bb);
+  const Box& box_of(uint32_t r) const { return refs->at(r).box; }
+
+  Box union_of(const std::vector<uint32_t>& l, size_t b, size_t e, int parts) const {
+    if (parts <= 1) {
+      Box r;
+      box_empty(r);
+      for (size_t i = b; i < e; ++i) box_add(r, box_of(l[i]));
+      return r;
     }
-    Box bf;
-    box_empty(bf);
-    double bc = INFINITY;
-    uint32_t bi = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-      box_add(bf, box_of(i));
-      double cost = 0.5 + (box_area(bf) / parent_area) * (double)(i + 1) + (suf[i] / parent_area) * (double)(n - 1 - i);
-      if (cost < bc) {
-        bc = cost;
-        bi = i;
-      }
-    }
-    *best_cost = bc;
-    *best_index = bi;
+    std::vector<Box> u(parts);
+    run_parts(parts, [&](int c) {
+      box_empty(u[c]);
+      for (size_t i = b + part_begin(e - b, parts, c), ie = b + part_begin(e - b, parts, c + 1); i < ie; ++i)
+        box_add(u[c], box_of(l[i]));
+    });
+    Box r;
+    box_empty(r);
+    for (const Box& x : u) box_add(r, x);
+    return r;
   }
-  void sweep(const std::vector<uint32_t>& l, double parent_area, double* best_cost, uint32_t* best_index) {
-    sweep_boxes((uint32_t)l.size(), [&](uint32_t i) -> const Box& { return refs->at(l[i]).box; }, parent_area,
-                best_cost, best_index);
+
+  void sweep(const std::vector<uint32_t>& l, double parent_area, int parts, double* best_cost,
+             uint32_t* best_index) const {
+    sweep_parts(l.size(), [&](size_t i) -> const Box& { return box_of(l[i]); }, parent_area, parts, best_cost,
+                best_index);
   }
 
   // axisIntersectTriangle: bounds of the triangle's part inside slab [p0, p1]
@@ -336,14 +487,77 @@ struct SBuild {
     return any;
   }
 
-  void node(uint32_t ni, std::vector<uint32_t> ax[3], int depth, int par) {
+  // Bins l[b, e) into kSpatialBins slabs of [lo, lo + span) along axis a.
+  void bin(const std::vector<uint32_t>& l, size_t b, size_t e, int a, double lo, double span, Bins& out) const {
+    out.cuts.reserve(e - b);
+    out.parts.reserve((e - b) + (e - b) / 2);
+    for (auto& sb : out.slab_box) box_empty(sb);
+    Box tmp[kSpatialBins];
+    for (size_t j = b; j < e; ++j) {
+      const uint32_t r = l[j];
+      const SRef& ref = refs->at(r);
+      const double rmin = ref.box.lo[a] - lo, rmax = ref.box.hi[a] - lo;
+      // a box starting on the node's upper face maps to slab 64: clamp
+      const int start = std::min(std::max(0, (int)std::floor((rmin / span) * kSpatialBins)), kSpatialBins - 1);
+      const int stop = std::max(std::min((int)std::floor((rmax / span) * kSpatialBins), kSpatialBins - 1), start);
+      Cut c{r, -1, -1, out.parts.size()};
+      if (start == stop) {
+        // within one slab: its fragment there is the reference itself
+        // (the slab's clip of the triangle contains the reference box)
+        c.first = c.last = start;
+        out.parts.push_back(ref.box);
+        box_add(out.slab_box[start], ref.box);
+        ++out.enter[start];
+        ++out.leave[start];
+        out.cuts.push_back(c);
+        continue;
+      }
+      for (int i = start; i <= stop; ++i) {
+        const double p0 = lo + (double)i * span / kSpatialBins;
+        const double p1 = lo + (double)(i + 1) * span / kSpatialBins;
+        Box fb;
+        if (!clip(ref.tri, a, p0, p1, &fb)) fb = Box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+        for (int k = 0; k < 3; ++k) {
+          fb.lo[k] = std::max(fb.lo[k], ref.box.lo[k]);
+          fb.hi[k] = std::min(fb.hi[k], ref.box.hi[k]);
+        }
+        const bool empty = !(fb.lo[0] <= fb.hi[0] && fb.lo[1] <= fb.hi[1] && fb.lo[2] <= fb.hi[2]);
+        if (!empty) {
+          if (c.first < 0) c.first = i;
+          c.last = i;
+          box_add(out.slab_box[i], fb);
+        }
+        tmp[i - start] = fb;
+      }
+      if (c.first < 0) {  // degenerate clip: keep the reference whole in its first slab
+        c.first = c.last = start;
+        out.parts.push_back(ref.box);
+        box_add(out.slab_box[start], ref.box);
+      } else {
+        out.parts.insert(out.parts.end(), tmp + (c.first - start), tmp + (c.last - start + 1));
+      }
+      ++out.enter[c.first];
+      ++out.leave[c.last];
+      out.cuts.push_back(c);
+    }
+  }
+
+  // Builds the subtree over the three centroid-sorted lists ax into node ni;
+  // width = threads for this node's own passes.  Returns the subtree's node
+  // count (kept in an internal node's `begin` until the compaction).
+  uint32_t node(uint32_t ni, std::vector<uint32_t> ax[3], int depth, int width) {
     int md = max_depth.load();
     while (depth > md && !max_depth.compare_exchange_weak(md, depth)) {
     }
     const uint32_t n = (uint32_t)ax[0].size();
-    Box bounds;
-    box_empty(bounds);
-    for (uint32_t r : ax[0]) box_add(bounds, refs->at(r).box);
+    const int parts = parts_for(width, n);
+    // per-axis passes run on three threads when the node's passes are split
+    auto each_axis = [&](auto&& f) {
+      if (parts > 1) run_parts(3, f);
+      else
+        for (int k = 0; k < 3; ++k) f(k);
+    };
+    const Box bounds = union_of(ax[0], 0, n, parts);
     {
       BNode& nd = pool->at(ni);
       for (int k = 0; k < 3; ++k) {
@@ -358,7 +572,7 @@ struct SBuild {
     for (int a = 0; a < 3; ++a) {
       double c;
       uint32_t i;
-      sweep(ax[a], parent_area, &c, &i);
+      sweep(ax[a], parent_area, parts, &c, &i);
       if (c < best_cost) {
         best_cost = c;
         best_axis = a;
@@ -372,12 +586,9 @@ struct SBuild {
     uint32_t split_index = best_index;
     if (best_axis >= 0) {
       // overlap of the object split's two child boxes (back box includes ref i)
-      Box f, b;
-      box_empty(f);
-      box_empty(b);
       const std::vector<uint32_t>& l = ax[best_axis];
-      for (uint32_t i = 0; i <= best_index; ++i) box_add(f, refs->at(l[i]).box);
-      for (uint32_t i = best_index; i < n; ++i) box_add(b, refs->at(l[i]).box);
+      Box f = union_of(l, 0, (size_t)best_index + 1, parts);
+      const Box b = union_of(l, best_index, n, parts);
       for (int k = 0; k < 3; ++k) {
         f.lo[k] = std::max(f.lo[k], b.lo[k]);
         f.hi[k] = std::min(f.hi[k], b.hi[k]);
@@ -386,68 +597,17 @@ struct SBuild {
         const int a = best_axis;
         const double lo = bounds.lo[a], span = bounds.hi[a] - lo;
         if (span != 0) {
-          // per reference: fragment boxes of the slabs it reaches (clipped
-          // triangle, within the reference's own box); per slab: bounds,
-          // references entering and leaving there
-          struct Cut {
-            uint32_t ref;
-            int first, last;
-            size_t part;  // parts[part + i - first]: fragment in slab i
-          };
-          std::vector<Cut> cuts;
-          std::vector<Box> parts;
-          cuts.reserve(n);
-          parts.reserve(n + n / 2);
-          Box tmp[kSpatialBins];
+          std::vector<Bins> bins(parts);
+          run_parts(parts, [&](int c) { bin(l, part_begin(n, parts, c), part_begin(n, parts, c + 1), a, lo, span, bins[c]); });
           Box slab_box[kSpatialBins];
           uint32_t enter[kSpatialBins] = {0}, leave[kSpatialBins] = {0};
           for (auto& sb : slab_box) box_empty(sb);
-          for (uint32_t r : l) {
-            const SRef& ref = refs->at(r);
-            const double rmin = ref.box.lo[a] - lo, rmax = ref.box.hi[a] - lo;
-            // a box starting on the node's upper face maps to slab 64: clamp
-            const int start = std::min(std::max(0, (int)std::floor((rmin / span) * kSpatialBins)), kSpatialBins - 1);
-            const int stop = std::max(std::min((int)std::floor((rmax / span) * kSpatialBins), kSpatialBins - 1), start);
-            Cut c{r, -1, -1, parts.size()};
-            if (start == stop) {
-              // within one slab: its fragment there is the reference itself
-              // (the slab's clip of the triangle contains the reference box)
-              c.first = c.last = start;
-              parts.push_back(ref.box);
-              box_add(slab_box[start], ref.box);
-              ++enter[start];
-              ++leave[start];
-              cuts.push_back(c);
-              continue;
+          for (const Bins& bn : bins)
+            for (int i = 0; i < kSpatialBins; ++i) {
+              box_add(slab_box[i], bn.slab_box[i]);
+              enter[i] += bn.enter[i];
+              leave[i] += bn.leave[i];
             }
-            for (int i = start; i <= stop; ++i) {
-              const double p0 = lo + (double)i * span / kSpatialBins;
-              const double p1 = lo + (double)(i + 1) * span / kSpatialBins;
-              Box fb;
-              if (!clip(ref.tri, a, p0, p1, &fb)) fb = Box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
-              for (int k = 0; k < 3; ++k) {
-                fb.lo[k] = std::max(fb.lo[k], ref.box.lo[k]);
-                fb.hi[k] = std::min(fb.hi[k], ref.box.hi[k]);
-              }
-              const bool empty = !(fb.lo[0] <= fb.hi[0] && fb.lo[1] <= fb.hi[1] && fb.lo[2] <= fb.hi[2]);
-              if (!empty) {
-                if (c.first < 0) c.first = i;
-                c.last = i;
-                box_add(slab_box[i], fb);
-              }
-              tmp[i - start] = fb;
-            }
-            if (c.first < 0) {  // degenerate clip: keep the reference whole in its first slab
-              c.first = c.last = start;
-              parts.push_back(ref.box);
-              box_add(slab_box[start], ref.box);
-            } else {
-              parts.insert(parts.end(), tmp + (c.first - start), tmp + (c.last - start + 1));
-            }
-            ++enter[c.first];
-            ++leave[c.last];
-            cuts.push_back(std::move(c));
-          }
           // sweep the 63 inner planes: cost 0.5 + A(L)/A(P) N_L + A(R)/A(P) N_R
           double suf_area[kSpatialBins + 1];
           uint32_t suf_n[kSpatialBins + 1];
@@ -483,35 +643,56 @@ struct SBuild {
           if (plane > 0 && sc < best_cost) {
             // references wholly on one side keep their id and box; a
             // straddling one becomes two, each clipped to its side
-            std::vector<SRef> halves;
-            for (const Cut& c : cuts) {
-              if (c.last < plane) {
-                sp_left.push_back(c.ref);
-              } else if (c.first >= plane) {
-                sp_right.push_back(c.ref);
-              } else {
-                Box lb, rb;
-                box_empty(lb);
-                box_empty(rb);
-                for (int i = c.first; i <= c.last; ++i) box_add(i < plane ? lb : rb, parts[c.part + i - c.first]);
-                const uint32_t t = refs->at(c.ref).tri;
-                halves.push_back(SRef{t, lb});
-                halves.push_back(SRef{t, rb});
+            struct Side {
+              std::vector<uint32_t> left, right;
+              std::vector<SRef> halves;
+            };
+            std::vector<Side> side(parts);
+            run_parts(parts, [&](int c) {
+              Side& s = side[c];
+              const Bins& bn = bins[c];
+              for (const Cut& cu : bn.cuts) {
+                if (cu.last < plane) {
+                  s.left.push_back(cu.ref);
+                } else if (cu.first >= plane) {
+                  s.right.push_back(cu.ref);
+                } else {
+                  Box lb, rb;
+                  box_empty(lb);
+                  box_empty(rb);
+                  for (int i = cu.first; i <= cu.last; ++i) box_add(i < plane ? lb : rb, bn.parts[cu.part + i - cu.first]);
+                  const uint32_t t = refs->at(cu.ref).tri;
+                  s.halves.push_back(SRef{t, lb});
+                  s.halves.push_back(SRef{t, rb});
+                }
               }
+            });
+            std::vector<Bins>().swap(bins);
+            size_t nh = 0, nsl = 0, nsr = 0;
+            for (const Side& s : side) {
+              nh += s.halves.size();
+              nsl += s.left.size();
+              nsr += s.right.size();
             }
-            const uint32_t first = halves.empty() ? 0u : refs->try_alloc((uint32_t)halves.size());
+            sp_left.reserve(nsl);
+            sp_right.reserve(nsr);
+            for (const Side& s : side) {
+              sp_left.insert(sp_left.end(), s.left.begin(), s.left.end());
+              sp_right.insert(sp_right.end(), s.right.begin(), s.right.end());
+            }
+            const uint32_t first = nh == 0 ? 0u : refs->try_alloc((uint32_t)nh);
             if (first != UINT32_MAX) {
-              for (size_t i = 0; i < halves.size(); ++i) {
-                refs->at(first + (uint32_t)i) = halves[i];
-                (i % 2 ? new_right : new_left).push_back(first + (uint32_t)i);
-              }
+              uint32_t id = first;
+              for (const Side& s : side)
+                for (size_t i = 0; i < s.halves.size(); ++i, ++id) {
+                  refs->at(id) = s.halves[i];
+                  (i % 2 ? new_right : new_left).push_back(id);
+                }
               spatial_split = true;
               split_cost = sc;
             } else {
-              sp_left.clear();
-              sp_right.clear();
-              new_left.clear();
-              new_right.clear();
+              std::vector<uint32_t>().swap(sp_left);
+              std::vector<uint32_t>().swap(sp_right);
             }
           }
         }
@@ -529,49 +710,66 @@ struct SBuild {
       nd.begin = (uint32_t)leaf_out.size();
       nd.count = n;
       for (uint32_t r : ax[best_axis]) leaf_out.push_back(refs->at(r).tri);
-      return;
+      return 1;
     }
     pool->at(ni).split_axis = (uint8_t)best_axis;
+    // sets mark[id] = v for every id of list s, on the node's threads
+    auto set_marks = [&](const std::vector<uint32_t>& s, size_t count, uint8_t v) {
+      run_parts(parts, [&](int c) {
+        for (size_t i = part_begin(count, parts, c), e = part_begin(count, parts, c + 1); i < e; ++i) mark[s[i]] = v;
+      });
+    };
     std::vector<uint32_t> lax[3], rax[3];
     if (spatial_split) {
       // whole references keep their centroid order in every axis list (a
       // subsequence of a sorted list is sorted); the new halves are sorted
-      // by centroid and merged in (stable: whole references first on ties)
+      // by centroid and merged in (stable: whole references first on ties).
+      // The halves' order for axis k is a stable re-sort of their order for
+      // axis k-1.
       spatial.fetch_add(1);
-      for (uint32_t r : sp_left) mark_at(r) = 1;
-      for (uint32_t r : sp_right) mark_at(r) = 2;
-      for (int k = 0; k < 3; ++k) {
-        auto by_centroid = [&](uint32_t x, uint32_t y) {
+      set_marks(sp_left, sp_left.size(), 1);
+      set_marks(sp_right, sp_right.size(), 2);
+      auto by_centroid = [&](int k) {
+        return [this, k](uint32_t x, uint32_t y) {
           const Box& bx = refs->at(x).box;
           const Box& by = refs->at(y).box;
           return (bx.lo[k] + bx.hi[k]) * 0.5 < (by.lo[k] + by.hi[k]) * 0.5;
         };
+      };
+      std::vector<uint32_t> nls[3], nrs[3];
+      for (int k = 0; k < 3; ++k) {
+        std::stable_sort(new_left.begin(), new_left.end(), by_centroid(k));
+        std::stable_sort(new_right.begin(), new_right.end(), by_centroid(k));
+        nls[k] = new_left;
+        nrs[k] = new_right;
+      }
+      each_axis([&](int k) {
         std::vector<uint32_t> kl, kr;
         kl.reserve(sp_left.size());
         kr.reserve(sp_right.size());
         for (uint32_t r : ax[k]) {
-          const uint8_t m = mark_at(r);
+          const uint8_t m = mark[r];
           if (m == 1) kl.push_back(r);
           else if (m == 2) kr.push_back(r);
         }
         std::vector<uint32_t>().swap(ax[k]);
-        std::stable_sort(new_left.begin(), new_left.end(), by_centroid);
-        std::stable_sort(new_right.begin(), new_right.end(), by_centroid);
-        lax[k].reserve(kl.size() + new_left.size());
-        rax[k].reserve(kr.size() + new_right.size());
-        std::merge(kl.begin(), kl.end(), new_left.begin(), new_left.end(), std::back_inserter(lax[k]), by_centroid);
-        std::merge(kr.begin(), kr.end(), new_right.begin(), new_right.end(), std::back_inserter(rax[k]), by_centroid);
-      }
-      for (uint32_t r : sp_left) mark_at(r) = 0;
-      for (uint32_t r : sp_right) mark_at(r) = 0;
+        lax[k].reserve(kl.size() + nls[k].size());
+        rax[k].reserve(kr.size() + nrs[k].size());
+        std::merge(kl.begin(), kl.end(), nls[k].begin(), nls[k].end(), std::back_inserter(lax[k]), by_centroid(k));
+        std::merge(kr.begin(), kr.end(), nrs[k].begin(), nrs[k].end(), std::back_inserter(rax[k]), by_centroid(k));
+      });
+      set_marks(sp_left, sp_left.size(), 0);
+      set_marks(sp_right, sp_right.size(), 0);
     } else {
-      const std::vector<uint32_t>& split = ax[best_axis];
-      for (uint32_t i = 0; i <= split_index; ++i) mark_at(split[i]) = 1;
-      for (int k = 0; k < 3; ++k) {
-        for (uint32_t r : ax[k]) (mark_at(r) ? lax[k] : rax[k]).push_back(r);
+      const size_t nl = (size_t)split_index + 1;
+      set_marks(ax[best_axis], nl, 1);
+      each_axis([&](int k) {
+        lax[k].reserve(nl);
+        rax[k].reserve(n - nl);
+        for (uint32_t r : ax[k]) (mark[r] ? lax[k] : rax[k]).push_back(r);
         std::vector<uint32_t>().swap(ax[k]);
-      }
-      for (uint32_t r : lax[0]) mark_at(r) = 0;
+      });
+      set_marks(lax[0], nl, 0);
     }
     uint32_t l = pool->alloc(), r = pool->alloc();
     {
@@ -579,17 +777,12 @@ struct SBuild {
       p.left = (int32_t)l;
       p.right = (int32_t)r;
     }
-    if (par > 0 && n > 32768) {
-      auto fut = std::async(std::launch::async, [&]() { node(l, lax, depth + 1, par - 1); });
-      node(r, rax, depth + 1, par - 1);
-      fut.get();
-    } else {
-      node(l, lax, depth + 1, 0);
-      node(r, rax, depth + 1, 0);
-    }
+    const int cw = std::max(1, width / 2);
+    uint32_t sl = 0, sr = 0;
+    tasks.fork(n, [&]() { sl = node(l, lax, depth + 1, cw); }, [&]() { sr = node(r, rax, depth + 1, cw); });
+    pool->at(ni).begin = 1 + sl + sr;
+    return 1 + sl + sr;
   }
-
-  uint8_t& mark_at(uint32_t id) { return mark[id]; }
 };
 
 // JS ToInt16 (ECMAScript ToInt16: truncate, wrap modulo 2^16)
@@ -756,7 +949,7 @@ void pk_make_wide4(const std::vector<pk::WNode>& n8, std::vector<pk::WNode4>& n4
   }
 }
 
-void pk_bvh_make_wide(pk_bvh* bvh);
+void pk_bvh_make_wide(pk_bvh* bvh, int threads);
 
 extern "C" {
 
@@ -776,11 +969,17 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
     return PK_E_LIMIT;
   }
   for (size_t i = 0; i < (size_t)nt; ++i)
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
       if (triangles[i * 4 + c] < 0 || (uint32_t)triangles[i * 4 + c] >= nv) {
         pk_set_error("pk_bvh_build: triangle " + std::to_string(i) + " references a vertex out of range");
         return PK_E_INVALID;
       }
+      const double* p = positions + (size_t)triangles[i * 4 + c] * 3;
+      if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) {
+        pk_set_error("pk_bvh_build: triangle " + std::to_string(i) + " has a vertex with a non-finite coordinate");
+        return PK_E_INVALID;
+      }
+    }
   if (num_threads <= 0) num_threads = (int)std::max(1u, std::thread::hardware_concurrency());
   auto* bvh = new pk_bvh();
   bvh->nv = nv;
@@ -789,34 +988,47 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
   // reference boxes and centroids (primitives.js:17-33)
   std::vector<Box> ref_box(nt);
   std::vector<double> cen((size_t)nt * 3);
-  for (uint32_t t = 0; t < nt; ++t) {
-    Box b;
-    box_empty(b);
-    for (int v = 0; v < 3; ++v) {
-      const double* p = positions + (size_t)triangles[(size_t)t * 4 + v] * 3;
-      for (int k = 0; k < 3; ++k) {
-        b.lo[k] = std::min(p[k], b.lo[k]);
-        b.hi[k] = std::max(p[k], b.hi[k]);
+  const int tparts = parts_for(num_threads, nt);
+  run_parts(tparts, [&](int c) {
+    for (size_t t = part_begin(nt, tparts, c), e = part_begin(nt, tparts, c + 1); t < e; ++t) {
+      Box b;
+      box_empty(b);
+      for (int v = 0; v < 3; ++v) {
+        const double* p = positions + (size_t)triangles[t * 4 + v] * 3;
+        for (int k = 0; k < 3; ++k) {
+          b.lo[k] = std::min(p[k], b.lo[k]);
+          b.hi[k] = std::max(p[k], b.hi[k]);
+        }
       }
+      ref_box[t] = b;
+      for (int k = 0; k < 3; ++k) cen[t * 3 + k] = (b.lo[k] + b.hi[k]) * 0.5;
     }
-    ref_box[t] = b;
-    for (int k = 0; k < 3; ++k) cen[(size_t)t * 3 + k] = (b.lo[k] + b.hi[k]) * 0.5;
-  }
+  });
   std::vector<uint32_t> arr((size_t)nt * 3);
   uint32_t* ax[3] = {arr.data(), arr.data() + nt, arr.data() + 2 * (size_t)nt};
-  {
-    std::vector<std::thread> ts;
-    for (int a = 0; a < 3; ++a)
-      ts.emplace_back([&, a]() {
-        std::iota(ax[a], ax[a] + nt, 0u);
-        std::stable_sort(ax[a], ax[a] + nt,
-                         [&](uint32_t x, uint32_t y) { return cen[(size_t)x * 3 + a] < cen[(size_t)y * 3 + a]; });
+  // per axis: triangles by centroid, ties by index (a stable sort of
+  // 0..nt-1), as (centroid, index) pairs sorted in parts on threads of their
+  // own and merged pairwise (stable: the earlier part first on ties)
+  run_parts(3, [&](int a) {
+    typedef std::pair<double, uint32_t> Key;
+    std::vector<Key> kv(nt);
+    for (uint32_t t = 0; t < nt; ++t) kv[t] = Key(cen[(size_t)t * 3 + a], t);
+    auto less = [](const Key& x, const Key& y) { return x.first < y.first; };
+    const int parts = parts_for(std::max(1, num_threads / 3), nt);
+    run_parts(parts, [&](int c) {
+      std::stable_sort(kv.begin() + part_begin(nt, parts, c), kv.begin() + part_begin(nt, parts, c + 1), less);
+    });
+    for (int w = 1; w < parts; w *= 2) {
+      const int merges = (parts - w + 2 * w - 1) / (2 * w);
+      run_parts(merges, [&](int m) {
+        const int c = m * 2 * w;
+        std::inplace_merge(kv.begin() + part_begin(nt, parts, c), kv.begin() + part_begin(nt, parts, c + w),
+                           kv.begin() + part_begin(nt, parts, std::min(c + 2 * w, parts)), less);
       });
-    for (auto& t : ts) t.join();
-  }
+    }
+    for (uint32_t i = 0; i < nt; ++i) ax[a][i] = kv[i].second;
+  });
   std::vector<double>().swap(cen);
-  int par = 0;
-  for (int t = num_threads; t > 1; t >>= 1) ++par;
   bvh->spatial_splits = (flags & PK_BVH_SPATIAL_SPLITS) != 0;
   bvh->sah_collapse = (flags & PK_BVH_SAH_COLLAPSE) != 0;
   // SBVH references: the triangles, then fragments (each spatial split
@@ -832,12 +1044,17 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
     b.leaf_out = bvh->leaf_tris.data();
     b.in_left.assign(nt, 0);
     b.pool = pool.get();
-    b.node(root, 0, nt, 1, par + 1);
+    b.tasks.threads = num_threads;
+    b.node(root, 0, nt, 1, num_threads);
     bvh->degenerate = b.degenerate.load();
     bvh->binary_depth = b.max_depth.load();
   } else {
     Pool<SRef> refs(ref_cap);
-    for (uint32_t t = 0; t < nt; ++t) refs.at(refs.alloc()) = SRef{t, ref_box[t]};
+    refs.try_alloc(nt);  // ids 0..nt-1: the triangles
+    run_parts(tparts, [&](int c) {
+      for (size_t t = part_begin(nt, tparts, c), e = part_begin(nt, tparts, c + 1); t < e; ++t)
+        refs.at((uint32_t)t) = SRef{(uint32_t)t, ref_box[t]};
+    });
     SBuild b;
     b.pos = positions;
     b.tris = triangles;
@@ -851,17 +1068,53 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
     std::vector<uint32_t> lists[3];
     for (int a = 0; a < 3; ++a) lists[a].assign(ax[a], ax[a] + nt);
     std::vector<uint32_t>().swap(arr);
-    b.node(root, lists, 1, par + 1);
+    b.tasks.threads = num_threads;
+    b.node(root, lists, 1, num_threads);
     bvh->degenerate = b.degenerate.load();
     bvh->binary_depth = b.max_depth.load();
     bvh->leaf_tris.swap(b.leaf_out);
     bvh->num_spatial_splits = b.spatial.load();
   }
   NodePool& pool_ref = *pool;
-  // compact: renumber nodes in allocation order (root is 0)
+  // compact: nodes renumbered depth-first (root 0, a node's left subtree
+  // before its right one), so that the passes over the tree walk memory in
+  // order whichever thread built which subtree.  Subtree sizes give every
+  // node's number up front: subtrees are copied on threads.
   const uint32_t nn = pool_ref.size();
   bvh->nodes.resize(nn);
-  for (uint32_t i = 0; i < nn; ++i) bvh->nodes[i] = pool_ref.at(i);
+  {
+    auto size_of = [&](uint32_t id) { return pool_ref.at(id).leaf ? 1u : pool_ref.at(id).begin; };
+    // copies pool subtree `id` to numbers p.. ; stops at subtrees of at most
+    // `grain` nodes when `rest` is given, listing them there
+    auto copy = [&](uint32_t id, uint32_t p, uint32_t grain, std::vector<std::pair<uint32_t, uint32_t>>* rest) {
+      std::vector<std::pair<uint32_t, uint32_t>> st{{id, p}};
+      while (!st.empty()) {
+        const auto [x, q] = st.back();
+        st.pop_back();
+        if (rest && size_of(x) <= grain) {
+          rest->emplace_back(x, q);
+          continue;
+        }
+        BNode& d = bvh->nodes[q];
+        d = pool_ref.at(x);
+        if (!d.leaf) {
+          const uint32_t l = (uint32_t)d.left, r = (uint32_t)d.right;
+          d.begin = 0;
+          d.left = (int32_t)(q + 1);
+          d.right = (int32_t)(q + 1 + size_of(l));
+          st.emplace_back(r, (uint32_t)d.right);
+          st.emplace_back(l, (uint32_t)d.left);
+        }
+      }
+    };
+    std::vector<std::pair<uint32_t, uint32_t>> jobs;
+    const int cparts = parts_for(num_threads, nn);
+    copy(0, 0, cparts > 1 ? nn / (uint32_t)(8 * cparts) : nn, &jobs);
+    run_parts(cparts, [&](int c) {
+      for (size_t j = part_begin(jobs.size(), cparts, c), e = part_begin(jobs.size(), cparts, c + 1); j < e; ++j)
+        copy(jobs[j].first, jobs[j].second, 0, nullptr);
+    });
+  }
   pool.reset();
   int largest = 0;
   for (auto& n : bvh->nodes)
@@ -878,7 +1131,7 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
   bvh->verts.resize((size_t)nv * 3);
   for (size_t i = 0; i < (size_t)nv * 3; ++i)
     bvh->verts[i] = (float)((positions[i] - bvh->norm_center[i % 3]) * bvh->norm_scale);
-  pk_bvh_make_wide(bvh);
+  pk_bvh_make_wide(bvh, num_threads);
   *out = bvh;
   return PK_OK;
 }
@@ -1017,7 +1270,7 @@ int pk_bvh_export_wide(const pk_bvh* bvh, void* nodes, void* tris, int32_t* tri_
 // breadth-first order so the top levels share lines; leaf triangles are laid
 // out in the order the collapse reaches them.
 // ---------------------------------------------------------------------------
-void pk_bvh_make_wide(pk_bvh* bvh) {
+void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
   std::vector<BNode>& nodes = bvh->nodes;
   // split oversized leaves into virtual halves for the collapse only
   const size_t n0 = nodes.size();
@@ -1111,8 +1364,10 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
   bvh->wnodes.clear();
   bvh->wtris.clear();
   bvh->wtri_index.clear();
-  bvh->wtris.reserve(bvh->nt);
-  bvh->wtri_index.reserve((size_t)bvh->nt * 4);
+  // the walk below numbers the triangle records; (binary leaf, its first
+  // record) pairs are filled in afterwards, on threads
+  std::vector<std::pair<int32_t, uint32_t>> fills;
+  uint32_t num_records = 0;
   struct QItem {
     int32_t bnode;
     uint32_t wnode;
@@ -1123,30 +1378,9 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
   q.push_back({0, 0, 1});
   int wdepth = 1;
   auto emit_leaf = [&](int32_t li, uint32_t* first) {
-    const BNode& n = nodes[li];
-    *first = (uint32_t)bvh->wtris.size();
-    for (uint32_t t = 0; t < n.count; ++t) {
-      uint32_t id = bvh->leaf_tris[n.begin + t];
-      const int32_t* tr = &bvh->tris[(size_t)id * 4];
-      const float* p0 = &V[(size_t)tr[0] * 3];
-      const float* p1 = &V[(size_t)tr[1] * 3];
-      const float* p2 = &V[(size_t)tr[2] * 3];
-      pk::TriRecord rec;
-      rec.v0x = p0[0];
-      rec.v0y = p0[1];
-      rec.v0z = p0[2];
-      rec.id = id;
-      rec.e1x = p1[0] - p0[0];
-      rec.e1y = p1[1] - p0[1];
-      rec.e1z = p1[2] - p0[2];
-      rec.pad0 = 0;
-      rec.e2x = p2[0] - p0[0];
-      rec.e2y = p2[1] - p0[1];
-      rec.e2z = p2[2] - p0[2];
-      rec.pad1 = 0;
-      bvh->wtris.push_back(rec);
-      for (int c = 0; c < 4; ++c) bvh->wtri_index.push_back(tr[c]);
-    }
+    *first = num_records;
+    fills.emplace_back(li, num_records);
+    num_records += nodes[li].count;
   };
   // SAH-optimal collapse (Ylitie et al. 2017): for every binary node x and
   // slot budget k, the least SAH cost of representing x's subtree with at most
@@ -1242,7 +1476,7 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
   };
   // a leaf child holding every triangle of binary subtree x, in order
   auto emit_subtree = [&](int32_t x, uint32_t* first, uint32_t* count) {
-    *first = (uint32_t)bvh->wtris.size();
+    *first = num_records;
     std::vector<int32_t> st{x};
     while (!st.empty()) {
       const int32_t y = st.back();
@@ -1255,7 +1489,7 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
         emit_leaf(y, &f);
       }
     }
-    *count = (uint32_t)bvh->wtris.size() - *first;
+    *count = num_records - *first;
   };
   std::vector<std::pair<int32_t, uint8_t>> chosen;
   while (!q.empty()) {
@@ -1342,4 +1576,36 @@ void pk_bvh_make_wide(pk_bvh* bvh) {
     bvh->wnodes[it.wnode] = w;
   }
   bvh->wide_depth = wdepth;
+  // triangle records: vertex 0 and the two edges of the normalized f32
+  // triangle, and its original index
+  bvh->wtris.resize(num_records);
+  bvh->wtri_index.resize((size_t)num_records * 4);
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int parts = parts_for(threads, num_records) > 1 ? std::min<int>(threads, (int)fills.size()) : 1;
+  run_parts(parts, [&](int c) {
+    for (size_t f = part_begin(fills.size(), parts, c), fe = part_begin(fills.size(), parts, c + 1); f < fe; ++f) {
+      const BNode& n = nodes[fills[f].first];
+      for (uint32_t t = 0; t < n.count; ++t) {
+        const uint32_t id = bvh->leaf_tris[n.begin + t];
+        const int32_t* tr = &bvh->tris[(size_t)id * 4];
+        const float* p0 = &V[(size_t)tr[0] * 3];
+        const float* p1 = &V[(size_t)tr[1] * 3];
+        const float* p2 = &V[(size_t)tr[2] * 3];
+        pk::TriRecord& rec = bvh->wtris[fills[f].second + t];
+        rec.v0x = p0[0];
+        rec.v0y = p0[1];
+        rec.v0z = p0[2];
+        rec.id = id;
+        rec.e1x = p1[0] - p0[0];
+        rec.e1y = p1[1] - p0[1];
+        rec.e1z = p1[2] - p0[2];
+        rec.pad0 = 0;
+        rec.e2x = p2[0] - p0[0];
+        rec.e2y = p2[1] - p0[1];
+        rec.e2z = p2[2] - p0[2];
+        rec.pad1 = 0;
+        for (int k = 0; k < 4; ++k) bvh->wtri_index[((size_t)fills[f].second + t) * 4 + k] = tr[k];
+      }
+    }
+  });
 }

@@ -363,7 +363,7 @@ bool wide_layout_ok(const pk_bvh& b, uint32_t top_nodes) {
 
 }  // namespace
 
-void pk_bvh_make_wide(pk_bvh* bvh);
+void pk_bvh_make_wide(pk_bvh* bvh, int threads);
 
 extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
                                       const pk_instance_desc* instances, uint32_t num_instances, int num_threads,
@@ -539,7 +539,7 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
     for (size_t i = 0; i < (size_t)w.nv * 3; ++i) w.verts[i] = (float)((scene->positions[i] - c[i % 3]) * s);
     w.inst_boxes = nbox;
     insert_placements(w.nodes, wbox);
-    pk_bvh_make_wide(&w);
+    pk_bvh_make_wide(&w, num_threads);
     bvh->wnodes.swap(w.wnodes);
     bvh->wtris.swap(w.wtris);
     bvh->wtri_index.swap(w.wtri_index);

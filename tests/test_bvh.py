@@ -251,6 +251,44 @@ def test_bvh_build_rejects_bad_input():
     assert e.value.code == -1 and "out of range" in str(e.value)
     with pytest.raises(pike_amd.PikeError):
         pike_amd.BVH(pos, np.zeros((0, 4), np.int32))
+    for bad in (np.nan, np.inf, -np.inf):
+        p = np.zeros((4, 3))
+        p[1, 0] = 1.0
+        p[2, 1] = 1.0
+        p[3, 2] = bad               # vertex 3: referenced by the second triangle only
+        t = np.array([[0, 1, 2, 0], [0, 1, 3, 0]], np.int32)
+        with pytest.raises(pike_amd.PikeError) as e:
+            pike_amd.BVH(p, t)
+        assert e.value.code == -1 and "non-finite" in str(e.value) and "triangle 1" in str(e.value)
+        pike_amd.BVH(p, t[:1])      # an unreferenced non-finite vertex is not an error
+
+
+def _exports(b):
+    return [a.view(np.uint8) for a in b.export_wide() + b.export_reference()]
+
+
+@pytest.mark.parametrize("spatial", [False, True])
+def test_build_does_not_depend_on_thread_count(spatial):
+    """Node passes split over threads (unions and sweeps in parts) and
+    subtrees handed to threads as they free up give the serial tree: every
+    byte of both layouts equal for 1, 2, 3 and 8 threads.  Sizes pass the
+    threshold of split passes (65,536 references) at the root; the terrain
+    has many equal centroid coordinates (ties), the slivers spatial splits."""
+    pos, _, tri = scenes.terrain(200, seed=5)       # 80,000 triangles
+    t = np.zeros((len(tri), 4), np.int32)
+    t[:, :3] = tri
+    cases = [(pos, t, (1, 2, 3, 8))] + ([_slivers(70000, 4) + ((1, 8),)] if spatial else [])
+    for p, tt, counts in cases:
+        ref = None
+        for threads in counts:
+            b = pike_amd.BVH(p, tt, threads=threads, spatial_splits=spatial, sah_collapse=True)
+            ex = _exports(b)
+            if ref is None:
+                ref = ex
+                assert b.info()["binary_nodes"] > 2 * 32768 // 8
+                continue
+            for x, y in zip(ref, ex):
+                np.testing.assert_array_equal(x, y)
 
 
 def _slivers(n, seed):
