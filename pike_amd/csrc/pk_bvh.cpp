@@ -1274,8 +1274,10 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
   std::vector<BNode>& nodes = bvh->nodes;
   // split oversized leaves into virtual halves for the collapse only
   const size_t n0 = nodes.size();
+  std::vector<size_t> work;
   for (size_t i = 0; i < n0; ++i) {
-    std::vector<size_t> work{i};
+    if (!nodes[i].leaf || nodes[i].count <= (uint32_t)pk::kLeafMax) continue;
+    work.assign(1, i);
     while (!work.empty()) {
       size_t j = work.back();
       work.pop_back();
@@ -1299,22 +1301,48 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
   // f32 boxes over the normalized vertices, bottom-up
   std::vector<FBox> fb(nodes.size());
   const std::vector<float>& V = bvh->verts;
-  std::vector<int32_t> order;
-  order.reserve(nodes.size());
+  // The bottom-up passes (these boxes, the collapse's dynamic program) visit
+  // the subtrees below a cut on threads, then the nodes above the cut.
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int tparts = parts_for(threads, nodes.size());
+  std::vector<int32_t> top, roots;  // above the cut (parents first); subtrees below it
   {
-    std::vector<int32_t> st{0};
+    int cut = 0;
+    while (tparts > 1 && (1 << cut) < 8 * tparts) ++cut;
+    std::vector<std::pair<int32_t, int>> st{{0, 0}};
     while (!st.empty()) {
-      int32_t i = st.back();
+      const auto [x, d] = st.back();
       st.pop_back();
-      order.push_back(i);
-      if (is_internal(i)) {
-        st.push_back(kid_l(i));
-        st.push_back(kid_r(i));
+      if (d >= cut || !is_internal(x)) {
+        roots.push_back(x);
+        continue;
       }
+      top.push_back(x);
+      st.push_back({kid_l(x), d + 1});
+      st.push_back({kid_r(x), d + 1});
     }
   }
-  for (size_t oi = order.size(); oi-- > 0;) {
-    int32_t i = order[oi];
+  auto for_bottom_up = [&](const auto& visit) {  // visit(x) after visit(children of x)
+    run_parts(tparts, [&](int c) {
+      std::vector<int32_t> order, st;
+      for (size_t j = part_begin(roots.size(), tparts, c), e = part_begin(roots.size(), tparts, c + 1); j < e; ++j) {
+        order.clear();
+        st.assign(1, roots[j]);
+        while (!st.empty()) {
+          const int32_t i = st.back();
+          st.pop_back();
+          order.push_back(i);
+          if (is_internal(i)) {
+            st.push_back(kid_l(i));
+            st.push_back(kid_r(i));
+          }
+        }
+        for (size_t oi = order.size(); oi-- > 0;) visit(order[oi]);
+      }
+    });
+    for (size_t t = top.size(); t-- > 0;) visit(top[t]);
+  };
+  for_bottom_up([&](int32_t i) {
     FBox& b = fb[i];
     for (int k = 0; k < 3; ++k) {
       b.lo[k] = INFINITY;
@@ -1354,7 +1382,7 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
           }
       }
     }
-  }
+  });
   auto area = [&](int32_t i) {
     const FBox& b = fb[i];
     double x = std::max((double)b.hi[0] - b.lo[0], 0.0), y = std::max((double)b.hi[1] - b.lo[1], 0.0),
@@ -1402,8 +1430,7 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
     slot_how.assign(nb * pk::kWide, kAsNode);
     split_at.assign(nb * pk::kWide, 1);
     tri_count.assign(nb, 0);
-    for (size_t oi = order.size(); oi-- > 0;) {  // children before parents
-      const int32_t x = order[oi];
+    for_bottom_up([&](int32_t x) {
       const float ax = (float)area(x);
       float* sc = &slot_cost[(size_t)x * pk::kWide];
       uint8_t* sh = &slot_how[(size_t)x * pk::kWide];
@@ -1415,7 +1442,7 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
           sc[k] = c;
           sh[k] = placement ? kAsPlacement : kAsLeaf;
         }
-        continue;
+        return;
       }
       const int32_t l = kid_l(x), r = kid_r(x);
       const uint32_t tl = tri_count[l], tr = tri_count[r];
@@ -1452,15 +1479,17 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
           sh[k - 1] = sh[0];
         }
       }
-    }
+    });
   }
   // the children of the wide node for internal binary node b under the
   // collapse's decisions: (binary node, how)
   std::vector<std::pair<int32_t, uint8_t>> pending;
+  std::vector<std::pair<int32_t, int>> expand_work;
   auto expand = [&](int32_t b, std::vector<std::pair<int32_t, uint8_t>>& out) {
     pending.clear();
     const int j = split_at[(size_t)b * pk::kWide + pk::kWide - 1];
-    std::vector<std::pair<int32_t, int>> work{{kid_r(b), pk::kWide - j}, {kid_l(b), j}};
+    std::vector<std::pair<int32_t, int>>& work = expand_work;
+    work.assign({{kid_r(b), pk::kWide - j}, {kid_l(b), j}});
     while (!work.empty()) {
       const auto [x, k] = work.back();
       work.pop_back();
@@ -1474,10 +1503,12 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
       }
     }
   };
+  std::vector<int32_t> subtree_stack;
   // a leaf child holding every triangle of binary subtree x, in order
   auto emit_subtree = [&](int32_t x, uint32_t* first, uint32_t* count) {
     *first = num_records;
-    std::vector<int32_t> st{x};
+    std::vector<int32_t>& st = subtree_stack;
+    st.assign(1, x);
     while (!st.empty()) {
       const int32_t y = st.back();
       st.pop_back();
@@ -1580,7 +1611,6 @@ void pk_bvh_make_wide(pk_bvh* bvh, int threads) {
   // triangle, and its original index
   bvh->wtris.resize(num_records);
   bvh->wtri_index.resize((size_t)num_records * 4);
-  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
   const int parts = parts_for(threads, num_records) > 1 ? std::min<int>(threads, (int)fills.size()) : 1;
   run_parts(parts, [&](int c) {
     for (size_t f = part_begin(fills.size(), parts, c), fe = part_begin(fills.size(), parts, c + 1); f < fe; ++f) {
