@@ -63,7 +63,7 @@ struct ViewDev {
 struct FrameDev {
   float4* qo[2];   // ray queues: origin.xyz, pixel id bits
   float4* qd[2];   //             direction.xyz
-  float4* thr;     // path throughput per path from bounce 1 on (camera rays carry 1)
+  float4* qt[2];   //             path throughput (bounce rays; camera rays carry 1 and have none)
   uint4* hits;     // {ray slot, leaf triangle, t bits, bary mask}; x = ~0: a hole
   float4* so;      // shadow queue: origin.xyz, pixel id bits
   float4* sd;      //               direction.xyz

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_raygen(FrameDev F) {
                pkm::mk(P.cby[0], P.cby[1], P.cby[2]), &o, &d);
   F.qo[0][idx] = make_float4(o.x, o.y, o.z, __uint_as_float(idx));
   F.qd[0][idx] = make_float4(d.x, d.y, d.z, 0.0f);
-  // (camera rays carry throughput 1: bounce 0 does not read F.thr)
+  // (camera rays carry throughput 1: bounce 0 does not read F.qt)
   F.color[(size_t)pixel * F.S + s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(64 * kWpb, trace_waves_per_simd(kLds)) k_trace
         // the wave would run the environment lookup for a lane or two)
         if (done && ray_missed(r)) {  // shade_miss.wgsl envColor x throughput
           const uint32_t pid = __float_as_uint(qo[slot].w);  // reread: no register held through the traversal
-          const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.thr[pid];
+          const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.qt[in][slot];
           add_color(F, ns, pid, pkm::mul(env_color(S, theta, r.d), pkm::mk(t4.x, t4.y, t4.z)));
         }
         // hit records go to this wave's reserved run of the hit queue, in
@@ -856,7 +856,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64, 7) k_shade(DevScene S, Frame
     // numbered pixel-major
     const uint32_t s = pid % ns;
     const uint32_t pixel = F.views ? pid / ns - F.views[view_bits].offset : pid / ns;
-    const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.thr[pid];  // camera rays: 1
+    const float4 t4 = bounce == 0 ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : F.qt[in][slot];  // camera rays: 1
     shade_hit<false>(S, P.env_theta, pkm::mk(o4.x, o4.y, o4.z), pkm::mk(d4.x, d4.y, d4.z), __uint_as_float(hr.z),
                      hr.y, hr.w, shade_seed((uint32_t)bounce, pixel, P.sample + s), pkm::mk(t4.x, t4.y, t4.z), o,
                      nullptr);
@@ -868,7 +868,7 @@ __global__ void __launch_bounds__(kShadeWaves * 64, 7) k_shade(DevScene S, Frame
   if (o.bounce) {
     F.qo[in ^ 1][bs] = make_float4(o.b_o.x, o.b_o.y, o.b_o.z, __uint_as_float(pid));
     F.qd[in ^ 1][bs] = make_float4(o.b_d.x, o.b_d.y, o.b_d.z, __uint_as_float(view_bits));
-    F.thr[pid] = make_float4(o.b_thr.x, o.b_thr.y, o.b_thr.z, 0.0f);
+    F.qt[in ^ 1][bs] = make_float4(o.b_thr.x, o.b_thr.y, o.b_thr.z, 0.0f);
   }
   if (o.shadow) {
     F.so[ss] = make_float4(o.s_o.x, o.s_o.y, o.s_o.z, __uint_as_float(pid));

@@ -763,8 +763,8 @@ static int alloc_paths_into(pk_frame* f, size_t paths, AllocList& list, pk::Fram
   for (int q = 0; q < 2; ++q) {
     if ((rc = dev_alloc(list, &F.qo[q], paths))) return fail(rc);
     if ((rc = dev_alloc(list, &F.qd[q], paths))) return fail(rc);
+    if ((rc = dev_alloc(list, &F.qt[q], paths))) return fail(rc);
   }
-  if ((rc = dev_alloc(list, &F.thr, paths))) return fail(rc);
   // the hit queue also holds the holes at the end of each trace_closest
   // wave's last run (fewer than kHitRun per wave of the largest grid)
   const size_t hit_cap = paths + (size_t)f->ctx->num_cus * kMaxWavesPerCu * pk::kHitRun;
