@@ -238,7 +238,10 @@ int pk_buffer_copy(pk_ctx* ctx, void* device_dst, const void* device_src, size_t
 /* ---- BVH (render/bvh.js) : host-side, no device needed ----
  * Builds the SAH binary BVH with bvh.js's decisions (full sweep, float64,
  * first minimum, leaf iff cost > N) and collapses it into the 128-byte wide
- * layout the kernels traverse. */
+ * layout the kernels traverse.  num_threads <= 0: every hardware thread; the
+ * result does not depend on the thread count.  PK_E_INVALID for an empty
+ * input, a vertex index out of range, or a referenced vertex with a
+ * non-finite coordinate. */
 int pk_bvh_build(const double* positions, uint32_t num_vertices, const int32_t* triangles,
                  uint32_t num_triangles, int num_threads, pk_bvh** out);
 /* The same with options.  PK_BVH_SPATIAL_SPLITS makes bvh.js's SBVH path
