@@ -135,6 +135,31 @@ def test_samples_per_pass_equals_one_sample_frames(ctx, samples, frames):
     np.testing.assert_array_equal(fr.read_image().view(np.uint32), acc_o.view(np.uint32))
 
 
+@pytest.mark.parametrize("W,H,samples", [(1, 1, 1), (1, 1, 5), (37, 23, 1), (37, 23, 3), (129, 3, 2)])
+def test_ragged_frames_match_fp32_oracle(ctx, W, H, samples):
+    """Frames whose pixel and path counts fill no launch block (one pixel; 37x23
+    and 129x3 against 256-thread raygen/accumulate blocks and 64-lane waves),
+    with one- and multi-sample passes: every pixel, the last included (the
+    reference's color buffer drops it, A.5; its accumulate bounds check is
+    off by one, A.6), equals the FP32 oracle bit for bit."""
+    env = scenes.lognormal_sun_env(64, 32, seed=3, sigma=1.0, sun_deg=6.0, sun_scale=30.0)
+    data = scenes.mixed_scene(5, 3000, env=env)
+    data.camera = scenes.camera((0.5, 0.5, 1.4), (0.5, 0.05, 0.45), 1.0)
+    bvh, scene, orc, cam = _setup(ctx, data)
+    depth, frames = 4, 5
+    fr = pike_amd.Frame(ctx, W, H)
+    fr.set_camera(cam)
+    fr.set_samples_per_pass(samples)
+    fr.render(scene, max_depth=depth, first_sample=0, num_frames=frames)
+    c = fr.counters()
+    assert c["frames"] == frames and c["stack_overflow"] == 0
+    img = fr.read_image()
+    assert img.shape == (H, W, 4)
+    acc_o, _, _ = orc.render(cam, W, H, depth, 0, frames, precision=32)
+    np.testing.assert_array_equal(img.view(np.uint32), acc_o.view(np.uint32))
+    assert np.all(np.isfinite(img)) and img[..., :3].max() > 0
+
+
 def _sun_scene(sampler):
     env = scenes.lognormal_sun_env(128, 64, seed=8, sigma=1.5, sun_deg=4.0, sun_scale=1e3)
     data = scenes.mixed_scene(11, 6000, env=env)
