@@ -159,6 +159,13 @@ def run_render(args, pike_amd, scenes):
     rays_total = cnt["rays_traced"] + cnt["shadow_rays"]
     log(f"render: {args.steps} passes of {S} samples per pixel ({samples} spp) in {dt:.3f}s = {spp_s:.2f} spp/s, "
         f"{rays_total / dt / 1e6:.0f} Mrays/s, overflow={cnt['stack_overflow']}")
+    # the first sample alone (a one-sample pass), for the CPU leg's
+    # full-size comparison with the oracle's first frame
+    chk = pike_amd.Frame(ctx, W, H)
+    chk.set_camera(cam)
+    chk.render(scene, max_depth=depth, first_sample=0, num_frames=1, use_graph=False)
+    first_frame = chk.read_image()
+    del chk
     # kernel-level profile: counted run for algorithmic bytes, uncounted for
     # time, over whole passes
     pf = args.profile_frames
@@ -170,7 +177,7 @@ def run_render(args, pike_amd, scenes):
                     kernel_ms=None, profile_frames=0, profile_counters=None,
                     roofline=dict(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
                                   traffic=None, bytes_per_launch=None, launch_ms=None),
-                    samples_per_step=S,
+                    samples_per_step=S, first_frame=first_frame,
                     triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler()), data, bvh, cam
     frame.reset()
     frame.count_nodes(True)
@@ -206,7 +213,7 @@ def run_render(args, pike_amd, scenes):
     result = dict(
         spp_s=spp_s, ms_per_step=dt / args.steps * 1e3, rays_per_s=rays_total / dt, counters=cnt,
         kernel_ms=ms, profile_frames=pf, profile_counters=cc, roofline=roof, samples_per_step=S,
-        triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler())
+        first_frame=first_frame, triangles=data.baked_triangles, autotune=tuned, scheduler=ctx.scheduler())
     return result, data, bvh, cam
 
 
@@ -511,7 +518,11 @@ def run_diff(args, pike_amd, scenes):
     return out
 
 
-def run_cpu_baseline(args, data, cam):
+def run_cpu_baseline(args, data, cam, first_frame=None):
+    """The oracle's FP32 restatement timed on whole frames of the render
+    workload; its first frame, computed to size the sample, is also compared
+    with the GPU's first sample at full size (returned second, or None)."""
+    import numpy as np
     from oracle.oracle import OracleScene
 
     # Every core this process may run on, up to the 16 host cores the GPU
@@ -524,8 +535,17 @@ def run_cpu_baseline(args, data, cam):
     log(f"oracle BVH ({time.time() - t0:.1f}s)")
     # one whole frame to size the sample, then whole frames up to ~cpu_seconds
     t0 = time.perf_counter()
-    orc.render(cam, args.width, args.height, args.depth, 0, 1, precision=32)
+    acc0, _, _ = orc.render(cam, args.width, args.height, args.depth, 0, 1, precision=32)
     t1 = time.perf_counter() - t0
+    parity = None
+    if first_frame is not None:
+        a, g = acc0[..., :3].astype(np.float64), first_frame[..., :3].astype(np.float64)
+        rel = np.abs(a - g) / np.maximum(np.maximum(np.abs(a), np.abs(g)), 1e-6)
+        bad = int(np.any(acc0.view(np.uint32) != first_frame.view(np.uint32), axis=-1).sum())
+        parity = dict(pixels=int(args.width * args.height), mismatched_pixels=bad, max_rel=float(rel.max()),
+                      compared=f"sample 0 at {args.width}x{args.height}, depth {args.depth}: every pixel's "
+                               "running mean (bit patterns) against the oracle's FP32 frame")
+        log(f"full-size parity: {bad} of {parity['pixels']} pixels differ from the oracle (max rel {rel.max():.2e})")
     frames = int(max(1, round(args.cpu_seconds / max(t1, 1e-3))))
     t0 = time.perf_counter()
     orc.render(cam, args.width, args.height, args.depth, 1, frames, precision=32)
@@ -536,7 +556,7 @@ def run_cpu_baseline(args, data, cam):
     return dict(value=v, unit="spp/s", cores=threads, kind="port", host_cpus=os.cpu_count(),
                 affinity_cpus=affinity, core_limit=16,
                 sample=f"oracle FP32 restatement (C++, std::thread), {frames} whole {args.width}x{args.height} "
-                       f"frames (1 sample per pixel each, depth {args.depth}) of the same scene and camera")
+                       f"frames (1 sample per pixel each, depth {args.depth}) of the same scene and camera"), parity
 
 
 def run_traversal(args, pike_amd, scenes):
@@ -693,12 +713,15 @@ def main():
         args.profile_frames = 0
 
     res, data, bvh, cam = run_render(args, pike_amd, scenes)
+    first_frame = res.pop("first_frame")
     extra = dict(mrays_per_s_render=res["rays_per_s"] / 1e6, msamples_per_s=res["spp_s"] * args.width * args.height / 1e6,
                  kernel_ms_per_profile=res["kernel_ms"], profile_frames=res["profile_frames"],
                  trace_closest_bytes_per_launch=res["roofline"]["bytes_per_launch"],
                  trace_closest_launch_ms=res["roofline"]["launch_ms"], counters=res["counters"],
                  autotune=res["autotune"])
-    cpu = None if args.no_cpu else run_cpu_baseline(args, data, cam)
+    cpu, parity = (None, None) if args.no_cpu else run_cpu_baseline(args, data, cam, first_frame)
+    if parity:
+        extra["full_size_parity"] = parity
     if not args.no_traversal:
         extra["traversal_10M_soup"] = run_traversal(args, pike_amd, scenes)
     if not args.no_instanced:
