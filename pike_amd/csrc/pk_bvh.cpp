@@ -28,7 +28,9 @@
 #include <future>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
+#include <system_error>
 #include <thread>
 
 #include "pk_internal.h"
@@ -59,7 +61,8 @@ inline double box_area(const Box& b) {
 }
 
 // fn(c) for c in [0, parts), part 0 on the calling thread, the others on
-// threads of their own; an exception from any part is rethrown after all end.
+// threads of their own (on the calling thread too when no thread can be
+// started); an exception from any part is rethrown after all end.
 template <class F> void run_parts(int parts, F&& fn) {
   if (parts <= 1) {
     fn(0);
@@ -75,7 +78,14 @@ template <class F> void run_parts(int parts, F&& fn) {
   };
   std::vector<std::thread> ts;
   ts.reserve(parts - 1);
-  for (int c = 1; c < parts; ++c) ts.emplace_back(guarded, c);
+  for (int c = 1; c < parts; ++c) {
+    try {
+      ts.emplace_back(guarded, c);
+    } catch (const std::system_error&) {
+      for (int d = c; d < parts; ++d) guarded(d);
+      break;
+    }
+  }
   guarded(0);
   for (auto& t : ts) t.join();
   for (auto& e : err)
@@ -107,10 +117,18 @@ struct Tasks {
       return;
     }
     working.fetch_add(1);
-    auto fut = std::async(std::launch::async, [&]() {
-      left();
+    std::future<void> fut;
+    try {
+      fut = std::async(std::launch::async, [&]() {
+        left();
+        working.fetch_sub(1);
+      });
+    } catch (const std::system_error&) {  // no thread to be had: both halves here
       working.fetch_sub(1);
-    });
+      left();
+      right();
+      return;
+    }
     right();
     working.fetch_sub(1);
     fut.get();
@@ -951,15 +969,8 @@ void pk_make_wide4(const std::vector<pk::WNode>& n8, std::vector<pk::WNode4>& n4
 
 void pk_bvh_make_wide(pk_bvh* bvh, int threads);
 
-extern "C" {
-
-int pk_bvh_build(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
-                 pk_bvh** out) {
-  return pk_bvh_build_ex(positions, nv, triangles, nt, num_threads, 0, out);
-}
-
-int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
-                    uint32_t flags, pk_bvh** out) {
+static int build_bvh(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
+                     uint32_t flags, pk_bvh** out) {
   if (!positions || !triangles || !out || nt == 0 || nv == 0) {
     pk_set_error("pk_bvh_build: empty or null input");
     return PK_E_INVALID;
@@ -982,6 +993,7 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
     }
   if (num_threads <= 0) num_threads = (int)std::max(1u, std::thread::hardware_concurrency());
   auto* bvh = new pk_bvh();
+  std::unique_ptr<pk_bvh> guard(bvh);
   bvh->nv = nv;
   bvh->nt = nt;
   bvh->tris.assign(triangles, triangles + (size_t)nt * 4);
@@ -1132,8 +1144,29 @@ int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangl
   for (size_t i = 0; i < (size_t)nv * 3; ++i)
     bvh->verts[i] = (float)((positions[i] - bvh->norm_center[i % 3]) * bvh->norm_scale);
   pk_bvh_make_wide(bvh, num_threads);
-  *out = bvh;
+  *out = guard.release();
   return PK_OK;
+}
+
+extern "C" {
+
+int pk_bvh_build(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
+                 pk_bvh** out) {
+  return pk_bvh_build_ex(positions, nv, triangles, nt, num_threads, 0, out);
+}
+
+// No exception leaves the C ABI: host memory or threads running out end the
+// build with PK_E_OOM (and free what it had built).
+int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
+                    uint32_t flags, pk_bvh** out) {
+  try {
+    return build_bvh(positions, nv, triangles, nt, num_threads, flags, out);
+  } catch (const std::bad_alloc&) {
+    pk_set_error("pk_bvh_build: out of host memory");
+  } catch (const std::exception& e) {
+    pk_set_error(std::string("pk_bvh_build: ") + e.what());
+  }
+  return PK_E_OOM;
 }
 
 int pk_bvh_destroy(pk_bvh* bvh) {

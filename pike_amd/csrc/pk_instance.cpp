@@ -21,7 +21,9 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <memory>
+#include <new>
 #include <numeric>
 #include <queue>
 #include <string>
@@ -365,9 +367,9 @@ bool wide_layout_ok(const pk_bvh& b, uint32_t top_nodes) {
 
 void pk_bvh_make_wide(pk_bvh* bvh, int threads);
 
-extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
-                                      const pk_instance_desc* instances, uint32_t num_instances, int num_threads,
-                                      uint32_t flags, pk_bvh** out) {
+static int build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
+                           const pk_instance_desc* instances, uint32_t num_instances, int num_threads, uint32_t flags,
+                           pk_bvh** out) {
   if (!scene || !out || (num_meshes && !meshes) || (num_instances && !instances)) {
     pk_set_error("pk_bvh_build_instanced: null argument");
     return PK_E_INVALID;
@@ -610,4 +612,19 @@ extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_
   bvh->wide_depth = top_wide_depth + blas_wide_depth;
   *out = guard.release();
   return PK_OK;
+}
+
+// No exception leaves the C ABI: host memory or threads running out end the
+// build with PK_E_OOM.
+extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
+                                      const pk_instance_desc* instances, uint32_t num_instances, int num_threads,
+                                      uint32_t flags, pk_bvh** out) {
+  try {
+    return build_instanced(scene, meshes, num_meshes, instances, num_instances, num_threads, flags, out);
+  } catch (const std::bad_alloc&) {
+    pk_set_error("pk_bvh_build_instanced: out of host memory");
+  } catch (const std::exception& e) {
+    pk_set_error(std::string("pk_bvh_build_instanced: ") + e.what());
+  }
+  return PK_E_OOM;
 }

@@ -263,6 +263,43 @@ def test_bvh_build_rejects_bad_input():
         pike_amd.BVH(p, t[:1])      # an unreferenced non-finite vertex is not an error
 
 
+def test_build_out_of_host_memory_fails_cleanly():
+    """A build that runs out of host memory returns PK_E_OOM with a message
+    instead of ending the process (no exception crosses the C ABI): a child
+    process limits its address space to 48 MB above what it holds, then
+    builds a ~1M-triangle SBVH, and afterwards a small scene again."""
+    import subprocess
+    import sys
+    code = r"""
+import resource, sys
+import numpy as np
+import pike_amd
+from pike_amd import scenes
+pos, _, tri = scenes.terrain(700, seed=1)
+t = np.zeros((len(tri), 4), np.int32)
+t[:, :3] = tri
+pos = np.ascontiguousarray(pos, np.float64)
+small_p, _, small_t = scenes.terrain(8, seed=1)
+st = np.zeros((len(small_t), 4), np.int32)
+st[:, :3] = small_t
+pike_amd.lib()
+vm = [int(l.split()[1]) * 1024 for l in open("/proc/self/status") if l.startswith("VmSize:")][0]
+resource.setrlimit(resource.RLIMIT_AS, (vm + (48 << 20), resource.getrlimit(resource.RLIMIT_AS)[1]))
+try:
+    pike_amd.BVH(pos, t, threads=4, spatial_splits=True, sah_collapse=True)
+    print("built")
+except pike_amd.PikeError as e:
+    print("code", e.code, str(e))
+b = pike_amd.BVH(small_p, st, threads=2)
+print("small", b.info()["leaf_triangles"])
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       cwd=str(__import__("pathlib").Path(__file__).resolve().parents[1]))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "code -3" in r.stdout and "pk_bvh_build" in r.stdout, r.stdout
+    assert "small 128" in r.stdout, r.stdout
+
+
 def _exports(b):
     return [a.view(np.uint8) for a in b.export_wide() + b.export_reference()]
 
