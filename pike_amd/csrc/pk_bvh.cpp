@@ -1155,143 +1155,147 @@ int pk_bvh_build(const double* positions, uint32_t nv, const int32_t* triangles,
   return pk_bvh_build_ex(positions, nv, triangles, nt, num_threads, 0, out);
 }
 
-// No exception leaves the C ABI: host memory or threads running out end the
-// build with PK_E_OOM (and free what it had built).
+// (a build that runs out of host memory frees what it had built)
 int pk_bvh_build_ex(const double* positions, uint32_t nv, const int32_t* triangles, uint32_t nt, int num_threads,
                     uint32_t flags, pk_bvh** out) {
-  try {
-    return build_bvh(positions, nv, triangles, nt, num_threads, flags, out);
-  } catch (const std::bad_alloc&) {
-    pk_set_error("pk_bvh_build: out of host memory");
-  } catch (const std::exception& e) {
-    pk_set_error(std::string("pk_bvh_build: ") + e.what());
-  }
-  return PK_E_OOM;
+  return pk_guard("pk_bvh_build", [&]() { return build_bvh(positions, nv, triangles, nt, num_threads, flags, out); });
 }
 
 int pk_bvh_destroy(pk_bvh* bvh) {
-  delete bvh;
-  return PK_OK;
+  return pk_guard("pk_bvh_destroy", [&]() -> int {
+    delete bvh;
+    return PK_OK;
+  });
 }
 
 int pk_bvh_info(const pk_bvh* bvh, int64_t* info) {
-  if (!bvh || !info) {
-    pk_set_error("pk_bvh_info: null argument");
-    return PK_E_INVALID;
-  }
-  info[0] = bvh->num_binary;
-  info[1] = (int64_t)bvh->wnodes.size();
-  info[2] = (int64_t)bvh->wtris.size();
-  info[3] = bvh->binary_depth;
-  info[4] = bvh->wide_depth;
-  info[5] = bvh->largest_leaf;
-  return PK_OK;
+  return pk_guard("pk_bvh_info", [&]() -> int {
+    if (!bvh || !info) {
+      pk_set_error("pk_bvh_info: null argument");
+      return PK_E_INVALID;
+    }
+    info[0] = bvh->num_binary;
+    info[1] = (int64_t)bvh->wnodes.size();
+    info[2] = (int64_t)bvh->wtris.size();
+    info[3] = bvh->binary_depth;
+    info[4] = bvh->wide_depth;
+    info[5] = bvh->largest_leaf;
+    return PK_OK;
+  });
 }
 
 int pk_bvh_normalizer(const pk_bvh* bvh, double* norm4) {
-  if (!bvh || !norm4) {
-    pk_set_error("pk_bvh_normalizer: null argument");
-    return PK_E_INVALID;
-  }
-  for (int k = 0; k < 3; ++k) norm4[k] = bvh->norm_center[k];
-  norm4[3] = bvh->norm_scale;
-  return PK_OK;
+  return pk_guard("pk_bvh_normalizer", [&]() -> int {
+    if (!bvh || !norm4) {
+      pk_set_error("pk_bvh_normalizer: null argument");
+      return PK_E_INVALID;
+    }
+    for (int k = 0; k < 3; ++k) norm4[k] = bvh->norm_center[k];
+    norm4[3] = bvh->norm_scale;
+    return PK_OK;
+  });
 }
 
 int pk_bvh_export_reference(const pk_bvh* bvh, int32_t* nodes5, int32_t* tris4, float* verts4, double* norm4) {
-  if (!bvh) {
-    pk_set_error("pk_bvh_export_reference: null bvh");
-    return PK_E_INVALID;
-  }
-  if (bvh->instanced) {
-    pk_set_error("pk_bvh_export_reference: an instanced BVH has no reference layout");
-    return PK_E_INVALID;
-  }
-  // BFS (bvh.js:46-74): siblings adjacent, left child of the k-th interior
-  // node at 2k+1.
-  std::deque<int32_t> q;
-  q.push_back(0);
-  int64_t out = 0, interior = 0, tri_index = 0;
-  while (!q.empty()) {
-    int32_t ni = q.front();
-    q.pop_front();
-    const BNode& n = bvh->nodes[ni];
-    double lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = (n.lo[k] - bvh->norm_center[k]) * bvh->norm_scale;
-      hi[k] = (n.hi[k] - bvh->norm_center[k]) * bvh->norm_scale;
+  return pk_guard("pk_bvh_export_reference", [&]() -> int {
+    if (!bvh) {
+      pk_set_error("pk_bvh_export_reference: null bvh");
+      return PK_E_INVALID;
     }
-    int32_t* w = nodes5 ? nodes5 + out * 5 : nullptr;
-    if (n.leaf) {
-      if (w) {
-        w[0] = -1;
-        w[1] = (int32_t)(((uint32_t)n.count << 24) | (uint32_t)tri_index);
+    if (bvh->instanced) {
+      pk_set_error("pk_bvh_export_reference: an instanced BVH has no reference layout");
+      return PK_E_INVALID;
+    }
+    // BFS (bvh.js:46-74): siblings adjacent, left child of the k-th interior
+    // node at 2k+1.
+    std::deque<int32_t> q;
+    q.push_back(0);
+    int64_t out = 0, interior = 0, tri_index = 0;
+    while (!q.empty()) {
+      int32_t ni = q.front();
+      q.pop_front();
+      const BNode& n = bvh->nodes[ni];
+      double lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = (n.lo[k] - bvh->norm_center[k]) * bvh->norm_scale;
+        hi[k] = (n.hi[k] - bvh->norm_center[k]) * bvh->norm_scale;
       }
-      if (tris4)
-        for (uint32_t i = 0; i < n.count; ++i)
-          std::memcpy(tris4 + (tri_index + i) * 4, &bvh->tris[(size_t)bvh->leaf_tris[n.begin + i] * 4], 16);
-      tri_index += n.count;
-    } else {
-      if (w) {
-        w[0] = (int32_t)(2 * interior + 1);
-        w[1] = -1;
+      int32_t* w = nodes5 ? nodes5 + out * 5 : nullptr;
+      if (n.leaf) {
+        if (w) {
+          w[0] = -1;
+          w[1] = (int32_t)(((uint32_t)n.count << 24) | (uint32_t)tri_index);
+        }
+        if (tris4)
+          for (uint32_t i = 0; i < n.count; ++i)
+            std::memcpy(tris4 + (tri_index + i) * 4, &bvh->tris[(size_t)bvh->leaf_tris[n.begin + i] * 4], 16);
+        tri_index += n.count;
+      } else {
+        if (w) {
+          w[0] = (int32_t)(2 * interior + 1);
+          w[1] = -1;
+        }
+        ++interior;
+        q.push_back(n.left);
+        q.push_back(n.right);
       }
-      ++interior;
-      q.push_back(n.left);
-      q.push_back(n.right);
+      if (w) {
+        w[2] = (int32_t)quantize_pair(hi[0], lo[0]);
+        w[3] = (int32_t)quantize_pair(hi[1], lo[1]);
+        w[4] = (int32_t)quantize_pair(hi[2], lo[2]);
+      }
+      ++out;
     }
-    if (w) {
-      w[2] = (int32_t)quantize_pair(hi[0], lo[0]);
-      w[3] = (int32_t)quantize_pair(hi[1], lo[1]);
-      w[4] = (int32_t)quantize_pair(hi[2], lo[2]);
+    if (verts4)
+      for (size_t i = 0; i < bvh->nv; ++i) {
+        verts4[i * 4 + 0] = bvh->verts[i * 3 + 0];
+        verts4[i * 4 + 1] = bvh->verts[i * 3 + 1];
+        verts4[i * 4 + 2] = bvh->verts[i * 3 + 2];
+        verts4[i * 4 + 3] = 0.0f;
+      }
+    if (norm4) {
+      for (int k = 0; k < 3; ++k) norm4[k] = bvh->norm_center[k];
+      norm4[3] = bvh->norm_scale;
     }
-    ++out;
-  }
-  if (verts4)
-    for (size_t i = 0; i < bvh->nv; ++i) {
-      verts4[i * 4 + 0] = bvh->verts[i * 3 + 0];
-      verts4[i * 4 + 1] = bvh->verts[i * 3 + 1];
-      verts4[i * 4 + 2] = bvh->verts[i * 3 + 2];
-      verts4[i * 4 + 3] = 0.0f;
-    }
-  if (norm4) {
-    for (int k = 0; k < 3; ++k) norm4[k] = bvh->norm_center[k];
-    norm4[3] = bvh->norm_scale;
-  }
-  return PK_OK;
+    return PK_OK;
+  });
 }
 
 int pk_bvh_export_wide4(const pk_bvh* bvh, void* nodes64, int64_t* count) {
-  if (!bvh || !count) {
-    pk_set_error("pk_bvh_export_wide4: null argument");
-    return PK_E_INVALID;
-  }
-  if (bvh->instanced) {
-    pk_set_error("pk_bvh_export_wide4: an instanced BVH has no 4-wide layout");
-    return PK_E_INVALID;
-  }
-  std::vector<pk::WNode4> n4;
-  pk_make_wide4(bvh->wnodes, n4);
-  if (nodes64) {
-    if (*count < (int64_t)n4.size()) {
-      pk_set_error("pk_bvh_export_wide4: output holds fewer than " + std::to_string(n4.size()) + " nodes");
-      return PK_E_LIMIT;
+  return pk_guard("pk_bvh_export_wide4", [&]() -> int {
+    if (!bvh || !count) {
+      pk_set_error("pk_bvh_export_wide4: null argument");
+      return PK_E_INVALID;
     }
-    std::memcpy(nodes64, n4.data(), n4.size() * sizeof(pk::WNode4));
-  }
-  *count = (int64_t)n4.size();
-  return PK_OK;
+    if (bvh->instanced) {
+      pk_set_error("pk_bvh_export_wide4: an instanced BVH has no 4-wide layout");
+      return PK_E_INVALID;
+    }
+    std::vector<pk::WNode4> n4;
+    pk_make_wide4(bvh->wnodes, n4);
+    if (nodes64) {
+      if (*count < (int64_t)n4.size()) {
+        pk_set_error("pk_bvh_export_wide4: output holds fewer than " + std::to_string(n4.size()) + " nodes");
+        return PK_E_LIMIT;
+      }
+      std::memcpy(nodes64, n4.data(), n4.size() * sizeof(pk::WNode4));
+    }
+    *count = (int64_t)n4.size();
+    return PK_OK;
+  });
 }
 
 int pk_bvh_export_wide(const pk_bvh* bvh, void* nodes, void* tris, int32_t* tri_index) {
-  if (!bvh) {
-    pk_set_error("pk_bvh_export_wide: null bvh");
-    return PK_E_INVALID;
-  }
-  if (nodes) std::memcpy(nodes, bvh->wnodes.data(), bvh->wnodes.size() * sizeof(pk::WNode));
-  if (tris) std::memcpy(tris, bvh->wtris.data(), bvh->wtris.size() * sizeof(pk::TriRecord));
-  if (tri_index) std::memcpy(tri_index, bvh->wtri_index.data(), bvh->wtri_index.size() * sizeof(int32_t));
-  return PK_OK;
+  return pk_guard("pk_bvh_export_wide", [&]() -> int {
+    if (!bvh) {
+      pk_set_error("pk_bvh_export_wide: null bvh");
+      return PK_E_INVALID;
+    }
+    if (nodes) std::memcpy(nodes, bvh->wnodes.data(), bvh->wnodes.size() * sizeof(pk::WNode));
+    if (tris) std::memcpy(tris, bvh->wtris.data(), bvh->wtris.size() * sizeof(pk::TriRecord));
+    if (tri_index) std::memcpy(tri_index, bvh->wtri_index.data(), bvh->wtri_index.size() * sizeof(int32_t));
+    return PK_OK;
+  });
 }
 
 }  // extern "C"

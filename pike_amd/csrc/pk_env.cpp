@@ -169,23 +169,25 @@ int pk_build_env_tables(const pk_scene_desc* d, EnvTables* e, std::string* err) 
 }
 
 extern "C" int pk_env_tables(const pk_scene_desc* d, int64_t* info4, int32_t* bins2, int32_t* coords2, float* pdf) {
-  if (!d || !info4) {
-    pk_set_error("pk_env_tables: null argument");
-    return PK_E_INVALID;
-  }
-  EnvTables e;
-  std::string err;
-  int rc = pk_build_env_tables(d, &e, &err);
-  if (rc) {
-    pk_set_error("pk_env_tables: " + err);
-    return rc;
-  }
-  info4[0] = (int64_t)(e.bins.size() / 2);
-  info4[1] = (int64_t)(e.coords.size() / 2);
-  info4[2] = e.width;
-  info4[3] = e.height;
-  if (bins2) std::copy(e.bins.begin(), e.bins.end(), bins2);
-  if (coords2) std::copy(e.coords.begin(), e.coords.end(), coords2);
-  if (pdf) std::copy(e.pdf.begin(), e.pdf.end(), pdf);
-  return PK_OK;
+  return pk_guard("pk_env_tables", [&]() -> int {
+    if (!d || !info4) {
+      pk_set_error("pk_env_tables: null argument");
+      return PK_E_INVALID;
+    }
+    EnvTables e;
+    std::string err;
+    int rc = pk_build_env_tables(d, &e, &err);
+    if (rc) {
+      pk_set_error("pk_env_tables: " + err);
+      return rc;
+    }
+    info4[0] = (int64_t)(e.bins.size() / 2);
+    info4[1] = (int64_t)(e.coords.size() / 2);
+    info4[2] = e.width;
+    info4[3] = e.height;
+    if (bins2) std::copy(e.bins.begin(), e.bins.end(), bins2);
+    if (coords2) std::copy(e.coords.begin(), e.coords.end(), coords2);
+    if (pdf) std::copy(e.pdf.begin(), e.pdf.end(), pdf);
+    return PK_OK;
+  });
 }

@@ -614,17 +614,10 @@ static int build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshe
   return PK_OK;
 }
 
-// No exception leaves the C ABI: host memory or threads running out end the
-// build with PK_E_OOM.
 extern "C" int pk_bvh_build_instanced(const pk_scene_desc* scene, const pk_mesh_desc* meshes, uint32_t num_meshes,
                                       const pk_instance_desc* instances, uint32_t num_instances, int num_threads,
                                       uint32_t flags, pk_bvh** out) {
-  try {
+  return pk_guard("pk_bvh_build_instanced", [&]() {
     return build_instanced(scene, meshes, num_meshes, instances, num_instances, num_threads, flags, out);
-  } catch (const std::bad_alloc&) {
-    pk_set_error("pk_bvh_build_instanced: out of host memory");
-  } catch (const std::exception& e) {
-    pk_set_error(std::string("pk_bvh_build_instanced: ") + e.what());
-  }
-  return PK_E_OOM;
+  });
 }

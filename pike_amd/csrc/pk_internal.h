@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -79,3 +81,23 @@ struct EnvTables {
 
 int pk_build_env_tables(const pk_scene_desc* d, EnvTables* out, std::string* err);
 void pk_set_error(const std::string& msg);
+
+// Every C entry point runs its body through pk_guard: no C++ exception leaves
+// the C ABI.  Host memory (or threads) running out ends the call with
+// PK_E_OOM and the error text; what the call had allocated may stay allocated.
+template <class F> int pk_guard(const char* what, F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    try {
+      pk_set_error(std::string(what) + ": out of host memory");
+    } catch (...) {
+    }
+  } catch (const std::exception& e) {
+    try {
+      pk_set_error(std::string(what) + ": " + e.what());
+    } catch (...) {
+    }
+  }
+  return PK_E_OOM;
+}

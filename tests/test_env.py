@@ -105,3 +105,36 @@ def test_cdf_sampler_pdf_normalized():
     pn, rn = p / p.sum(), r / r.sum()
     assert np.abs(pn - rn).sum() < 1e-4
     assert np.abs(pn - rn).max() < 2 * 2.0 ** -24
+
+
+def test_env_tables_out_of_host_memory_fails_cleanly():
+    """pk_env_tables, like every C entry point, returns PK_E_OOM with a
+    message when host memory runs out instead of ending the process: a child
+    process limits its address space to 48 MB above what it holds, asks for
+    the tables of a 4096x2048 float map, then for a small map's."""
+    import pathlib
+    import subprocess
+    import sys
+    code = r"""
+import ctypes, resource
+import numpy as np
+import pike_amd
+from pike_amd import scenes
+big = scenes.config1(seed=1, num_spheres=2, subdiv=0)
+big.env = np.random.default_rng(1).random((2048, 4096, 4), dtype=np.float32)
+small = scenes.config1(seed=1, num_spheres=2, subdiv=0)
+small.env = np.random.default_rng(2).random((16, 32, 4), dtype=np.float32)
+db, ds = big.desc(), small.desc()
+info = np.zeros(4, np.int64)
+lib = pike_amd.lib()
+vm = [int(l.split()[1]) * 1024 for l in open("/proc/self/status") if l.startswith("VmSize:")][0]
+resource.setrlimit(resource.RLIMIT_AS, (vm + (48 << 20), resource.getrlimit(resource.RLIMIT_AS)[1]))
+rc = lib.pk_env_tables(ctypes.byref(db), info.ctypes.data, None, None, None)
+print("big", rc, lib.pk_last_error().decode())
+print("small", lib.pk_env_tables(ctypes.byref(ds), info.ctypes.data, None, None, None), int(info[1]))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       cwd=str(pathlib.Path(__file__).resolve().parents[1]))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "big -3 pk_env_tables: out of host memory" in r.stdout, r.stdout
+    assert "small 0 512" in r.stdout, r.stdout
