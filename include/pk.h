@@ -33,6 +33,9 @@
 extern "C" {
 #endif
 
+/* Every entry point returns one of these (pk_last_error() holds the text of
+ * the last failure on the calling thread); none lets a C++ exception escape:
+ * host memory or threads running out return PK_E_OOM. */
 #define PK_OK 0
 #define PK_E_INVALID (-1)     /* bad argument or handle */
 #define PK_E_NO_DEVICE (-2)   /* HIP device missing or unusable */
