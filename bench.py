@@ -673,7 +673,7 @@ def main():
                     help="rays of the soup checked against the oracle's canonical hits (0: none)")
     ap.add_argument("--spp-per-pass", type=int, default=1,
                     help="samples per pixel traced per wavefront pass of the 1080p render (without autotuning)")
-    ap.add_argument("--max-spp-per-pass", type=int, default=128, help="autotuner's largest samples per pass")
+    ap.add_argument("--max-spp-per-pass", type=int, default=512, help="autotuner's largest samples per pass")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-diff-checks", action="store_true")
     ap.add_argument("--diff-res", type=int, default=512)

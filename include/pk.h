@@ -213,7 +213,8 @@ int pk_ctx_get_scheduler(const pk_ctx* ctx, int* scheduler);
  * after. */
 int pk_autotune_render(pk_frame* frame, pk_scene* scene, uint32_t max_depth, uint32_t frames, float* ms2);
 /* Queue capacity: chooses the frame's samples per pass (1, 2, 4, ... up to
- * max_samples, below 2^28 paths) by timing max_samples samples of
+ * max_samples, below 2^30 paths and within device memory: a size whose
+ * queues cannot be allocated ends the search) by timing max_samples samples of
  * graph-replayed passes of this scene, camera and resolution; leaves the
  * fastest set and the frame reset.  ms2 = milliseconds per sample at the
  * frame's previous setting (0 if it was not among the candidates) and at the
@@ -343,7 +344,7 @@ int pk_render(pk_frame* frame, pk_scene* scene, const pk_render_params* params);
 /* Samples per pass (no reference counterpart; default 1): each pass of
  * pk_render and of the per-pass entry points traces `samples` consecutive
  * samples of every pixel as one wavefront (width*height*samples paths, below
- * 2^28), so small frames fill the GPU.  The accumulated image equals that of
+ * 2^30; 176 bytes of queues per path), so small frames fill the GPU.  The accumulated image equals that of
  * one-sample frames bit for bit.  On a batch frame (pk_batch_create) it sets
  * the samples per round of pk_batch_render (total view pixels * samples
  * paths).  Reallocates the frame's queues. */

@@ -912,8 +912,8 @@ int pk_frame_set_samples_per_pass(pk_frame* f, uint32_t samples) {
       return PK_E_INVALID;
     }
     const uint64_t paths = (uint64_t)f->W * f->H * samples;
-    if (paths >= (1ull << 28)) {
-      pk_set_error("pk_frame_set_samples_per_pass: width*height*samples must stay below 2^28 paths");
+    if (paths >= (1ull << 30)) {
+      pk_set_error("pk_frame_set_samples_per_pass: width*height*samples must stay below 2^30 paths");
       return PK_E_LIMIT;
     }
     if (samples == f->dev.S) return PK_OK;
@@ -2044,7 +2044,7 @@ extern "C" int pk_autotune_render(pk_frame* f, pk_scene* s, uint32_t depth, uint
 // Queue capacity: samples per pass in 1, 2, 4, ... up to max_samples (and
 // below kTunePathCap paths), each timed over max_samples samples of
 // graph-replayed passes; keeps the fastest per sample.
-constexpr uint64_t kTunePathCap = 1ull << 28;  // the frame's own limit
+constexpr uint64_t kTunePathCap = 1ull << 30;  // the frame's own limit
 
 extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t depth, uint32_t max_samples,
                                             float* ms2) {
@@ -2066,7 +2066,10 @@ extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t d
     int rc = PK_OK;
     for (uint32_t S = 1; S <= max_samples; S *= 2) {
       if ((uint64_t)f->W * f->H * S >= kTunePathCap) break;
-      if ((rc = pk_frame_set_samples_per_pass(f, S))) break;
+      if ((rc = pk_frame_set_samples_per_pass(f, S))) {
+        if (rc == PK_E_OOM && S > 1) rc = PK_OK;  // device memory bounds the capacity: the search ends there
+        break;
+      }
       pk_render_params warm = p;
       warm.num_frames = S;
       if ((rc = pk_render(f, s, &warm))) break;

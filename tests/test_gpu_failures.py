@@ -124,6 +124,17 @@ def test_memory_cap_fails_cleanly():
         fr.render(scene, max_depth=3, num_frames=4)
         ctx.set_memory_limit(0)
         assert np.isfinite(fr.read_image()).all() and fr.read_image()[..., :3].max() > 0
+        # the samples-per-pass search ends at the first size whose queues do
+        # not fit (1 and 2 fit, 4 needs the old and new queues together)
+        fr.set_samples_per_pass(1)
+        u1 = ctx.memory_used()
+        fr.set_samples_per_pass(2)
+        per_sample = ctx.memory_used() - u1
+        fr.set_samples_per_pass(1)
+        ctx.set_memory_limit(u1 + 3 * per_sample)
+        fr.autotune_samples(scene, max_depth=3, max_samples=64)
+        assert fr.samples_per_pass in (1, 2)
+        ctx.set_memory_limit(0)
         used = ctx.memory_used()
         fr.close()
         scene.close()
