@@ -161,11 +161,13 @@ def run_render(args, pike_amd, scenes):
         f"{rays_total / dt / 1e6:.0f} Mrays/s, overflow={cnt['stack_overflow']}")
     # the first sample alone (a one-sample pass), for the CPU leg's
     # full-size comparison with the oracle's first frame
-    chk = pike_amd.Frame(ctx, W, H)
-    chk.set_camera(cam)
-    chk.render(scene, max_depth=depth, first_sample=0, num_frames=1, use_graph=False)
-    first_frame = chk.read_image()
-    del chk
+    first_frame = None
+    if not args.no_cpu:
+        chk = pike_amd.Frame(ctx, W, H)
+        chk.set_camera(cam)
+        chk.render(scene, max_depth=depth, first_sample=0, num_frames=1, use_graph=False)
+        first_frame = chk.read_image()
+        del chk
     # kernel-level profile: counted run for algorithmic bytes, uncounted for
     # time, over whole passes
     pf = args.profile_frames
