@@ -331,11 +331,11 @@ def run_batching(args, pike_amd, scenes, ctx):
         t0 = time.perf_counter()
         plain_ms()
         t_plain_ms = time.perf_counter() - t0
-        # batched rounds of 1, 2, 4, ... samples per view pixel (below 2^28
+        # batched rounds of 1, 2, 4, ... samples per view pixel (below 2^30
         # paths per round, the frame's limit); the fastest is the batched figure
         t_round = {}
-        for spr in (1, 2, 4, 8):
-            if total_px * spr >= (1 << 28):
+        for spr in (1, 2, 4, 8, 16, 32):
+            if total_px * spr >= (1 << 30):
                 break
             batch.set_samples_per_round(spr)
             batched()  # warm-up: records this shape's graph
