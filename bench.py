@@ -234,7 +234,14 @@ def run_instanced(args, pike_amd, scenes, ctx, spp_per_pass):
     scene = pike_amd.Scene(ctx, data, bvh)
     frame = pike_amd.Frame(ctx, args.width, args.height)
     frame.set_camera(scenes.normalized_camera(data.camera, bvh.normalizer()))
-    frame.set_samples_per_pass(spp_per_pass)
+    while True:  # the render leg's pass size, halved while its queues do not fit
+        try:
+            frame.set_samples_per_pass(spp_per_pass)
+            break
+        except pike_amd.PikeError as e:
+            if e.code != pike_amd._ffi.PK_E_OOM or spp_per_pass == 1:
+                raise
+            spp_per_pass //= 2
     if args.scheduler == "auto":
         frame.autotune_scheduler(scene, max_depth=args.depth, samples=2 * spp_per_pass)
     else:
