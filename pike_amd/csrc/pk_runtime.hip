@@ -2066,6 +2066,9 @@ extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t d
     int rc = PK_OK;
     for (uint32_t S = 1; S <= max_samples; S *= 2) {
       if ((uint64_t)f->W * f->H * S >= kTunePathCap) break;
+      // down to one sample first: a resize holds the old queues and the new
+      // ones together, and the largest sizes only fit alone
+      if (S > 2 && (rc = pk_frame_set_samples_per_pass(f, 1))) break;
       if ((rc = pk_frame_set_samples_per_pass(f, S))) {
         if (rc == PK_E_OOM && S > 1) rc = PK_OK;  // device memory bounds the capacity: the search ends there
         break;
@@ -2086,7 +2089,8 @@ extern "C" int pk_autotune_samples_per_pass(pk_frame* f, pk_scene* s, uint32_t d
         best_S = S;
       }
     }
-    int rc2 = pk_frame_set_samples_per_pass(f, best_S);
+    int rc2 = best_S == f->dev.S ? PK_OK : pk_frame_set_samples_per_pass(f, 1);
+    if (!rc2) rc2 = pk_frame_set_samples_per_pass(f, best_S);
     if (rc || rc2) return rc ? rc : rc2;
     if (ms2) {
       ms2[0] = at_start;
