@@ -266,10 +266,13 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
   bool active = false, done = false, exhausted = false;
   uint32_t slot = 0;
   // slots this wave has claimed and not yet started (wave-uniform); claims
-  // take kClaimChunk slots per atomic, since atomics on one counter
+  // take `claim` slots per atomic, since atomics on one counter
   // serialize at the L2
   uint32_t pool_lo = 0, pool_hi = 0;
   bool claimed_all = false;
+  // closest-hit launches of a quarter billion rays or more claim twice as
+  // many slots per atomic: their tails are short next to their work
+  const uint32_t claim = !kAnyHit && n >= (1u << 28) ? 2u * kClaimChunk : kClaimChunk;
   while (true) {
     if (wave_step<kAnyHit, kCount, kInst, kLds, kNW>(S, r, active, stk, st)) {
       active = false;
@@ -291,10 +294,10 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
     bool got = false;
     if (have < nidle && !claimed_all) {
       uint32_t b = 0;
-      if (lane == 0) b = atomicAdd(cursor, kClaimChunk);
+      if (lane == 0) b = atomicAdd(cursor, claim);
       fresh = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(b, 0));  // wave-uniform: a scalar
       got = true;
-      claimed_all = fresh >= n || n - fresh <= kClaimChunk;
+      claimed_all = fresh >= n || n - fresh <= claim;
     }
     if (!active) {
       // idle lanes take the pool's slots first, then the fresh chunk's
@@ -310,7 +313,7 @@ __device__ inline void trace_loop(const DevScene& S, uint32_t n, uint32_t* curso
       pool_lo += nidle;
     } else if (got) {
       pool_lo = fresh + (nidle - have);
-      pool_hi = fresh + kClaimChunk;
+      pool_hi = fresh + claim;
     } else {
       pool_lo = pool_hi;
     }

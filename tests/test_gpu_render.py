@@ -200,3 +200,26 @@ def test_converged_image_matches_fp64_oracle(ctx, sampler):
     rel = np.abs(gpu - ref).sum() / np.abs(ref).sum()
     print("converged image mean relative L1 vs FP64:", rel)
     assert rel < 1e-3, rel
+
+
+def test_quarter_billion_path_pass_equals_small_passes(ctx):
+    """A pass of 2^28 paths (512x512 at 1024 samples: 47 GB of queues) takes
+    the closest-hit kernel's large-launch claim size; its image equals that of
+    16-sample passes bit for bit."""
+    env = scenes.lognormal_sun_env(64, 32, seed=4, sigma=1.0, sun_deg=6.0, sun_scale=30.0)
+    data = scenes.mixed_scene(9, 3000, env=env)
+    data.camera = scenes.camera((0.5, 0.5, 1.4), (0.5, 0.05, 0.45), 1.0)
+    bvh, scene, _, cam = _setup(ctx, data)
+    W = H = 512
+    imgs = []
+    for S in (1024, 16):
+        fr = pike_amd.Frame(ctx, W, H)
+        fr.set_camera(cam)
+        fr.set_samples_per_pass(S)
+        fr.render(scene, max_depth=2, first_sample=0, num_frames=1024)
+        c = fr.counters()
+        assert c["frames"] == 1024 and c["stack_overflow"] == 0
+        imgs.append(fr.read_image())
+        fr.close()
+    assert W * H * 1024 >= 1 << 28
+    np.testing.assert_array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
