@@ -220,9 +220,10 @@ __device__ inline void add_color(const FrameDev& F, uint32_t ns, uint32_t pid, v
 // owns one ray at a time; a lane that finishes stays idle until at least
 // `refill_min` lanes of the wave are idle, then the wave compacts the
 // finished results (flush) and refills the idle lanes from the slots it has
-// claimed, claiming kClaimChunk more with one atomic when those run out.
-// This keeps lanes busy on incoherent rays while the queue cursor, whose
-// atomics serialize at the L2, sees one atomic per 256 rays; each wave also
+// claimed, claiming kClaimChunk (2 kClaimChunk in the largest closest-hit
+// launches) more with one atomic when those run out.  This keeps lanes busy
+// on incoherent rays while the queue cursor, whose atomics serialize at the
+// L2, sees one atomic per 256 or 512 rays; each wave also
 // works through contiguous (neighbouring-pixel) slots.  Every wave leaves the
 // loop once the queue is exhausted and its own lanes are done.
 constexpr uint32_t kClaimChunk = 256;  // slots per cursor atomic (>= 64)
